@@ -1,0 +1,233 @@
+"""caps_amd: the CaPS plane-sphere round-trip-matrix path on MI355X (gfx950).
+
+Python view of ``libcaps_amd.so`` (the libcaps C ABI; see include/libcaps.h).
+Every numerical call runs the library's HIP kernels; importing this module
+without the built library, or calling it without a usable gfx950 GPU, fails
+loudly (the library aborts with a message, as the reference's TERMINATE
+does).  There is no CPU fallback.
+
+    from caps_amd import Caps
+    c = Caps(R=200e-6, L=1e-6, ldim=1200)
+    c.set_drude(9.0, 0.035)          # eV, both bodies
+    c.logdetD(165.4564, 0)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcaps_amd.so")
+
+HBAR_EV = 6.582119514e-16   # ħ in eV s (reference src/include/constants.h:46)
+CAPS_C = 299792458.0
+KB, HBAR = 1.38064852e-23, 1.0545718e-34
+
+DETALG = {"HODLR": 0, "LU": 1, "QR": 2, "CHOLESKY": 3}
+
+_lib = None
+
+
+def lib():
+    """The loaded libcaps_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libcaps_amd.so is not built; run `python caps_amd/build.py` "
+                               "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        d, i, p = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        sigs = {
+            "caps_init": (p, [d, d]), "caps_free": (None, [p]),
+            "caps_set_ldim": (i, [p, i]), "caps_get_ldim": (i, [p]),
+            "caps_set_epsrel": (i, [p, d]), "caps_get_epsrel": (d, [p]),
+            "caps_set_detalg": (i, [p, i]), "caps_get_detalg": (i, [p]),
+            "caps_set_epsilonm1": (None, [p, p, p]),
+            "caps_set_epsilonm1_plate": (None, [p, p, p]),
+            "caps_set_epsilonm1_sphere": (None, [p, p, p]),
+            "caps_epsilonm1_plate": (d, [p, d]), "caps_epsilonm1_sphere": (d, [p, d]),
+            "caps_lnLambda": (d, [i, i, i]),
+            "caps_estimate_lminmax": (i, [p, i, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+            "caps_mie": (None, [p, d, i, dp, dp]), "caps_mie_perf": (None, [p, d, i, dp, dp]),
+            "caps_fresnel": (None, [p, d, d, dp, dp]),
+            "caps_logdetD": (d, [p, d, i]),
+            "caps_logdetD_batch": (i, [p, dp, ip, i, dp]),
+            "caps_last_timing": (None, [p, dp]),
+            "caps_M_init": (p, [p, i, d]), "caps_M_elem": (d, [p, i, i, ctypes.c_char, ctypes.c_char]),
+            "caps_M_free": (None, [p]), "caps_kernel_M": (d, [i, i, p]),
+            "caps_ht_drude": (d, [p]),
+            "caps_free_energy": (d, [p, d, d, d, i, i, ctypes.POINTER(ctypes.c_long)]),
+            "capsx_lu_logdet_batch": (i, [dp, i, i, dp]),
+            "capsx_mfma_probe": (i, [dp, dp, dp]),
+            "capsx_block_tables": (i, [p, d, i, ip, ip, dp, ip, ip, dp, ctypes.POINTER(ctypes.c_byte), i, i]),
+            "capsx_block_nodes": (i, [p, d, i]),
+            "capsx_M_copy": (i, [p, dp]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Caps:
+    """A caps_t object (reference: src/include/libcaps.h:41-78)."""
+
+    def __init__(self, R, L, ldim=None, epsrel=None, detalg=None):
+        self.ptr = lib().caps_init(R, L)
+        if not self.ptr:
+            raise ValueError("caps_init: R and L must be positive")
+        self.R, self.L = R, L
+        self._ud = None
+        if ldim is not None:
+            self.set_ldim(ldim)
+        if epsrel is not None:
+            lib().caps_set_epsrel(self.ptr, epsrel)
+        if detalg is not None:
+            self.set_detalg(detalg)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().caps_free(self.ptr)
+            self.ptr = None
+
+    @property
+    def ldim(self):
+        return lib().caps_get_ldim(self.ptr)
+
+    def set_ldim(self, ldim):
+        if not lib().caps_set_ldim(self.ptr, ldim):
+            raise ValueError("ldim must be >= 1")
+
+    def set_detalg(self, name):
+        lib().caps_set_detalg(self.ptr, DETALG[name] if isinstance(name, str) else int(name))
+
+    def set_drude(self, omegap_eV, gamma_eV, body="both"):
+        """Drude metal ε(iξ)-1 = ωp²/(ξ(ξ+γ)) (the reference's
+        caps_epsilonm1_drude, userdata {ωp, γ} in rad/s)."""
+        ud = (ctypes.c_double * 2)(omegap_eV / HBAR_EV, gamma_eV / HBAR_EV)
+        self._ud = getattr(self, "_ud_list", []) + [ud]
+        self._ud_list = self._ud
+        fn = ctypes.cast(lib().caps_epsilonm1_drude, ctypes.c_void_p)
+        setter = {"both": lib().caps_set_epsilonm1, "plate": lib().caps_set_epsilonm1_plate,
+                  "sphere": lib().caps_set_epsilonm1_sphere}[body]
+        setter(self.ptr, fn, ctypes.cast(ud, ctypes.c_void_p))
+
+    def lminmax(self, m):
+        a, b = ctypes.c_size_t(), ctypes.c_size_t()
+        lib().caps_estimate_lminmax(self.ptr, m, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def logdetD(self, xi_, m):
+        return lib().caps_logdetD(self.ptr, float(xi_), int(m))
+
+    def logdetD_batch(self, xis, ms):
+        xis = np.ascontiguousarray(xis, dtype=np.float64)
+        ms = np.ascontiguousarray(ms, dtype=np.int32)
+        if xis.shape != ms.shape:
+            raise ValueError("xi_ and m must have the same length")
+        out = np.empty(len(xis))
+        rc = lib().caps_logdetD_batch(self.ptr, _dptr(xis), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      len(xis), _dptr(out))
+        if rc != 0:
+            raise ValueError("caps_logdetD_batch rejected its input")
+        return out
+
+    def last_timing(self):
+        t = np.zeros(6)
+        lib().caps_last_timing(self.ptr, _dptr(t))
+        return dict(zip(["mie", "K", "I", "assemble", "lu", "total"], t))
+
+    def mie(self, xi_, l):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        lib().caps_mie(self.ptr, xi_, l, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def fresnel(self, xi_, k_):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        lib().caps_fresnel(self.ptr, xi_, k_, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def matrix(self, xi_, m):
+        """The round-trip matrix M (N x N, N = 2 ldim) and lmin, via caps_M_init."""
+        M = lib().caps_M_init(self.ptr, int(m), float(xi_))
+        N = 2 * self.ldim
+        out = np.empty(N * N)
+        lib().capsx_M_copy(M, _dptr(out))
+        lmin = ctypes.cast(M, ctypes.POINTER(ctypes.c_int))[3]   # caps_M_t: caps*, m, lmin
+        obj = _MObj(M)
+        return out.reshape(N, N, order="F"), lmin, obj
+
+    def tables(self, xi_, m):
+        """K table (log|K_ν^p|, ν = mu..) and I table (log|I|, sign) of a block."""
+        mu, nK, lo, nl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib().capsx_block_tables(self.ptr, xi_, m, ctypes.byref(mu), ctypes.byref(nK), None, ctypes.byref(lo),
+                                 ctypes.byref(nl), None, None, 0, 0)
+        lnK = np.empty(2 * nK.value)
+        logI = np.empty(2 * nl.value * nl.value)
+        sgn = np.empty(2 * nl.value * nl.value, dtype=np.int8)
+        rc = lib().capsx_block_tables(self.ptr, xi_, m, ctypes.byref(mu), ctypes.byref(nK), _dptr(lnK),
+                                      ctypes.byref(lo), ctypes.byref(nl), _dptr(logI),
+                                      sgn.ctypes.data_as(ctypes.POINTER(ctypes.c_byte)), nK.value, nl.value)
+        assert rc == 0
+        return {"mu": mu.value, "lnK": lnK.reshape(2, nK.value), "lo": lo.value,
+                "logI": logI.reshape(2, nl.value, nl.value), "sgnI": sgn.reshape(2, nl.value, nl.value)}
+
+    def nodes(self, xi_, m):
+        return lib().capsx_block_nodes(self.ptr, xi_, m)
+
+    def ht_drude(self):
+        return lib().caps_ht_drude(self.ptr)
+
+    def free_energy(self, T, cutoff=1e-9, epsrel=1e-6, mode="batched", drude_xi0=True):
+        """F (L+R)/(ħc) at temperature T with the reference driver's sums
+        (src/caps.c:307-396, 722-850); returns (F, number of determinants)."""
+        nb = ctypes.c_long()
+        F = lib().caps_free_energy(self.ptr, T, cutoff, epsrel, {"plain": 0, "batched": 1}[mode],
+                                   1 if drude_xi0 else 0, ctypes.byref(nb))
+        return F, nb.value
+
+
+class _MObj:
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def elem(self, l1, l2, p1, p2):
+        return lib().caps_M_elem(self.ptr, l1, l2, p1.encode(), p2.encode())
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().caps_M_free(self.ptr)
+            self.ptr = None
+
+
+def lu_logdet_batch(mats):
+    """log|det A| of square matrices on the GPU's batched LU (test hook)."""
+    mats = np.asarray(mats, dtype=np.float64)
+    n, N, _ = mats.shape
+    buf = np.ascontiguousarray(np.transpose(mats, (0, 2, 1)))   # column-major per matrix
+    out = np.empty(n)
+    if lib().capsx_lu_logdet_batch(_dptr(buf), N, n, _dptr(out)) != 0:
+        raise ValueError("bad LU batch")
+    return out
+
+
+def mfma_probe(A, B):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    D = np.empty((16, 16))
+    lib().capsx_mfma_probe(_dptr(A), _dptr(B), _dptr(D))
+    return D
+
+
+def T_scaled(R, L, T):
+    """2π k_B (R+L) T/(ħc): the Matsubara step in units of c/(L+R)."""
+    import math
+    return 2 * math.pi * KB * (R + L) * T / (HBAR * CAPS_C)

@@ -1,0 +1,60 @@
+// Layout of one batch of (ξ, m) round-trip-matrix blocks in HBM.
+//
+// A batch is a set of blocks of one caps object.  The host fills one
+// BlockDesc per block; every kernel takes the descriptor array and the arena
+// base pointer and finds its block through blockIdx (the batch dimension).
+// All per-block tables live in one device arena, at the offsets recorded in
+// the descriptor (in bytes, 256-byte aligned).
+#pragma once
+#include <stdint.h>
+
+namespace capsd {
+
+struct BlockDesc {
+    // physics of the block
+    double xi_;          // ξ(L+R)/c
+    double alpha;        // 2 ξ(L+R)/c
+    double epsm1_plate;  // ε(iξ)-1 of the plate (+inf: perfect reflector)
+    double epsm1_sphere; // ε(iξ)-1 of the sphere
+    double LbyR;
+    int m, lmin, ldim, N;   // N = 2 ldim, window l in [lmin, lmin+ldim)
+    int is_pr;              // plate is a perfect reflector: I_TE = -|I_TM|
+    // K_ν table: ν in [mu, nu_max]
+    int mu, nu_max, nK;
+    // I table: l in [lo, lo+nl)
+    int lo, nl;
+    // quadrature grid of the K integrals: v_k = v0 + k h, k < n_nodes,
+    // x(v) = 1 + A² softplus(v)²
+    double v0, h, A;
+    int n_nodes, n_waves;
+    // matrix
+    int lda;
+    int pad_;
+    // byte offsets into the arena
+    int64_t off_mie;    // 2 * ldim doubles: log|a_l|, log|b_l|
+    int64_t off_kpart;  // n_waves * nK * KPart
+    int64_t off_k;      // 2 pols * nK * (double mant, int exp, double lnK)
+    int64_t off_i;      // 2 pols * nl*nl doubles, then 2 pols * nl*nl int8 signs
+    int64_t off_mat;    // lda * N doubles (column-major)
+    int64_t off_diag;   // N doubles: diagonal of M
+    int64_t off_piv;    // N ints: LU pivots
+};
+
+// Partial K sums of one wave of quadrature nodes for one ν.
+struct KPart {
+    double s_te, s_tm;
+    int e;
+    int pad_;
+};
+
+// Per-block scalar results.
+struct BlockOut {
+    double trace;       // tr M (Kahan)
+    double logdet;      // log|det(1-M)| from the LU (NAN if skipped)
+    double frob2;       // ||M||_F^2 (LU mode only)
+    double tr2;         // tr M^2 (LU mode only)
+    int skip;           // LU skipped (trace shortcut)
+    int status;         // 0 ok; nonzero: numerical failure flag
+};
+
+}  // namespace capsd

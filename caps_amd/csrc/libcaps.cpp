@@ -1,0 +1,961 @@
+// Host side of libcaps_amd: the libcaps C ABI over the gfx950 kernels.
+//
+// One device context per caps_t (stream, events, a growable HBM arena,
+// pinned staging buffers), created on the first GPU call.  A batch of
+// (ξ, m) blocks is laid out in the arena (BlockDesc offsets), the pipeline
+// Mie -> K -> I -> assemble(1-M) -> trace -> LU runs asynchronously on the
+// context's stream, and only the per-block scalars come back.  The
+// determinant shortcuts of the reference (src/matrix.c:78-82, 144-145,
+// 494-505) are applied to those scalars on the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "blocks.h"
+#include "../../include/libcaps.h"
+
+using capsd::BlockDesc;
+using capsd::BlockOut;
+using capsd::KPart;
+
+extern "C" {
+void capsd_launch_mie(const BlockDesc*, char*, int, int, hipStream_t);
+void capsd_launch_K(const BlockDesc*, char*, const double*, int, int, int, int*, hipStream_t);
+void capsd_launch_I(const BlockDesc*, char*, const double*, int, int64_t, int*, hipStream_t);
+void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int, hipStream_t);
+void capsd_launch_trace(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
+void capsd_launch_mie_points(int, const double*, const int*, double, const double*, double*, hipStream_t);
+void capsd_launch_fresnel_points(int, const double*, const double*, const double*, double*, hipStream_t);
+void capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t);
+int capsd_lu_init(void);
+int capsd_lu_max_dim(void);
+int capsd_lu_nb(void);
+int capsd_lu_maxmove(void);
+void capsd_launch_mfma_probe(const double*, const double*, double*, hipStream_t);
+}
+
+#define TERMINATE(cond, ...)                                                                      \
+    do {                                                                                          \
+        if (cond) {                                                                               \
+            fprintf(stderr, "Fatal error: ");                                                     \
+            fprintf(stderr, __VA_ARGS__);                                                         \
+            fprintf(stderr, " (in %s, %s:%d)\n", __func__, __FILE__, __LINE__);                   \
+            abort();                                                                              \
+        }                                                                                         \
+    } while (0)
+
+#define HIPCHECK(x)                                                                               \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        TERMINATE(e_ != hipSuccess, "HIP error %s at %s", hipGetErrorString(e_), #x);             \
+    } while (0)
+
+namespace {
+
+struct LuSide {
+    int64_t off_panel, off_logs, off_moved;
+};
+
+double* g_lfac_dev = nullptr;
+std::once_flag g_dev_once;
+
+void device_init()
+{
+    std::call_once(g_dev_once, [] {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        TERMINATE(e != hipSuccess || n < 1, "libcaps_amd needs a gfx950 GPU; none usable (%s)",
+                  hipGetErrorString(e));
+        hipDeviceProp_t prop;
+        HIPCHECK(hipGetDeviceProperties(&prop, 0));
+        TERMINATE(strncmp(prop.gcnArchName, "gfx950", 6) != 0, "libcaps_amd is built for gfx950, device is %s",
+                  prop.gcnArchName);
+        std::vector<double> lf(1024);
+        for (int i = 0; i < 1024; i++) lf[i] = lgamma((double)i + 1.0);
+        HIPCHECK(hipMalloc(&g_lfac_dev, 1024 * sizeof(double)));
+        HIPCHECK(hipMemcpy(g_lfac_dev, lf.data(), 1024 * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHECK((hipError_t)capsd_lu_init());
+    });
+}
+
+struct Ctx {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[7];
+    char* arena = nullptr;
+    size_t arena_size = 0;
+    BlockDesc* d_desc = nullptr;
+    LuSide* d_side = nullptr;
+    BlockOut* d_out = nullptr;
+    int* d_status = nullptr;
+    int cap_blocks = 0;
+    BlockDesc* h_desc = nullptr;  // pinned
+    LuSide* h_side = nullptr;
+    BlockOut* h_out = nullptr;
+    int* h_status = nullptr;
+    double timing[6] = {0, 0, 0, 0, 0, 0};
+    std::mutex mu;
+};
+
+Ctx* ctx_of(caps_t* self)
+{
+    device_init();
+    if (!self->priv) {
+        Ctx* c = new Ctx();
+        HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
+        HIPCHECK(hipMalloc(&c->d_status, sizeof(int)));
+        HIPCHECK(hipHostMalloc((void**)&c->h_status, sizeof(int), hipHostMallocDefault));
+        self->priv = c;
+    }
+    return (Ctx*)self->priv;
+}
+
+void ctx_free(Ctx* c)
+{
+    if (!c) return;
+    hipStreamSynchronize(c->stream);
+    if (c->arena) hipFree(c->arena);
+    if (c->d_desc) hipFree(c->d_desc);
+    if (c->d_side) hipFree(c->d_side);
+    if (c->d_out) hipFree(c->d_out);
+    if (c->d_status) hipFree(c->d_status);
+    if (c->h_desc) hipHostFree(c->h_desc);
+    if (c->h_side) hipHostFree(c->h_side);
+    if (c->h_out) hipHostFree(c->h_out);
+    if (c->h_status) hipHostFree(c->h_status);
+    for (auto& e : c->ev) hipEventDestroy(e);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void ensure_blocks(Ctx* c, int n)
+{
+    if (n <= c->cap_blocks) return;
+    const int cap = std::max(n, 2 * c->cap_blocks);
+    if (c->d_desc) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        hipFree(c->d_desc);
+        hipFree(c->d_side);
+        hipFree(c->d_out);
+        hipHostFree(c->h_desc);
+        hipHostFree(c->h_side);
+        hipHostFree(c->h_out);
+    }
+    HIPCHECK(hipMalloc(&c->d_desc, cap * sizeof(BlockDesc)));
+    HIPCHECK(hipMalloc(&c->d_side, cap * sizeof(LuSide)));
+    HIPCHECK(hipMalloc(&c->d_out, cap * sizeof(BlockOut)));
+    HIPCHECK(hipHostMalloc((void**)&c->h_desc, cap * sizeof(BlockDesc), hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc((void**)&c->h_side, cap * sizeof(LuSide), hipHostMallocDefault));
+    HIPCHECK(hipHostMalloc((void**)&c->h_out, cap * sizeof(BlockOut), hipHostMallocDefault));
+    c->cap_blocks = cap;
+}
+
+void ensure_arena(Ctx* c, size_t bytes)
+{
+    if (bytes <= c->arena_size) return;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->arena) HIPCHECK(hipFree(c->arena));
+    c->arena = nullptr;
+    const size_t sz = std::max(bytes, (size_t)(c->arena_size * 1.25));
+    HIPCHECK(hipMalloc(&c->arena, sz));
+    c->arena_size = sz;
+}
+
+inline int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
+
+double env_double(const char* name, double dflt)
+{
+    const char* s = getenv(name);
+    return s ? atof(s) : dflt;
+}
+
+// Quadrature grid of the K integrals of one block (see k_K_partial):
+// x(v) = 1 + A² softplus(v)², A = 0.5/sqrt(α), step h, from v0 to the end of
+// the tail of the largest ν (integrand below e^{-70} of its peak).
+void grid_for(BlockDesc& d)
+{
+    const double alpha = d.alpha;
+    const double h = env_double("CAPS_AMD_QUAD_H", 0.45);
+    const double v0 = env_double("CAPS_AMD_QUAD_V0", -19.0);
+    const double A = 0.5 / sqrt(alpha);
+    const double nu = d.nu_max + 0.5;
+    const double xp = sqrt(1 + (nu / alpha) * (nu / alpha));
+    auto g = [&](double x) { return -alpha * x + nu * acosh(x); };
+    const double gp = g(xp);
+    double x = xp;
+    int guard = 0;
+    while (g(x) > gp - 70 && guard++ < 100000) x = 1 + (x - 1) * 1.05 + 1.0 / alpha;
+    x = std::max(x, 1 + 80.0 / alpha);
+    const double s = sqrt(x - 1) / A;
+    const double vhi = s > 30 ? s : log(expm1(s));
+    d.v0 = v0;
+    d.h = h;
+    d.A = A;
+    d.n_nodes = (int)ceil((vhi - v0) / h) + 1;
+    d.n_waves = (d.n_nodes + 63) / 64;
+}
+
+struct BatchPlan {
+    int n = 0, maxN = 0, max_ldim = 0, max_waves = 0, max_nK = 0;
+    int64_t max_ntri = 0;
+    size_t bytes = 0;
+};
+
+// Fill descriptors for blocks [0, n) and their arena offsets.
+BatchPlan plan_batch(caps_t* self, Ctx* c, const double* xi_, const int* m, int n, int with_lu)
+{
+    BatchPlan p;
+    p.n = n;
+    const int ldim = self->ldim;
+    const int N = 2 * ldim;
+    const int lda = (N + 15) & ~15;
+    const int maxmove = capsd_lu_maxmove(), nb = capsd_lu_nb();
+    int64_t off = 0;
+    double last_xi = NAN, ep = 0, es = 0;
+    for (int b = 0; b < n; b++) {
+        BlockDesc& d = c->h_desc[b];
+        memset(&d, 0, sizeof(d));
+        d.xi_ = xi_[b];
+        d.alpha = 2 * xi_[b];
+        if (!(xi_[b] == last_xi)) {
+            ep = caps_epsilonm1_plate(self, xi_[b]);
+            es = caps_epsilonm1_sphere(self, xi_[b]);
+            last_xi = xi_[b];
+        }
+        d.epsm1_plate = ep;
+        d.epsm1_sphere = es;
+        d.LbyR = self->LbyR;
+        d.m = m[b];
+        size_t lmin, lmax;
+        caps_estimate_lminmax(self, m[b], &lmin, &lmax);
+        d.lmin = (int)lmin;
+        d.ldim = ldim;
+        d.N = N;
+        d.lda = lda;
+        d.is_pr = std::isinf(caps_epsilonm1_plate(self, INFINITY)) ? 1 : 0;
+        d.mu = m[b] > 0 ? 2 * m[b] : 2;
+        d.lo = d.lmin - 1;
+        d.nl = ldim + 2;
+        d.nu_max = 2 * (d.lo + d.nl - 1);
+        d.nK = d.nu_max - d.mu + 1;
+        grid_for(d);
+        d.off_mie = off;
+        off = align256(off + 2 * (int64_t)ldim * 8);
+        d.off_kpart = off;
+        off = align256(off + (int64_t)d.n_waves * d.nK * sizeof(KPart));
+        d.off_k = off;
+        off = align256(off + 2 * (int64_t)d.nK * 24);
+        d.off_i = off;
+        off = align256(off + 2 * (int64_t)d.nl * d.nl * 9);
+        d.off_mat = off;
+        off = align256(off + (int64_t)lda * N * 8);
+        d.off_diag = off;
+        off = align256(off + (int64_t)N * 8);
+        d.off_piv = off;
+        off = align256(off + (int64_t)N * 4);
+        LuSide& s = c->h_side[b];
+        if (with_lu) {
+            s.off_panel = off;
+            off = align256(off + (int64_t)lda * nb * 8);
+            s.off_logs = off;
+            off = align256(off + (int64_t)N * 8);
+            s.off_moved = off;
+            off = align256(off + (1 + 2 * (int64_t)maxmove) * 4);
+        } else
+            s.off_panel = s.off_logs = s.off_moved = 0;
+        p.maxN = std::max(p.maxN, N);
+        p.max_ldim = std::max(p.max_ldim, ldim);
+        p.max_waves = std::max(p.max_waves, d.n_waves);
+        p.max_nK = std::max(p.max_nK, d.nK);
+        p.max_ntri = std::max(p.max_ntri, (int64_t)d.nl * (d.nl + 1) / 2);
+    }
+    p.bytes = (size_t)off;
+    return p;
+}
+
+size_t block_bytes_estimate(caps_t* self, int m)
+{
+    const int ldim = self->ldim, N = 2 * ldim;
+    const size_t lda = (N + 15) & ~15;
+    size_t lmin, lmax;
+    caps_estimate_lminmax(self, m, &lmin, &lmax);
+    const size_t nK = 2 * (lmin + ldim) + 4;
+    return (size_t)2 * (ldim + 2) * (ldim + 2) * 9 + lda * N * 8 + lda * 64 * 8 + nK * (24 * 2 + 24 * 64) +
+           (size_t)N * 32 + 65536;
+}
+
+// Run the pipeline for n blocks already planned; mode 0: LU path, 1: M only.
+void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts)
+{
+    const int n = p.n;
+    hipStream_t s = c->stream;
+    ensure_arena(c, p.bytes);
+    HIPCHECK(hipMemcpyAsync(c->d_desc, c->h_desc, n * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(c->d_side, c->h_side, n * sizeof(LuSide), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemsetAsync(c->d_status, 0, sizeof(int), s));
+    HIPCHECK(hipEventRecord(c->ev[0], s));
+    capsd_launch_mie(c->d_desc, c->arena, n, p.max_ldim, s);
+    HIPCHECK(hipEventRecord(c->ev[1], s));
+    capsd_launch_K(c->d_desc, c->arena, g_lfac_dev, n, p.max_waves, p.max_nK, c->d_status, s);
+    HIPCHECK(hipEventRecord(c->ev[2], s));
+    capsd_launch_I(c->d_desc, c->arena, g_lfac_dev, n, p.max_ntri, c->d_status, s);
+    HIPCHECK(hipEventRecord(c->ev[3], s));
+    capsd_launch_assemble(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, mode, s);
+    if (mode == 0) capsd_launch_trace(c->d_desc, c->arena, c->d_out, n, lu_mode_shortcuts, s);
+    HIPCHECK(hipEventRecord(c->ev[4], s));
+    if (mode == 0) capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, p.maxN, s);
+    HIPCHECK(hipEventRecord(c->ev[5], s));
+    HIPCHECK(hipGetLastError());
+    if (mode == 0) HIPCHECK(hipMemcpyAsync(c->h_out, c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    float ms;
+    for (int i = 0; i < 5; i++) {
+        HIPCHECK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+        c->timing[i] = ms;
+    }
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[5]));
+    c->timing[5] = ms;
+    TERMINATE(*c->h_status != 0, "non-finite k-integral (flags %d) in a batch of %d blocks (xi_=%g, m=%d ...)",
+              *c->h_status, n, c->h_desc[0].xi_, c->h_desc[0].m);
+}
+
+// log det(1-M) from the block scalars with the reference's shortcut rules.
+double finish_block(const BlockOut& o, int lu_mode)
+{
+    const double tr = o.trace;
+    if (fabs(tr) < 1e-8) return -tr;
+    if (lu_mode && o.skip == 2) return -tr - o.tr2 / 2;
+    const double ld = o.logdet;
+    if (!lu_mode && fabs(tr) > fabs(ld)) return -tr;
+    return ld;
+}
+
+bool is_lu_mode(detalg_t a) { return a == DETALG_LU || a == DETALG_QR; }
+
+// Largest number of blocks per launch sequence that fits the HBM budget.
+int chunk_blocks(caps_t* self, const int* m, int n)
+{
+    const double gb = env_double("CAPS_AMD_ARENA_GB", 64.0);
+    const size_t budget = (size_t)(gb * 1e9);
+    size_t acc = 0;
+    int k = 0;
+    for (; k < n; k++) {
+        acc += block_bytes_estimate(self, m[k]);
+        if (acc > budget && k > 0) break;
+    }
+    return std::max(k, 1);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+caps_t* caps_init(double R, double L)
+{
+    if (L <= 0 || R <= 0) return NULL;
+    caps_t* self = (caps_t*)calloc(1, sizeof(caps_t));
+    TERMINATE(!self, "out of memory");
+    const double LbyR = L / R;
+    self->LbyR = LbyR;
+    self->L = L;
+    self->R = R;
+    self->calL = L + R;
+    self->y = -log(2.0) - log1p(LbyR);
+    self->ldim = (int)ceil(std::max((double)CAPS_MINIMUM_LDIM, CAPS_FACTOR_LDIM / LbyR));
+    self->epsilonm1_plate = caps_epsilonm1_perf;
+    self->epsilonm1_sphere = caps_epsilonm1_perf;
+    self->userdata_plate = NULL;
+    self->userdata_sphere = NULL;
+    self->epsrel = CAPS_EPSREL;
+    self->detalg = DETALG_HODLR;
+    self->priv = NULL;
+    return self;
+}
+
+void caps_free(caps_t* self)
+{
+    if (!self) return;
+    ctx_free((Ctx*)self->priv);
+    free(self);
+}
+
+void caps_build(FILE* stream, const char* prefix)
+{
+    if (!prefix) prefix = "";
+    fprintf(stream, "%sversion: 0.5 (libcaps_amd, gfx950)\n", prefix);
+    fprintf(stream, "%scompiler: hipcc/clang %s\n", prefix, __clang_version__);
+    fprintf(stream, "%scompile time: %s %s\n", prefix, __DATE__, __TIME__);
+}
+
+void caps_info(caps_t* self, FILE* stream, const char* prefix)
+{
+    const char* a;
+    if (!prefix) prefix = "";
+    switch (self->detalg) {
+    case DETALG_HODLR: a = "HODLR"; break;
+    case DETALG_LU: a = "LU"; break;
+    case DETALG_QR: a = "QR"; break;
+    case DETALG_CHOLESKY: a = "CHOLESKY"; break;
+    default: a = "unknown";
+    }
+    fprintf(stream, "%sL/R    = %.16g\n", prefix, self->LbyR);
+    fprintf(stream, "%sL      = %.16g\n", prefix, self->L);
+    fprintf(stream, "%sR      = %.16g\n", prefix, self->R);
+    fprintf(stream, "%sldim   = %d\n", prefix, self->ldim);
+    fprintf(stream, "%sepsrel = %.1e\n", prefix, self->epsrel);
+    fprintf(stream, "%sdetalg = %s\n", prefix, a);
+}
+
+int caps_set_epsrel(caps_t* self, double epsrel)
+{
+    if (epsrel <= 0) return 0;
+    self->epsrel = epsrel;
+    return 1;
+}
+double caps_get_epsrel(caps_t* self) { return self->epsrel; }
+
+int caps_set_ldim(caps_t* self, int ldim)
+{
+    if (ldim < 1) return 0;
+    self->ldim = ldim;
+    return 1;
+}
+int caps_get_ldim(caps_t* self) { return self->ldim; }
+
+int caps_set_detalg(caps_t* self, detalg_t detalg)
+{
+    self->detalg = detalg;
+    return 1;
+}
+detalg_t caps_get_detalg(caps_t* self) { return self->detalg; }
+
+void caps_set_epsilonm1(caps_t* self, double (*f)(double, void*), void* userdata)
+{
+    caps_set_epsilonm1_plate(self, f, userdata);
+    caps_set_epsilonm1_sphere(self, f, userdata);
+}
+void caps_set_epsilonm1_plate(caps_t* self, double (*f)(double, void*), void* userdata)
+{
+    self->epsilonm1_plate = f;
+    self->userdata_plate = userdata;
+}
+void caps_set_epsilonm1_sphere(caps_t* self, double (*f)(double, void*), void* userdata)
+{
+    self->epsilonm1_sphere = f;
+    self->userdata_sphere = userdata;
+}
+
+double caps_epsilonm1_plate(caps_t* self, double xi_)
+{
+    return self->epsilonm1_plate(xi_ * CAPS_c / self->calL, self->userdata_plate);
+}
+double caps_epsilonm1_sphere(caps_t* self, double xi_)
+{
+    return self->epsilonm1_sphere(xi_ * CAPS_c / self->calL, self->userdata_sphere);
+}
+double caps_epsilonm1_perf(double, void*) { return INFINITY; }
+double caps_epsilonm1_drude(double xi, void* userdata)
+{
+    const double* p = (const double*)userdata;
+    return p[0] * p[0] / (xi * (xi + p[1]));
+}
+
+static double host_lfac(unsigned n)
+{
+    if (n < 1024) return lgamma((double)n + 1.0);
+    const double C = 0.91893853320467274178;
+    return (n + 0.5) * log((double)n) - n + C + (1. / 12) / n;
+}
+
+double caps_lnLambda(int l1, int l2, int m)
+{
+    return (log(2.0 * l1 + 1) + log(2.0 * l2 + 1) - log((double)l1) - log(l1 + 1.0) - log((double)l2) - log(l2 + 1.0)
+            + host_lfac(l1 - m) + host_lfac(l2 - m) - host_lfac(l1 + m) - host_lfac(l2 + m)) / 2.0;
+}
+
+int caps_estimate_lminmax(caps_t* self, int m, size_t* lmin_p, size_t* lmax_p)
+{
+    const int ldim = self->ldim;
+    if (m == 0) {
+        *lmin_p = 1;
+        *lmax_p = 1 + ldim;
+        return 0;
+    }
+    const int l = (int)round(m / sqrt(2 * self->LbyR));
+    int lmin = l - ldim / 2;
+    if (lmin < m) lmin = m;
+    *lmin_p = lmin;
+    *lmax_p = lmin + ldim;
+    return l;
+}
+
+void caps_mie(caps_t* self, double xi_, int l, double* lna, double* lnb)
+{
+    Ctx* c = ctx_of(self);
+    std::lock_guard<std::mutex> lock(c->mu);
+    const double eps = caps_epsilonm1_sphere(self, xi_);
+    double* d;
+    HIPCHECK(hipMalloc(&d, 6 * sizeof(double)));
+    double h[3] = {xi_, eps, 0};
+    int li = l;
+    HIPCHECK(hipMemcpyAsync(d, h, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(d + 2, &li, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    capsd_launch_mie_points(1, d, (const int*)(d + 2), self->LbyR, d + 1, d + 4, c->stream);
+    double r[2];
+    HIPCHECK(hipMemcpyAsync(r, d + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipFree(d));
+    *lna = r[0];
+    *lnb = r[1];
+}
+
+void caps_mie_perf(caps_t* self, double xi_, int l, double* lna, double* lnb)
+{
+    double (*f)(double, void*) = self->epsilonm1_sphere;
+    void* u = self->userdata_sphere;
+    self->epsilonm1_sphere = caps_epsilonm1_perf;
+    caps_mie(self, xi_, l, lna, lnb);
+    self->epsilonm1_sphere = f;
+    self->userdata_sphere = u;
+}
+
+void caps_fresnel(caps_t* self, double xi_, double k_, double* r_TE, double* r_TM)
+{
+    Ctx* c = ctx_of(self);
+    std::lock_guard<std::mutex> lock(c->mu);
+    const double eps = caps_epsilonm1_plate(self, xi_);
+    double* d;
+    HIPCHECK(hipMalloc(&d, 6 * sizeof(double)));
+    double h[3] = {xi_, k_, eps};
+    HIPCHECK(hipMemcpyAsync(d, h, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    capsd_launch_fresnel_points(1, d, d + 1, d + 2, d + 4, c->stream);
+    double r[2];
+    HIPCHECK(hipMemcpyAsync(r, d + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipFree(d));
+    *r_TE = r[0];
+    *r_TM = r[1];
+}
+
+int caps_logdetD_batch(caps_t* self, const double* xi_, const int* m, int n, double* out)
+{
+    if (n <= 0) return 0;
+    for (int i = 0; i < n; i++) {
+        if (!(xi_[i] > 0) || m[i] < 0) {
+            fprintf(stderr, "caps_logdetD_batch: block %d has xi_=%g, m=%d (need xi_ > 0, m >= 0)\n", i, xi_[i], m[i]);
+            return 1;
+        }
+    }
+    TERMINATE(2 * self->ldim > capsd_lu_max_dim(), "ldim=%d exceeds the LU's maximum dimension %d", self->ldim,
+              capsd_lu_max_dim());
+    Ctx* c = ctx_of(self);
+    std::lock_guard<std::mutex> lock(c->mu);
+    const int lu_mode = is_lu_mode(self->detalg) ? 1 : 0;
+    double tot[6] = {0, 0, 0, 0, 0, 0};
+    for (int b0 = 0; b0 < n;) {
+        const int k = chunk_blocks(self, m + b0, n - b0);
+        ensure_blocks(c, k);
+        BatchPlan p = plan_batch(self, c, xi_ + b0, m + b0, k, 1);
+        run_batch(self, c, p, 0, lu_mode);
+        for (int i = 0; i < k; i++) out[b0 + i] = finish_block(c->h_out[i], lu_mode);
+        for (int i = 0; i < 6; i++) tot[i] += c->timing[i];
+        b0 += k;
+    }
+    memcpy(c->timing, tot, sizeof(tot));
+    return 0;
+}
+
+double caps_logdetD(caps_t* self, double xi_, int m)
+{
+    TERMINATE(xi_ <= 0, "Matsubara frequency must be positive");
+    double out;
+    caps_logdetD_batch(self, &xi_, &m, 1, &out);
+    return out;
+}
+
+void caps_last_timing(caps_t* self, double ms[6])
+{
+    Ctx* c = ctx_of(self);
+    for (int i = 0; i < 6; i++) ms[i] = c->timing[i];
+}
+
+// ---- round-trip matrix object ----------------------------------------------
+
+struct caps_block_state {
+    double* M;   // N x N, column-major, lda = N
+    int N;
+};
+
+caps_M_t* caps_M_init(caps_t* caps, int m, double xi_)
+{
+    TERMINATE(xi_ <= 0, "Matsubara frequency must be positive");
+    Ctx* c = ctx_of(caps);
+    std::lock_guard<std::mutex> lock(c->mu);
+    ensure_blocks(c, 1);
+    BatchPlan p = plan_batch(caps, c, &xi_, &m, 1, 0);
+    run_batch(caps, c, p, 1, 0);
+    const BlockDesc& d = c->h_desc[0];
+    caps_M_t* self = (caps_M_t*)calloc(1, sizeof(caps_M_t));
+    self->caps = caps;
+    self->m = m;
+    self->lmin = d.lmin;
+    self->ldim = d.ldim;
+    self->xi_ = xi_;
+    self->integration_plasma = NULL;
+    caps_block_state* st = (caps_block_state*)calloc(1, sizeof(caps_block_state));
+    st->N = d.N;
+    st->M = (double*)malloc((size_t)d.N * d.N * sizeof(double));
+    HIPCHECK(hipMemcpy2D(st->M, d.N * sizeof(double), c->arena + d.off_mat, d.lda * sizeof(double),
+                         d.N * sizeof(double), d.N, hipMemcpyDeviceToHost));
+    self->al = (double*)malloc(d.ldim * sizeof(double));
+    self->bl = (double*)malloc(d.ldim * sizeof(double));
+    HIPCHECK(hipMemcpy(self->al, c->arena + d.off_mie, d.ldim * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(self->bl, c->arena + d.off_mie + d.ldim * 8, d.ldim * sizeof(double), hipMemcpyDeviceToHost));
+    self->integration = st;
+    return self;
+}
+
+double caps_M_elem(caps_M_t* self, int l1, int l2, char p1, char p2)
+{
+    const int i = 2 * (l1 - self->lmin) + (p1 == 'M'), j = 2 * (l2 - self->lmin) + (p2 == 'M');
+    TERMINATE(i < 0 || j < 0 || i >= 2 * self->ldim || j >= 2 * self->ldim,
+              "l1=%d, l2=%d outside the window [%d, %d)", l1, l2, self->lmin, self->lmin + self->ldim);
+    const caps_block_state* st = self->integration;
+    return st->M[i + (size_t)j * st->N];
+}
+
+double caps_kernel_M(int i, int j, void* args_)
+{
+    caps_M_t* a = (caps_M_t*)args_;
+    const char pol[2] = {'E', 'M'};
+    return caps_M_elem(a, a->lmin + i / 2, a->lmin + j / 2, pol[i % 2], pol[j % 2]);
+}
+
+void caps_M_free(caps_M_t* self)
+{
+    if (!self) return;
+    if (self->integration) {
+        free(self->integration->M);
+        free(self->integration);
+    }
+    free(self->al);
+    free(self->bl);
+    free(self);
+}
+
+// ---- generic determinant ----------------------------------------------------
+
+double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int sym_spd, detalg_t detalg)
+{
+    (void)sym_spd;
+    TERMINATE(dim <= 0 || dim > capsd_lu_max_dim(), "dimension %d outside [1, %d]", dim, capsd_lu_max_dim());
+    std::vector<double> diag(dim);
+    for (int i = 0; i < dim; i++) diag[i] = kernel(i, i, args);
+    double tr = 0, comp = 0;
+    for (int i = 0; i < dim; i++) {
+        const double y = diag[i] - comp, t = tr + y;
+        comp = (t - tr) - y;
+        tr = t;
+    }
+    if (fabs(tr) < 1e-8) return -tr;
+    const int lda = (dim + 15) & ~15;
+    std::vector<double> A((size_t)lda * dim, 0.0);
+    for (int i = 0; i < dim; i++) A[i + (size_t)i * lda] = diag[i];
+    for (int md = 1; md < dim; md++)
+        for (int k = 0; k < dim - md; k++) {
+            A[k + (size_t)(md + k) * lda] = kernel(k, md + k, args);
+            A[(md + k) + (size_t)k * lda] = kernel(md + k, k, args);
+        }
+    const int lu_mode = is_lu_mode(detalg);
+    if (lu_mode) {
+        double norm = 0;
+        for (int j = 0; j < dim; j++)
+            for (int i = 0; i < dim; i++) norm += A[i + (size_t)j * lda] * A[i + (size_t)j * lda];
+        norm = sqrt(norm);
+        if (norm < 1) {
+            std::vector<double> row(dim);
+            for (int i = 0; i < dim; i++) {
+                double s = 0;
+                for (int j = 0; j < dim; j++) s += A[i + (size_t)j * lda] * A[j + (size_t)i * lda];
+                row[i] = s;
+            }
+            double t2 = 0, c2 = 0;
+            for (int i = 0; i < dim; i++) {
+                const double y = row[i] - c2, t = t2 + y;
+                c2 = (t - t2) - y;
+                t2 = t;
+            }
+            const double mercator = -tr - t2 / 2;
+            const double err = fabs(norm * norm / 2 + norm + log1p(-norm));
+            if (fabs(err / mercator) < 1e-8) return mercator;
+        }
+    }
+    for (size_t i = 0; i < A.size(); i++) A[i] = -A[i];
+    for (int i = 0; i < dim; i++) A[i + (size_t)i * lda] += 1;
+    // one-block batch with a user matrix
+    caps_t* tmp = caps_init(1, 1);
+    Ctx* c = ctx_of(tmp);
+    ensure_blocks(c, 1);
+    BlockDesc& d = c->h_desc[0];
+    memset(&d, 0, sizeof(d));
+    d.N = dim;
+    d.ldim = (dim + 1) / 2;
+    d.lda = lda;
+    int64_t off = 0;
+    d.off_mat = off;
+    off = align256(off + (int64_t)lda * dim * 8);
+    LuSide& s = c->h_side[0];
+    s.off_panel = off;
+    off = align256(off + (int64_t)lda * capsd_lu_nb() * 8);
+    s.off_logs = off;
+    off = align256(off + (int64_t)dim * 8);
+    s.off_moved = off;
+    off = align256(off + (1 + 2 * (int64_t)capsd_lu_maxmove()) * 4);
+    ensure_arena(c, off);
+    BlockOut o;
+    memset(&o, 0, sizeof(o));
+    HIPCHECK(hipMemcpyAsync(c->d_desc, &d, sizeof(BlockDesc), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->d_side, &s, sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->d_out, &o, sizeof(BlockOut), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->arena, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, 1, dim, c->stream);
+    HIPCHECK(hipMemcpyAsync(&o, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    caps_free(tmp);
+    const double ld = o.logdet;
+    if (!lu_mode && fabs(tr) > fabs(ld)) return -tr;
+    return ld;
+}
+
+// ---- ξ = 0 Drude term --------------------------------------------------------
+
+double caps_ht_drude(caps_t* caps)
+{
+    const double x = caps->LbyR, x2 = x * x;
+    const double Z = (1 + x) * (1 - sqrt((x2 + 2 * x) / (x2 + 2 * x + 1)));
+    double s1 = 0, s2 = 0;
+    for (int l = 1;; l++) {
+        const double a = pow(Z, 2 * l), b = a * Z;
+        const double v1 = (2 * l + 1) * log1p(-b), v2 = b * (1 - a) / (1 - b);
+        s1 += v1;
+        s2 += v2;
+        if (fabs(v1 / s1) < 1e-15 && fabs(v2 / s2) < 1e-15) break;
+    }
+    return 0.5 * (s1 + log1p(-(1 - Z * Z) * s2));
+}
+
+// ---- free energy (reference driver semantics) -------------------------------
+
+// Σ'_m logdetD(ξ, m) with the reference's m-cutoff: m runs 0, 1, 2, ... and
+// stops at the first m whose value v satisfies v == 0 or v/v_0 < cutoff
+// (that m included); m = 0 is halved.  Reference: src/caps.c:307-396 with a
+// single worker (sequential semantics).  mode 0: one block per call; mode 1:
+// blocks in batches of the predicted cutoff (*mpred, updated), extended
+// until the cutoff is seen; both give the same set of m and the same sum.
+static double F_xi(caps_t* self, double xi_, double cutoff, int mode, int* mpred, long* nblocks)
+{
+    std::vector<double> terms;
+    int mstop = -1;
+    if (mode == 0) {
+        for (int m = 0;; m++) {
+            TERMINATE(m >= 100000, "sum over m did not converge at xi_=%g", xi_);
+            const double v = caps_logdetD(self, xi_, m);
+            terms.push_back(v);
+            (*nblocks)++;
+            if (v == 0 || v / terms[0] < cutoff) {
+                mstop = m;
+                break;
+            }
+        }
+    } else {
+        int next = 0;
+        int chunk = std::max(*mpred + 1, 16);
+        while (mstop < 0) {
+            TERMINATE(next >= 100000, "sum over m did not converge at xi_=%g", xi_);
+            std::vector<double> xs(chunk, xi_), vals(chunk);
+            std::vector<int> ms(chunk);
+            for (int i = 0; i < chunk; i++) ms[i] = next + i;
+            TERMINATE(caps_logdetD_batch(self, xs.data(), ms.data(), chunk, vals.data()) != 0, "batch failed");
+            *nblocks += chunk;
+            for (int i = 0; i < chunk; i++) {
+                terms.push_back(vals[i]);
+                const double v = vals[i];
+                if (v == 0 || v / terms[0] < cutoff) {
+                    mstop = next + i;
+                    break;
+                }
+            }
+            next += chunk;
+            chunk = std::max(16, chunk / 2);
+        }
+        *mpred = mstop;
+    }
+    terms.resize(mstop + 1);
+    terms[0] /= 2;
+    double s = 0, c = 0;
+    for (double t : terms) {
+        const double y = t - c, tt = s + y;
+        c = (tt - s) - y;
+        s = tt;
+    }
+    return s;
+}
+
+double caps_free_energy(caps_t* self, double T, double cutoff, double epsrel, int mode, int drude_xi0,
+                        long* nblocks)
+{
+    TERMINATE(!(T > 0), "caps_free_energy needs T > 0 (got %g)", T);
+    const double T_scaled = 2 * M_PI * CAPS_kB * (self->R + self->L) * T / (CAPS_hbar * CAPS_c);
+    long nb = 0;
+    std::vector<double> v;
+    TERMINATE(!drude_xi0, "the xi = 0 term is implemented for the Drude model only");
+    v.push_back(caps_ht_drude(self));
+    int mpred = 0;
+    for (int n = 1;; n++) {
+        const double xi = n * T_scaled;
+        v.push_back(F_xi(self, xi, cutoff, mode, &mpred, &nb));
+        if (fabs(v[n] / v[0]) < epsrel) break;
+        TERMINATE(n > 100000, "Matsubara sum did not converge");
+    }
+    v[0] /= 2;
+    double s = 0, c = 0;
+    for (double t : v) {
+        const double y = t - c, tt = s + y;
+        c = (tt - s) - y;
+        s = tt;
+    }
+    if (nblocks) *nblocks = nb;
+    return T_scaled / M_PI * s;
+}
+
+// ---- LU of user matrices and the MFMA probe (test hooks) --------------------
+
+// Extension (test hook): log|det A| of n column-major dim x dim matrices
+// (lda = dim) on the GPU's batched LU.  Returns 0 on success.
+int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
+{
+    if (dim <= 0 || dim > capsd_lu_max_dim() || n <= 0) return 1;
+    caps_t* tmp = caps_init(1, 1);
+    Ctx* c = ctx_of(tmp);
+    ensure_blocks(c, n);
+    const int lda = dim;
+    int64_t off = 0;
+    for (int b = 0; b < n; b++) {
+        BlockDesc& d = c->h_desc[b];
+        memset(&d, 0, sizeof(d));
+        d.N = dim;
+        d.ldim = (dim + 1) / 2;
+        d.lda = lda;
+        d.off_mat = off;
+        off = align256(off + (int64_t)lda * dim * 8);
+        LuSide& s = c->h_side[b];
+        s.off_panel = off;
+        off = align256(off + (int64_t)lda * capsd_lu_nb() * 8);
+        s.off_logs = off;
+        off = align256(off + (int64_t)dim * 8);
+        s.off_moved = off;
+        off = align256(off + (1 + 2 * (int64_t)capsd_lu_maxmove()) * 4);
+    }
+    ensure_arena(c, off);
+    std::vector<BlockOut> o(n);
+    memset(o.data(), 0, n * sizeof(BlockOut));
+    HIPCHECK(hipMemcpyAsync(c->d_desc, c->h_desc, n * sizeof(BlockDesc), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->d_side, c->h_side, n * sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->d_out, o.data(), n * sizeof(BlockOut), hipMemcpyHostToDevice, c->stream));
+    for (int b = 0; b < n; b++)
+        HIPCHECK(hipMemcpyAsync(c->arena + c->h_desc[b].off_mat, A + (size_t)b * dim * dim,
+                                (size_t)dim * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipEventRecord(c->ev[0], c->stream));
+    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, dim, c->stream);
+    HIPCHECK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHECK(hipMemcpyAsync(o.data(), c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    float ms;
+    HIPCHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    for (int b = 0; b < n; b++) out[b] = o[b].logdet;
+    caps_free(tmp);
+    return 0;
+}
+
+// Extension (test hook): D = A(16x4) B(4x16) through one v_mfma_f64_16x16x4f64.
+int capsx_mfma_probe(const double* A, const double* B, double* D)
+{
+    device_init();
+    double *dA, *dB, *dD;
+    HIPCHECK(hipMalloc(&dA, 64 * 8));
+    HIPCHECK(hipMalloc(&dB, 64 * 8));
+    HIPCHECK(hipMalloc(&dD, 256 * 8));
+    HIPCHECK(hipMemcpy(dA, A, 64 * 8, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(dB, B, 64 * 8, hipMemcpyHostToDevice));
+    capsd_launch_mfma_probe(dA, dB, dD, 0);
+    HIPCHECK(hipDeviceSynchronize());
+    HIPCHECK(hipMemcpy(D, dD, 256 * 8, hipMemcpyDeviceToHost));
+    hipFree(dA);
+    hipFree(dB);
+    hipFree(dD);
+    return 0;
+}
+
+// Extension (test hook): the K table (log|K_ν^p|, ν = mu..nu_max) and the I
+// table (log|I|, sign) of one block, as the kernels compute them.
+int capsx_block_tables(caps_t* self, double xi_, int m, int* mu, int* nK, double* lnK /* 2*nK */, int* lo,
+                       int* nl, double* logI /* 2*nl*nl */, signed char* sgnI /* 2*nl*nl */, int cap_nK, int cap_nl)
+{
+    Ctx* c = ctx_of(self);
+    std::lock_guard<std::mutex> lock(c->mu);
+    ensure_blocks(c, 1);
+    BatchPlan p = plan_batch(self, c, &xi_, &m, 1, 0);
+    run_batch(self, c, p, 1, 0);
+    const BlockDesc& d = c->h_desc[0];
+    *mu = d.mu;
+    *nK = d.nK;
+    *lo = d.lo;
+    *nl = d.nl;
+    if (d.nK > cap_nK || d.nl > cap_nl) return 1;
+    if (lnK) {
+        HIPCHECK(hipMemcpy(lnK, c->arena + d.off_k + 24 * (int64_t)d.nK, 2 * d.nK * sizeof(double),
+                           hipMemcpyDeviceToHost));
+    }
+    if (logI) {
+        HIPCHECK(hipMemcpy(logI, c->arena + d.off_i, 2 * (size_t)d.nl * d.nl * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(sgnI, c->arena + d.off_i + 16 * (int64_t)d.nl * d.nl, 2 * (size_t)d.nl * d.nl,
+                           hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+// Extension (test hook): copy the N x N matrix M of a caps_M_t (column-major).
+int capsx_M_copy(caps_M_t* self, double* out)
+{
+    const caps_block_state* st = self->integration;
+    memcpy(out, st->M, (size_t)st->N * st->N * sizeof(double));
+    return st->N;
+}
+
+// Extension: number of quadrature nodes of the K integrals of a block.
+int capsx_block_nodes(caps_t* self, double xi_, int m)
+{
+    BlockDesc d;
+    memset(&d, 0, sizeof(d));
+    size_t lmin, lmax;
+    caps_estimate_lminmax(self, m, &lmin, &lmax);
+    d.alpha = 2 * xi_;
+    d.mu = m > 0 ? 2 * m : 2;
+    d.lo = (int)lmin - 1;
+    d.nl = self->ldim + 2;
+    d.nu_max = 2 * (d.lo + d.nl - 1);
+    grid_for(d);
+    return d.n_nodes;
+}
+
+}  // extern "C"

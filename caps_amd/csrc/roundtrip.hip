@@ -1,0 +1,577 @@
+// Round-trip matrix of (ξ, m) blocks on gfx950: Mie coefficients, the
+// k-integrals K_ν, the integrals I_{l1 l2} and the matrix 1 - M.
+//
+// Every kernel is batched: blockIdx.y (or .z) selects the (ξ, m) block, the
+// remaining grid dimensions cover the work of one block, lanes beyond a
+// block's own extent idle.  All arithmetic is FP64.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "blocks.h"
+#include "device_math.h"
+
+namespace capsd {
+
+template <typename T>
+__device__ __forceinline__ T* at(char* base, int64_t off) { return reinterpret_cast<T*>(base + off); }
+
+// ---------------------------------------------------------------------------
+// Mie coefficients: one lane per l of the block's window.
+// Reference: src/libcaps.c:557-602 (caps_mie), cached per block as in
+// src/libcaps.c:690-694 and 766-770.
+__global__ void __launch_bounds__(256) k_mie(const BlockDesc* __restrict__ desc, char* __restrict__ arena)
+{
+    const BlockDesc& d = desc[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ldim) return;
+    double* al = at<double>(arena, d.off_mie);
+    double lna, lnb;
+    mie(d.xi_, d.LbyR, d.epsm1_sphere, d.lmin + i, lna, lnb);
+    al[i] = lna;
+    al[d.ldim + i] = lnb;
+}
+
+// Mie coefficients at arbitrary (ξ, l) points (scalar API caps_mie).
+__global__ void k_mie_points(int n, const double* __restrict__ xi_, const int* __restrict__ l, double LbyR,
+                             const double* __restrict__ epsm1, double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a, b;
+    mie(xi_[i], LbyR, epsm1[i], l[i], a, b);
+    out[2 * i] = a;
+    out[2 * i + 1] = b;
+}
+
+// Fresnel coefficients at arbitrary points (scalar API caps_fresnel).
+__global__ void k_fresnel_points(int n, const double* __restrict__ xi_, const double* __restrict__ k_,
+                                 const double* __restrict__ epsm1, double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double te, tm;
+    fresnel(epsm1[i], xi_[i], k_[i], te, tm);
+    out[2 * i] = te;
+    out[2 * i + 1] = tm;
+}
+
+// ---------------------------------------------------------------------------
+// K_ν for all ν of a block.
+//
+// K_ν^p = ∫_1^∞ r_p(x) e^{-αx} P_ν^μ(x) w(x) dx, μ = 2m (m > 0) or 2 (m = 0),
+// w = 1/(x²-1) (m > 0) or 1 (m = 0); the reference integrates each ν
+// adaptively (src/integration.c:257-427).  Here all ν share one quadrature
+// grid: the trapezoidal rule in v with x(v) = 1 + A² softplus(v)², which
+// sends x = 1 to v = -∞ (integrand decays like e^{2v}) and makes the step a
+// fixed fraction of the local peak width sqrt(x/α) for large x, so the rule
+// converges exponentially for every ν at once.  At each node (one lane) the
+// values P_ν^μ(x) for ν = μ, μ+1, ... follow from the upward recurrence in the
+// degree (DLMF 14.10.3, the recurrence of src/plm.c:72-112 continued over
+// all ν), kept as mantissa × 2^E.  For every ν the wave sums its 64 node
+// terms after aligning them to the wave's largest exponent (ldexp, exact);
+// one KPart per (ν, wave) goes to HBM and k_K_combine adds the waves.
+
+__device__ __forceinline__ int wave_max_int(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                  const double* __restrict__ lfac_tab)
+{
+    const BlockDesc& d = desc[blockIdx.y];
+    const int wave = blockIdx.x;
+    if (wave >= d.n_waves) return;  // uniform over the workgroup
+    const int lane = threadIdx.x;
+    const int k = wave * 64 + lane;
+    const bool active = k < d.n_nodes;
+    const int mu = d.mu;
+
+    // node, weight and Fresnel factors (once per node)
+    double x = 2, mant_te = 0, mant_tm = 0;
+    int E = -1000000;
+    if (active) {
+        const double v = d.v0 + k * d.h;
+        const double L = v > 40 ? v : log1p(exp(v));
+        const double sg = 1 / (1 + exp(-v));
+        const double xm1 = d.A * d.A * L * L;
+        x = 1 + xm1;
+        const double x2m1 = xm1 * (x + 1);
+        const double dx = 2 * d.A * d.A * L * sg;
+        const double lpmm = lfac(lfac_tab, 2 * mu) - mu * LOG2 - lfac(lfac_tab, mu) + 0.5 * mu * log(x2m1);
+        double lw = log(d.h * dx) - d.alpha * x + lpmm;
+        if (d.m > 0) lw -= log(x2m1);
+        double rte, rtm;
+        fresnel(d.epsm1_plate, d.xi_, d.xi_ * sqrt(x2m1), rte, rtm);
+        const double lw2 = lw * (1 / LOG2);
+        const double fl = floor(lw2);
+        E = (int)fl;
+        const double w = exp((lw2 - fl) * LOG2);
+        mant_te = w * fabs(rte);
+        mant_tm = w * fabs(rtm);
+    }
+
+    KPart* part = at<KPart>(arena, d.off_kpart);
+    __shared__ double c1s[64], c2s[64];
+    double p_prev = 0, p = 1;
+    for (int base = mu; base <= d.nu_max; base += 64) {
+        // recurrence coefficients for the next 64 steps, one division per lane
+        {
+            const int nu = base + lane;
+            const double ll = (double)(nu + 1 - mu);
+            c1s[lane] = (2.0 * nu + 1) / ll;
+            c2s[lane] = (double)(nu + mu) / ll;
+        }
+        __syncthreads();
+        const int nend = min(base + 64, d.nu_max + 1);
+        for (int nu = base; nu < nend; nu++) {
+            const int e_lane = active ? E + ilogb(p) : -1000000;
+            const int F = wave_max_int(e_lane);
+            const double s_te = wave_sum(active ? ldexp(p * mant_te, E - F) : 0.0);
+            const double s_tm = wave_sum(active ? ldexp(p * mant_tm, E - F) : 0.0);
+            if (lane == 0) {
+                KPart kp;
+                kp.s_te = s_te;
+                kp.s_tm = s_tm;
+                kp.e = F;
+                kp.pad_ = 0;
+                part[(int64_t)(nu - mu) * d.n_waves + wave] = kp;
+            }
+            const double pn = c1s[nu - base] * x * p - c2s[nu - base] * p_prev;
+            p_prev = p;
+            p = pn;
+            if (p > 0x1p600) {
+                p *= 0x1p-600;
+                p_prev *= 0x1p-600;
+                E += 600;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Sum the waves' partial sums: log|K_ν|, and K_ν = kmant · 2^kexp with
+// |kmant| in [1, 2) carrying the sign (TE negative, TM positive).
+__global__ void __launch_bounds__(256) k_K_combine(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                   int* __restrict__ status)
+{
+    const BlockDesc& d = desc[blockIdx.y];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.nK) return;
+    const KPart* part = at<KPart>(arena, d.off_kpart) + (int64_t)j * d.n_waves;
+    int F = -2000000000;
+    for (int w = 0; w < d.n_waves; w++)
+        if (part[w].s_tm > 0 || part[w].s_te > 0) F = max(F, part[w].e);
+    double ste = 0, stm = 0;
+    for (int w = 0; w < d.n_waves; w++) {
+        ste += ldexp(part[w].s_te, part[w].e - F);
+        stm += ldexp(part[w].s_tm, part[w].e - F);
+    }
+    double* kmant = at<double>(arena, d.off_k);          // [2][nK]
+    int* kexp = at<int>(arena, d.off_k + 16 * (int64_t)d.nK);  // [2][nK]
+    double* lnk = at<double>(arena, d.off_k + 24 * (int64_t)d.nK);  // [2][nK]
+    const double s[2] = {ste, stm};
+    for (int pol = 0; pol < 2; pol++) {
+        const double v = s[pol];
+        if (!(v > 0) || !isfinite(v)) {
+            if (!(pol == 0 && d.is_pr)) atomicOr(status, 1);
+            kmant[pol * d.nK + j] = 0;
+            kexp[pol * d.nK + j] = 0;
+            lnk[pol * d.nK + j] = -INFINITY;
+            continue;
+        }
+        const int e = ilogb(v);
+        const double mnt = ldexp(v, -e);
+        kmant[pol * d.nK + j] = pol == 0 ? -mnt : mnt;
+        kexp[pol * d.nK + j] = F + e;
+        lnk[pol * d.nK + j] = log(v) + (double)F * LOG2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// I_{l1 l2}^p for all pairs of the block's table, both polarizations per
+// lane (the a_q depend on l1, l2, m only).
+//
+// I = e^{log a0} Σ_q a_q K_{l1+l2-2q}: eqs. (20), (24), (26)-(30), (35) of the
+// reference (src/integration.c:497-609), with its rescaling of a_q (here by
+// powers of two) and its early exit after three consecutive terms below
+// e^{-60} times the first one (per polarization).  Terms are summed in
+// linear arithmetic after aligning the K exponents, as logadd_ms does
+// (src/misc.c:95-109).  Symmetric in l1, l2; zero below l = max(m,1);
+// perfect reflectors: I_TE = -|I_TM| (src/integration.c:627-671).
+
+__device__ __forceinline__ double alpha_pnnu(double p, double n, double nu)
+{
+    return ((p * p - (n + nu + 1) * (n + nu + 1)) * (p * p - (n - nu) * (n - nu))) / (4 * p * p - 1);
+}
+
+__global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                           const double* __restrict__ lfac_tab, int* __restrict__ status)
+{
+    const BlockDesc& d = desc[blockIdx.y];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nl = d.nl;
+    const int64_t ntri = (int64_t)nl * (nl + 1) / 2;
+    if (t >= ntri) return;
+    // triangular index -> (i1 >= i2)
+    int i1 = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((int64_t)i1 * (i1 + 1) / 2 > t) i1--;
+    while ((int64_t)(i1 + 1) * (i1 + 2) / 2 <= t) i1++;
+    const int i2 = (int)(t - (int64_t)i1 * (i1 + 1) / 2);
+    const int l1 = d.lo + i1, l2 = d.lo + i2;
+
+    double* logI = at<double>(arena, d.off_i);                           // [2][nl*nl]
+    int8_t* sgnI = at<int8_t>(arena, d.off_i + 16 * (int64_t)nl * nl);   // [2][nl*nl]
+    const int64_t a = (int64_t)i1 * nl + i2, b = (int64_t)i2 * nl + i1;
+
+    const int m_ = d.m > 0 ? d.m : 1;
+    if (l2 < m_ || l2 < d.m) {
+        for (int pol = 0; pol < 2; pol++) {
+            logI[pol * nl * nl + a] = -INFINITY;
+            logI[pol * nl * nl + b] = -INFINITY;
+            sgnI[pol * nl * nl + a] = 0;
+            sgnI[pol * nl * nl + b] = 0;
+        }
+        return;
+    }
+
+    const double* kmant = at<double>(arena, d.off_k);
+    const int* kexp = at<int>(arena, d.off_k + 16 * (int64_t)d.nK);
+    const int mu = d.mu;
+    const int npol = d.is_pr ? 1 : 2;   // pol index 1 = TM; PR: TM only
+    const int pol0 = d.is_pr ? 1 : 0;
+
+    const double n = l1, nu = l2, m = m_;
+    const double n4 = l1 + l2 - 2 * m_;
+    const int l12 = l1 + l2;
+    int qmax = (l12 - 2 * m_) / 2;
+    qmax = min(qmax, min(l1, l2));
+    const double Ap = -2 * m * (n - nu) * (n + nu + 1);
+    const double log_a0 = lfac(lfac_tab, 2 * l1) - lfac(lfac_tab, l1) + lfac(lfac_tab, 2 * l2) - lfac(lfac_tab, l2)
+                        + lfac(lfac_tab, l12) - lfac(lfac_tab, 2 * l12) + lfac(lfac_tab, l12 - 2 * m_)
+                        - lfac(lfac_tab, l1 - m_) - lfac(lfac_tab, l2 - m_);
+
+    const double thr = 8.7565107626965203e-27;  // e^{-60}
+    double sum[2] = {0, 0}, t0abs[2] = {0, 0};
+    int E0[2] = {0, 0}, cnt[2] = {0, 0};
+    bool done[2] = {false, false};
+    for (int pp = 0; pp < npol; pp++) {
+        const int pol = pol0 + pp;
+        const int j = l12 - mu;
+        E0[pol] = kexp[pol * d.nK + j];
+        sum[pol] = kmant[pol * d.nK + j];
+        t0abs[pol] = fabs(sum[pol]);
+    }
+
+    double aqm2 = 1, aqm1 = 0, aq = 1;  // a_{q-2}, a_{q-1}, a_q
+    int S = 0;                           // a_q carries a factor 2^S
+    for (int q = 1; q <= qmax; q++) {
+        if (q == 1) {
+            aq = (n + nu - 1.5) * (1 - (2 * n + 2 * nu - 1) / (n4 * (n4 - 1))
+                 * ((m - n) * (m - n + 1) / (2 * n - 1) + (m - nu) * (m - nu + 1) / (2 * nu - 1)));
+            aqm1 = 1;
+        } else if (q == 2) {
+            aqm2 = aqm1;
+            aqm1 = aq;
+            aq = (2 * n + 2 * nu - 1) * (2 * n + 2 * nu - 7) / 4 * ((2 * n + 2 * nu - 3) / (n4 * (n4 - 1))
+                 * ((2 * n + 2 * nu - 5) / (2 * (n4 - 2) * (n4 - 3))
+                 * ((m - n) * (m - n + 1) * (m - n + 2) * (m - n + 3) / (2 * n - 1) / (2 * n - 3)
+                 + 2 * (m - n) * (m - n + 1) * (m - nu) * (m - nu + 1) / ((2 * n - 1) * (2 * nu - 1))
+                 + (m - nu) * (m - nu + 1) * (m - nu + 2) * (m - nu + 3) / (2 * nu - 1) / (2 * nu - 3))
+                 - (m - n) * (m - n + 1) / (2 * n - 1) - (m - nu) * (m - nu + 1) / (2 * nu - 1)) + 0.5);
+        } else {
+            aqm2 = aqm1;
+            aqm1 = aq;
+            const double pq = n + nu - 2 * q, p1 = pq - 2 * m, p2 = pq + 2 * m;
+            if (Ap != 0) {
+                const double c0 = (pq + 2) * (pq + 3) * (p1 + 1) * (p1 + 2) * Ap * alpha_pnnu(pq + 1, n, nu);
+                const double c1 = Ap * (Ap * Ap + (pq + 1) * (pq + 3) * (p1 + 2) * (p2 + 2) * alpha_pnnu(pq + 2, n, nu)
+                                  + (pq + 2) * (pq + 4) * (p1 + 3) * (p2 + 3) * alpha_pnnu(pq + 3, n, nu));
+                const double c2 = -(pq + 2) * (pq + 3) * (p2 + 3) * (p2 + 4) * Ap * alpha_pnnu(pq + 4, n, nu);
+                aq = (c1 * aqm1 + c2 * aqm2) / c0;
+            } else {
+                aq = (pq + 1) * (p2 + 2) * alpha_pnnu(pq + 2, n, nu) * aqm1
+                     / ((pq + 2) * (p1 + 1) * alpha_pnnu(pq + 1, n, nu));
+            }
+            const double fa = fabs(aq);
+            if (fa > 0x1p332) {
+                const int e = ilogb(aq);
+                aq = ldexp(aq, -e);
+                aqm1 = ldexp(aqm1, -e);
+                S += e;
+            } else if (fa < 0x1p-332 && fa > 0) {
+                const int e = ilogb(aq);
+                aq = ldexp(aq, -e);
+                aqm1 = ldexp(aqm1, -e);
+                S += e;
+            }
+        }
+        const int j = l12 - 2 * q - mu;
+        bool all_done = true;
+        for (int pp = 0; pp < npol; pp++) {
+            const int pol = pol0 + pp;
+            if (done[pol]) continue;
+            const double term = ldexp(aq * kmant[pol * d.nK + j], S + kexp[pol * d.nK + j] - E0[pol]);
+            sum[pol] += term;
+            if (q >= 3) {
+                if (fabs(term) < thr * t0abs[pol]) {
+                    if (++cnt[pol] >= 3) done[pol] = true;
+                } else
+                    cnt[pol] = 0;
+            }
+            all_done = all_done && done[pol];
+        }
+        if (all_done) break;
+    }
+
+    for (int pp = 0; pp < npol; pp++) {
+        const int pol = pol0 + pp;
+        const double s = sum[pol];
+        const double li = log_a0 + log(fabs(s)) + (double)E0[pol] * LOG2;
+        if (!isfinite(li)) atomicOr(status, 2);
+        const int8_t sg = (s > 0) - (s < 0);
+        logI[pol * nl * nl + a] = li;
+        logI[pol * nl * nl + b] = li;
+        sgnI[pol * nl * nl + a] = sg;
+        sgnI[pol * nl * nl + b] = sg;
+    }
+    if (d.is_pr) {
+        logI[a] = logI[nl * nl + a];
+        logI[b] = logI[nl * nl + b];
+        sgnI[a] = -1;
+        sgnI[b] = -1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Matrix assembly: one lane per 2x2 (E,M) tile (l1, l2).  Writes
+// mode 0: 1 - M (the matrix the LU factors), mode 1: M, column-major with
+// leading dimension lda; row i = 2(l1-lmin) + p1, column j = 2(l2-lmin) + p2
+// (the interleaved index map of src/libcaps.c:717-743).  Elements follow
+// src/libcaps.c:761-831 and A/B/C/D src/integration.c:762-888.
+
+struct LogVal {
+    double v;
+    int s;
+};
+
+__device__ __forceinline__ LogVal getI(const double* logI, const int8_t* sgnI, const BlockDesc& d, int pol, int l1, int l2)
+{
+    if (l1 < d.m || l2 < d.m) return {-INFINITY, 0};
+    const int64_t idx = (int64_t)pol * d.nl * d.nl + (int64_t)(l1 - d.lo) * d.nl + (l2 - d.lo);
+    return {logI[idx], (int)sgnI[idx]};
+}
+
+__device__ __forceinline__ int sgnd(double v) { return (v > 0) - (v < 0); }
+
+__device__ LogVal intA(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol)
+{
+    if (d.m == 0) return {-INFINITY, 0};
+    const LogVal I = getI(lI, sI, d, pol, l1, l2);
+    return {2 * logi(d.m) + I.v, I.s};
+}
+
+__device__ LogVal intB(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol)
+{
+    const int m = d.m;
+    if (m == 0) return getI(lI, sI, d, pol, l1, l2);
+    const LogVal I1 = getI(lI, sI, d, pol, l1 - 1, l2 - 1);
+    const LogVal I2 = getI(lI, sI, d, pol, l1 + 1, l2 - 1);
+    const LogVal I3 = getI(lI, sI, d, pol, l1 - 1, l2 + 1);
+    const LogVal I4 = getI(lI, sI, d, pol, l1 + 1, l2 + 1);
+    const double den = (2 * l1 + 1.) * (2 * l2 + 1.);
+    const double mx = fmax(fmax(I1.v, I2.v), fmax(I3.v, I4.v));
+    double I = (l1 + 1.) * (l1 + m) * (l2 + 1.) * (l2 + m) / den * I1.s * exp(I1.v - mx);
+    I -= l1 * (l1 - m + 1.) * (l2 + 1.) * (l2 + m) / den * I2.s * exp(I2.v - mx);
+    I -= (l1 + 1.) * (l1 + m) * l2 * (l2 - m + 1.) / den * I3.s * exp(I3.v - mx);
+    I += l1 * (l1 - m + 1.) * l2 * (l2 - m + 1.) / den * I4.s * exp(I4.v - mx);
+    return {mx + log(fabs(I)), sgnd(I)};
+}
+
+__device__ LogVal intC(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol)
+{
+    const int m = d.m;
+    if (m == 0) return {-INFINITY, 0};
+    const LogVal I1 = getI(lI, sI, d, pol, l1, l2 - 1);
+    const LogVal I2 = getI(lI, sI, d, pol, l1, l2 + 1);
+    const double den = 2 * l2 + 1;
+    double I = -(l2 + 1.) * (l2 + m) / den * I1.s * exp(I1.v - I2.v);
+    I += l2 * (l2 - m + 1.) / den * I2.s;
+    return {logi(m) + I2.v + log(fabs(I)), sgnd(I)};
+}
+
+__device__ __forceinline__ double term(LogVal t, double pref) { return t.s == 0 ? 0.0 : t.s * exp(t.v + pref); }
+
+__global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                  const double* __restrict__ lfac_tab, int mode)
+{
+    const BlockDesc& d = desc[blockIdx.z];
+    const int i2 = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int j2 = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i2 >= d.ldim || j2 >= d.ldim) return;
+    const int l1 = d.lmin + i2, l2 = d.lmin + j2, m = d.m;
+    const double* lI = at<double>(arena, d.off_i);
+    const int8_t* sI = at<int8_t>(arena, d.off_i + 16 * (int64_t)d.nl * d.nl);
+    const double* al = at<double>(arena, d.off_mie);
+    const double* bl = al + d.ldim;
+    const double lnL = (logi(2 * l1 + 1) + logi(2 * l2 + 1) - logi(l1) - logi(l1 + 1) - logi(l2) - logi(l2 + 1)
+                        + lfac(lfac_tab, l1 - m) + lfac(lfac_tab, l2 - m) - lfac(lfac_tab, l1 + m)
+                        - lfac(lfac_tab, l2 + m)) / 2.0;
+    const double a1 = al[i2], b1 = bl[i2], a2 = al[j2], b2 = bl[j2];
+    const int TEp = 0, TMp = 1;
+    // EE: √(a_l1 a_l2)(B_TM - A_TE);  MM: √(b_l1 b_l2)(A_TM - B_TE)
+    const double pEE = (a1 + a2) / 2 + lnL, pMM = (b1 + b2) / 2 + lnL;
+    const double EE = term(intB(lI, sI, d, l1, l2, TMp), pEE) - term(intA(lI, sI, d, l1, l2, TEp), pEE);
+    const double MM = term(intA(lI, sI, d, l1, l2, TMp), pMM) - term(intB(lI, sI, d, l1, l2, TEp), pMM);
+    double EM = 0, ME = 0;
+    if (m != 0) {
+        // EM: D_TM - C_TE with (a_l1 + b_l2)/2;  ME: C_TM - D_TE with (b_l1 + a_l2)/2;  D(l1,l2) = C(l2,l1)
+        const double pEM = (a1 + b2) / 2 + lnL, pME = (b1 + a2) / 2 + lnL;
+        EM = term(intC(lI, sI, d, l2, l1, TMp), pEM) - term(intC(lI, sI, d, l1, l2, TEp), pEM);
+        ME = term(intC(lI, sI, d, l1, l2, TMp), pME) - term(intC(lI, sI, d, l2, l1, TEp), pME);
+    }
+    double* A = at<double>(arena, d.off_mat);
+    const int64_t r = 2 * i2, c = 2 * j2, lda = d.lda;
+    const double s = mode == 0 ? -1.0 : 1.0;
+    const double one = mode == 0 ? 1.0 : 0.0;
+    A[r + c * lda] = s * EE + (i2 == j2 ? one : 0.0);
+    A[r + (c + 1) * lda] = s * EM;
+    A[r + 1 + c * lda] = s * ME;
+    A[r + 1 + (c + 1) * lda] = s * MM + (i2 == j2 ? one : 0.0);
+    if (i2 == j2) {
+        double* dg = at<double>(arena, d.off_diag);
+        dg[r] = EE;
+        dg[r + 1] = MM;
+    }
+}
+
+// Trace of M (Kahan, in index order as src/matrix.c:63-68) and the trace
+// shortcut flag |tr M| < 1e-8 (src/matrix.c:78-82).  One lane per block.
+__global__ void k_trace(const BlockDesc* __restrict__ desc, char* __restrict__ arena, BlockOut* __restrict__ out,
+                        int nblocks)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    const BlockDesc& d = desc[b];
+    const double* dg = at<double>(arena, d.off_diag);
+    double s = 0, c = 0;
+    for (int i = 0; i < d.N; i++) {
+        const double y = dg[i] - c;
+        const double t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    out[b].trace = s;
+    out[b].skip = fabs(s) < 1e-8 ? 1 : 0;
+    out[b].logdet = NAN;
+    out[b].status = 0;
+}
+
+// ||M||_F² and tr M² = Σ_ij M_ij M_ji of M (from the stored 1 - M and the
+// exact diagonal of M) and the LU-mode Mercator shortcut of
+// src/matrix.c:489-505: if ||M||_F < 1 and the bound on the neglected terms,
+// |n²/2 + n + log(1-n)|, is below 1e-8 of -tr M - tr M²/2, the block's LU is
+// skipped (skip = 2) and the host returns the two-term series.  One
+// workgroup per block.
+__global__ void __launch_bounds__(256) k_frobenius(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                   BlockOut* __restrict__ out)
+{
+    const BlockDesc& d = desc[blockIdx.x];
+    const double* A = at<double>(arena, d.off_mat);
+    const double* dg = at<double>(arena, d.off_diag);
+    double f = 0, t2 = 0;
+    for (int64_t idx = threadIdx.x; idx < (int64_t)d.N * d.N; idx += blockDim.x) {
+        const int i = (int)(idx % d.N), j = (int)(idx / d.N);
+        const double mij = i == j ? dg[i] : -A[i + (int64_t)j * d.lda];
+        const double mji = i == j ? dg[i] : -A[j + (int64_t)i * d.lda];
+        f += mij * mij;
+        t2 += mij * mji;
+    }
+    __shared__ double sf[256], st[256];
+    sf[threadIdx.x] = f;
+    st[threadIdx.x] = t2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sf[threadIdx.x] += sf[threadIdx.x + o];
+            st[threadIdx.x] += st[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        BlockOut& o = out[blockIdx.x];
+        o.frob2 = sf[0];
+        o.tr2 = st[0];
+        const double norm = sqrt(sf[0]);
+        if (!o.skip && norm < 1) {
+            const double mercator = -o.trace - st[0] / 2;
+            const double err = fabs(norm * norm / 2 + norm + log1p(-norm));
+            if (fabs(err / mercator) < 1e-8) o.skip = 2;
+        }
+    }
+}
+
+}  // namespace capsd
+
+// ---------------------------------------------------------------------------
+// Launchers (C linkage, used by the host library).
+using namespace capsd;
+
+extern "C" {
+
+void capsd_launch_mie(const BlockDesc* desc, char* arena, int nblocks, int max_ldim, hipStream_t s)
+{
+    dim3 g((max_ldim + 255) / 256, nblocks);
+    hipLaunchKernelGGL(k_mie, g, dim3(256), 0, s, desc, arena);
+}
+
+void capsd_launch_K(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int max_waves,
+                    int max_nK, int* status, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_K_partial, dim3(max_waves, nblocks), dim3(64), 0, s, desc, arena, lfac_tab);
+    hipLaunchKernelGGL(k_K_combine, dim3((max_nK + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, status);
+}
+
+void capsd_launch_I(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int64_t max_ntri,
+                    int* status, hipStream_t s)
+{
+    dim3 g((unsigned)((max_ntri + 255) / 256), nblocks);
+    hipLaunchKernelGGL(k_I, g, dim3(256), 0, s, desc, arena, lfac_tab, status);
+}
+
+void capsd_launch_assemble(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int max_ldim,
+                           int mode, hipStream_t s)
+{
+    dim3 g((max_ldim + 15) / 16, (max_ldim + 15) / 16, nblocks);
+    hipLaunchKernelGGL(k_assemble, g, dim3(256), 0, s, desc, arena, lfac_tab, mode);
+}
+
+void capsd_launch_trace(const BlockDesc* desc, char* arena, BlockOut* out, int nblocks, int frobenius, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_trace, dim3((nblocks + 63) / 64), dim3(64), 0, s, desc, arena, out, nblocks);
+    if (frobenius) hipLaunchKernelGGL(k_frobenius, dim3(nblocks), dim3(256), 0, s, desc, arena, out);
+}
+
+void capsd_launch_mie_points(int n, const double* xi_, const int* l, double LbyR, const double* epsm1, double* out,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mie_points, dim3((n + 63) / 64), dim3(64), 0, s, n, xi_, l, LbyR, epsm1, out);
+}
+
+void capsd_launch_fresnel_points(int n, const double* xi_, const double* k_, const double* epsm1, double* out,
+                                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fresnel_points, dim3((n + 63) / 64), dim3(64), 0, s, n, xi_, k_, epsm1, out);
+}
+
+}  // extern "C"

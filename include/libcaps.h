@@ -1,0 +1,193 @@
+/*
+ * libcaps_amd: the CaPS plane-sphere round-trip-matrix library, MI355X
+ * (gfx950) edition.
+ *
+ * C ABI of libcaps_amd.so.  Every entry point keeps the name, argument
+ * meaning, ownership rule and error behaviour of the CaPS v0.5 function it
+ * replaces (cited per declaration, reference file:line), so that existing
+ * callers (the block CLI caps_logdetD, the MPI driver caps, the AuAl
+ * example, cass, capc) link against this library unchanged.  The numerical
+ * work behind caps_logdetD, caps_M_init and kernel_logdet runs in FP64 HIP
+ * kernels on the GPU; the library fails loudly (abort with a message) when
+ * no gfx950 device is usable.
+ *
+ * Additions that have no counterpart in the reference are marked "extension"
+ * and carry the prefix caps_ or capsx_.
+ */
+#ifndef LIBCAPS_AMD_H
+#define LIBCAPS_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#include <stdio.h>
+#include <stdint.h>
+
+/* Polarization of a k-integral.  Reference: src/include/libcaps.h:17. */
+typedef enum { TE, TM } polarization_t;
+
+/* Sign of a log-represented number.  Reference: src/include/constants.h:15. */
+typedef signed char sign_t;
+
+/* Determinant algorithm.  Reference: src/include/matrix.h:14.  Here
+ * DETALG_HODLR (the default) and DETALG_CHOLESKY select the dense
+ * determinant with the HODLR-mode shortcuts of kernel_logdet, DETALG_LU and
+ * DETALG_QR the dense LU with the LU-mode shortcuts (Mercator series).  All
+ * of them factor with the GPU's blocked LU with partial pivoting. */
+typedef enum { DETALG_HODLR, DETALG_LU, DETALG_QR, DETALG_CHOLESKY } detalg_t;
+
+/* Defaults.  Reference: src/include/libcaps.h:29-33. */
+#define CAPS_MINIMUM_LDIM 20
+#define CAPS_FACTOR_LDIM 5
+#define CAPS_EPSREL 1e-8
+
+/* Physical constants.  Reference: src/include/constants.h:45-48. */
+#define CAPS_hbar    1.0545718e-34
+#define CAPS_hbar_eV 6.582119514e-16
+#define CAPS_kB      1.38064852e-23
+#define CAPS_c       299792458.
+
+/* The CaPS object.  Reference: src/include/libcaps.h:41-78 (same fields in
+ * the same order; one private pointer appended for the device context, so
+ * the offsets of all public fields are those of the reference).  Do not
+ * modify the fields yourself. */
+typedef struct caps
+{
+    double L;
+    double R;
+    double calL;
+    double LbyR;
+    double y;
+    double (*epsilonm1_plate)(double xi_, void *userdata);
+    void *userdata_plate;
+    double (*epsilonm1_sphere)(double xi_, void *userdata);
+    void *userdata_sphere;
+    int ldim;
+    double epsrel;
+    detalg_t detalg;
+    void *priv;      /* extension: device context (owned by the library) */
+} caps_t;
+
+/* Opaque per-block state (the reference's integration_t).  */
+typedef struct caps_block_state integration_t;
+typedef struct caps_block_state_plasma integration_plasma_t;
+
+/* Round-trip matrix object of one (ξ, m) block.  Reference:
+ * src/include/libcaps.h:101-109 (same fields and order).  al/bl hold
+ * log|a_l|, log|b_l| for l = lmin .. lmin+ldim-1 once caps_M_init returns. */
+typedef struct
+{
+    caps_t *caps;
+    int m, lmin, ldim;
+    integration_t *integration;
+    integration_plasma_t *integration_plasma;
+    double xi_;
+    double *al, *bl;
+} caps_M_t;
+
+/* ---- object, setters, info: src/include/libcaps.h:113-140 ------------- */
+
+/* Reference: src/libcaps.c:183-214.  NULL if R <= 0 or L <= 0. */
+caps_t *caps_init(double R, double L);
+/* Reference: src/libcaps.c:223-226. */
+void caps_free(caps_t *self);
+/* Reference: src/libcaps.c:239-262. */
+void caps_build(FILE *stream, const char *prefix);
+/* Reference: src/libcaps.c:273-294. */
+void caps_info(caps_t *self, FILE *stream, const char *prefix);
+
+/* Reference: src/libcaps.c:306-313, 322-325.  Return 1 on success, 0 on
+ * invalid input (epsrel <= 0). */
+int    caps_set_epsrel(caps_t *self, double epsrel);
+double caps_get_epsrel(caps_t *self);
+/* Reference: src/libcaps.c:436-443, 453-456 (0 if ldim < 1). */
+int caps_set_ldim(caps_t *self, int ldim);
+int caps_get_ldim(caps_t *self);
+/* Reference: src/libcaps.c:401-411, 420-423. */
+int      caps_set_detalg(caps_t *self, detalg_t detalg);
+detalg_t caps_get_detalg(caps_t *self);
+
+/* ε(iξ)-1 callbacks, evaluated on the host once per ξ and body.
+ * Reference: src/libcaps.c:337-381; userdata is borrowed. */
+void caps_set_epsilonm1(caps_t *self, double (*epsilonm1)(double xi_, void *userdata), void *userdata);
+void caps_set_epsilonm1_plate(caps_t *self, double (*epsilonm1)(double xi_, void *userdata), void *userdata);
+void caps_set_epsilonm1_sphere(caps_t *self, double (*epsilonm1)(double xi_, void *userdata), void *userdata);
+/* Reference: src/libcaps.c:104-120. */
+double caps_epsilonm1_plate(caps_t *self, double xi_);
+double caps_epsilonm1_sphere(caps_t *self, double xi_);
+/* Reference: src/libcaps.c:129-132 (+∞) and 150-156 (Drude, userdata =
+ * {ωp, γ} in rad/s). */
+double caps_epsilonm1_perf(double xi_, void *userdata);
+double caps_epsilonm1_drude(double xi_, void *userdata);
+
+/* ---- scalar physics: src/include/libcaps.h:116,142-157 ---------------- */
+
+/* log Λ_{l1 l2}^{(m)}.  Reference: src/libcaps.c:45-48. */
+double caps_lnLambda(int l1, int l2, int m);
+/* ℓ-window [lmin, lmax = lmin+ldim).  Reference: src/libcaps.c:63-88. */
+int caps_estimate_lminmax(caps_t *self, int m, size_t *lmin_p, size_t *lmax_p);
+/* log|a_l|, log|b_l| (GPU).  Reference: src/libcaps.c:485-520, 557-602. */
+void caps_mie(caps_t *self, double xi_, int l, double *lna, double *lnb);
+void caps_mie_perf(caps_t *self, double xi_, int l, double *lna, double *lnb);
+/* Fresnel coefficients.  Reference: src/libcaps.c:616-645. */
+void caps_fresnel(caps_t *self, double xi_, double k_, double *r_TE, double *r_TM);
+
+/* ---- round-trip matrix and determinant: src/include/libcaps.h:145-150 -- */
+
+/* Builds the whole block on the GPU (Mie, K, I, matrix) and keeps a host
+ * copy.  Aborts if xi_ <= 0.  Reference: src/libcaps.c:673-697. */
+caps_M_t *caps_M_init(caps_t *self, int m, double xi_);
+/* Element M^{(m)}_{l1 l2}(p1,p2), p = 'E' or 'M', lmin <= l1,l2 < lmin+ldim.
+ * Reference: src/libcaps.c:761-831. */
+double caps_M_elem(caps_M_t *self, int l1, int l2, char p1, char p2);
+/* Reference: src/libcaps.c:717-743 (interleaved 2x2 index map). */
+double caps_kernel_M(int i, int j, void *args_);
+/* Reference: src/libcaps.c:840-846. */
+void caps_M_free(caps_M_t *self);
+
+/* log det(1 - M^{(m)}(ξ)), GPU.  Aborts if xi_ <= 0.  Reference:
+ * src/libcaps.c:950-962. */
+double caps_logdetD(caps_t *self, double xi_, int m);
+
+/* log det(1 - A) of an N x N kernel given as a host callback; the matrix
+ * is filled on the host and factored on the GPU.  sym_spd is accepted for
+ * compatibility.  Reference: src/matrix.c:57-149. */
+double kernel_logdet(int dim, double (*kernel)(int, int, void *), void *args, int sym_spd, detalg_t detalg);
+
+/* ξ = 0 Drude term.  Reference: src/libcaps.c:1043-1065. */
+double caps_ht_drude(caps_t *caps);
+
+/* ---- extensions ------------------------------------------------------- */
+
+/* Extension: n blocks (xi_[i], m[i]) of one caps object in one batched GPU
+ * launch sequence; results in out[i], identical to caps_logdetD(self,
+ * xi_[i], m[i]).  Returns 0 on success, nonzero on invalid input (a
+ * message goes to stderr).  Replaces the reference's one-block-per-MPI-task
+ * loop (src/caps.c:163-232, 869-957). */
+int caps_logdetD_batch(caps_t *self, const double *xi_, const int *m, int n, double *out);
+
+/* Extension: per-stage timing of the last batch on this caps object, in
+ * milliseconds: [0] Mie, [1] K, [2] I, [3] assembly, [4] LU, [5] total
+ * (HIP events on the library's stream). */
+void caps_last_timing(caps_t *self, double ms[6]);
+
+/* Extension: free energy of the plane-sphere system at temperature T (K)
+ * in units of k_B T for T > 0, with the Matsubara sum and m-sum of the
+ * reference driver (src/caps.c:307-396, 722-850: m-sum stops at the first
+ * m with logdetD/logdetD(m=0) < cutoff, Matsubara sum stops at
+ * |v_n/v_0| < epsrel, m = 0 and n = 0 halved).  mode 0 = plain (one block
+ * at a time, in F_xi order), mode 1 = batched (all m of a Matsubara term in
+ * one launch, verified against the same cutoff rule).  drude_xi0: nonzero
+ * adds the analytic ξ = 0 Drude term (caps_ht_drude), as the reference
+ * driver does for finite ωp and γ > 0.  nblocks (may be NULL) receives the
+ * number of determinants computed.  Returns F·(L+R)/(ħc)·... exactly as the
+ * reference prints: F/(k_B T) summed terms times 1 (see caps_amd docs). */
+double caps_free_energy(caps_t *self, double T, double cutoff, double epsrel, int mode,
+                        int drude_xi0, long *nblocks);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,230 @@
+"""Write parity fixtures from the reference CaPS library compiled from its own
+sources (oracle/build_ref.sh -> oracle/_ref/libcaps_ref.so).
+
+Run in a container that holds the reference tree; the JSON it writes holds
+only inputs and outputs (no reference code) and is what the tests read on
+machines without the reference.
+
+    python tests/golden/gen_ref_vectors.py blocks      # block logdetD values
+    python tests/golden/gen_ref_vectors.py tables      # K, I and M elements
+    python tests/golden/gen_ref_vectors.py free_energy # C2 free energy (~10 min, 8 cores)
+"""
+import ctypes
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+LIB = os.path.join(ROOT, "oracle", "_ref", "libcaps_ref.so")
+HBAR_EV = 6.582119514e-16
+KB, HBAR, C = 1.38064852e-23, 1.0545718e-34, 299792458.0
+
+_lib = None
+_keep = []
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        # LAPACK symbols of the reference's dense route stay unresolved; the
+        # default (HODLR) route never calls them, so bind lazily.
+        L = ctypes.CDLL(LIB, mode=os.RTLD_LAZY)
+        d, i, p = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        L.caps_init.restype = p
+        L.caps_init.argtypes = [d, d]
+        L.caps_logdetD.restype = d
+        L.caps_logdetD.argtypes = [p, d, i]
+        L.caps_set_ldim.argtypes = [p, i]
+        L.caps_set_epsilonm1.argtypes = [p, p, p]
+        L.caps_ht_drude.restype = d
+        L.caps_ht_drude.argtypes = [p]
+        L.caps_integrate_init.restype = p
+        L.caps_integrate_init.argtypes = [p, d, i, d]
+        L.caps_integrate_K.restype = d
+        L.caps_integrate_K.argtypes = [p, i, i, ctypes.POINTER(ctypes.c_byte)]
+        L.caps_integrate_I.restype = d
+        L.caps_integrate_I.argtypes = [p, i, i, i, ctypes.POINTER(ctypes.c_byte)]
+        L.caps_M_init.restype = p
+        L.caps_M_init.argtypes = [p, i, d]
+        L.caps_M_elem.restype = d
+        L.caps_M_elem.argtypes = [p, i, i, ctypes.c_char, ctypes.c_char]
+        L.caps_M_free.argtypes = [p]
+        L.caps_estimate_lminmax.restype = i
+        L.caps_estimate_lminmax.argtypes = [p, i, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
+        _lib = L
+    return _lib
+
+
+def make_caps(R, L, ldim, drude=None):
+    c = lib().caps_init(R, L)
+    lib().caps_set_ldim(c, ldim)
+    if drude is not None:
+        ud = (ctypes.c_double * 2)(*drude)
+        _keep.append(ud)
+        lib().caps_set_epsilonm1(c, ctypes.cast(lib().caps_epsilonm1_drude, ctypes.c_void_p),
+                                 ctypes.cast(ud, ctypes.c_void_p))
+    return c
+
+
+_caps_cache = {}
+
+
+def _block(args):
+    R, L, ldim, drude, xi, m = args
+    key = (R, L, ldim, drude)
+    if key not in _caps_cache:
+        _caps_cache[key] = make_caps(R, L, ldim, drude)
+    t = time.time()
+    v = lib().caps_logdetD(_caps_cache[key], xi, m)
+    return v, time.time() - t
+
+
+def drude_c2():
+    return (9.0 / HBAR_EV, 0.035 / HBAR_EV)
+
+
+def gen_blocks(pool):
+    rows = []
+    jobs = []
+    # C1: perfect reflectors, R/L = 20, ldim = 120, xi = 1, m = 0..120
+    for m in range(121):
+        jobs.append(("C1", (20e-6, 1e-6, 120, None, 1.0, m)))
+    # C2 samples: Drude 9 eV / 35 meV, R/L = 200, ldim = 1200
+    Ts = 2 * math.pi * KB * (200e-6 + 1e-6) * 300 / (HBAR * C)
+    for n, ms in [(1, [0, 1, 2, 5, 10, 25, 50, 75, 100, 125]), (2, [0, 20, 60]), (5, [0, 30]), (9, [0, 10])]:
+        for m in ms:
+            jobs.append(("C2", (200e-6, 1e-6, 1200, drude_c2(), n * Ts, m)))
+    # C3-like: perfect reflectors at other aspect ratios (small enough for CI)
+    for (RbyL, xi, m) in [(50, 0.5, 0), (50, 30.0, 12), (80, 150.0, 40), (60, 3.0, 75)]:
+        jobs.append(("C3s", (RbyL * 1e-6, 1e-6, 6 * RbyL, None, xi, m)))
+    # Drude at R/L = 50 (both bodies), small ldim
+    for (xi, m) in [(0.3, 0), (3.0, 2), (40.0, 9)]:
+        jobs.append(("drude50", (50e-6, 1e-6, 150, drude_c2(), xi, m)))
+    res = pool.map(_block, [j[1] for j in jobs], chunksize=1)
+    for (tag, (R, L, ldim, drude, xi, m)), (v, dt) in zip(jobs, res):
+        rows.append({"set": tag, "R": R, "L": L, "ldim": ldim, "omegap": None if drude is None else drude[0],
+                     "gamma": None if drude is None else drude[1], "xi_": xi, "m": m, "logdetD": v,
+                     "ref_seconds": dt})
+    return rows
+
+
+def gen_tables():
+    """K_ν, I_{l1 l2} and matrix elements of a few small blocks."""
+    out = []
+    cases = [
+        ("pr", 20e-6, 1e-6, 40, None, 1.0, 0),
+        ("pr", 20e-6, 1e-6, 40, None, 1.0, 3),
+        ("pr", 100e-6, 1e-6, 60, None, 0.1, 1),
+        ("pr", 100e-6, 1e-6, 60, None, 10.0, 20),
+        ("drude", 200e-6, 1e-6, 50, drude_c2(), 165.4564, 0),
+        ("drude", 200e-6, 1e-6, 50, drude_c2(), 165.4564, 7),
+        ("drude", 200e-6, 1e-6, 50, drude_c2(), 1489.1076, 30),
+    ]
+    L_ = lib()
+    for tag, R, L, ldim, drude, xi, m in cases:
+        c = make_caps(R, L, ldim, drude)
+        lmin, lmax = ctypes.c_size_t(), ctypes.c_size_t()
+        L_.caps_estimate_lminmax(c, m, ctypes.byref(lmin), ctypes.byref(lmax))
+        lmin = lmin.value
+        integ = L_.caps_integrate_init(c, xi, m, 1e-8)
+        s = ctypes.c_byte()
+        mu = 2 * m if m > 0 else 2
+        K = []
+        for nu in sorted(set([mu, mu + 1, mu + 2, mu + 5, mu + 17, mu + 40, 2 * (lmin + ldim) - 3])):
+            for p in (0, 1):
+                v = L_.caps_integrate_K(integ, nu, p, ctypes.byref(s))
+                K.append([nu, p, v, s.value])
+        I = []
+        ls = sorted(set([max(lmin, max(m, 1)), lmin + 1, lmin + 7, lmin + ldim // 2, lmin + ldim - 1]))
+        for l1 in ls:
+            for l2 in ls:
+                for p in (0, 1):
+                    v = L_.caps_integrate_I(integ, l1, l2, p, ctypes.byref(s))
+                    I.append([l1, l2, p, v, s.value])
+        Mobj = L_.caps_M_init(c, m, xi)
+        E = []
+        for l1 in ls:
+            for l2 in ls:
+                for p1 in "EM":
+                    for p2 in "EM":
+                        E.append([l1, l2, p1, p2, L_.caps_M_elem(Mobj, l1, l2, p1.encode(), p2.encode())])
+        L_.caps_M_free(Mobj)
+        out.append({"kind": tag, "R": R, "L": L, "ldim": ldim, "omegap": None if drude is None else drude[0],
+                    "gamma": None if drude is None else drude[1], "xi_": xi, "m": m, "lmin": lmin,
+                    "K": K, "I": I, "M": E})
+    return out
+
+
+def gen_free_energy(pool):
+    """Drude 9 eV / 35 meV, T = 300 K, R = 200 µm, L = 1 µm, ldim = 1200,
+    cutoff 1e-9, epsrel 1e-6: the Matsubara and m sums of src/caps.c:307-396,
+    722-850 with sequential (one-worker) semantics over the reference's
+    caps_logdetD.  Blocks are evaluated 8 at a time; the set of m summed is
+    the sequential one (first m with v/v0 < cutoff, included)."""
+    R, L, ldim, T = 200e-6, 1e-6, 1200, 300.0
+    cutoff, epsrel = 1e-9, 1e-6
+    drude = drude_c2()
+    Ts = 2 * math.pi * KB * (R + L) * T / (HBAR * C)
+    c = make_caps(R, L, ldim, drude)
+    v0 = lib().caps_ht_drude(c)
+    v = [v0]
+    per_xi = []
+    nblocks = 0
+    t0 = time.time()
+    for n in range(1, 1000):
+        xi = n * Ts
+        terms = []
+        mstop = None
+        m0 = 0
+        while mstop is None:
+            chunk = list(range(m0, m0 + 8))
+            res = pool.map(_block, [(R, L, ldim, drude, xi, m) for m in chunk], chunksize=1)
+            for m, (val, dt) in zip(chunk, res):
+                terms.append(val)
+                nblocks += 1
+                if val == 0 or val / terms[0] < cutoff:
+                    mstop = m
+                    break
+            m0 += 8
+        terms = terms[: mstop + 1]
+        per_xi.append({"n": n, "xi_": xi, "mstop": mstop, "terms": terms})
+        t = [terms[0] / 2] + terms[1:]
+        vn = math.fsum(t)
+        v.append(vn)
+        print("n=%d xi=%.6f mstop=%d v=%.16g  (%.0fs)" % (n, xi, mstop, vn, time.time() - t0), flush=True)
+        if abs(vn / v0) < epsrel:
+            break
+    vv = [v[0] / 2] + v[1:]
+    F = Ts / math.pi * math.fsum(vv)
+    return {"R": R, "L": L, "T": T, "ldim": ldim, "omegap": drude[0], "gamma": drude[1], "cutoff": cutoff,
+            "epsrel": epsrel, "T_scaled": Ts, "xi0_drude": v0, "per_xi": per_xi, "F": F,
+            "determinants": nblocks, "seconds_8_workers": time.time() - t0}
+
+
+def main():
+    what = sys.argv[1] if len(sys.argv) > 1 else "blocks"
+    if what == "tables":
+        data = gen_tables()
+        name = "ref_tables.json"
+    else:
+        with mp.Pool(8) as pool:
+            if what == "blocks":
+                data = gen_blocks(pool)
+                name = "ref_blocks.json"
+            elif what == "free_energy":
+                data = gen_free_energy(pool)
+                name = "ref_free_energy_c2.json"
+            else:
+                sys.exit("unknown target " + what)
+    with open(os.path.join(HERE, name), "w") as f:
+        json.dump({"source": "reference CaPS v0.5 compiled from its sources (oracle/build_ref.sh), "
+                             "default detalg (HODLR), epsrel 1e-8", "data": data}, f, indent=0)
+    print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

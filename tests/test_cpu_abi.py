@@ -1,0 +1,63 @@
+"""CPU-side checks of the drop-in boundary: libcaps_amd.so loads and exports
+every function include/libcaps.h declares (no compute calls: no GPU here)."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "libcaps.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b(\w+)\s*\([^;{]*\)\s*;", text, flags=re.M)
+    return sorted(set(n for n in names if not n.startswith("__")))
+
+
+def test_header_declares_reference_api():
+    names = declared_functions()
+    for n in ["caps_init", "caps_free", "caps_logdetD", "caps_M_init", "caps_M_elem", "caps_M_free",
+              "caps_kernel_M", "kernel_logdet", "caps_set_ldim", "caps_set_epsilonm1", "caps_mie",
+              "caps_fresnel", "caps_lnLambda", "caps_estimate_lminmax", "caps_ht_drude",
+              "caps_logdetD_batch", "caps_free_energy"]:
+        assert n in names, n
+
+
+def test_library_exports_every_declared_symbol():
+    import caps_amd
+    L = ctypes.CDLL(caps_amd.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_caps_struct_layout_matches_reference():
+    """Public fields of caps_t sit at the reference's offsets
+    (src/include/libcaps.h:41-78): doubles L, R, calL, LbyR, y, then the two
+    callback/userdata pairs, ldim, epsrel, detalg."""
+    import caps_amd
+    L = caps_amd.lib()
+    p = L.caps_init(2.0, 0.5)
+    d = ctypes.cast(p, ctypes.POINTER(ctypes.c_double))
+    assert (d[0], d[1], d[2], d[3]) == (0.5, 2.0, 2.5, 0.25)
+    ldim = ctypes.cast(p + 9 * 8, ctypes.POINTER(ctypes.c_int))[0]
+    assert ldim == 20
+    epsrel = ctypes.cast(p + 10 * 8, ctypes.POINTER(ctypes.c_double))[0]
+    assert epsrel == 1e-8
+    L.caps_set_ldim(p, 77)
+    assert ctypes.cast(p + 9 * 8, ctypes.POINTER(ctypes.c_int))[0] == 77
+    assert L.caps_set_ldim(p, 0) == 0 and L.caps_set_epsrel(p, -1.0) == 0
+    L.caps_free(p)
+    assert L.caps_init(-1.0, 1.0) is None
+
+
+def test_host_only_entry_points():
+    import caps_amd
+    L = caps_amd.lib()
+    import pyoracle as po
+    for l1, l2, m in [(3, 5, 1), (100, 120, 40), (2000, 2100, 5)]:
+        assert abs(L.caps_lnLambda(l1, l2, m) - po.lib().orc_lnLambda(l1, l2, m)) < 1e-12 * max(1, abs(L.caps_lnLambda(l1, l2, m)))
+    c = caps_amd.Caps(200e-6, 1e-6, ldim=1200)
+    assert c.lminmax(0) == (1, 1201)
+    assert c.lminmax(100) == (400, 1600)
+    assert c.lminmax(200) == (1400, 2600)
+    assert abs(c.ht_drude() - po.lib().orc_ht_drude(1e-6 / 200e-6)) < 1e-14 * abs(c.ht_drude())
