@@ -1,0 +1,146 @@
+"""Benchmark: logdetD blocks/s of the round-trip-matrix path on one MI355X.
+
+Workload (one "step"): the T = 300 K free energy of a Drude sphere (ωp = 9 eV,
+γ = 35 meV) above a Drude plate, R = 200 µm, L = 1 µm (R/L = 200),
+ldim = 1200 (blocks 2400 x 2400, FP64), full Matsubara sum with the
+reference driver's cutoffs (m-cutoff 1e-9, Matsubara epsrel 1e-6).  Every
+block entering the energy is computed on the GPU; the batched form runs all
+m of a Matsubara frequency in one launch sequence.  The same step in the
+plain one-block-at-a-time form is timed once beside it.
+
+Prints one JSON line (the driver's contract), with a roofline entry for the
+dominant kernel (HIP events on the library's stream) and a CPU baseline: the
+reference CaPS library compiled from its sources (oracle/_ref), or the CPU
+oracle when that is absent, timed on a bounded sample of the same blocks.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+R, L, LDIM, T = 200e-6, 1e-6, 1200, 300.0
+OMEGAP_EV, GAMMA_EV = 9.0, 0.035
+CUTOFF, EPSREL = 1e-9, 1e-6
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def gpu_step(c, mode):
+    t0 = time.perf_counter()
+    F, nblocks = c.free_energy(T, cutoff=CUTOFF, epsrel=EPSREL, mode=mode)
+    return F, nblocks, time.perf_counter() - t0
+
+
+def cpu_baseline(sample_blocks=8):
+    """Reference caps_logdetD (compiled from its sources) on C2 blocks
+    (ξ_1, m = 0..sample_blocks-1), one block per process, on the host cores."""
+    import multiprocessing as mp
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libcaps_ref.so")
+    cores = min(len(os.sched_getaffinity(0)), 16, sample_blocks)
+    from caps_amd import T_scaled
+    xi = T_scaled(R, L, T)
+    jobs = [(xi, m) for m in range(sample_blocks)]
+    kind = "reference" if os.path.exists(ref_so) else "port"
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(cores) as pool:
+        vals = pool.map(_cpu_block if kind == "reference" else _cpu_block_oracle, jobs, chunksize=1)
+    dt = time.perf_counter() - t0
+    return {"value": len(jobs) / dt, "unit": "blocks/s", "cores": cores, "kind": kind,
+            "sample": "%d blocks of the workload (xi_1=%.4f, m=0..%d, N=2400), one per process, %s" % (
+                len(jobs), xi, sample_blocks - 1,
+                "reference CaPS caps_logdetD (default HODLR), compiled from its sources" if kind == "reference"
+                else "CPU oracle (dense LU)"),
+            "seconds": dt, "values": vals}
+
+
+def _cpu_block(job):
+    import ctypes
+    xi, m = job
+    lib = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libcaps_ref.so"), mode=os.RTLD_LAZY)
+    lib.caps_init.restype = ctypes.c_void_p
+    lib.caps_init.argtypes = [ctypes.c_double, ctypes.c_double]
+    lib.caps_logdetD.restype = ctypes.c_double
+    lib.caps_logdetD.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+    lib.caps_set_ldim.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.caps_set_epsilonm1.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    hb = 6.582119514e-16
+    ud = (ctypes.c_double * 2)(OMEGAP_EV / hb, GAMMA_EV / hb)
+    c = lib.caps_init(R, L)
+    lib.caps_set_ldim(c, LDIM)
+    lib.caps_set_epsilonm1(c, ctypes.cast(lib.caps_epsilonm1_drude, ctypes.c_void_p), ctypes.cast(ud, ctypes.c_void_p))
+    return lib.caps_logdetD(c, xi, m)
+
+
+def _cpu_block_oracle(job):
+    xi, m = job
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    hb = po.HBAR_EV
+    c = po.caps(R, L, ldim=LDIM, drude=(OMEGAP_EV / hb, GAMMA_EV / hb))
+    return po.logdetD(c, xi, m)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (every (xi, m) block fits one MI355X); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import caps_amd
+    caps_amd.lib()
+    c = caps_amd.Caps(R, L, ldim=LDIM)
+    c.set_drude(OMEGAP_EV, GAMMA_EV)
+
+    for _ in range(args.warmup):
+        gpu_step(c, "batched")
+    caps_amd.lib().capsx_stats_reset(c.ptr)
+    times, F, nb = [], None, 0
+    for _ in range(args.steps):
+        F, nb, dt = gpu_step(c, "batched")
+        times.append(dt)
+    stats = c.stats()
+    total = sum(times)
+    value = nb * args.steps / total
+
+    plain = None
+    if not args.no_plain:
+        Fp, nbp, dtp = gpu_step(c, "plain")
+        plain = {"blocks_per_s": nbp / dtp, "seconds": dtp, "F": Fp, "blocks": nbp}
+
+    # roofline of the dominant kernel, from the library's per-kernel events
+    roof = stats["roofline"]
+    cpu = None if args.no_cpu else cpu_baseline()
+    line = {
+        "metric": "logdetD blocks/s @R/L=200 FP64 (batched form; plain form and LU FP64 TF/s alongside)",
+        "value": value, "unit": "blocks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000 * total / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (analytic Drude material, no dataset)",
+        "config": {"workload": "free energy T=300K, Drude 9eV/35meV both bodies, R/L=200 (R=200um, L=1um), "
+                               "ldim=1200 (N=2400), m-cutoff 1e-9, Matsubara epsrel 1e-6",
+                   "blocks_per_step": nb, "F": F, "form": "batched"},
+        "plain": plain,
+        "stage_ms_per_step": {k: v / args.steps for k, v in stats["stage_ms"].items()},
+        "lu_tflops": stats["lu_tflops"],
+        "roofline": roof,
+        "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -64,6 +64,7 @@ def lib():
             "capsx_block_tables": (i, [p, d, i, ip, ip, dp, ip, ip, dp, ctypes.POINTER(ctypes.c_byte), i, i]),
             "capsx_block_nodes": (i, [p, d, i]),
             "capsx_M_copy": (i, [p, dp]),
+            "capsx_stats": (None, [p, dp]), "capsx_stats_reset": (None, [p]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -144,6 +145,19 @@ class Caps:
         t = np.zeros(6)
         lib().caps_last_timing(self.ptr, _dptr(t))
         return dict(zip(["mie", "K", "I", "assemble", "lu", "total"], t))
+
+    def stats(self, reset=False):
+        """Accumulated stage times and work counters of this object's batches."""
+        a = np.zeros(11)
+        lib().capsx_stats(self.ptr, _dptr(a))
+        if reset:
+            lib().capsx_stats_reset(self.ptr)
+        names = ["mie", "K", "I", "assemble", "lu", "total"]
+        return {"stage_ms": dict(zip(names, a[:6])), "blocks": int(a[6]), "blocks_factored": int(a[7]),
+                "lu_flops": a[8], "batches": int(a[9]), "I_integrals": a[10]}
+
+    def reset_stats(self):
+        lib().capsx_stats_reset(self.ptr)
 
     def mie(self, xi_, l):
         a, b = ctypes.c_double(), ctypes.c_double()

@@ -99,6 +99,9 @@ struct Ctx {
     BlockOut* h_out = nullptr;
     int* h_status = nullptr;
     double timing[6] = {0, 0, 0, 0, 0, 0};
+    // accumulated since capsx_stats_reset: stage ms [0..5], blocks, blocks
+    // factored (not skipped), algorithmic LU flops, batches, I pairs
+    double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     std::mutex mu;
 };
 
@@ -324,6 +327,18 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     c->timing[5] = ms;
     TERMINATE(*c->h_status != 0, "non-finite k-integral (flags %d) in a batch of %d blocks (xi_=%g, m=%d ...)",
               *c->h_status, n, c->h_desc[0].xi_, c->h_desc[0].m);
+    for (int i = 0; i < 6; i++) c->acc[i] += c->timing[i];
+    c->acc[6] += n;
+    c->acc[9] += 1;
+    for (int b = 0; b < n; b++) {
+        const BlockDesc& d = c->h_desc[b];
+        c->acc[10] += (double)d.nl * (d.nl + 1) / 2 * (d.is_pr ? 1 : 2);
+        if (mode == 0 && c->h_out[b].skip == 0) {
+            const double N = d.N;
+            c->acc[7] += 1;
+            c->acc[8] += 2.0 / 3 * N * N * N - 0.5 * N * N + 5.0 / 6 * N;
+        }
+    }
 }
 
 // log det(1-M) from the block scalars with the reference's shortcut rules.
@@ -940,6 +955,22 @@ int capsx_M_copy(caps_M_t* self, double* out)
     const caps_block_state* st = self->integration;
     memcpy(out, st->M, (size_t)st->N * st->N * sizeof(double));
     return st->N;
+}
+
+// Extension: accumulated statistics since the last reset: out[0..5] stage
+// milliseconds (Mie, K, I, assembly, LU, total), out[6] blocks, out[7]
+// blocks factored, out[8] algorithmic LU flops Σ(2/3 N³ - 1/2 N² + 5/6 N),
+// out[9] batches, out[10] (l1 >= l2, polarization) I integrals.
+void capsx_stats(caps_t* self, double out[11])
+{
+    Ctx* c = ctx_of(self);
+    for (int i = 0; i < 11; i++) out[i] = c->acc[i];
+}
+
+void capsx_stats_reset(caps_t* self)
+{
+    Ctx* c = ctx_of(self);
+    for (int i = 0; i < 11; i++) c->acc[i] = 0;
 }
 
 // Extension: number of quadrature nodes of the K integrals of a block.

@@ -329,7 +329,8 @@ __global__ void __launch_bounds__(256) k_lu_gemm(const BlockDesc* __restrict__ d
             acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
         }
     }
-    // D layout: lane l, register q holds row 4*(l/16)+q, column l%16 of the 16x16 tile
+    // D layout (measured with k_mfma_probe): lane l, register q holds row
+    // l/16 + 4q, column l%16 of the 16x16 tile
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -338,7 +339,7 @@ __global__ void __launch_bounds__(256) k_lu_gemm(const BlockDesc* __restrict__ d
             if (col >= N) continue;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int row = m0 + wr + 16 * i + 4 * lk + q;
+                const int row = m0 + wr + 16 * i + lk + 4 * q;
                 if (row < N) A[row + (int64_t)col * lda] -= acc[i][j][q];
             }
         }
@@ -373,7 +374,7 @@ __global__ void k_mfma_probe(const double* __restrict__ A, const double* __restr
     double4_t acc = (double4_t){0, 0, 0, 0};
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc, 0, 0, 0);
 #pragma unroll
-    for (int q = 0; q < 4; q++) D[(4 * (lane >> 4) + q) * 16 + (lane & 15)] = acc[q];
+    for (int q = 0; q < 4; q++) D[((lane >> 4) + 4 * q) * 16 + (lane & 15)] = acc[q];
 }
 
 }  // namespace capsd
