@@ -34,8 +34,8 @@ HBM_PEAK_GBS = 8000.0
 
 def gpu_step(c, mode):
     t0 = time.perf_counter()
-    F, nblocks = c.free_energy(T, cutoff=CUTOFF, epsrel=EPSREL, mode=mode)
-    return F, nblocks, time.perf_counter() - t0
+    F, ncomputed, nused = c.free_energy(T, cutoff=CUTOFF, epsrel=EPSREL, mode=mode)
+    return F, nused, ncomputed, time.perf_counter() - t0
 
 
 def cpu_baseline(sample_blocks=8):
@@ -108,22 +108,35 @@ def main():
 
     for _ in range(args.warmup):
         gpu_step(c, "batched")
-    caps_amd.lib().capsx_stats_reset(c.ptr)
-    times, F, nb = [], None, 0
+    c.profile(True)
+    c.reset_stats()
+    times, F, nb, ncomp = [], None, 0, 0
     for _ in range(args.steps):
-        F, nb, dt = gpu_step(c, "batched")
+        F, nb, ncomp, dt = gpu_step(c, "batched")
         times.append(dt)
-    stats = c.stats()
+        print("bench: batched step %.3f s, %d blocks in the energy (%d computed), F=%.16g" % (dt, nb, ncomp, F),
+              file=sys.stderr, flush=True)
+    st = c.stats(reset=True)
+    c.profile(False)
     total = sum(times)
     value = nb * args.steps / total
 
     plain = None
     if not args.no_plain:
-        Fp, nbp, dtp = gpu_step(c, "plain")
-        plain = {"blocks_per_s": nbp / dtp, "seconds": dtp, "F": Fp, "blocks": nbp}
+        Fp, nbp, _, dtp = gpu_step(c, "plain")
+        plain = {"blocks_per_s": nbp / dtp, "seconds": dtp, "F": Fp, "blocks": nbp,
+                 "speedup_batched_vs_plain": (nb / (total / args.steps)) / (nbp / dtp)}
+        print("bench: plain step %.3f s, %d blocks" % (dtp, nbp), file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel, from the library's per-kernel events
-    roof = stats["roofline"]
+    # dominant kernel: the LU's trailing update (k_lu_gemm), FP64 MFMA
+    gemm_s = st["gemm_ms"] / 1e3
+    achieved = st["gemm_flops"] / gemm_s / 1e12 if gemm_s > 0 else None
+    roof = {"kernel": "k_lu_gemm", "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS if achieved else None, "traffic": None,
+            "launches": st["gemm_launches"],
+            "avg_launch_us": 1e3 * st["gemm_ms"] / max(1, st["gemm_launches"]),
+            "flops_per_launch": st["gemm_flops"] / max(1, st["gemm_launches"])}
+    lu_s = st["stage_ms"]["lu"] / 1e3
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "logdetD blocks/s @R/L=200 FP64 (batched form; plain form and LU FP64 TF/s alongside)",
@@ -132,10 +145,11 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (analytic Drude material, no dataset)",
         "config": {"workload": "free energy T=300K, Drude 9eV/35meV both bodies, R/L=200 (R=200um, L=1um), "
                                "ldim=1200 (N=2400), m-cutoff 1e-9, Matsubara epsrel 1e-6",
-                   "blocks_per_step": nb, "F": F, "form": "batched"},
+                   "blocks_per_step": nb, "blocks_computed_per_step": ncomp, "F": F, "form": "batched"},
         "plain": plain,
-        "stage_ms_per_step": {k: v / args.steps for k, v in stats["stage_ms"].items()},
-        "lu_tflops": stats["lu_tflops"],
+        "stage_ms_per_step": {k: v / args.steps for k, v in st["stage_ms"].items()},
+        "lu_tflops": st["lu_flops"] / lu_s / 1e12 if lu_s > 0 else None,
+        "blocks_factored_per_step": st["blocks_factored"] / args.steps,
         "roofline": roof,
         "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }

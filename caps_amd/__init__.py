@@ -58,13 +58,16 @@ def lib():
             "caps_M_init": (p, [p, i, d]), "caps_M_elem": (d, [p, i, i, ctypes.c_char, ctypes.c_char]),
             "caps_M_free": (None, [p]), "caps_kernel_M": (d, [i, i, p]),
             "caps_ht_drude": (d, [p]),
-            "caps_free_energy": (d, [p, d, d, d, i, i, ctypes.POINTER(ctypes.c_long)]),
+            "caps_free_energy": (d, [p, d, d, d, i, i, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
+            "caps_logdetD0": (None, [p, i, d, dp, dp, dp]),
+            "caps_ht_perf": (d, [p, d]),
             "capsx_lu_logdet_batch": (i, [dp, i, i, dp]),
             "capsx_mfma_probe": (i, [dp, dp, dp]),
             "capsx_block_tables": (i, [p, d, i, ip, ip, dp, ip, ip, dp, ctypes.POINTER(ctypes.c_byte), i, i]),
             "capsx_block_nodes": (i, [p, d, i]),
             "capsx_M_copy": (i, [p, dp]),
             "capsx_stats": (None, [p, dp]), "capsx_stats_reset": (None, [p]),
+            "capsx_profile": (None, [p, i]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -148,13 +151,18 @@ class Caps:
 
     def stats(self, reset=False):
         """Accumulated stage times and work counters of this object's batches."""
-        a = np.zeros(11)
+        a = np.zeros(14)
         lib().capsx_stats(self.ptr, _dptr(a))
         if reset:
             lib().capsx_stats_reset(self.ptr)
         names = ["mie", "K", "I", "assemble", "lu", "total"]
         return {"stage_ms": dict(zip(names, a[:6])), "blocks": int(a[6]), "blocks_factored": int(a[7]),
-                "lu_flops": a[8], "batches": int(a[9]), "I_integrals": a[10]}
+                "lu_flops": a[8], "batches": int(a[9]), "I_integrals": a[10],
+                "gemm_ms": a[11], "gemm_flops": a[12], "gemm_launches": int(a[13])}
+
+    def profile(self, on=True):
+        """Time every launch of the LU's trailing-update kernel with HIP events."""
+        lib().capsx_profile(self.ptr, 1 if on else 0)
 
     def reset_stats(self):
         lib().capsx_stats_reset(self.ptr)
@@ -202,11 +210,21 @@ class Caps:
 
     def free_energy(self, T, cutoff=1e-9, epsrel=1e-6, mode="batched", drude_xi0=True):
         """F (L+R)/(ħc) at temperature T with the reference driver's sums
-        (src/caps.c:307-396, 722-850); returns (F, number of determinants)."""
-        nb = ctypes.c_long()
+        (src/caps.c:307-396, 722-850); returns (F, determinants computed,
+        determinants entering the sums)."""
+        nb, nu = ctypes.c_long(), ctypes.c_long()
         F = lib().caps_free_energy(self.ptr, T, cutoff, epsrel, {"plain": 0, "batched": 1}[mode],
-                                   1 if drude_xi0 else 0, ctypes.byref(nb))
-        return F, nb.value
+                                   1 if drude_xi0 else 0, ctypes.byref(nb), ctypes.byref(nu))
+        return F, nb.value, nu.value
+
+    def logdetD0(self, m):
+        """ξ = 0 EE and MM blocks (perfect reflector / Drude)."""
+        ee, mm = ctypes.c_double(), ctypes.c_double()
+        lib().caps_logdetD0(self.ptr, m, 0.0, ctypes.byref(ee), ctypes.byref(mm), None)
+        return ee.value, mm.value
+
+    def ht_perf(self, eps=1e-13):
+        return lib().caps_ht_perf(self.ptr, eps)
 
 
 class _MObj:

@@ -32,7 +32,7 @@ void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int
 void capsd_launch_trace(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
 void capsd_launch_mie_points(int, const double*, const int*, double, const double*, double*, hipStream_t);
 void capsd_launch_fresnel_points(int, const double*, const double*, const double*, double*, hipStream_t);
-void capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t);
+void capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t, hipEvent_t*);
 int capsd_lu_init(void);
 int capsd_lu_max_dim(void);
 int capsd_lu_nb(void);
@@ -101,7 +101,10 @@ struct Ctx {
     double timing[6] = {0, 0, 0, 0, 0, 0};
     // accumulated since capsx_stats_reset: stage ms [0..5], blocks, blocks
     // factored (not skipped), algorithmic LU flops, batches, I pairs
-    double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // per-launch timing of the LU's trailing-update kernel (capsx_profile)
+    int prof = 0;
+    std::vector<hipEvent_t> gemm_ev;
     std::mutex mu;
 };
 
@@ -312,7 +315,14 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     capsd_launch_assemble(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, mode, s);
     if (mode == 0) capsd_launch_trace(c->d_desc, c->arena, c->d_out, n, lu_mode_shortcuts, s);
     HIPCHECK(hipEventRecord(c->ev[4], s));
-    if (mode == 0) capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, p.maxN, s);
+    const int npanels = (p.maxN + capsd_lu_nb() - 1) / capsd_lu_nb();
+    if (c->prof && (int)c->gemm_ev.size() < 2 * npanels) {
+        const size_t old = c->gemm_ev.size();
+        c->gemm_ev.resize(2 * npanels);
+        for (size_t i = old; i < c->gemm_ev.size(); i++) HIPCHECK(hipEventCreate(&c->gemm_ev[i]));
+    }
+    if (mode == 0)
+        capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, p.maxN, s, c->prof ? c->gemm_ev.data() : nullptr);
     HIPCHECK(hipEventRecord(c->ev[5], s));
     HIPCHECK(hipGetLastError());
     if (mode == 0) HIPCHECK(hipMemcpyAsync(c->h_out, c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, s));
@@ -328,6 +338,19 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     TERMINATE(*c->h_status != 0, "non-finite k-integral (flags %d) in a batch of %d blocks (xi_=%g, m=%d ...)",
               *c->h_status, n, c->h_desc[0].xi_, c->h_desc[0].m);
     for (int i = 0; i < 6; i++) c->acc[i] += c->timing[i];
+    if (c->prof && mode == 0) {
+        const int nb = capsd_lu_nb();
+        for (int pi = 0, k = 0; k < p.maxN; k += nb, pi++) {
+            if (p.maxN - k - nb <= 0) continue;
+            HIPCHECK(hipEventElapsedTime(&ms, c->gemm_ev[2 * pi], c->gemm_ev[2 * pi + 1]));
+            c->acc[11] += ms;
+            c->acc[13] += 1;
+            for (int b = 0; b < n; b++) {
+                const double r = c->h_desc[b].N - k - nb;
+                if (c->h_out[b].skip == 0 && r > 0) c->acc[12] += 2.0 * r * r * nb;
+            }
+        }
+    }
     c->acc[6] += n;
     c->acc[9] += 1;
     for (int b = 0; b < n; b++) {
@@ -743,7 +766,7 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     HIPCHECK(hipMemcpyAsync(c->d_side, &s, sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->d_out, &o, sizeof(BlockOut), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->arena, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, 1, dim, c->stream);
+    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, 1, dim, c->stream, nullptr);
     HIPCHECK(hipMemcpyAsync(&o, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     caps_free(tmp);
@@ -769,6 +792,73 @@ double caps_ht_drude(caps_t* caps)
     return 0.5 * (s1 + log1p(-(1 - Z * Z) * s2));
 }
 
+// ---- ξ = 0 blocks (perfect reflector / Drude EE and MM) ----------------------
+
+// Closed-form ξ = 0 round-trip kernels.  Reference: src/libcaps.c:860-941
+// (EE: exp((l1+l2+1) y + log (l1+l2)! - ½ Σ log(l±m)!); MM additionally
+// × sqrt(l1 l2/((l1+1)(l2+1)))).  args is a caps_M_t (caps, m, lmin).
+double caps_kernel_M0_EE(int i, int j, void* args_)
+{
+    caps_M_t* a = (caps_M_t*)args_;
+    const double y = a->caps->y;
+    const int l1 = i + a->lmin, l2 = j + a->lmin, m = a->m;
+    return exp((l1 + l2 + 1) * y + host_lfac(l1 + l2)
+               - 0.5 * (host_lfac(l1 + m) + host_lfac(l1 - m) + host_lfac(l2 + m) + host_lfac(l2 - m)));
+}
+
+double caps_kernel_M0_MM(int i, int j, void* args_)
+{
+    caps_M_t* a = (caps_M_t*)args_;
+    const double y = a->caps->y;
+    const int l1 = i + a->lmin, l2 = j + a->lmin, m = a->m;
+    return exp((l1 + l2 + 1) * y + host_lfac(l1 + l2)
+               - 0.5 * (host_lfac(l1 + m) + host_lfac(l1 - m) + host_lfac(l2 + m) + host_lfac(l2 - m)))
+           * sqrt(((double)l1 * (double)l2) / ((l1 + 1.) * (l2 + 1.)));
+}
+
+// log det of the ξ = 0 EE and MM blocks (ldim x ldim, dense LU on the GPU
+// through kernel_logdet).  The plasma-model MM block needs the ξ = 0 plasma
+// integrals (src/integration.c:897-1031), which this library does not
+// provide: a non-NULL MM_plasma aborts with a message.
+// Reference: src/libcaps.c:984-1017.
+void caps_logdetD0(caps_t* self, int m, double omegap, double* EE, double* MM, double* MM_plasma)
+{
+    (void)omegap;
+    TERMINATE(MM_plasma != NULL, "caps_logdetD0: the plasma-model MM block (xi = 0) is not implemented");
+    size_t lmin, lmax;
+    caps_estimate_lminmax(self, m, &lmin, &lmax);
+    caps_M_t args;
+    memset(&args, 0, sizeof(args));
+    args.caps = self;
+    args.m = m;
+    args.lmin = (int)lmin;
+    args.ldim = self->ldim;
+    if (EE) *EE = kernel_logdet(self->ldim, caps_kernel_M0_EE, &args, 2, self->detalg);
+    if (MM) *MM = kernel_logdet(self->ldim, caps_kernel_M0_MM, &args, 2, self->detalg);
+}
+
+// ξ = 0 free energy of perfect reflectors: Drude EE term plus the MM blocks,
+// m = 0 from Bimonte's closed form, m >= 1 until |v/Σ| < eps.
+// Reference: src/libcaps.c:1089-1116.
+double caps_ht_perf(caps_t* caps, double eps)
+{
+    const double LbyR = caps->LbyR;
+    const double drude = caps_ht_drude(caps);
+    double MM = 0;
+    const double Z = 1. / (1 + LbyR + sqrt(LbyR * (2 + LbyR)));
+    for (int l = 0;; l++) {
+        const double v = 0.5 * log1p(-pow(Z, 2 * l + 3));
+        MM += v;
+        if (fabs(v / MM) < eps) break;
+    }
+    for (int m = 1;; m++) {
+        double v;
+        caps_logdetD0(caps, m, 0, NULL, &v, NULL);
+        MM += v;
+        if (fabs(v / MM) < eps) return drude + MM;
+    }
+}
+
 // ---- free energy (reference driver semantics) -------------------------------
 
 // Σ'_m logdetD(ξ, m) with the reference's m-cutoff: m runs 0, 1, 2, ... and
@@ -777,7 +867,7 @@ double caps_ht_drude(caps_t* caps)
 // single worker (sequential semantics).  mode 0: one block per call; mode 1:
 // blocks in batches of the predicted cutoff (*mpred, updated), extended
 // until the cutoff is seen; both give the same set of m and the same sum.
-static double F_xi(caps_t* self, double xi_, double cutoff, int mode, int* mpred, long* nblocks)
+static double F_xi(caps_t* self, double xi_, double cutoff, int mode, int* mpred, long* nblocks, long* nused)
 {
     std::vector<double> terms;
     int mstop = -1;
@@ -816,6 +906,7 @@ static double F_xi(caps_t* self, double xi_, double cutoff, int mode, int* mpred
         *mpred = mstop;
     }
     terms.resize(mstop + 1);
+    *nused += mstop + 1;
     terms[0] /= 2;
     double s = 0, c = 0;
     for (double t : terms) {
@@ -827,18 +918,18 @@ static double F_xi(caps_t* self, double xi_, double cutoff, int mode, int* mpred
 }
 
 double caps_free_energy(caps_t* self, double T, double cutoff, double epsrel, int mode, int drude_xi0,
-                        long* nblocks)
+                        long* nblocks, long* nused)
 {
     TERMINATE(!(T > 0), "caps_free_energy needs T > 0 (got %g)", T);
     const double T_scaled = 2 * M_PI * CAPS_kB * (self->R + self->L) * T / (CAPS_hbar * CAPS_c);
-    long nb = 0;
+    long nb = 0, nu = 0;
     std::vector<double> v;
     TERMINATE(!drude_xi0, "the xi = 0 term is implemented for the Drude model only");
     v.push_back(caps_ht_drude(self));
     int mpred = 0;
     for (int n = 1;; n++) {
         const double xi = n * T_scaled;
-        v.push_back(F_xi(self, xi, cutoff, mode, &mpred, &nb));
+        v.push_back(F_xi(self, xi, cutoff, mode, &mpred, &nb, &nu));
         if (fabs(v[n] / v[0]) < epsrel) break;
         TERMINATE(n > 100000, "Matsubara sum did not converge");
     }
@@ -850,6 +941,7 @@ double caps_free_energy(caps_t* self, double T, double cutoff, double epsrel, in
         s = tt;
     }
     if (nblocks) *nblocks = nb;
+    if (nused) *nused = nu;
     return T_scaled / M_PI * s;
 }
 
@@ -891,7 +983,7 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
         HIPCHECK(hipMemcpyAsync(c->arena + c->h_desc[b].off_mat, A + (size_t)b * dim * dim,
                                 (size_t)dim * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, dim, c->stream);
+    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, dim, c->stream, nullptr);
     HIPCHECK(hipEventRecord(c->ev[1], c->stream));
     HIPCHECK(hipMemcpyAsync(o.data(), c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -961,16 +1053,26 @@ int capsx_M_copy(caps_M_t* self, double* out)
 // milliseconds (Mie, K, I, assembly, LU, total), out[6] blocks, out[7]
 // blocks factored, out[8] algorithmic LU flops Σ(2/3 N³ - 1/2 N² + 5/6 N),
 // out[9] batches, out[10] (l1 >= l2, polarization) I integrals.
-void capsx_stats(caps_t* self, double out[11])
+// With capsx_profile on: out[11] milliseconds in the LU's trailing-update
+// kernel (k_lu_gemm, HIP events around each launch), out[12] its algorithmic
+// flops Σ 2 (N-k-64)² 64 over factored blocks, out[13] its launches.
+void capsx_stats(caps_t* self, double out[14])
 {
     Ctx* c = ctx_of(self);
-    for (int i = 0; i < 11; i++) out[i] = c->acc[i];
+    for (int i = 0; i < 14; i++) out[i] = c->acc[i];
 }
 
 void capsx_stats_reset(caps_t* self)
 {
     Ctx* c = ctx_of(self);
-    for (int i = 0; i < 11; i++) c->acc[i] = 0;
+    for (int i = 0; i < 14; i++) c->acc[i] = 0;
+}
+
+// Extension: per-launch events around the LU's trailing-update kernel.
+void capsx_profile(caps_t* self, int on)
+{
+    Ctx* c = ctx_of(self);
+    c->prof = on;
 }
 
 // Extension: number of quadrature nodes of the K integrals of a block.

@@ -72,9 +72,9 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 
     __shared__ int perm[R * PT];
-    __shared__ double red_v[PT / 64];
-    __shared__ int red_i[PT / 64];
-    __shared__ double prow[LEAF], xrow[LEAF];
+    __shared__ double red_v2[2][PT / 64];
+    __shared__ int red_i2[2][PT / 64];
+    __shared__ double prow2[2][LEAF], xrow2[2][LEAF];
     __shared__ double U[NB * LEAF];       // U part of the current leaf, [row][col]
     __shared__ double Lt[NB * NB];        // L11 of the panel so far, [row][col]
     __shared__ int nmoved;
@@ -112,20 +112,36 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
                 }
             }
             __syncthreads();
-            // rows < c0 take the solved U; rows >= c0 get the left-looking update
+            // rows < c0 take the solved U; rows >= c0 get the left-looking
+            // update, one earlier leaf (8 columns of L) per step with all of
+            // the step's loads issued before their first use
 #pragma unroll
             for (int i = 0; i < R; i++) {
                 const int r = tid + i * PT;
                 if (r < c0) {
 #pragma unroll
                     for (int j = 0; j < LEAF; j++) a[i][j] = U[r * LEAF + j];
-                } else if (r < np) {
-                    const int64_t pr = k + perm[r];
-                    for (int cc = 0; cc < c0; cc++) {
-                        const double l = A[pr + (k + cc) * lda];
+                }
+            }
+            for (int cc0 = 0; cc0 < c0; cc0 += LEAF) {
+                double Lr[R][LEAF];
 #pragma unroll
-                        for (int j = 0; j < LEAF; j++) a[i][j] -= l * U[cc * LEAF + j];
-                    }
+                for (int i = 0; i < R; i++) {
+                    const int r = tid + i * PT;
+                    const bool row_ok = r >= c0 && r < np;
+                    const int64_t pr = k + (row_ok ? perm[r] : 0);
+#pragma unroll
+                    for (int jj = 0; jj < LEAF; jj++) Lr[i][jj] = row_ok ? A[pr + (k + cc0 + jj) * lda] : 0.0;
+                }
+#pragma unroll
+                for (int jj = 0; jj < LEAF; jj++) {
+                    double u[LEAF];
+#pragma unroll
+                    for (int j = 0; j < LEAF; j++) u[j] = U[(cc0 + jj) * LEAF + j];
+#pragma unroll
+                    for (int i = 0; i < R; i++)
+#pragma unroll
+                        for (int j = 0; j < LEAF; j++) a[i][j] -= Lr[i][jj] * u[j];
                 }
             }
         }
@@ -134,6 +150,13 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
         for (int j = 0; j < LEAF; j++) {
             if (j >= w) break;
             const int c = c0 + j;
+            // per-column scratch alternates between two buffers, so one
+            // barrier after the reduction and one after the pivot row is
+            // published suffice
+            double* red_v = red_v2[c & 1];
+            int* red_i = red_i2[c & 1];
+            double* prow = prow2[c & 1];
+            double* xrow = xrow2[c & 1];
             double bv = -1.0;
             int bi = 0x7fffffff;
 #pragma unroll
@@ -208,7 +231,6 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
                         if (jj > j) a[i][jj] -= l * prow[jj];
                 }
             }
-            __syncthreads();
         }
         // write the leaf back through the row map
 #pragma unroll
@@ -409,19 +431,24 @@ int capsd_lu_init(void)
                                     MAXMOVE * 64 * sizeof(double));
 }
 
+// gemm_ev: NULL, or 2 events per panel recorded around each k_lu_gemm
+// launch (per-kernel timing of the trailing update).
 void capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockOut* out, int nblocks, int maxN,
-                     hipStream_t s)
+                     hipStream_t s, hipEvent_t* gemm_ev)
 {
     const LuSide* sd = (const LuSide*)side;
-    for (int k = 0; k < maxN; k += NB) {
+    int p = 0;
+    for (int k = 0; k < maxN; k += NB, p++) {
         const int R = (maxN - k + PT - 1) / PT;
         launch_panel(R, desc, arena, sd, out, nblocks, k, s);
         const int rest = maxN - k - NB;
         if (rest > 0) {
             hipLaunchKernelGGL(k_lu_swap_trsm, dim3((rest + 63) / 64, nblocks), dim3(64),
                                MAXMOVE * 64 * sizeof(double), s, desc, arena, sd, out, k);
+            if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p], s);
             hipLaunchKernelGGL(k_lu_gemm, dim3((rest + 63) / 64, (rest + 63) / 64, nblocks), dim3(256), 0, s, desc,
                                arena, sd, out, k);
+            if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
         }
     }
     hipLaunchKernelGGL(k_lu_finalize, dim3((nblocks + 63) / 64), dim3(64), 0, s, desc, arena, sd, out, nblocks);

@@ -158,6 +158,17 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void *), void *args, in
 /* ξ = 0 Drude term.  Reference: src/libcaps.c:1043-1065. */
 double caps_ht_drude(caps_t *caps);
 
+/* ξ = 0 closed-form kernels (args: caps_M_t with caps, m, lmin).
+ * Reference: src/libcaps.c:860-873, 925-941. */
+double caps_kernel_M0_EE(int i, int j, void *args);
+double caps_kernel_M0_MM(int i, int j, void *args);
+/* ξ = 0 blocks EE and MM (ldim x ldim); MM_plasma must be NULL (the plasma
+ * ξ = 0 integrals are not part of this library: abort with a message).
+ * Reference: src/libcaps.c:984-1017. */
+void caps_logdetD0(caps_t *self, int m, double omegap, double *EE, double *MM, double *MM_plasma);
+/* ξ = 0 perfect-reflector free energy.  Reference: src/libcaps.c:1089-1116. */
+double caps_ht_perf(caps_t *caps, double eps);
+
 /* ---- extensions ------------------------------------------------------- */
 
 /* Extension: n blocks (xi_[i], m[i]) of one caps object in one batched GPU
@@ -180,11 +191,13 @@ void caps_last_timing(caps_t *self, double ms[6]);
  * at a time, in F_xi order), mode 1 = batched (all m of a Matsubara term in
  * one launch, verified against the same cutoff rule).  drude_xi0: nonzero
  * adds the analytic ξ = 0 Drude term (caps_ht_drude), as the reference
- * driver does for finite ωp and γ > 0.  nblocks (may be NULL) receives the
- * number of determinants computed.  Returns F·(L+R)/(ħc)·... exactly as the
- * reference prints: F/(k_B T) summed terms times 1 (see caps_amd docs). */
+ * driver does for finite ωp and γ > 0 (required: other ξ = 0 terms abort).
+ * nblocks (may be NULL) receives the number of determinants computed, nused
+ * (may be NULL) the number that enter the sums.  Returns E·(L+R)/(ħc) =
+ * T_scaled/π Σ'_n v_n, T_scaled = 2π k_B (L+R) T/(ħc), the value the
+ * reference driver prints. */
 double caps_free_energy(caps_t *self, double T, double cutoff, double epsrel, int mode,
-                        int drude_xi0, long *nblocks);
+                        int drude_xi0, long *nblocks, long *nused);
 
 #ifdef __cplusplus
 }
