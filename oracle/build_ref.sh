@@ -35,3 +35,15 @@ for f in $HSRC; do g++ $CXXFLAGS -c "$REF/src/libhodlr/src/$f" -o "$OBJ/h_${f%.c
 for p in "${pids[@]}"; do wait "$p"; done
 g++ -shared -o "$OUT/libcaps_ref.so" "$OBJ"/*.o -lm
 echo "built $OUT/libcaps_ref.so"
+
+# Drop-in check: the reference's block CLI (src/caps_logdetD.c, with the
+# material/utility sources it needs for its own option handling) compiled
+# against the reference's headers and linked, unchanged, against
+# libcaps_amd.so instead of the reference library.
+AMD=$HERE/../caps_amd/libcaps_amd.so
+if [ -f "$AMD" ]; then
+    gcc -O2 -std=c99 -D_GNU_SOURCE -w -I"$REF/src/include" \
+        "$REF/src/caps_logdetD.c" "$REF/src/material.c" "$REF/src/utils.c" "$REF/src/misc.c" \
+        -L"$(dirname "$AMD")" -lcaps_amd -Wl,-rpath,'$ORIGIN/../../caps_amd' -lm -o "$OUT/caps_logdetD_amd"
+    echo "built $OUT/caps_logdetD_amd (reference CLI linked against libcaps_amd.so)"
+fi

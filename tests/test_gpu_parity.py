@@ -210,3 +210,23 @@ def test_logdetD_vs_oracle_lu_mode(ca):
         v = c.logdetD(xi, m)
         ref = po.logdetD(oc, xi, m)
         assert close_block(v, ref), (xi, m, v, ref)
+
+
+CLI = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "caps_logdetD_amd")
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="reference CLI relinked against libcaps_amd.so not built here")
+@pytest.mark.parametrize("args,expect", [
+    (["-R", "100e-6", "-L", "1e-6", "--xi", "1", "-m", "1", "-l", "501"], -6.463973151333012),
+    (["-R", "100e-6", "-L", "1e-6", "--xi", "2", "-m", "1", "-l", "501", "-d", "LU"], -6.349155228127988),
+])
+def test_reference_cli_relinked(args, expect):
+    """The reference's block CLI (src/caps_logdetD.c), linked unchanged
+    against libcaps_amd.so, reproduces the manual's single-block answers
+    (docs/manual/index.rst:866,888)."""
+    import subprocess
+    out = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    last = [ln for ln in out.stdout.splitlines() if ln and not ln.startswith("#")][-1]
+    v = float(last.split(",")[4])
+    assert abs(v - expect) <= 1e-9 * abs(expect), (v, expect, out.stdout)
