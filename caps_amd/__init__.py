@@ -211,9 +211,11 @@ class Caps:
     def free_energy(self, T, cutoff=1e-9, epsrel=1e-6, mode="batched", drude_xi0=True):
         """F (L+R)/(ħc) at temperature T with the reference driver's sums
         (src/caps.c:307-396, 722-850); returns (F, determinants computed,
-        determinants entering the sums)."""
+        determinants entering the sums).  mode: "plain" (one block at a
+        time), "batched" (planned from the first two Matsubara terms) or
+        "surrogate" (planned by an attached caps_amd.surrogate.MLP)."""
         nb, nu = ctypes.c_long(), ctypes.c_long()
-        F = lib().caps_free_energy(self.ptr, T, cutoff, epsrel, {"plain": 0, "batched": 1}[mode],
+        F = lib().caps_free_energy(self.ptr, T, cutoff, epsrel, {"plain": 0, "batched": 1, "surrogate": 2}[mode],
                                    1 if drude_xi0 else 0, ctypes.byref(nb), ctypes.byref(nu))
         return F, nb.value, nu.value
 

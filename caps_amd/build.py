@@ -13,7 +13,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-result", "-Wno-unused-variable", "-Wno-unused-value"]
-SOURCES = ["roundtrip.hip", "lu.hip", "libcaps.cpp"]
+SOURCES = ["roundtrip.hip", "lu.hip", "mlp.hip", "libcaps.cpp"]
 
 
 def build(verbose=True):

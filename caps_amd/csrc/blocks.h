@@ -31,7 +31,7 @@ struct BlockDesc {
     int lda;
     int pad_;
     // byte offsets into the arena
-    int64_t off_mie;    // 2 * ldim doubles: log|a_l|, log|b_l|
+    int64_t off_mie;    // 4 * ldim doubles: log|a_l|, log|b_l|, ea(l), eb(l)
     int64_t off_kpart;  // n_waves * nK * KPart
     int64_t off_k;      // 2 pols * nK * (double mant, int exp, double lnK)
     int64_t off_i;      // 2 pols * nl*nl doubles, then 2 pols * nl*nl int8 signs

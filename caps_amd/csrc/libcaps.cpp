@@ -24,8 +24,10 @@ using capsd::BlockDesc;
 using capsd::BlockOut;
 using capsd::KPart;
 
+struct capsx_mlp;
 extern "C" {
-void capsd_launch_mie(const BlockDesc*, char*, int, int, hipStream_t);
+void capsx_mlp_predict(capsx_mlp* h, const float* X, int n, float* Y);
+void capsd_launch_mie(const BlockDesc*, char*, const double*, int, int, hipStream_t);
 void capsd_launch_K(const BlockDesc*, char*, const double*, int, int, int, int*, hipStream_t);
 void capsd_launch_I(const BlockDesc*, char*, const double*, int, int64_t, int*, hipStream_t);
 void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int, hipStream_t);
@@ -84,9 +86,15 @@ void device_init()
     });
 }
 
+constexpr int MAX_LU_STREAMS = 4;
+
 struct Ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[7];
+    // the LU of a batch is split into groups of blocks on these streams so
+    // that one group's panel factorizations overlap another's updates
+    hipStream_t lu_stream[MAX_LU_STREAMS];
+    hipEvent_t lu_done[MAX_LU_STREAMS];
     char* arena = nullptr;
     size_t arena_size = 0;
     BlockDesc* d_desc = nullptr;
@@ -105,6 +113,11 @@ struct Ctx {
     // per-launch timing of the LU's trailing-update kernel (capsx_profile)
     int prof = 0;
     std::vector<hipEvent_t> gemm_ev;
+    // planner surrogate (capsx_set_surrogate) and the material features it
+    // is evaluated with
+    struct capsx_mlp* sur = nullptr;
+    float sur_feat[2] = {0, 0};
+    long plan_batches = 0;
     std::mutex mu;
 };
 
@@ -115,6 +128,10 @@ Ctx* ctx_of(caps_t* self)
         Ctx* c = new Ctx();
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (auto& e : c->ev) HIPCHECK(hipEventCreate(&e));
+        for (int g = 0; g < MAX_LU_STREAMS; g++) {
+            HIPCHECK(hipStreamCreateWithFlags(&c->lu_stream[g], hipStreamNonBlocking));
+            HIPCHECK(hipEventCreateWithFlags(&c->lu_done[g], hipEventDisableTiming));
+        }
         HIPCHECK(hipMalloc(&c->d_status, sizeof(int)));
         HIPCHECK(hipHostMalloc((void**)&c->h_status, sizeof(int), hipHostMallocDefault));
         self->priv = c;
@@ -136,6 +153,10 @@ void ctx_free(Ctx* c)
     if (c->h_out) hipHostFree(c->h_out);
     if (c->h_status) hipHostFree(c->h_status);
     for (auto& e : c->ev) hipEventDestroy(e);
+    for (int g = 0; g < MAX_LU_STREAMS; g++) {
+        hipStreamDestroy(c->lu_stream[g]);
+        hipEventDestroy(c->lu_done[g]);
+    }
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -252,7 +273,7 @@ BatchPlan plan_batch(caps_t* self, Ctx* c, const double* xi_, const int* m, int 
         d.nK = d.nu_max - d.mu + 1;
         grid_for(d);
         d.off_mie = off;
-        off = align256(off + 2 * (int64_t)ldim * 8);
+        off = align256(off + 4 * (int64_t)ldim * 8);
         d.off_kpart = off;
         off = align256(off + (int64_t)d.n_waves * d.nK * sizeof(KPart));
         d.off_k = off;
@@ -306,7 +327,7 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     HIPCHECK(hipMemcpyAsync(c->d_side, c->h_side, n * sizeof(LuSide), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemsetAsync(c->d_status, 0, sizeof(int), s));
     HIPCHECK(hipEventRecord(c->ev[0], s));
-    capsd_launch_mie(c->d_desc, c->arena, n, p.max_ldim, s);
+    capsd_launch_mie(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, s);
     HIPCHECK(hipEventRecord(c->ev[1], s));
     capsd_launch_K(c->d_desc, c->arena, g_lfac_dev, n, p.max_waves, p.max_nK, c->d_status, s);
     HIPCHECK(hipEventRecord(c->ev[2], s));
@@ -321,8 +342,24 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
         c->gemm_ev.resize(2 * npanels);
         for (size_t i = old; i < c->gemm_ev.size(); i++) HIPCHECK(hipEventCreate(&c->gemm_ev[i]));
     }
-    if (mode == 0)
-        capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, p.maxN, s, c->prof ? c->gemm_ev.data() : nullptr);
+    if (mode == 0) {
+        // profiled runs keep one group so each update launch is timed alone
+        int groups = c->prof ? 1 : (int)env_double("CAPS_AMD_LU_STREAMS", 3);
+        groups = std::max(1, std::min(groups, std::min(MAX_LU_STREAMS, n / 4)));
+        if (groups == 1) {
+            capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, p.maxN, s,
+                            c->prof ? c->gemm_ev.data() : nullptr);
+        } else {
+            for (int g = 0; g < groups; g++) {
+                const int b0 = (int)((int64_t)n * g / groups), b1 = (int)((int64_t)n * (g + 1) / groups);
+                HIPCHECK(hipStreamWaitEvent(c->lu_stream[g], c->ev[4], 0));
+                capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, b1 - b0, p.maxN,
+                                c->lu_stream[g], nullptr);
+                HIPCHECK(hipEventRecord(c->lu_done[g], c->lu_stream[g]));
+                HIPCHECK(hipStreamWaitEvent(s, c->lu_done[g], 0));
+            }
+        }
+    }
     HIPCHECK(hipEventRecord(c->ev[5], s));
     HIPCHECK(hipGetLastError());
     if (mode == 0) HIPCHECK(hipMemcpyAsync(c->h_out, c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, s));
@@ -380,7 +417,7 @@ bool is_lu_mode(detalg_t a) { return a == DETALG_LU || a == DETALG_QR; }
 // Largest number of blocks per launch sequence that fits the HBM budget.
 int chunk_blocks(caps_t* self, const int* m, int n)
 {
-    const double gb = env_double("CAPS_AMD_ARENA_GB", 64.0);
+    const double gb = env_double("CAPS_AMD_ARENA_GB", 160.0);
     const size_t budget = (size_t)(gb * 1e9);
     size_t acc = 0;
     int k = 0;
@@ -861,55 +898,13 @@ double caps_ht_perf(caps_t* caps, double eps)
 
 // ---- free energy (reference driver semantics) -------------------------------
 
-// Σ'_m logdetD(ξ, m) with the reference's m-cutoff: m runs 0, 1, 2, ... and
-// stops at the first m whose value v satisfies v == 0 or v/v_0 < cutoff
-// (that m included); m = 0 is halved.  Reference: src/caps.c:307-396 with a
-// single worker (sequential semantics).  mode 0: one block per call; mode 1:
-// blocks in batches of the predicted cutoff (*mpred, updated), extended
-// until the cutoff is seen; both give the same set of m and the same sum.
-static double F_xi(caps_t* self, double xi_, double cutoff, int mode, int* mpred, long* nblocks, long* nused)
+namespace {
+
+// Kahan sum in index order (src/misc.c:26-40).
+double kahan(const std::vector<double>& x)
 {
-    std::vector<double> terms;
-    int mstop = -1;
-    if (mode == 0) {
-        for (int m = 0;; m++) {
-            TERMINATE(m >= 100000, "sum over m did not converge at xi_=%g", xi_);
-            const double v = caps_logdetD(self, xi_, m);
-            terms.push_back(v);
-            (*nblocks)++;
-            if (v == 0 || v / terms[0] < cutoff) {
-                mstop = m;
-                break;
-            }
-        }
-    } else {
-        int next = 0;
-        int chunk = std::max(*mpred + 1, 16);
-        while (mstop < 0) {
-            TERMINATE(next >= 100000, "sum over m did not converge at xi_=%g", xi_);
-            std::vector<double> xs(chunk, xi_), vals(chunk);
-            std::vector<int> ms(chunk);
-            for (int i = 0; i < chunk; i++) ms[i] = next + i;
-            TERMINATE(caps_logdetD_batch(self, xs.data(), ms.data(), chunk, vals.data()) != 0, "batch failed");
-            *nblocks += chunk;
-            for (int i = 0; i < chunk; i++) {
-                terms.push_back(vals[i]);
-                const double v = vals[i];
-                if (v == 0 || v / terms[0] < cutoff) {
-                    mstop = next + i;
-                    break;
-                }
-            }
-            next += chunk;
-            chunk = std::max(16, chunk / 2);
-        }
-        *mpred = mstop;
-    }
-    terms.resize(mstop + 1);
-    *nused += mstop + 1;
-    terms[0] /= 2;
     double s = 0, c = 0;
-    for (double t : terms) {
+    for (double t : x) {
         const double y = t - c, tt = s + y;
         c = (tt - s) - y;
         s = tt;
@@ -917,31 +912,200 @@ static double F_xi(caps_t* self, double xi_, double cutoff, int mode, int* mpred
     return s;
 }
 
+// The m-sum of one Matsubara term from the values computed so far
+// (v[m] for m = 0 .. v.size()-1): m runs 0, 1, 2, ... and stops at the first
+// m with v == 0 or v/v_0 < cutoff, that m included; m = 0 is halved.
+// Reference: src/caps.c:307-396 with a single worker (sequential
+// semantics).  Returns false if the cutoff is not reached within v.
+bool msum(const std::vector<double>& v, double cutoff, double* out, int* mstop)
+{
+    for (size_t m = 0; m < v.size(); m++) {
+        if (v[m] == 0 || v[m] / v[0] < cutoff) {
+            std::vector<double> t(v.begin(), v.begin() + m + 1);
+            t[0] /= 2;
+            *out = kahan(t);
+            *mstop = (int)m;
+            return true;
+        }
+    }
+    return false;
+}
+
+struct Term {
+    double xi;
+    std::vector<double> v;   // logdetD for m = 0 .. v.size()-1
+    bool done = false;
+    int mstop = -1;
+    double value = 0;
+};
+
+// Evaluate the listed (term, m-range) requests in one batch.
+void run_requests(caps_t* self, std::vector<Term>& terms, const std::vector<std::pair<int, int>>& req,
+                  long* computed)
+{
+    std::vector<double> xs;
+    std::vector<int> ms, owner;
+    for (auto& r : req) {
+        Term& t = terms[r.first];
+        for (int m = (int)t.v.size(); m < r.second; m++) {
+            xs.push_back(t.xi);
+            ms.push_back(m);
+            owner.push_back(r.first);
+        }
+    }
+    if (xs.empty()) return;
+    std::vector<double> out(xs.size());
+    TERMINATE(caps_logdetD_batch(self, xs.data(), ms.data(), (int)xs.size(), out.data()) != 0, "batch failed");
+    *computed += (long)xs.size();
+    for (size_t i = 0; i < xs.size(); i++) terms[owner[i]].v.push_back(out[i]);
+}
+
+}  // namespace
+
 double caps_free_energy(caps_t* self, double T, double cutoff, double epsrel, int mode, int drude_xi0,
                         long* nblocks, long* nused)
 {
     TERMINATE(!(T > 0), "caps_free_energy needs T > 0 (got %g)", T);
-    const double T_scaled = 2 * M_PI * CAPS_kB * (self->R + self->L) * T / (CAPS_hbar * CAPS_c);
-    long nb = 0, nu = 0;
-    std::vector<double> v;
     TERMINATE(!drude_xi0, "the xi = 0 term is implemented for the Drude model only");
+    const double T_scaled = 2 * M_PI * CAPS_kB * (self->R + self->L) * T / (CAPS_hbar * CAPS_c);
+    long computed = 0, used = 0;
+    std::vector<double> v;      // Matsubara terms, v[0] = ξ = 0 term
     v.push_back(caps_ht_drude(self));
-    int mpred = 0;
-    for (int n = 1;; n++) {
-        const double xi = n * T_scaled;
-        v.push_back(F_xi(self, xi, cutoff, mode, &mpred, &nb, &nu));
-        if (fabs(v[n] / v[0]) < epsrel) break;
-        TERMINATE(n > 100000, "Matsubara sum did not converge");
+    std::vector<Term> terms(1);  // terms[n] for n >= 1
+    auto term = [&](int n) -> Term& {
+        while ((int)terms.size() <= n) {
+            Term t;
+            t.xi = terms.size() * T_scaled;
+            terms.push_back(t);
+        }
+        return terms[n];
+    };
+
+    if (mode == 0) {
+        // plain: one block at a time in F_xi order (src/caps.c:307-396, 836-846)
+        for (int n = 1;; n++) {
+            Term& t = term(n);
+            for (int m = 0;; m++) {
+                TERMINATE(m >= 100000, "sum over m did not converge at xi_=%g", t.xi);
+                t.v.push_back(caps_logdetD(self, t.xi, m));
+                computed++;
+                if (msum(t.v, cutoff, &t.value, &t.mstop)) break;
+            }
+            used += t.mstop + 1;
+            v.push_back(t.value);
+            if (fabs(v[n] / v[0]) < epsrel) break;
+            TERMINATE(n > 100000, "Matsubara sum did not converge");
+        }
+    } else {
+        // batched (mode 1): terms 1 and 2 in growing chunks; from their
+        // m-cutoffs and values, one batch with all further terms up to the
+        // predicted end of the Matsubara sum.  Surrogate-planned (mode 2):
+        // the MLP predicts log|logdetD(ξ_n, m)| for all candidate blocks, the
+        // predicted m-cutoffs and end of the Matsubara sum give one batch
+        // for the whole energy.  In both, every term is then checked against
+        // the same cutoff rules in order and extended if the plan fell
+        // short, so the summed blocks are those of the plain form.
+        Ctx* cx = ctx_of(self);
+        auto finish_term = [&](int n, int first, int step) {
+            Term& t = term(n);
+            int upto = first;
+            while (true) {
+                run_requests(self, terms, {{n, upto}}, &computed);
+                cx->plan_batches++;
+                if (msum(t.v, cutoff, &t.value, &t.mstop)) break;
+                TERMINATE(upto >= 100000, "sum over m did not converge at xi_=%g", t.xi);
+                upto += step;
+            }
+            t.done = true;
+        };
+        int n = 0;
+        bool stop = false;
+        auto accept = [&](int j) {
+            Term& t = term(j);
+            used += t.mstop + 1;
+            v.push_back(t.value);
+            n = j;
+            stop = fabs(v[j] / v[0]) < epsrel;
+        };
+        if (mode == 2) {
+            TERMINATE(!cx->sur, "caps_free_energy mode 2 needs a surrogate (capsx_set_surrogate)");
+            const int NCAP = 48;
+            const int MCAP = std::min(self->ldim - 1, 4000);
+            const int per = MCAP + 1;
+            std::vector<float> X((size_t)NCAP * per * 5), P((size_t)NCAP * per);
+            for (int j = 1; j <= NCAP; j++)
+                for (int m = 0; m <= MCAP; m++) {
+                    float* x = &X[((size_t)(j - 1) * per + m) * 5];
+                    x[0] = (float)log(1 / self->LbyR);
+                    x[1] = (float)log(j * T_scaled);
+                    x[2] = (float)m / self->ldim;
+                    x[3] = cx->sur_feat[0];
+                    x[4] = cx->sur_feat[1];
+                }
+            capsx_mlp_predict(cx->sur, X.data(), NCAP * per, P.data());
+            std::vector<std::pair<int, int>> req;
+            const double lcut = log(cutoff);
+            for (int j = 1; j <= NCAP; j++) {
+                const float* pr = &P[(size_t)(j - 1) * per];
+                int ms = MCAP;
+                for (int m = 1; m <= MCAP; m++)
+                    if (pr[m] - pr[0] < lcut) {
+                        ms = m;
+                        break;
+                    }
+                double vj = exp((double)pr[0]) / 2;
+                for (int m = 1; m <= ms; m++) vj += exp((double)pr[m]);
+                const int margin = std::max(4, ms / 25);
+                term(j);
+                req.push_back({j, std::min(self->ldim * 8, ms + margin + 1)});
+                if (vj / fabs(v[0]) < epsrel) break;
+            }
+            run_requests(self, terms, req, &computed);
+            cx->plan_batches++;
+            for (size_t r = 0; r < req.size() && !stop; r++) {
+                const int j = req[r].first;
+                Term& t = term(j);
+                if (!msum(t.v, cutoff, &t.value, &t.mstop)) finish_term(j, (int)t.v.size() + 16, 16);
+                accept(j);
+            }
+        }
+        if (!stop && n < 1) {
+            finish_term(1, 64, 32);
+            accept(1);
+        }
+        if (!stop && n < 2) {
+            finish_term(2, term(1).mstop + 12, 16);
+            accept(2);
+        }
+        while (!stop) {
+            // prediction: geometric decay of v_n, m-cutoff growing by at most
+            // the last step, 8 blocks of margin, one extra term
+            const double r = std::min(0.95, std::max(1e-6, fabs(v[n] / v[n - 1])));
+            const double need = epsrel * fabs(v[0]) / fabs(v[n]);
+            int more = need < 1 ? (int)ceil(log(need) / log(r)) + 1 : 1;
+            more = std::max(1, std::min(more, 64));
+            const int dm = std::max(0, term(n).mstop - term(n - 1).mstop);
+            const int mp = term(n).mstop + dm + 8;
+            std::vector<std::pair<int, int>> req;
+            for (int j = n + 1; j <= n + more; j++) {
+                term(j);
+                req.push_back({j, mp + 1});
+            }
+            run_requests(self, terms, req, &computed);
+            cx->plan_batches++;
+            const int n0 = n;
+            for (int j = n0 + 1; j <= n0 + more && !stop; j++) {
+                Term& t = term(j);
+                if (!msum(t.v, cutoff, &t.value, &t.mstop)) finish_term(j, (int)t.v.size() + 16, 16);
+                accept(j);
+            }
+            TERMINATE(n > 100000, "Matsubara sum did not converge");
+        }
     }
     v[0] /= 2;
-    double s = 0, c = 0;
-    for (double t : v) {
-        const double y = t - c, tt = s + y;
-        c = (tt - s) - y;
-        s = tt;
-    }
-    if (nblocks) *nblocks = nb;
-    if (nused) *nused = nu;
+    const double s = kahan(v);
+    if (nblocks) *nblocks = computed;
+    if (nused) *nused = used;
     return T_scaled / M_PI * s;
 }
 
@@ -1073,6 +1237,27 @@ void capsx_profile(caps_t* self, int on)
 {
     Ctx* c = ctx_of(self);
     c->prof = on;
+}
+
+// Extension: attach a planner surrogate (capsx_mlp, borrowed) and the
+// material features it is evaluated with (log ωp/eV, log γ/eV, with the
+// sentinels of caps_amd/surrogate.py for perfect reflectors and plasma).
+void capsx_set_surrogate(caps_t* self, struct capsx_mlp* mlp, float f_omegap, float f_gamma)
+{
+    Ctx* c = ctx_of(self);
+    c->sur = mlp;
+    c->sur_feat[0] = f_omegap;
+    c->sur_feat[1] = f_gamma;
+}
+
+// Extension: launch sequences (batches) the free-energy driver used since
+// the last reset; resets when reset != 0.
+long capsx_plan_batches(caps_t* self, int reset)
+{
+    Ctx* c = ctx_of(self);
+    const long r = c->plan_batches;
+    if (reset) c->plan_batches = 0;
+    return r;
 }
 
 // Extension: number of quadrature nodes of the K integrals of a block.

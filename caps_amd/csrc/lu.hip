@@ -53,10 +53,11 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx)
     }
 }
 
-template <int R>
-__global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1))) k_lu_panel(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                                 const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
-                                                 int k)
+// Panel body: R rows per lane, leaves of LF columns, NT threads; the
+// logical->physical row map lives in dynamic LDS (R*NT ints).
+template <int R, int LF, int NT>
+__device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                              const LuSide* __restrict__ side, const BlockOut* __restrict__ out, int k)
 {
     const int b = blockIdx.x;
     const BlockDesc& d = desc[b];
@@ -71,44 +72,44 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     int* moved = at<int>(arena, side[b].off_moved);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 
-    __shared__ int perm[R * PT];
-    __shared__ double red_v2[2][PT / 64];
-    __shared__ int red_i2[2][PT / 64];
-    __shared__ double prow2[2][LEAF], xrow2[2][LEAF];
-    __shared__ double U[NB * LEAF];       // U part of the current leaf, [row][col]
+    extern __shared__ int perm[];   // R * NT
+    __shared__ double red_v2[2][NT / 64];
+    __shared__ int red_i2[2][NT / 64];
+    __shared__ double prow2[2][LF], xrow2[2][LF];
+    __shared__ double U[NB * LF];       // U part of the current leaf, [row][col]
     __shared__ double Lt[NB * NB];        // L11 of the panel so far, [row][col]
     __shared__ int nmoved;
 
-    for (int r = tid; r < np; r += PT) perm[r] = r;
+    for (int r = tid; r < np; r += NT) perm[r] = r;
     if (tid == 0) nmoved = 0;
     __syncthreads();
 
-    double a[R][LEAF];
-    for (int c0 = 0; c0 < nbk; c0 += LEAF) {
-        const int w = min(LEAF, nbk - c0);
+    double a[R][LF];
+    for (int c0 = 0; c0 < nbk; c0 += LF) {
+        const int w = min(LF, nbk - c0);
         // load the leaf (logical rows, through the row map)
 #pragma unroll
         for (int i = 0; i < R; i++) {
-            const int r = tid + i * PT;
+            const int r = tid + i * NT;
 #pragma unroll
-            for (int j = 0; j < LEAF; j++)
+            for (int j = 0; j < LF; j++)
                 a[i][j] = (r < np && j < w) ? A[(k + perm[r]) + (k + c0 + j) * lda] : 0.0;
         }
         if (c0 > 0) {
             // U part: solve L11[0:c0,0:c0] U = leaf[0:c0] (unit lower), one lane per column
-            for (int idx = tid; idx < c0 * c0; idx += PT) {
+            for (int idx = tid; idx < c0 * c0; idx += NT) {
                 const int rr = idx / c0, cc = idx % c0;
                 Lt[rr * NB + cc] = A[(k + perm[rr]) + (k + cc) * lda];
             }
-            for (int idx = tid; idx < c0 * LEAF; idx += PT) {
-                const int rr = idx / LEAF, cc = idx % LEAF;
-                U[rr * LEAF + cc] = cc < w ? A[(k + perm[rr]) + (k + c0 + cc) * lda] : 0.0;
+            for (int idx = tid; idx < c0 * LF; idx += NT) {
+                const int rr = idx / LF, cc = idx % LF;
+                U[rr * LF + cc] = cc < w ? A[(k + perm[rr]) + (k + c0 + cc) * lda] : 0.0;
             }
             __syncthreads();
-            if (tid < LEAF) {
+            if (tid < LF) {
                 for (int i = 0; i < c0; i++) {
-                    const double xi = U[i * LEAF + tid];
-                    for (int jj = i + 1; jj < c0; jj++) U[jj * LEAF + tid] -= Lt[jj * NB + i] * xi;
+                    const double xi = U[i * LF + tid];
+                    for (int jj = i + 1; jj < c0; jj++) U[jj * LF + tid] -= Lt[jj * NB + i] * xi;
                 }
             }
             __syncthreads();
@@ -117,37 +118,37 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
             // the step's loads issued before their first use
 #pragma unroll
             for (int i = 0; i < R; i++) {
-                const int r = tid + i * PT;
+                const int r = tid + i * NT;
                 if (r < c0) {
 #pragma unroll
-                    for (int j = 0; j < LEAF; j++) a[i][j] = U[r * LEAF + j];
+                    for (int j = 0; j < LF; j++) a[i][j] = U[r * LF + j];
                 }
             }
-            for (int cc0 = 0; cc0 < c0; cc0 += LEAF) {
-                double Lr[R][LEAF];
+            for (int cc0 = 0; cc0 < c0; cc0 += LF) {
+                double Lr[R][LF];
 #pragma unroll
                 for (int i = 0; i < R; i++) {
-                    const int r = tid + i * PT;
+                    const int r = tid + i * NT;
                     const bool row_ok = r >= c0 && r < np;
                     const int64_t pr = k + (row_ok ? perm[r] : 0);
 #pragma unroll
-                    for (int jj = 0; jj < LEAF; jj++) Lr[i][jj] = row_ok ? A[pr + (k + cc0 + jj) * lda] : 0.0;
+                    for (int jj = 0; jj < LF; jj++) Lr[i][jj] = row_ok ? A[pr + (k + cc0 + jj) * lda] : 0.0;
                 }
 #pragma unroll
-                for (int jj = 0; jj < LEAF; jj++) {
-                    double u[LEAF];
+                for (int jj = 0; jj < LF; jj++) {
+                    double u[LF];
 #pragma unroll
-                    for (int j = 0; j < LEAF; j++) u[j] = U[(cc0 + jj) * LEAF + j];
+                    for (int j = 0; j < LF; j++) u[j] = U[(cc0 + jj) * LF + j];
 #pragma unroll
                     for (int i = 0; i < R; i++)
 #pragma unroll
-                        for (int j = 0; j < LEAF; j++) a[i][j] -= Lr[i][jj] * u[j];
+                        for (int j = 0; j < LF; j++) a[i][j] -= Lr[i][jj] * u[j];
                 }
             }
         }
         // factor the leaf with partial pivoting (unrolled: j indexes registers)
 #pragma unroll
-        for (int j = 0; j < LEAF; j++) {
+        for (int j = 0; j < LF; j++) {
             if (j >= w) break;
             const int c = c0 + j;
             // per-column scratch alternates between two buffers, so one
@@ -161,7 +162,7 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
             int bi = 0x7fffffff;
 #pragma unroll
             for (int i = 0; i < R; i++) {
-                const int r = tid + i * PT;
+                const int r = tid + i * NT;
                 const double v = fabs(a[i][j]);
                 if (r >= c && r < np && (v > bv || (v == bv && r < bi))) {
                     bv = v;
@@ -176,27 +177,27 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
             __syncthreads();
             double gv = red_v[0];
             int p = red_i[0];
-            for (int q = 1; q < PT / 64; q++)
+            for (int q = 1; q < NT / 64; q++)
                 if (red_v[q] > gv || (red_v[q] == gv && red_i[q] < p)) {
                     gv = red_v[q];
                     p = red_i[q];
                 }
             // owners of rows p and c publish them
-            const int own_p = p % PT, own_c = c % PT;
+            const int own_p = p % NT, own_c = c % NT;
             if (tid == own_p) {
 #pragma unroll
                 for (int i = 0; i < R; i++)
-                    if (tid + i * PT == p) {
+                    if (tid + i * NT == p) {
 #pragma unroll
-                        for (int jj = 0; jj < LEAF; jj++) prow[jj] = a[i][jj];
+                        for (int jj = 0; jj < LF; jj++) prow[jj] = a[i][jj];
                     }
             }
             if (tid == own_c && p != c) {
 #pragma unroll
                 for (int i = 0; i < R; i++)
-                    if (tid + i * PT == c) {
+                    if (tid + i * NT == c) {
 #pragma unroll
-                        for (int jj = 0; jj < LEAF; jj++) xrow[jj] = a[i][jj];
+                        for (int jj = 0; jj < LF; jj++) xrow[jj] = a[i][jj];
                     }
             }
             if (tid == 0) {
@@ -211,23 +212,23 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
             const double u = prow[j];
 #pragma unroll
             for (int i = 0; i < R; i++) {
-                const int r = tid + i * PT;
+                const int r = tid + i * NT;
                 if (r == c) {
 #pragma unroll
-                    for (int jj = 0; jj < LEAF; jj++) a[i][jj] = prow[jj];
+                    for (int jj = 0; jj < LF; jj++) a[i][jj] = prow[jj];
                 } else if (r == p) {
 #pragma unroll
-                    for (int jj = 0; jj < LEAF; jj++) a[i][jj] = xrow[jj];
+                    for (int jj = 0; jj < LF; jj++) a[i][jj] = xrow[jj];
                 }
             }
 #pragma unroll
             for (int i = 0; i < R; i++) {
-                const int r = tid + i * PT;
+                const int r = tid + i * NT;
                 if (r > c && r < np) {
                     const double l = a[i][j] / u;
                     a[i][j] = l;
 #pragma unroll
-                    for (int jj = 0; jj < LEAF; jj++)
+                    for (int jj = 0; jj < LF; jj++)
                         if (jj > j) a[i][jj] -= l * prow[jj];
                 }
             }
@@ -235,18 +236,18 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
         // write the leaf back through the row map
 #pragma unroll
         for (int i = 0; i < R; i++) {
-            const int r = tid + i * PT;
+            const int r = tid + i * NT;
             if (r < np) {
                 const int64_t pr = k + perm[r];
 #pragma unroll
-                for (int j = 0; j < LEAF; j++)
+                for (int j = 0; j < LF; j++)
                     if (j < w) A[pr + (k + c0 + j) * lda] = a[i][j];
             }
         }
         __syncthreads();
     }
     // panel in pivoted (logical) order, and the list of moved rows
-    for (int r = tid; r < np; r += PT) {
+    for (int r = tid; r < np; r += NT) {
         const int64_t pr = k + perm[r];
         for (int j = 0; j < nbk; j++) P[r + j * lda] = A[pr + (k + j) * lda];
         if (perm[r] != r) {
@@ -259,6 +260,25 @@ __global__ void __launch_bounds__(PT) __attribute__((amdgpu_waves_per_eu(1, 1)))
     }
     __syncthreads();
     if (tid == 0) moved[0] = min(nmoved, MAXMOVE);
+}
+
+
+// Up to 12 rows per lane: leaves of 8 columns; 16 or 24 rows: leaves of 4
+// (register budget of one wave per SIMD).
+template <int R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+k_lu_panel(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
+           const BlockOut* __restrict__ out, int k)
+{
+    lu_panel_body<R, (R <= 12 ? 8 : 4), 256>(desc, arena, side, out, k);
+}
+
+// Large matrices (N up to 24 x 1024 rows): 1024 lanes, leaves of 2 columns.
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_lu_panel_big(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
+               const BlockOut* __restrict__ out, int k)
+{
+    lu_panel_body<24, 2, 1024>(desc, arena, side, out, k);
 }
 
 // Row permutation of the trailing columns and U12 = L11^{-1} A12.  One lane
@@ -298,11 +318,18 @@ __global__ void __launch_bounds__(64) k_lu_swap_trsm(const BlockDesc* __restrict
     for (int i = 0; i < NB; i++) Ac[k + i] = x[i];
 }
 
-// A22 -= L21 U12 with FP64 MFMA.  Workgroup tile 64x64 of A22, 4 waves of
-// 32x32 (2x2 tiles of 16x16), K = NB = 64 staged in LDS.
+// A22 -= L21 U12 with FP64 MFMA.  Workgroup tile 128x128 of A22, 4 waves
+// of 64x64 (4x4 tiles of 16x16 per wave, 16 accumulators), K = NB = 64
+// staged through LDS 16 at a time.  LDS rows are padded to 144 doubles so
+// the four k-rows a wavefront reads in one instruction fall on disjoint bank
+// halves.
 typedef double double4_t __attribute__((ext_vector_type(4)));
+constexpr int GT = 128;          // workgroup tile
+constexpr int KS = 16;           // K per LDS stage
+constexpr int GTP = GT + 16;     // padded LDS row
 
-__global__ void __launch_bounds__(256) k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                  const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
                                                  int k)
 {
@@ -310,61 +337,100 @@ __global__ void __launch_bounds__(256) k_lu_gemm(const BlockDesc* __restrict__ d
     const BlockDesc& d = desc[b];
     if (out[b].skip) return;
     const int N = d.N;
-    const int m0 = k + NB + blockIdx.x * 64;   // rows of A22
-    const int n0 = k + NB + blockIdx.y * 64;   // columns of A22
+    const int m0 = k + NB + blockIdx.x * GT;   // rows of A22
+    const int n0 = k + NB + blockIdx.y * GT;   // columns of A22
     if (m0 >= N || n0 >= N) return;
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
     const double* P = at<double>(arena, side[b].off_panel);
-    constexpr int KH = NB / 2;
-    __shared__ double As[KH][64 + 1];   // [kk][row]
-    __shared__ double Bs[KH][64 + 1];   // [kk][col]
+    // one LDS buffer: the A and B stages, reused for the C strips at the end
+    __shared__ double smem[2 * KS * GTP];
+    double(*As)[GTP] = reinterpret_cast<double(*)[GTP]>(smem);            // [kk][row]
+    double(*Bs)[GTP] = reinterpret_cast<double(*)[GTP]>(smem + KS * GTP);  // [kk][col]
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
-    const int wr = (wv & 1) * 32, wc = (wv >> 1) * 32;
-    double4_t acc[2][2];
+    const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
+    double4_t acc[4][4];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = (double4_t){0, 0, 0, 0};
+        for (int j = 0; j < 4; j++) acc[i][j] = (double4_t){0, 0, 0, 0};
     const int li = lane & 15, lk = lane >> 4;
-    for (int ks = 0; ks < NB; ks += KH) {
-        __syncthreads();
-        for (int idx = tid; idx < KH * 64; idx += 256) {
-            const int kk = idx / 64, r = idx % 64;
+    // the next stage's operands are fetched into registers while the
+    // current stage runs on the MFMA units
+    constexpr int PER = (KS * GT) / 256;
+    double pa[PER], pb[PER];
+    auto fetch = [&](int ks) {
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            const int idx = tid + 256 * t;
+            const int kk = idx / GT, r = idx % GT;
             const int row = m0 + r;
-            As[kk][r] = row < N ? P[(row - k) + (ks + kk) * lda] : 0.0;
-            const int kk2 = idx % KH, c = idx / KH;
+            pa[t] = row < N ? P[(row - k) + (int64_t)(ks + kk) * lda] : 0.0;
+            const int kk2 = idx % KS, c = idx / KS;
             const int colg = n0 + c;
-            Bs[kk2][c] = colg < N ? A[(k + ks + kk2) + colg * lda] : 0.0;
+            pb[t] = colg < N ? A[(k + ks + kk2) + (int64_t)colg * lda] : 0.0;
+        }
+    };
+    fetch(0);
+    for (int ks = 0; ks < NB; ks += KS) {
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            const int idx = tid + 256 * t;
+            As[idx / GT][idx % GT] = pa[t];
+            Bs[idx % KS][idx / KS] = pb[t];
         }
         __syncthreads();
-#pragma unroll 4
-        for (int kk = 0; kk < KH; kk += 4) {
-            const double a0 = As[kk + lk][wr + li];
-            const double a1 = As[kk + lk][wr + 16 + li];
-            const double b0 = Bs[kk + lk][wc + li];
-            const double b1 = Bs[kk + lk][wc + 16 + li];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        if (ks + KS < NB) fetch(ks + KS);
+#pragma unroll
+        for (int kk = 0; kk < KS; kk += 4) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) av[i] = As[kk + lk][wr + 16 * i + li];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bv[j] = Bs[kk + lk][wc + 16 * j + li];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
     }
     // D layout (measured with k_mfma_probe): lane l, register q holds row
-    // l/16 + 4q, column l%16 of the 16x16 tile
+    // l/16 + 4q, column l%16 of the 16x16 tile.  Each wave passes its
+    // 64x64 result through LDS one 16-row strip at a time so that the
+    // read-modify-write of C touches whole 128-byte column segments: lane
+    // (r = l%16, c = l/16) updates rows r of columns c, c+4, ..., c+60, all
+    // loads of a strip issued before its stores.
+    double* Ts = smem + wv * (16 * 64 + 4);        // per-wave strip [col][row], 16 x 64
+    __syncthreads();                                // As/Bs are reused
+    const int rr = lane & 15, cc = lane >> 4;
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 4; i++) {
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int col = n0 + wc + 16 * j + li;
-            if (col >= N) continue;
+        for (int j = 0; j < 4; j++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int row = m0 + wr + 16 * i + lk + 4 * q;
-                if (row < N) A[row + (int64_t)col * lda] -= acc[i][j][q];
-            }
+            for (int q = 0; q < 4; q++) Ts[(16 * j + li) * 16 + (lk + 4 * q)] = acc[i][j][q];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int row = m0 + wr + 16 * i + rr;
+        double cv[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const int col = n0 + wc + cc + 4 * t;
+            cv[t] = (row < N && col < N) ? A[row + (int64_t)col * lda] : 0.0;
         }
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const int col = n0 + wc + cc + 4 * t;
+            if (row < N && col < N) A[row + (int64_t)col * lda] = cv[t] - Ts[(cc + 4 * t) * 16 + rr];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
 }
 
 // Kahan sum of log|u_ii| (src/matrix.c:462-471).
@@ -407,28 +473,34 @@ extern "C" {
 
 // Rows per lane the panel kernel needs for a matrix of dimension N.
 int capsd_lu_rows_per_lane(int N) { return (N + PT - 1) / PT; }
-int capsd_lu_max_dim(void) { return 24 * PT; }
+int capsd_lu_max_dim(void) { return 24 * 1024; }
 int capsd_lu_nb(void) { return NB; }
 int capsd_lu_maxmove(void) { return MAXMOVE; }
 
 static void launch_panel(int R, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
                          int k, hipStream_t s)
 {
-#define PANEL(RR) hipLaunchKernelGGL(k_lu_panel<RR>, dim3(nb), dim3(PT), 0, s, desc, arena, side, out, k)
+#define PANEL(RR) \
+    hipLaunchKernelGGL(k_lu_panel<RR>, dim3(nb), dim3(256), (RR) * 256 * sizeof(int), s, desc, arena, side, out, k)
     if (R <= 2) PANEL(2);
     else if (R <= 4) PANEL(4);
     else if (R <= 8) PANEL(8);
     else if (R <= 12) PANEL(12);
     else if (R <= 16) PANEL(16);
-    else PANEL(24);
+    else if (R <= 24) PANEL(24);
+    else hipLaunchKernelGGL(k_lu_panel_big, dim3(nb), dim3(1024), 24 * 1024 * sizeof(int), s, desc, arena, side, out, k);
 #undef PANEL
 }
 
 // Whole LU of a batch (host loop over panels; all launches on stream s).
 int capsd_lu_init(void)
 {
-    return (int)hipFuncSetAttribute((const void*)k_lu_swap_trsm, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    MAXMOVE * 64 * sizeof(double));
+    hipError_t e = hipFuncSetAttribute((const void*)k_lu_swap_trsm, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       MAXMOVE * 64 * sizeof(double));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_lu_panel_big, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                24 * 1024 * sizeof(int));
+    return (int)e;
 }
 
 // gemm_ev: NULL, or 2 events per panel recorded around each k_lu_gemm
@@ -446,8 +518,8 @@ void capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, Block
             hipLaunchKernelGGL(k_lu_swap_trsm, dim3((rest + 63) / 64, nblocks), dim3(64),
                                MAXMOVE * 64 * sizeof(double), s, desc, arena, sd, out, k);
             if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p], s);
-            hipLaunchKernelGGL(k_lu_gemm, dim3((rest + 63) / 64, (rest + 63) / 64, nblocks), dim3(256), 0, s, desc,
-                               arena, sd, out, k);
+            hipLaunchKernelGGL(k_lu_gemm, dim3((rest + GT - 1) / GT, (rest + GT - 1) / GT, nblocks), dim3(256), 0, s,
+                               desc, arena, sd, out, k);
             if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
         }
     }

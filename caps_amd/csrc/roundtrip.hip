@@ -20,16 +20,27 @@ __device__ __forceinline__ T* at(char* base, int64_t off) { return reinterpret_c
 // Mie coefficients: one lane per l of the block's window.
 // Reference: src/libcaps.c:557-602 (caps_mie), cached per block as in
 // src/libcaps.c:690-694 and 766-770.
-__global__ void __launch_bounds__(256) k_mie(const BlockDesc* __restrict__ desc, char* __restrict__ arena)
+//
+// Besides log|a_l|, log|b_l| it stores the per-l halves of the element
+// prefactors: log Λ_{l1 l2} = (g(l1) + g(l2))/2 with
+// g(l) = log(2l+1) - log l - log(l+1) + log(l-m)! - log(l+m)! (the
+// separable form of src/libcaps.c:45-48), so ea(l) = (log|a_l| + g(l))/2 and
+// eb(l) = (log|b_l| + g(l))/2 give every element prefactor as a sum of two.
+__global__ void __launch_bounds__(256) k_mie(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                             const double* __restrict__ lfac_tab)
 {
     const BlockDesc& d = desc[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.ldim) return;
-    double* al = at<double>(arena, d.off_mie);
+    double* al = at<double>(arena, d.off_mie);   // [a | b | ea | eb], ldim each
     double lna, lnb;
-    mie(d.xi_, d.LbyR, d.epsm1_sphere, d.lmin + i, lna, lnb);
+    const int l = d.lmin + i, m = d.m;
+    mie(d.xi_, d.LbyR, d.epsm1_sphere, l, lna, lnb);
+    const double g = logi(2 * l + 1) - logi(l) - logi(l + 1) + lfac(lfac_tab, l - m) - lfac(lfac_tab, l + m);
     al[i] = lna;
     al[d.ldim + i] = lnb;
+    al[2 * d.ldim + i] = (lna + g) / 2;
+    al[3 * d.ldim + i] = (lnb + g) / 2;
 }
 
 // Mie coefficients at arbitrary (ξ, l) points (scalar API caps_mie).
@@ -275,6 +286,9 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
 
     double aqm2 = 1, aqm1 = 0, aq = 1;  // a_{q-2}, a_{q-1}, a_q
     int S = 0;                           // a_q carries a factor 2^S
+    // α(p+3), α(p+4) of a step are α(p+1), α(p+2) of the step before
+    double al_next3 = 0, al_next4 = 0;
+    bool have_al = false;
     for (int q = 1; q <= qmax; q++) {
         if (q == 1) {
             aq = (n + nu - 1.5) * (1 - (2 * n + 2 * nu - 1) / (n4 * (n4 - 1))
@@ -293,15 +307,20 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
             aqm2 = aqm1;
             aqm1 = aq;
             const double pq = n + nu - 2 * q, p1 = pq - 2 * m, p2 = pq + 2 * m;
+            const double al1 = alpha_pnnu(pq + 1, n, nu), al2 = alpha_pnnu(pq + 2, n, nu);
+            const double al3 = have_al ? al_next3 : alpha_pnnu(pq + 3, n, nu);
+            const double al4 = have_al ? al_next4 : alpha_pnnu(pq + 4, n, nu);
+            al_next3 = al1;
+            al_next4 = al2;
+            have_al = true;
             if (Ap != 0) {
-                const double c0 = (pq + 2) * (pq + 3) * (p1 + 1) * (p1 + 2) * Ap * alpha_pnnu(pq + 1, n, nu);
-                const double c1 = Ap * (Ap * Ap + (pq + 1) * (pq + 3) * (p1 + 2) * (p2 + 2) * alpha_pnnu(pq + 2, n, nu)
-                                  + (pq + 2) * (pq + 4) * (p1 + 3) * (p2 + 3) * alpha_pnnu(pq + 3, n, nu));
-                const double c2 = -(pq + 2) * (pq + 3) * (p2 + 3) * (p2 + 4) * Ap * alpha_pnnu(pq + 4, n, nu);
+                const double c0 = (pq + 2) * (pq + 3) * (p1 + 1) * (p1 + 2) * Ap * al1;
+                const double c1 = Ap * (Ap * Ap + (pq + 1) * (pq + 3) * (p1 + 2) * (p2 + 2) * al2
+                                  + (pq + 2) * (pq + 4) * (p1 + 3) * (p2 + 3) * al3);
+                const double c2 = -(pq + 2) * (pq + 3) * (p2 + 3) * (p2 + 4) * Ap * al4;
                 aq = (c1 * aqm1 + c2 * aqm2) / c0;
             } else {
-                aq = (pq + 1) * (p2 + 2) * alpha_pnnu(pq + 2, n, nu) * aqm1
-                     / ((pq + 2) * (p1 + 1) * alpha_pnnu(pq + 1, n, nu));
+                aq = (pq + 1) * (p2 + 2) * al2 * aqm1 / ((pq + 2) * (p1 + 1) * al1);
             }
             const double fa = fabs(aq);
             if (fa > 0x1p332) {
@@ -372,45 +391,42 @@ __device__ __forceinline__ LogVal getI(const double* logI, const int8_t* sgnI, c
     return {logI[idx], (int)sgnI[idx]};
 }
 
-__device__ __forceinline__ int sgnd(double v) { return (v > 0) - (v < 0); }
+// Element contributions in linear form: for a prefactor p (log of the Mie
+// and Λ factors), X·e^p is summed directly from the I values,
+//   A·e^p = m² s e^{I + p},
+//   B·e^p = Σ_k c_k s_k e^{I_k + p}   (4 terms; m = 0: I itself),
+//   C·e^p = m [-(l2+1)(l2+m)/(2l2+1) s_1 e^{I_1+p} + l2(l2-m+1)/(2l2+1) s_2 e^{I_2+p}],
+// the same quantities as src/integration.c:762-888 followed by the exp of
+// src/libcaps.c:784-829, without the intermediate log.  Terms below e^{-745}
+// vanish, as they do there.
+__device__ __forceinline__ double ex(LogVal t, double p) { return t.s == 0 ? 0.0 : t.s * exp(t.v + p); }
 
-__device__ LogVal intA(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol)
+__device__ double linA(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol, double p)
 {
-    if (d.m == 0) return {-INFINITY, 0};
-    const LogVal I = getI(lI, sI, d, pol, l1, l2);
-    return {2 * logi(d.m) + I.v, I.s};
+    if (d.m == 0) return 0.0;
+    return (double)d.m * d.m * ex(getI(lI, sI, d, pol, l1, l2), p);
 }
 
-__device__ LogVal intB(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol)
+__device__ double linB(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol, double p)
 {
     const int m = d.m;
-    if (m == 0) return getI(lI, sI, d, pol, l1, l2);
-    const LogVal I1 = getI(lI, sI, d, pol, l1 - 1, l2 - 1);
-    const LogVal I2 = getI(lI, sI, d, pol, l1 + 1, l2 - 1);
-    const LogVal I3 = getI(lI, sI, d, pol, l1 - 1, l2 + 1);
-    const LogVal I4 = getI(lI, sI, d, pol, l1 + 1, l2 + 1);
+    if (m == 0) return ex(getI(lI, sI, d, pol, l1, l2), p);
     const double den = (2 * l1 + 1.) * (2 * l2 + 1.);
-    const double mx = fmax(fmax(I1.v, I2.v), fmax(I3.v, I4.v));
-    double I = (l1 + 1.) * (l1 + m) * (l2 + 1.) * (l2 + m) / den * I1.s * exp(I1.v - mx);
-    I -= l1 * (l1 - m + 1.) * (l2 + 1.) * (l2 + m) / den * I2.s * exp(I2.v - mx);
-    I -= (l1 + 1.) * (l1 + m) * l2 * (l2 - m + 1.) / den * I3.s * exp(I3.v - mx);
-    I += l1 * (l1 - m + 1.) * l2 * (l2 - m + 1.) / den * I4.s * exp(I4.v - mx);
-    return {mx + log(fabs(I)), sgnd(I)};
+    double r = (l1 + 1.) * (l1 + m) * (l2 + 1.) * (l2 + m) / den * ex(getI(lI, sI, d, pol, l1 - 1, l2 - 1), p);
+    r -= l1 * (l1 - m + 1.) * (l2 + 1.) * (l2 + m) / den * ex(getI(lI, sI, d, pol, l1 + 1, l2 - 1), p);
+    r -= (l1 + 1.) * (l1 + m) * l2 * (l2 - m + 1.) / den * ex(getI(lI, sI, d, pol, l1 - 1, l2 + 1), p);
+    r += l1 * (l1 - m + 1.) * l2 * (l2 - m + 1.) / den * ex(getI(lI, sI, d, pol, l1 + 1, l2 + 1), p);
+    return r;
 }
 
-__device__ LogVal intC(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol)
+__device__ double linC(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol, double p)
 {
     const int m = d.m;
-    if (m == 0) return {-INFINITY, 0};
-    const LogVal I1 = getI(lI, sI, d, pol, l1, l2 - 1);
-    const LogVal I2 = getI(lI, sI, d, pol, l1, l2 + 1);
+    if (m == 0) return 0.0;
     const double den = 2 * l2 + 1;
-    double I = -(l2 + 1.) * (l2 + m) / den * I1.s * exp(I1.v - I2.v);
-    I += l2 * (l2 - m + 1.) / den * I2.s;
-    return {logi(m) + I2.v + log(fabs(I)), sgnd(I)};
+    return m * (-(l2 + 1.) * (l2 + m) / den * ex(getI(lI, sI, d, pol, l1, l2 - 1), p)
+                + l2 * (l2 - m + 1.) / den * ex(getI(lI, sI, d, pol, l1, l2 + 1), p));
 }
-
-__device__ __forceinline__ double term(LogVal t, double pref) { return t.s == 0 ? 0.0 : t.s * exp(t.v + pref); }
 
 __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                   const double* __restrict__ lfac_tab, int mode)
@@ -422,32 +438,29 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     const int l1 = d.lmin + i2, l2 = d.lmin + j2, m = d.m;
     const double* lI = at<double>(arena, d.off_i);
     const int8_t* sI = at<int8_t>(arena, d.off_i + 16 * (int64_t)d.nl * d.nl);
-    const double* al = at<double>(arena, d.off_mie);
-    const double* bl = al + d.ldim;
-    const double lnL = (logi(2 * l1 + 1) + logi(2 * l2 + 1) - logi(l1) - logi(l1 + 1) - logi(l2) - logi(l2 + 1)
-                        + lfac(lfac_tab, l1 - m) + lfac(lfac_tab, l2 - m) - lfac(lfac_tab, l1 + m)
-                        - lfac(lfac_tab, l2 + m)) / 2.0;
-    const double a1 = al[i2], b1 = bl[i2], a2 = al[j2], b2 = bl[j2];
+    const double* ea = at<double>(arena, d.off_mie) + 2 * d.ldim;
+    const double* eb = ea + d.ldim;
+    const double ea1 = ea[i2], eb1 = eb[i2], ea2 = ea[j2], eb2 = eb[j2];
     const int TEp = 0, TMp = 1;
-    // EE: √(a_l1 a_l2)(B_TM - A_TE);  MM: √(b_l1 b_l2)(A_TM - B_TE)
-    const double pEE = (a1 + a2) / 2 + lnL, pMM = (b1 + b2) / 2 + lnL;
-    const double EE = term(intB(lI, sI, d, l1, l2, TMp), pEE) - term(intA(lI, sI, d, l1, l2, TEp), pEE);
-    const double MM = term(intA(lI, sI, d, l1, l2, TMp), pMM) - term(intB(lI, sI, d, l1, l2, TEp), pMM);
+    // EE: √(a_l1 a_l2) Λ (B_TM - A_TE);  MM: √(b_l1 b_l2) Λ (A_TM - B_TE)
+    const double pEE = ea1 + ea2, pMM = eb1 + eb2;
+    const double EE = linB(lI, sI, d, l1, l2, TMp, pEE) - linA(lI, sI, d, l1, l2, TEp, pEE);
+    const double MM = linA(lI, sI, d, l1, l2, TMp, pMM) - linB(lI, sI, d, l1, l2, TEp, pMM);
     double EM = 0, ME = 0;
     if (m != 0) {
         // EM: D_TM - C_TE with (a_l1 + b_l2)/2;  ME: C_TM - D_TE with (b_l1 + a_l2)/2;  D(l1,l2) = C(l2,l1)
-        const double pEM = (a1 + b2) / 2 + lnL, pME = (b1 + a2) / 2 + lnL;
-        EM = term(intC(lI, sI, d, l2, l1, TMp), pEM) - term(intC(lI, sI, d, l1, l2, TEp), pEM);
-        ME = term(intC(lI, sI, d, l1, l2, TMp), pME) - term(intC(lI, sI, d, l2, l1, TEp), pME);
+        const double pEM = ea1 + eb2, pME = eb1 + ea2;
+        EM = linC(lI, sI, d, l2, l1, TMp, pEM) - linC(lI, sI, d, l1, l2, TEp, pEM);
+        ME = linC(lI, sI, d, l1, l2, TMp, pME) - linC(lI, sI, d, l2, l1, TEp, pME);
     }
     double* A = at<double>(arena, d.off_mat);
     const int64_t r = 2 * i2, c = 2 * j2, lda = d.lda;
-    const double s = mode == 0 ? -1.0 : 1.0;
+    const double sg = mode == 0 ? -1.0 : 1.0;
     const double one = mode == 0 ? 1.0 : 0.0;
-    A[r + c * lda] = s * EE + (i2 == j2 ? one : 0.0);
-    A[r + (c + 1) * lda] = s * EM;
-    A[r + 1 + c * lda] = s * ME;
-    A[r + 1 + (c + 1) * lda] = s * MM + (i2 == j2 ? one : 0.0);
+    A[r + c * lda] = sg * EE + (i2 == j2 ? one : 0.0);
+    A[r + (c + 1) * lda] = sg * EM;
+    A[r + 1 + c * lda] = sg * ME;
+    A[r + 1 + (c + 1) * lda] = sg * MM + (i2 == j2 ? one : 0.0);
     if (i2 == j2) {
         double* dg = at<double>(arena, d.off_diag);
         dg[r] = EE;
@@ -529,10 +542,11 @@ using namespace capsd;
 
 extern "C" {
 
-void capsd_launch_mie(const BlockDesc* desc, char* arena, int nblocks, int max_ldim, hipStream_t s)
+void capsd_launch_mie(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int max_ldim,
+                      hipStream_t s)
 {
     dim3 g((max_ldim + 255) / 256, nblocks);
-    hipLaunchKernelGGL(k_mie, g, dim3(256), 0, s, desc, arena);
+    hipLaunchKernelGGL(k_mie, g, dim3(256), 0, s, desc, arena, lfac_tab);
 }
 
 void capsd_launch_K(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int max_waves,

@@ -230,3 +230,25 @@ def test_reference_cli_relinked(args, expect):
     last = [ln for ln in out.stdout.splitlines() if ln and not ln.startswith("#")][-1]
     v = float(last.split(",")[4])
     assert abs(v - expect) <= 1e-9 * abs(expect), (v, expect, out.stdout)
+
+
+def test_lu_big_panel_path(ca):
+    """N above the 256-lane panel's 6144 rows: the 1024-lane panel kernel."""
+    rng = np.random.default_rng(11)
+    N = 6600
+    X = rng.normal(size=(N, N)) / math.sqrt(N)
+    A = np.eye(N) - 0.3 * (X + X.T) / 2
+    A[:5, :5] = rng.normal(size=(5, 5))     # force row interchanges in the first panel
+    v = ca.lu_logdet_batch(A[None])[0]
+    ref = np.linalg.slogdet(A)[1]
+    assert abs(v - ref) < 1e-9 * abs(ref), (v, ref)
+
+
+@pytest.mark.parametrize("row", [r for r in GOLD if r["ldim"] == 10000 and (r["xi_"], r["m"]) in
+                                 [(500.5, 1), (1.0, 0), (1000.0, 10)]],
+                         ids=lambda r: "RbyL1000_xi%g_m%d" % (r["xi_"], r["m"]))
+def test_logdetD_reference_goldens_ldim10000(ca, row):
+    """The reference's R/L = 1000, ldim = 10000 goldens (N = 20000)."""
+    c = ca.Caps(row["R"], row["L"], ldim=row["ldim"])
+    v = c.logdetD(row["xi_"], row["m"])
+    assert relerr(v, row["expect"]) < row["tol"], (v, row["expect"])
