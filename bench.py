@@ -88,6 +88,41 @@ def _cpu_block_oracle(job):
     return po.logdetD(c, xi, m)
 
 
+def mixed_aspect(ca, steps):
+    """The 64-block mixed-geometry batch (perfect reflectors, R/L log-U[50,
+    500], ldim = ceil(6 R/L), seed 0) as one batched call; blocks/s, LU
+    TF/s, and the largest deviation from the reference values held in
+    tests/golden/ref_mixed_aspect.json."""
+    from caps_amd.workloads import mixed_aspect_blocks, make_objects
+    blocks = mixed_aspect_blocks()
+    objs = make_objects(blocks)
+    xs = [b["xi_"] for b in blocks]
+    ms = [b["m"] for b in blocks]
+    ca.logdetD_multi(objs, xs, ms)
+    objs[0].reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = ca.logdetD_multi(objs, xs, ms)
+    dt = (time.perf_counter() - t0) / steps
+    st = objs[0].stats(reset=True)
+    lu_s = st["stage_ms"]["lu"] / 1e3
+    err = None
+    ref = os.path.join(ROOT, "tests", "golden", "ref_mixed_aspect.json")
+    if os.path.exists(ref):
+        with open(ref) as f:
+            rows = json.load(f)["data"]
+        err = max(abs(v - r["logdetD"]) / abs(r["logdetD"]) for v, r in zip(out, rows))
+    print("bench: mixed-aspect batch %.3f s for %d blocks" % (dt, len(blocks)), file=sys.stderr, flush=True)
+    return {"workload": "64 perfect-reflector blocks, R/L log-U[50,500], ldim=ceil(6 R/L), xi_ U[0.1,300], "
+                        "m U[0,ldim/4], seed 0, one batch",
+            "blocks_per_s": len(blocks) / dt, "seconds_per_batch": dt,
+            "dims": [min(2 * b["ldim"] for b in blocks), max(2 * b["ldim"] for b in blocks)],
+            "stage_ms_per_batch": {k: v / steps for k, v in st["stage_ms"].items()},
+            "lu_tflops": st["lu_flops"] / lu_s / 1e12 if lu_s > 0 else None,
+            "blocks_factored": st["blocks_factored"] / steps,
+            "max_rel_err_vs_reference": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +131,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every (xi, m) block fits one MI355X); "
@@ -154,6 +190,7 @@ def main():
             "avg_launch_us": 1e3 * prof["gemm_ms"] / max(1, prof["gemm_launches"]),
             "flops_per_launch": prof["gemm_flops"] / max(1, prof["gemm_launches"])}
     lu_s = st["stage_ms"]["lu"] / 1e3
+    mixed = None if args.no_mixed else mixed_aspect(caps_amd, args.steps)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "logdetD blocks/s @R/L=200 FP64 (batched+surrogate form; plain form and LU FP64 TF/s alongside)",
@@ -169,6 +206,7 @@ def main():
         "lu_tflops": st["lu_flops"] / lu_s / 1e12 if lu_s > 0 else None,
         "blocks_factored_per_step": st["blocks_factored"] / args.steps,
         "roofline": roof,
+        "mixed_aspect_batch": mixed,
         "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }
     print(json.dumps(line))

@@ -54,6 +54,7 @@ def lib():
             "caps_fresnel": (None, [p, d, d, dp, dp]),
             "caps_logdetD": (d, [p, d, i]),
             "caps_logdetD_batch": (i, [p, dp, ip, i, dp]),
+            "capsx_logdetD_multi": (i, [ctypes.POINTER(p), dp, ip, i, dp]),
             "caps_last_timing": (None, [p, dp]),
             "caps_M_init": (p, [p, i, d]), "caps_M_elem": (d, [p, i, i, ctypes.c_char, ctypes.c_char]),
             "caps_M_free": (None, [p]), "caps_kernel_M": (d, [i, i, p]),
@@ -240,6 +241,24 @@ class _MObj:
         if getattr(self, "ptr", None):
             lib().caps_M_free(self.ptr)
             self.ptr = None
+
+
+def logdetD_multi(objs, xis, ms):
+    """logdetD of blocks i = (objs[i], xis[i], ms[i]) in one batched launch
+    sequence; the objects may differ in R/L, ldim and material (uneven
+    dimensions 2 ldim).  Runs on objs[0]'s device context, whose stage
+    timings (last_timing) then cover the whole batch."""
+    xis = np.ascontiguousarray(xis, dtype=np.float64)
+    ms = np.ascontiguousarray(ms, dtype=np.int32)
+    if not (len(objs) == len(xis) == len(ms)):
+        raise ValueError("objs, xi_ and m must have the same length")
+    ptrs = (ctypes.c_void_p * len(objs))(*[o.ptr for o in objs])
+    out = np.empty(len(xis))
+    rc = lib().capsx_logdetD_multi(ptrs, _dptr(xis), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   len(xis), _dptr(out))
+    if rc != 0:
+        raise ValueError("capsx_logdetD_multi rejected its input")
+    return out
 
 
 def lu_logdet_batch(mats):

@@ -235,25 +235,28 @@ struct BatchPlan {
 };
 
 // Fill descriptors for blocks [0, n) and their arena offsets.
-BatchPlan plan_batch(caps_t* self, Ctx* c, const double* xi_, const int* m, int n, int with_lu)
+BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const int* m, int n, int with_lu)
 {
     BatchPlan p;
     p.n = n;
-    const int ldim = self->ldim;
-    const int N = 2 * ldim;
-    const int lda = (N + 15) & ~15;
     const int maxmove = capsd_lu_maxmove(), nb = capsd_lu_nb();
     int64_t off = 0;
     double last_xi = NAN, ep = 0, es = 0;
+    caps_t* last_obj = nullptr;
     for (int b = 0; b < n; b++) {
+        caps_t* self = objs[b];
+        const int ldim = self->ldim;
+        const int N = 2 * ldim;
+        const int lda = (N + 15) & ~15;
         BlockDesc& d = c->h_desc[b];
         memset(&d, 0, sizeof(d));
         d.xi_ = xi_[b];
         d.alpha = 2 * xi_[b];
-        if (!(xi_[b] == last_xi)) {
+        if (!(xi_[b] == last_xi) || self != last_obj) {
             ep = caps_epsilonm1_plate(self, xi_[b]);
             es = caps_epsilonm1_sphere(self, xi_[b]);
             last_xi = xi_[b];
+            last_obj = self;
         }
         d.epsm1_plate = ep;
         d.epsm1_sphere = es;
@@ -304,6 +307,12 @@ BatchPlan plan_batch(caps_t* self, Ctx* c, const double* xi_, const int* m, int 
     }
     p.bytes = (size_t)off;
     return p;
+}
+
+BatchPlan plan_batch(caps_t* self, Ctx* c, const double* xi_, const int* m, int n, int with_lu)
+{
+    std::vector<caps_t*> objs(n, self);
+    return plan_batch_multi(objs.data(), c, xi_, m, n, with_lu);
 }
 
 size_t block_bytes_estimate(caps_t* self, int m)
@@ -415,14 +424,14 @@ double finish_block(const BlockOut& o, int lu_mode)
 bool is_lu_mode(detalg_t a) { return a == DETALG_LU || a == DETALG_QR; }
 
 // Largest number of blocks per launch sequence that fits the HBM budget.
-int chunk_blocks(caps_t* self, const int* m, int n)
+int chunk_blocks_multi(caps_t* const* objs, const int* m, int n)
 {
     const double gb = env_double("CAPS_AMD_ARENA_GB", 160.0);
     const size_t budget = (size_t)(gb * 1e9);
     size_t acc = 0;
     int k = 0;
     for (; k < n; k++) {
-        acc += block_bytes_estimate(self, m[k]);
+        acc += block_bytes_estimate(objs[k], m[k]);
         if (acc > budget && k > 0) break;
     }
     return std::max(k, 1);
@@ -622,32 +631,45 @@ void caps_fresnel(caps_t* self, double xi_, double k_, double* r_TE, double* r_T
     *r_TM = r[1];
 }
 
-int caps_logdetD_batch(caps_t* self, const double* xi_, const int* m, int n, double* out)
+// Extension: n blocks, block i of caps object objs[i] (objects may differ in
+// R/L, ldim, material), in one batched launch sequence on objs[0]'s device
+// context.  All objects must use the same detalg.  Returns 0 on success.
+int capsx_logdetD_multi(caps_t* const* objs, const double* xi_, const int* m, int n, double* out)
 {
     if (n <= 0) return 0;
     for (int i = 0; i < n; i++) {
-        if (!(xi_[i] > 0) || m[i] < 0) {
+        if (!objs[i] || !(xi_[i] > 0) || m[i] < 0) {
             fprintf(stderr, "caps_logdetD_batch: block %d has xi_=%g, m=%d (need xi_ > 0, m >= 0)\n", i, xi_[i], m[i]);
             return 1;
         }
+        TERMINATE(2 * objs[i]->ldim > capsd_lu_max_dim(), "ldim=%d exceeds the LU's maximum dimension %d",
+                  objs[i]->ldim, capsd_lu_max_dim());
+        if (is_lu_mode(objs[i]->detalg) != is_lu_mode(objs[0]->detalg)) {
+            fprintf(stderr, "capsx_logdetD_multi: all objects must use the same determinant mode\n");
+            return 1;
+        }
     }
-    TERMINATE(2 * self->ldim > capsd_lu_max_dim(), "ldim=%d exceeds the LU's maximum dimension %d", self->ldim,
-              capsd_lu_max_dim());
-    Ctx* c = ctx_of(self);
+    Ctx* c = ctx_of(objs[0]);
     std::lock_guard<std::mutex> lock(c->mu);
-    const int lu_mode = is_lu_mode(self->detalg) ? 1 : 0;
+    const int lu_mode = is_lu_mode(objs[0]->detalg) ? 1 : 0;
     double tot[6] = {0, 0, 0, 0, 0, 0};
     for (int b0 = 0; b0 < n;) {
-        const int k = chunk_blocks(self, m + b0, n - b0);
+        const int k = chunk_blocks_multi(objs + b0, m + b0, n - b0);
         ensure_blocks(c, k);
-        BatchPlan p = plan_batch(self, c, xi_ + b0, m + b0, k, 1);
-        run_batch(self, c, p, 0, lu_mode);
+        BatchPlan p = plan_batch_multi(objs + b0, c, xi_ + b0, m + b0, k, 1);
+        run_batch(objs[0], c, p, 0, lu_mode);
         for (int i = 0; i < k; i++) out[b0 + i] = finish_block(c->h_out[i], lu_mode);
         for (int i = 0; i < 6; i++) tot[i] += c->timing[i];
         b0 += k;
     }
     memcpy(c->timing, tot, sizeof(tot));
     return 0;
+}
+
+int caps_logdetD_batch(caps_t* self, const double* xi_, const int* m, int n, double* out)
+{
+    std::vector<caps_t*> objs(n > 0 ? n : 1, self);
+    return capsx_logdetD_multi(objs.data(), xi_, m, n, out);
 }
 
 double caps_logdetD(caps_t* self, double xi_, int m)

@@ -178,6 +178,16 @@ double caps_ht_perf(caps_t *caps, double eps);
  * loop (src/caps.c:163-232, 869-957). */
 int caps_logdetD_batch(caps_t *self, const double *xi_, const int *m, int n, double *out);
 
+/* Extension: n blocks, block i being (objs[i], xi_[i], m[i]), in one batched
+ * launch sequence; the objects may differ in R/L, ldim and material, so the
+ * blocks have uneven dimensions 2 ldim.  out[i] equals caps_logdetD(objs[i],
+ * xi_[i], m[i]).  All objects must use the same determinant mode (HODLR
+ * shortcut rules, or LU/QR shortcut rules); the batch runs on objs[0]'s
+ * device context.  Returns 0 on success, nonzero on invalid input.  Serves
+ * drivers that sweep several geometries at once (the reference's workers
+ * build one caps object per geometry, src/caps.c:869-957, at 913). */
+int capsx_logdetD_multi(caps_t *const *objs, const double *xi_, const int *m, int n, double *out);
+
 /* Extension: per-stage timing of the last batch on this caps object, in
  * milliseconds: [0] Mie, [1] K, [2] I, [3] assembly, [4] LU, [5] total
  * (HIP events on the library's stream). */

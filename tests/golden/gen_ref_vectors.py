@@ -8,6 +8,7 @@ machines without the reference.
     python tests/golden/gen_ref_vectors.py blocks      # block logdetD values
     python tests/golden/gen_ref_vectors.py tables      # K, I and M elements
     python tests/golden/gen_ref_vectors.py free_energy # C2 free energy (~10 min, 8 cores)
+    python tests/golden/gen_ref_vectors.py mixed       # 64 mixed-aspect PR blocks
 """
 import ctypes
 import json
@@ -205,6 +206,28 @@ def gen_free_energy(pool):
             "determinants": nblocks, "seconds_8_workers": time.time() - t0}
 
 
+def gen_mixed(pool):
+    """The 64 perfect-reflector blocks of caps_amd.workloads.mixed_aspect_blocks
+    (seed 0): R/L log-U[50, 500], ldim = ceil(6 R/L), xi_ U[0.1, 300],
+    m U[0, ldim/4]."""
+    sys.path.insert(0, ROOT)
+    from caps_amd.workloads import mixed_aspect_blocks
+    blocks = mixed_aspect_blocks()
+    t0 = time.time()
+    # largest first so the pool's tail is short
+    order = sorted(range(len(blocks)), key=lambda i: -blocks[i]["ldim"])
+    res = pool.map(_block, [(blocks[i]["R"], blocks[i]["L"], blocks[i]["ldim"], None, blocks[i]["xi_"],
+                             blocks[i]["m"]) for i in order], chunksize=1)
+    vals = [None] * len(blocks)
+    for i, r in zip(order, res):
+        vals[i] = r
+    rows = []
+    for b, (v, dt) in zip(blocks, vals):
+        rows.append(dict(b, logdetD=v, ref_seconds=dt))
+    print("mixed: %d blocks in %.0fs (8 workers)" % (len(rows), time.time() - t0))
+    return rows
+
+
 def main():
     what = sys.argv[1] if len(sys.argv) > 1 else "blocks"
     if what == "tables":
@@ -218,6 +241,9 @@ def main():
             elif what == "free_energy":
                 data = gen_free_energy(pool)
                 name = "ref_free_energy_c2.json"
+            elif what == "mixed":
+                data = gen_mixed(pool)
+                name = "ref_mixed_aspect.json"
             else:
                 sys.exit("unknown target " + what)
     with open(os.path.join(HERE, name), "w") as f:
