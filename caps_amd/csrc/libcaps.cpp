@@ -34,7 +34,8 @@ void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int
 void capsd_launch_trace(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
 void capsd_launch_mie_points(int, const double*, const int*, double, const double*, double*, hipStream_t);
 void capsd_launch_fresnel_points(int, const double*, const double*, const double*, double*, hipStream_t);
-void capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t, hipEvent_t*);
+int capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t, int, hipEvent_t*, int*);
+int capsd_lu_max_updates(int);
 int capsd_lu_init(void);
 int capsd_lu_max_dim(void);
 int capsd_lu_nb(void);
@@ -113,6 +114,7 @@ struct Ctx {
     // per-launch timing of the LU's trailing-update kernel (capsx_profile)
     int prof = 0;
     std::vector<hipEvent_t> gemm_ev;
+    std::vector<int> gemm_info;
     // planner surrogate (capsx_set_surrogate) and the material features it
     // is evaluated with
     struct capsx_mlp* sur = nullptr;
@@ -201,6 +203,10 @@ double env_double(const char* name, double dflt)
     const char* s = getenv(name);
     return s ? atof(s) : dflt;
 }
+
+// LU schedule: 1 = 64-column steps, 2 = 128-column steps of two panels with
+// one K = 128 trailing update (CAPS_AMD_LU_LEVELS, default 2).
+int lu_levels() { return env_double("CAPS_AMD_LU_LEVELS", 2) >= 2 ? 2 : 1; }
 
 // Quadrature grid of the K integrals of one block (see k_K_partial):
 // x(v) = 1 + A² softplus(v)², A = 0.5/sqrt(α), step h, from v0 to the end of
@@ -292,11 +298,11 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         LuSide& s = c->h_side[b];
         if (with_lu) {
             s.off_panel = off;
-            off = align256(off + (int64_t)lda * nb * 8);
+            off = align256(off + 2 * (int64_t)lda * nb * 8);
             s.off_logs = off;
             off = align256(off + (int64_t)N * 8);
             s.off_moved = off;
-            off = align256(off + (1 + 2 * (int64_t)maxmove) * 4);
+            off = align256(off + 2 * (1 + 2 * (int64_t)maxmove) * 4);
         } else
             s.off_panel = s.off_logs = s.off_moved = 0;
         p.maxN = std::max(p.maxN, N);
@@ -322,7 +328,7 @@ size_t block_bytes_estimate(caps_t* self, int m)
     size_t lmin, lmax;
     caps_estimate_lminmax(self, m, &lmin, &lmax);
     const size_t nK = 2 * (lmin + ldim) + 4;
-    return (size_t)2 * (ldim + 2) * (ldim + 2) * 9 + lda * N * 8 + lda * 64 * 8 + nK * (24 * 2 + 24 * 64) +
+    return (size_t)2 * (ldim + 2) * (ldim + 2) * 9 + lda * N * 8 + lda * 128 * 8 + nK * (24 * 2 + 24 * 64) +
            (size_t)N * 32 + 65536;
 }
 
@@ -345,25 +351,28 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     capsd_launch_assemble(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, mode, s);
     if (mode == 0) capsd_launch_trace(c->d_desc, c->arena, c->d_out, n, lu_mode_shortcuts, s);
     HIPCHECK(hipEventRecord(c->ev[4], s));
-    const int npanels = (p.maxN + capsd_lu_nb() - 1) / capsd_lu_nb();
-    if (c->prof && (int)c->gemm_ev.size() < 2 * npanels) {
+    const int nupd = capsd_lu_max_updates(p.maxN);
+    if (c->prof && (int)c->gemm_ev.size() < 2 * nupd) {
         const size_t old = c->gemm_ev.size();
-        c->gemm_ev.resize(2 * npanels);
+        c->gemm_ev.resize(2 * nupd);
+        c->gemm_info.resize(2 * nupd);
         for (size_t i = old; i < c->gemm_ev.size(); i++) HIPCHECK(hipEventCreate(&c->gemm_ev[i]));
     }
+    const int levels = lu_levels();
+    int nlaunched = 0;
     if (mode == 0) {
         // profiled runs keep one group so each update launch is timed alone
         int groups = c->prof ? 1 : (int)env_double("CAPS_AMD_LU_STREAMS", 3);
         groups = std::max(1, std::min(groups, std::min(MAX_LU_STREAMS, n / 4)));
         if (groups == 1) {
-            capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, p.maxN, s,
-                            c->prof ? c->gemm_ev.data() : nullptr);
+            nlaunched = capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, p.maxN, s, levels,
+                                        c->prof ? c->gemm_ev.data() : nullptr, c->prof ? c->gemm_info.data() : nullptr);
         } else {
             for (int g = 0; g < groups; g++) {
                 const int b0 = (int)((int64_t)n * g / groups), b1 = (int)((int64_t)n * (g + 1) / groups);
                 HIPCHECK(hipStreamWaitEvent(c->lu_stream[g], c->ev[4], 0));
                 capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, b1 - b0, p.maxN,
-                                c->lu_stream[g], nullptr);
+                                c->lu_stream[g], levels, nullptr, nullptr);
                 HIPCHECK(hipEventRecord(c->lu_done[g], c->lu_stream[g]));
                 HIPCHECK(hipStreamWaitEvent(s, c->lu_done[g], 0));
             }
@@ -385,15 +394,19 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
               *c->h_status, n, c->h_desc[0].xi_, c->h_desc[0].m);
     for (int i = 0; i < 6; i++) c->acc[i] += c->timing[i];
     if (c->prof && mode == 0) {
-        const int nb = capsd_lu_nb();
-        for (int pi = 0, k = 0; k < p.maxN; k += nb, pi++) {
-            if (p.maxN - k - nb <= 0) continue;
+        // algorithmic flops of each update launch: 2 x rows x columns x depth
+        const double nb = capsd_lu_nb();
+        for (int pi = 0; pi < nlaunched; pi++) {
+            const int k = c->gemm_info[2 * pi], umode = c->gemm_info[2 * pi + 1];
             HIPCHECK(hipEventElapsedTime(&ms, c->gemm_ev[2 * pi], c->gemm_ev[2 * pi + 1]));
             c->acc[11] += ms;
             c->acc[13] += 1;
             for (int b = 0; b < n; b++) {
-                const double r = c->h_desc[b].N - k - nb;
-                if (c->h_out[b].skip == 0 && r > 0) c->acc[12] += 2.0 * r * r * nb;
+                if (c->h_out[b].skip) continue;
+                const double r1 = c->h_desc[b].N - k - nb, r2 = r1 - nb;
+                if (umode == 0 && r1 > 0) c->acc[12] += 2.0 * r1 * r1 * nb;
+                if (umode == 1 && r1 > 0) c->acc[12] += 2.0 * r1 * std::min(r1, nb) * nb;
+                if (umode == 2 && r2 > 0) c->acc[12] += 2.0 * r2 * r2 * 2 * nb;
             }
         }
     }
@@ -813,11 +826,11 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     off = align256(off + (int64_t)lda * dim * 8);
     LuSide& s = c->h_side[0];
     s.off_panel = off;
-    off = align256(off + (int64_t)lda * capsd_lu_nb() * 8);
+    off = align256(off + 2 * (int64_t)lda * capsd_lu_nb() * 8);
     s.off_logs = off;
     off = align256(off + (int64_t)dim * 8);
     s.off_moved = off;
-    off = align256(off + (1 + 2 * (int64_t)capsd_lu_maxmove()) * 4);
+    off = align256(off + 2 * (1 + 2 * (int64_t)capsd_lu_maxmove()) * 4);
     ensure_arena(c, off);
     BlockOut o;
     memset(&o, 0, sizeof(o));
@@ -825,7 +838,7 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     HIPCHECK(hipMemcpyAsync(c->d_side, &s, sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->d_out, &o, sizeof(BlockOut), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->arena, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, 1, dim, c->stream, nullptr);
+    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, 1, dim, c->stream, lu_levels(), nullptr, nullptr);
     HIPCHECK(hipMemcpyAsync(&o, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     caps_free(tmp);
@@ -1153,11 +1166,11 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
         off = align256(off + (int64_t)lda * dim * 8);
         LuSide& s = c->h_side[b];
         s.off_panel = off;
-        off = align256(off + (int64_t)lda * capsd_lu_nb() * 8);
+        off = align256(off + 2 * (int64_t)lda * capsd_lu_nb() * 8);
         s.off_logs = off;
         off = align256(off + (int64_t)dim * 8);
         s.off_moved = off;
-        off = align256(off + (1 + 2 * (int64_t)capsd_lu_maxmove()) * 4);
+        off = align256(off + 2 * (1 + 2 * (int64_t)capsd_lu_maxmove()) * 4);
     }
     ensure_arena(c, off);
     std::vector<BlockOut> o(n);
@@ -1169,7 +1182,7 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
         HIPCHECK(hipMemcpyAsync(c->arena + c->h_desc[b].off_mat, A + (size_t)b * dim * dim,
                                 (size_t)dim * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, dim, c->stream, nullptr);
+    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, dim, c->stream, lu_levels(), nullptr, nullptr);
     HIPCHECK(hipEventRecord(c->ev[1], c->stream));
     HIPCHECK(hipMemcpyAsync(o.data(), c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));

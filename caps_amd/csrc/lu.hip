@@ -2,16 +2,21 @@
 // partial pivoting (the dgetrf_ route of src/matrix.c:555-577 and the Kahan
 // sum of log|u_ii| of src/matrix.c:462-471), on gfx950.
 //
-// Per panel of NB = 64 columns, three batched kernels:
+// Per panel of NB = 64 columns, three batched kernels (or, with two levels,
+// per step of 128 columns: panel, swaps + TRSM and a narrow update of the
+// second panel's columns, panel, swaps + a 128-row TRSM, one K = 128
+// update of the trailing matrix):
 //   k_lu_panel      one workgroup per matrix factors the N-k x 64 panel:
-//                   left-looking over leaves of 8 columns, each leaf held in
-//                   registers (R rows per lane), pivot search reduced over the
-//                   wavefronts, rows swapped through a logical->physical row
-//                   map in LDS (no row moves in HBM during the panel); writes
-//                   the factored panel in pivoted order to a side buffer, the
+//                   left-looking over leaves of 2..8 columns, each leaf held
+//                   in registers (R rows per lane; 256, 512 or 1024 lanes by
+//                   panel height), pivot search reduced over the wavefronts,
+//                   rows swapped through a logical->physical row map in LDS
+//                   (no row moves in HBM during the panel); writes the
+//                   factored panel in pivoted order to a side buffer, the
 //                   list of moved rows and log|u_ii|;
 //   k_lu_swap_trsm  applies the row permutation to the trailing columns and
-//                   solves U12 = L11^{-1} A12 (one lane per column);
+//                   solves U12 = L11^{-1} A12 (one lane per column, the
+//                   64 x 64 tile staged through LDS);
 //   k_lu_gemm       A22 -= L21 U12 on FP64 MFMA (v_mfma_f64_16x16x4f64).
 // k_lu_finalize sums log|u_ii| with Kahan compensation.
 // Matrices are column-major with leading dimension lda.
@@ -24,8 +29,6 @@
 namespace capsd {
 
 constexpr int NB = 64;       // panel width
-constexpr int LEAF = 8;      // leaf width inside the panel
-constexpr int PT = 256;      // threads of the panel kernel
 constexpr int MAXMOVE = 2 * NB;
 
 template <typename T>
@@ -35,10 +38,21 @@ __device__ __forceinline__ T* at(char* base, int64_t off) { return reinterpret_c
 //   off_piv:              N ints (unused by the logdet; kept for the layout)
 // and, appended by the host after the matrix:
 struct LuSide {
-    int64_t off_panel;   // lda x NB doubles: factored panel, pivoted order
+    int64_t off_panel;   // 2 x lda x NB doubles: the two factored panels of a
+                         // 2 NB-column step, pivoted order
     int64_t off_logs;    // N doubles: log|u_ii|
-    int64_t off_moved;   // 1 + 2*MAXMOVE ints: count, (dst, src) pairs
+    int64_t off_moved;   // 2 x (1 + 2*MAXMOVE) ints: per panel count, (dst, src) pairs
 };
+constexpr int MOVED_STRIDE = 1 + 2 * MAXMOVE;
+
+__device__ __forceinline__ double* panel_buf(char* arena, const LuSide& sd, int64_t lda, int pb)
+{
+    return at<double>(arena, sd.off_panel) + (pb ? lda * NB : 0);
+}
+__device__ __forceinline__ int* moved_buf(char* arena, const LuSide& sd, int pb)
+{
+    return at<int>(arena, sd.off_moved) + (pb ? MOVED_STRIDE : 0);
+}
 
 __device__ __forceinline__ void wave_argmax(double& v, int& idx)
 {
@@ -57,7 +71,8 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx)
 // logical->physical row map lives in dynamic LDS (R*NT ints).
 template <int R, int LF, int NT>
 __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                              const LuSide* __restrict__ side, const BlockOut* __restrict__ out, int k)
+                                              const LuSide* __restrict__ side, const BlockOut* __restrict__ out, int k,
+                                              int pb)
 {
     const int b = blockIdx.x;
     const BlockDesc& d = desc[b];
@@ -66,18 +81,18 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
     const int64_t lda = d.lda;
     const int np = N - k;                 // panel rows
     const int nbk = min(NB, np);          // panel columns
-    double* A = at<double>(arena, d.off_mat);
-    double* P = at<double>(arena, side[b].off_panel);
-    double* logs = at<double>(arena, side[b].off_logs);
-    int* moved = at<int>(arena, side[b].off_moved);
+    double* __restrict__ A = at<double>(arena, d.off_mat);
+    double* __restrict__ P = panel_buf(arena, side[b], lda, pb);
+    double* __restrict__ logs = at<double>(arena, side[b].off_logs);
+    int* __restrict__ moved = moved_buf(arena, side[b], pb);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 
     extern __shared__ int perm[];   // R * NT
     __shared__ double red_v2[2][NT / 64];
     __shared__ int red_i2[2][NT / 64];
     __shared__ double prow2[2][LF], xrow2[2][LF];
-    __shared__ double U[NB * LF];       // U part of the current leaf, [row][col]
-    __shared__ double Lt[NB * NB];        // L11 of the panel so far, [row][col]
+    __shared__ double Uc[2][LF * LF];   // solved U rows of one chunk, [row][col]
+    __shared__ double pivv[NB];         // |pivot| per panel column
     __shared__ int nmoved;
 
     for (int r = tid; r < np; r += NT) perm[r] = r;
@@ -95,54 +110,52 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
             for (int j = 0; j < LF; j++)
                 a[i][j] = (r < np && j < w) ? A[(k + perm[r]) + (k + c0 + j) * lda] : 0.0;
         }
-        if (c0 > 0) {
-            // U part: solve L11[0:c0,0:c0] U = leaf[0:c0] (unit lower), one lane per column
-            for (int idx = tid; idx < c0 * c0; idx += NT) {
-                const int rr = idx / c0, cc = idx % c0;
-                Lt[rr * NB + cc] = A[(k + perm[rr]) + (k + cc) * lda];
-            }
-            for (int idx = tid; idx < c0 * LF; idx += NT) {
-                const int rr = idx / LF, cc = idx % LF;
-                U[rr * LF + cc] = cc < w ? A[(k + perm[rr]) + (k + c0 + cc) * lda] : 0.0;
-            }
-            __syncthreads();
-            if (tid < LF) {
-                for (int i = 0; i < c0; i++) {
-                    const double xi = U[i * LF + tid];
-                    for (int jj = i + 1; jj < c0; jj++) U[jj * LF + tid] -= Lt[jj * NB + i] * xi;
-                }
-            }
-            __syncthreads();
-            // rows < c0 take the solved U; rows >= c0 get the left-looking
-            // update, one earlier leaf (8 columns of L) per step with all of
-            // the step's loads issued before their first use
+        // left-looking update with the panel's earlier columns, LF columns of
+        // L per chunk.  Rows [cc0, cc0+LF) of the leaf (lanes cc0.. of wave 0,
+        // register row 0) become final U rows by a forward substitution
+        // inside the wavefront; every row below then subtracts the chunk's
+        // rank-LF product.
+        for (int cc0 = 0; cc0 < c0; cc0 += LF) {
+            const int buf = (cc0 / LF) & 1;
+            double Lr[R][LF];
 #pragma unroll
             for (int i = 0; i < R; i++) {
                 const int r = tid + i * NT;
-                if (r < c0) {
+                const bool row_ok = r > cc0 && r < np;
+                const int64_t pr = k + (row_ok ? perm[r] : 0);
 #pragma unroll
-                    for (int j = 0; j < LF; j++) a[i][j] = U[r * LF + j];
+                for (int jj = 0; jj < LF; jj++)
+                    Lr[i][jj] = (row_ok && cc0 + jj < r) ? A[pr + (k + cc0 + jj) * lda] : 0.0;
+            }
+            if (wid == 0) {
+#pragma unroll
+                for (int t = 0; t < LF; t++) {
+                    const int q = cc0 + t;
+                    double uq[LF];
+#pragma unroll
+                    for (int j = 0; j < LF; j++) uq[j] = __shfl(a[0][j], q, 64);
+                    if (lane > q && lane < cc0 + LF) {
+#pragma unroll
+                        for (int j = 0; j < LF; j++) a[0][j] -= Lr[0][t] * uq[j];
+                    }
+                }
+                if (lane >= cc0 && lane < cc0 + LF) {
+#pragma unroll
+                    for (int j = 0; j < LF; j++) Uc[buf][(lane - cc0) * LF + j] = a[0][j];
                 }
             }
-            for (int cc0 = 0; cc0 < c0; cc0 += LF) {
-                double Lr[R][LF];
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < LF; t++) {
+                double u[LF];
+#pragma unroll
+                for (int j = 0; j < LF; j++) u[j] = Uc[buf][t * LF + j];
 #pragma unroll
                 for (int i = 0; i < R; i++) {
-                    const int r = tid + i * NT;
-                    const bool row_ok = r >= c0 && r < np;
-                    const int64_t pr = k + (row_ok ? perm[r] : 0);
+                    if (i > 0 || tid >= cc0 + LF) {
 #pragma unroll
-                    for (int jj = 0; jj < LF; jj++) Lr[i][jj] = row_ok ? A[pr + (k + cc0 + jj) * lda] : 0.0;
-                }
-#pragma unroll
-                for (int jj = 0; jj < LF; jj++) {
-                    double u[LF];
-#pragma unroll
-                    for (int j = 0; j < LF; j++) u[j] = U[(cc0 + jj) * LF + j];
-#pragma unroll
-                    for (int i = 0; i < R; i++)
-#pragma unroll
-                        for (int j = 0; j < LF; j++) a[i][j] -= Lr[i][jj] * u[j];
+                        for (int j = 0; j < LF; j++) a[i][j] -= Lr[i][t] * u[j];
+                    }
                 }
             }
         }
@@ -206,7 +219,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
                     perm[c] = perm[p];
                     perm[p] = t;
                 }
-                logs[k + c] = log(fabs(gv));
+                pivv[c] = gv;
             }
             __syncthreads();
             const double u = prow[j];
@@ -246,10 +259,20 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
         }
         __syncthreads();
     }
-    // panel in pivoted (logical) order, and the list of moved rows
+    // log|u_ii|, the panel in pivoted (logical) order, and the list of moved
+    // rows
+    if (tid < nbk) logs[k + tid] = log(pivv[tid]);
     for (int r = tid; r < np; r += NT) {
         const int64_t pr = k + perm[r];
-        for (int j = 0; j < nbk; j++) P[r + j * lda] = A[pr + (k + j) * lda];
+#pragma unroll
+        for (int j0 = 0; j0 < NB; j0 += 16) {
+            double v[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) v[j] = j0 + j < nbk ? A[pr + (k + j0 + j) * lda] : 0.0;
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                if (j0 + j < nbk) P[r + (j0 + j) * lda] = v[j];
+        }
         if (perm[r] != r) {
             const int slot = atomicAdd(&nmoved, 1);
             if (slot < MAXMOVE) {
@@ -263,63 +286,172 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
 }
 
 
-// Up to 12 rows per lane: leaves of 8 columns; 16 or 24 rows: leaves of 4
-// (register budget of one wave per SIMD).
-template <int R>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+// R rows per lane, leaves of LF columns, NT lanes (NT/256 waves per SIMD,
+// so 512/(NT/256) registers per lane hold the leaf and one chunk of L).
+template <int R, int LF, int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 256, NT / 256)))
 k_lu_panel(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
-           const BlockOut* __restrict__ out, int k)
+           const BlockOut* __restrict__ out, int k, int pb)
 {
-    lu_panel_body<R, (R <= 12 ? 8 : 4), 256>(desc, arena, side, out, k);
+    lu_panel_body<R, LF, NT>(desc, arena, side, out, k, pb);
 }
 
-// Large matrices (N up to 24 x 1024 rows): 1024 lanes, leaves of 2 columns.
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_lu_panel_big(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
-               const BlockOut* __restrict__ out, int k)
+// Row interchanges of one column (lane) through LDS: every source row is
+// read before any row is written.  moved: count, then (dst, src) pairs of
+// rows relative to r0.
+__device__ __forceinline__ void swap_rows_col(double* __restrict__ Ac, const int* __restrict__ moved, int r0,
+                                              double* tmpS, int lane)
 {
-    lu_panel_body<24, 2, 1024>(desc, arena, side, out, k);
+    const int nm = moved[0];
+#pragma unroll 8
+    for (int t = 0; t < nm; t++) tmpS[t * 64 + lane] = Ac[r0 + moved[2 + 2 * t]];
+#pragma unroll 8
+    for (int t = 0; t < nm; t++) Ac[r0 + moved[1 + 2 * t]] = tmpS[t * 64 + lane];
 }
 
-// Row permutation of the trailing columns and U12 = L11^{-1} A12.  One lane
-// per trailing column, 64 columns per workgroup.  The moved rows of all 64
-// columns are gathered into LDS before any is written (dynamic LDS:
-// MAXMOVE x 64 doubles).
+// Row permutation of trailing columns and U12 = L11^{-1} A12.  One lane per
+// column, 64 columns per workgroup, columns [k+NB, min(N, k+NB+climit)).
+// Rows moved by the panel's pivoting are exchanged first; the 64 x 64 tile
+// A12 (rows k..k+63) then goes through LDS: coalesced loads (lane = row), a
+// forward substitution per column with the unit lower L11 of panel buffer
+// pb (uniform loads), coalesced stores.  Dynamic LDS: MAXMOVE x 64 doubles.
 __global__ void __launch_bounds__(64) k_lu_swap_trsm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                      const LuSide* __restrict__ side,
-                                                     const BlockOut* __restrict__ out, int k)
+                                                     const BlockOut* __restrict__ out, int k, int pb, int climit)
 {
     const int b = blockIdx.y;
     const BlockDesc& d = desc[b];
     if (out[b].skip || k + NB >= d.N) return;
     const int N = d.N;
+    const int cend = min(N, k + NB + climit);
     const int64_t lda = d.lda;
-    if (blockIdx.x * 64 + k + NB >= N) return;
-    const int col = k + NB + blockIdx.x * 64 + threadIdx.x;
+    const int c0 = k + NB + blockIdx.x * 64;
+    if (c0 >= cend) return;
+    const int lane = threadIdx.x;
+    const int col = c0 + lane;
+    const int ncol = min(64, cend - c0);
     double* A = at<double>(arena, d.off_mat);
-    const double* P = at<double>(arena, side[b].off_panel);
-    const int* moved = at<int>(arena, side[b].off_moved);
-    extern __shared__ double tmpS[];   // [MAXMOVE][64]
-    if (col >= N) return;
-    double* Ac = A + (int64_t)col * lda;
-    const int nm = moved[0];
-    for (int t = 0; t < nm; t++) tmpS[t * 64 + threadIdx.x] = Ac[k + moved[2 + 2 * t]];
-    for (int t = 0; t < nm; t++) Ac[k + moved[1 + 2 * t]] = tmpS[t * 64 + threadIdx.x];
-    // forward substitution with the unit lower L11 (uniform loads of P)
-    double x[NB];
+    const double* P = panel_buf(arena, side[b], lda, pb);
+    extern __shared__ double tmpS[];   // [MAXMOVE][64]; then the tile [col][65]
+    if (col < cend) swap_rows_col(A + (int64_t)col * lda, moved_buf(arena, side[b], pb), k, tmpS, lane);
+    __syncthreads();
+    double* Tt = tmpS;
+#pragma unroll 16
+    for (int c = 0; c < ncol; c++) Tt[c * 65 + lane] = A[(k + lane) + (int64_t)(c0 + c) * lda];
+    __syncthreads();
+    if (col < cend) {
+        double x[NB];
 #pragma unroll
-    for (int i = 0; i < NB; i++) x[i] = Ac[k + i];
+        for (int i = 0; i < NB; i++) x[i] = Tt[lane * 65 + i];
 #pragma unroll
-    for (int i = 0; i < NB; i++) {
+        for (int i = 0; i < NB; i++) {
 #pragma unroll
-        for (int j = i + 1; j < NB; j++) x[j] -= P[j + (int64_t)i * lda] * x[i];
+            for (int j = i + 1; j < NB; j++) x[j] -= P[j + (int64_t)i * lda] * x[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) Tt[lane * 65 + i] = x[i];
     }
+    __syncthreads();
+#pragma unroll 16
+    for (int c = 0; c < ncol; c++) A[(k + lane) + (int64_t)(c0 + c) * lda] = Tt[c * 65 + lane];
+}
+
+// Second panel's row interchanges applied to the rows of the first panel's
+// L below it: rows 64.. of panel buffer 0 (matrix rows k+64..) follow the
+// pivoting of the panel at k+64.  One lane per column of the panel.
+__global__ void __launch_bounds__(64) k_lu_swap_panel(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                      const LuSide* __restrict__ side,
+                                                      const BlockOut* __restrict__ out, int k)
+{
+    const int b = blockIdx.x;
+    const BlockDesc& d = desc[b];
+    if (out[b].skip || k + 2 * NB >= d.N) return;
+    extern __shared__ double tmpS[];   // [MAXMOVE][64]
+    double* P1 = panel_buf(arena, side[b], d.lda, 0);
+    swap_rows_col(P1 + (int64_t)threadIdx.x * d.lda, moved_buf(arena, side[b], 1), NB, tmpS, threadIdx.x);
+}
+
+// Trailing columns [k+2NB, N) of a 2NB-column step: the row interchanges of
+// both panels (the first relative to row k, then the second relative to
+// k+NB), then U12 = L11^{-1} A12 with the 2NB x 2NB unit lower
+//   L11 = [ L11(1)            0      ]
+//         [ L21(1)[0:NB)    L11(2)   ]
+// by blocks: X1 = L11(1)^{-1} B1, B2 -= L21(1) X1, X2 = L11(2)^{-1} B2.
+// The 128 x 64 tile goes through LDS as in k_lu_swap_trsm.  Dynamic LDS:
+// max(MAXMOVE x 64, 64 x 129) doubles.
+constexpr int T2P = 2 * NB + 1;
+__global__ void __launch_bounds__(64) k_lu_swap_trsm2(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                      const LuSide* __restrict__ side,
+                                                      const BlockOut* __restrict__ out, int k)
+{
+    const int b = blockIdx.y;
+    const BlockDesc& d = desc[b];
+    if (out[b].skip || k + 2 * NB >= d.N) return;
+    const int N = d.N;
+    const int64_t lda = d.lda;
+    const int c0 = k + 2 * NB + blockIdx.x * 64;
+    if (c0 >= N) return;
+    const int lane = threadIdx.x;
+    const int col = c0 + lane;
+    const int ncol = min(64, N - c0);
+    double* A = at<double>(arena, d.off_mat);
+    const double* P1 = panel_buf(arena, side[b], lda, 0);
+    const double* P2 = panel_buf(arena, side[b], lda, 1);
+    extern __shared__ double tmpS[];
+    if (col < N) {
+        double* Ac = A + (int64_t)col * lda;
+        swap_rows_col(Ac, moved_buf(arena, side[b], 0), k, tmpS, lane);
+        swap_rows_col(Ac, moved_buf(arena, side[b], 1), k + NB, tmpS, lane);
+    }
+    __syncthreads();
+    double* Tt = tmpS;
+#pragma unroll 8
+    for (int c = 0; c < ncol; c++) {
+        Tt[c * T2P + lane] = A[(k + lane) + (int64_t)(c0 + c) * lda];
+        Tt[c * T2P + NB + lane] = A[(k + NB + lane) + (int64_t)(c0 + c) * lda];
+    }
+    __syncthreads();
+    if (col < N) {
+        double* T = Tt + lane * T2P;
+        {
+            double x[NB];
 #pragma unroll
-    for (int i = 0; i < NB; i++) Ac[k + i] = x[i];
+            for (int i = 0; i < NB; i++) x[i] = T[i];
+#pragma unroll
+            for (int i = 0; i < NB; i++) {
+#pragma unroll
+                for (int j = i + 1; j < NB; j++) x[j] -= P1[j + (int64_t)i * lda] * x[i];
+            }
+#pragma unroll
+            for (int i = 0; i < NB; i++) T[i] = x[i];
+            for (int i2 = 0; i2 < NB; i2++) {
+                double acc = T[NB + i2];
+#pragma unroll
+                for (int j = 0; j < NB; j++) acc -= P1[(NB + i2) + (int64_t)j * lda] * x[j];
+                T[NB + i2] = acc;
+            }
+        }
+        double x[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) x[i] = T[NB + i];
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+#pragma unroll
+            for (int j = i + 1; j < NB; j++) x[j] -= P2[j + (int64_t)i * lda] * x[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) T[NB + i] = x[i];
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int c = 0; c < ncol; c++) {
+        A[(k + lane) + (int64_t)(c0 + c) * lda] = Tt[c * T2P + lane];
+        A[(k + NB + lane) + (int64_t)(c0 + c) * lda] = Tt[c * T2P + NB + lane];
+    }
 }
 
 // A22 -= L21 U12 with FP64 MFMA.  Workgroup tile 128x128 of A22, 4 waves
-// of 64x64 (4x4 tiles of 16x16 per wave, 16 accumulators), K = NB = 64
+// of 64x64 (4x4 tiles of 16x16 per wave, 16 accumulators), K = 64 or 128
 // staged through LDS 16 at a time.  LDS rows are padded to 144 doubles so
 // the four k-rows a wavefront reads in one instruction fall on disjoint bank
 // halves.
@@ -331,18 +463,38 @@ constexpr int GTP = GT + 16;     // padded LDS row
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                  const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
-                                                 int k)
+                                                 int k, int mode)
 {
     const int b = blockIdx.z;
     const BlockDesc& d = desc[b];
     if (out[b].skip) return;
     const int N = d.N;
-    const int m0 = k + NB + blockIdx.x * GT;   // rows of A22
-    const int n0 = k + NB + blockIdx.y * GT;   // columns of A22
-    if (m0 >= N || n0 >= N) return;
+    // the updated region [r0, rend) x [c0, cend) and the depth KK:
+    //   mode 0: one panel at k, all of A22;
+    //   mode 1: first panel of a 2NB step, the second panel's columns only;
+    //   mode 2: both panels of a 2NB step, A22 below and right of them
+    int r0, c0, cend, KK;
+    if (mode == 0) {
+        r0 = c0 = k + NB;
+        cend = N;
+        KK = NB;
+    } else if (mode == 1) {
+        r0 = c0 = k + NB;
+        cend = min(N, k + 2 * NB);
+        KK = NB;
+    } else {
+        r0 = c0 = k + 2 * NB;
+        cend = N;
+        KK = 2 * NB;
+    }
+    const int rend = N;
+    const int m0 = r0 + blockIdx.x * GT;   // rows of the update
+    const int n0 = c0 + blockIdx.y * GT;   // columns of the update
+    if (m0 >= rend || n0 >= cend) return;
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
-    const double* P = at<double>(arena, side[b].off_panel);
+    const double* P1 = panel_buf(arena, side[b], lda, 0);
+    const double* P2 = panel_buf(arena, side[b], lda, 1);
     // one LDS buffer: the A and B stages, reused for the C strips at the end
     __shared__ double smem[2 * KS * GTP];
     double(*As)[GTP] = reinterpret_cast<double(*)[GTP]>(smem);            // [kk][row]
@@ -361,19 +513,21 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
     constexpr int PER = (KS * GT) / 256;
     double pa[PER], pb[PER];
     auto fetch = [&](int ks) {
+        // L columns [ks, ks+KS) (of panel 0 or 1), in the panels' logical rows
+        const double* Lp = ks < NB ? P1 + (int64_t)ks * lda - k : P2 + (int64_t)(ks - NB) * lda - (k + NB);
 #pragma unroll
         for (int t = 0; t < PER; t++) {
             const int idx = tid + 256 * t;
             const int kk = idx / GT, r = idx % GT;
             const int row = m0 + r;
-            pa[t] = row < N ? P[(row - k) + (int64_t)(ks + kk) * lda] : 0.0;
+            pa[t] = row < rend ? Lp[row + (int64_t)kk * lda] : 0.0;
             const int kk2 = idx % KS, c = idx / KS;
             const int colg = n0 + c;
-            pb[t] = colg < N ? A[(k + ks + kk2) + (int64_t)colg * lda] : 0.0;
+            pb[t] = colg < cend ? A[(k + ks + kk2) + (int64_t)colg * lda] : 0.0;
         }
     };
     fetch(0);
-    for (int ks = 0; ks < NB; ks += KS) {
+    for (int ks = 0; ks < KK; ks += KS) {
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < PER; t++) {
@@ -382,7 +536,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
             Bs[idx % KS][idx / KS] = pb[t];
         }
         __syncthreads();
-        if (ks + KS < NB) fetch(ks + KS);
+        if (ks + KS < KK) fetch(ks + KS);
 #pragma unroll
         for (int kk = 0; kk < KS; kk += 4) {
             double av[4], bv[4];
@@ -420,12 +574,12 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
 #pragma unroll
         for (int t = 0; t < 16; t++) {
             const int col = n0 + wc + cc + 4 * t;
-            cv[t] = (row < N && col < N) ? A[row + (int64_t)col * lda] : 0.0;
+            cv[t] = (row < rend && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
         }
 #pragma unroll
         for (int t = 0; t < 16; t++) {
             const int col = n0 + wc + cc + 4 * t;
-            if (row < N && col < N) A[row + (int64_t)col * lda] = cv[t] - Ts[(cc + 4 * t) * 16 + rr];
+            if (row < rend && col < cend) A[row + (int64_t)col * lda] = cv[t] - Ts[(cc + 4 * t) * 16 + rr];
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -471,59 +625,98 @@ using namespace capsd;
 
 extern "C" {
 
-// Rows per lane the panel kernel needs for a matrix of dimension N.
-int capsd_lu_rows_per_lane(int N) { return (N + PT - 1) / PT; }
 int capsd_lu_max_dim(void) { return 24 * 1024; }
 int capsd_lu_nb(void) { return NB; }
 int capsd_lu_maxmove(void) { return MAXMOVE; }
 
-static void launch_panel(int R, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
-                         int k, hipStream_t s)
+static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
+                         int k, int pb, hipStream_t s)
 {
-#define PANEL(RR) \
-    hipLaunchKernelGGL(k_lu_panel<RR>, dim3(nb), dim3(256), (RR) * 256 * sizeof(int), s, desc, arena, side, out, k)
-    if (R <= 2) PANEL(2);
-    else if (R <= 4) PANEL(4);
-    else if (R <= 8) PANEL(8);
-    else if (R <= 12) PANEL(12);
-    else if (R <= 16) PANEL(16);
-    else if (R <= 24) PANEL(24);
-    else hipLaunchKernelGGL(k_lu_panel_big, dim3(nb), dim3(1024), 24 * 1024 * sizeof(int), s, desc, arena, side, out, k);
+#define PANEL(RR, LF, NT) \
+    hipLaunchKernelGGL((k_lu_panel<RR, LF, NT>), dim3(nb), dim3(NT), (RR) * (NT) * sizeof(int), s, desc, arena, side, \
+                       out, k, pb)
+    if (rows <= 2 * 256) PANEL(2, 8, 256);
+    else if (rows <= 4 * 256) PANEL(4, 8, 256);
+    else if (rows <= 8 * 256) PANEL(8, 8, 256);
+    else if (rows <= 6 * 512) PANEL(6, 4, 512);
+    else if (rows <= 12 * 512) PANEL(12, 4, 512);
+    else PANEL(24, 2, 1024);
 #undef PANEL
 }
 
-// Whole LU of a batch (host loop over panels; all launches on stream s).
+constexpr size_t SWAP_LDS = MAXMOVE * 64 * sizeof(double);
+constexpr size_t SWAP2_LDS = (MAXMOVE * 64 > 64 * T2P ? MAXMOVE * 64 : 64 * T2P) * sizeof(double);
+
 int capsd_lu_init(void)
 {
     hipError_t e = hipFuncSetAttribute((const void*)k_lu_swap_trsm, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       MAXMOVE * 64 * sizeof(double));
+                                       SWAP_LDS);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_lu_panel_big, hipFuncAttributeMaxDynamicSharedMemorySize,
+        e = hipFuncSetAttribute((const void*)k_lu_swap_panel, hipFuncAttributeMaxDynamicSharedMemorySize, SWAP_LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_lu_swap_trsm2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                SWAP2_LDS);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_lu_panel<24, 2, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 24 * 1024 * sizeof(int));
     return (int)e;
 }
 
-// gemm_ev: NULL, or 2 events per panel recorded around each k_lu_gemm
-// launch (per-kernel timing of the trailing update).
-void capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockOut* out, int nblocks, int maxN,
-                     hipStream_t s, hipEvent_t* gemm_ev)
+// Upper bound of the number of trailing-update launches of one LU.
+int capsd_lu_max_updates(int maxN) { return (maxN + NB - 1) / NB + 2; }
+
+// Whole LU of a batch (host loop over panels; all launches on stream s).
+// levels 1: NB-column steps (panel, swaps + TRSM, K = NB update);
+// levels 2: 2NB-column steps, the second panel factored after a narrow
+// update of its columns, then one K = 2NB update of the trailing matrix.
+// gemm_ev: NULL, or 2 events per update launch recorded around it;
+// gemm_info receives (k, mode) of each update launch.  Returns the number
+// of update launches.
+int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockOut* out, int nblocks, int maxN,
+                    hipStream_t s, int levels, hipEvent_t* gemm_ev, int* gemm_info)
 {
     const LuSide* sd = (const LuSide*)side;
     int p = 0;
-    for (int k = 0; k < maxN; k += NB, p++) {
-        const int R = (maxN - k + PT - 1) / PT;
-        launch_panel(R, desc, arena, sd, out, nblocks, k, s);
-        const int rest = maxN - k - NB;
-        if (rest > 0) {
-            hipLaunchKernelGGL(k_lu_swap_trsm, dim3((rest + 63) / 64, nblocks), dim3(64),
-                               MAXMOVE * 64 * sizeof(double), s, desc, arena, sd, out, k);
-            if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p], s);
-            hipLaunchKernelGGL(k_lu_gemm, dim3((rest + GT - 1) / GT, (rest + GT - 1) / GT, nblocks), dim3(256), 0, s,
-                               desc, arena, sd, out, k);
-            if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
+    auto update = [&](int k, int mode, int rows, int cols) {
+        if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p], s);
+        hipLaunchKernelGGL(k_lu_gemm, dim3((rows + GT - 1) / GT, (cols + GT - 1) / GT, nblocks), dim3(256), 0, s,
+                           desc, arena, sd, out, k, mode);
+        if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
+        if (gemm_info) {
+            gemm_info[2 * p] = k;
+            gemm_info[2 * p + 1] = mode;
+        }
+        p++;
+    };
+    if (levels <= 1) {
+        for (int k = 0; k < maxN; k += NB) {
+            launch_panel(maxN - k, desc, arena, sd, out, nblocks, k, 0, s);
+            const int rest = maxN - k - NB;
+            if (rest > 0) {
+                hipLaunchKernelGGL(k_lu_swap_trsm, dim3((rest + 63) / 64, nblocks), dim3(64), SWAP_LDS, s, desc,
+                                   arena, sd, out, k, 0, rest);
+                update(k, 0, rest, rest);
+            }
+        }
+    } else {
+        for (int k = 0; k < maxN; k += 2 * NB) {
+            launch_panel(maxN - k, desc, arena, sd, out, nblocks, k, 0, s);
+            const int rest = maxN - k - NB;
+            if (rest <= 0) continue;
+            hipLaunchKernelGGL(k_lu_swap_trsm, dim3(1, nblocks), dim3(64), SWAP_LDS, s, desc, arena, sd, out, k, 0,
+                               NB);
+            update(k, 1, rest, NB);
+            launch_panel(rest, desc, arena, sd, out, nblocks, k + NB, 1, s);
+            const int rest2 = rest - NB;
+            if (rest2 <= 0) continue;
+            hipLaunchKernelGGL(k_lu_swap_panel, dim3(nblocks), dim3(64), SWAP_LDS, s, desc, arena, sd, out, k);
+            hipLaunchKernelGGL(k_lu_swap_trsm2, dim3((rest2 + 63) / 64, nblocks), dim3(64), SWAP2_LDS, s, desc,
+                               arena, sd, out, k);
+            update(k, 2, rest2, rest2);
         }
     }
     hipLaunchKernelGGL(k_lu_finalize, dim3((nblocks + 63) / 64), dim3(64), 0, s, desc, arena, sd, out, nblocks);
+    return p;
 }
 
 void capsd_launch_mfma_probe(const double* A, const double* B, double* D, hipStream_t s)

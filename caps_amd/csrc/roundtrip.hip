@@ -221,9 +221,18 @@ __global__ void __launch_bounds__(256) k_K_combine(const BlockDesc* __restrict__
 // (src/misc.c:95-109).  Symmetric in l1, l2; zero below l = max(m,1);
 // perfect reflectors: I_TE = -|I_TM| (src/integration.c:627-671).
 
-__device__ __forceinline__ double alpha_pnnu(double p, double n, double nu)
+// α(p, n, ν) = Nα(p)/Dα(p) with Nα = (p² - (n+ν+1)²)(p² - (n-ν)²),
+// Dα = 4p² - 1 (the reference's alpha, src/integration.c:485-488).  The
+// recurrence keeps numerator and denominator apart so that one step of
+// eq. (35) costs a single division.
+struct AlphaND {
+    double N, D;
+};
+
+__device__ __forceinline__ AlphaND alpha_nd(double p, double s2, double d2)
 {
-    return ((p * p - (n + nu + 1) * (n + nu + 1)) * (p * p - (n - nu) * (n - nu))) / (4 * p * p - 1);
+    const double p2 = p * p;
+    return {(p2 - s2) * (p2 - d2), 4 * p2 - 1};
 }
 
 __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
@@ -287,7 +296,8 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     double aqm2 = 1, aqm1 = 0, aq = 1;  // a_{q-2}, a_{q-1}, a_q
     int S = 0;                           // a_q carries a factor 2^S
     // α(p+3), α(p+4) of a step are α(p+1), α(p+2) of the step before
-    double al_next3 = 0, al_next4 = 0;
+    const double s2 = (n + nu + 1) * (n + nu + 1), d2 = (n - nu) * (n - nu);
+    AlphaND al_next3 = {0, 1}, al_next4 = {0, 1};
     bool have_al = false;
     for (int q = 1; q <= qmax; q++) {
         if (q == 1) {
@@ -307,20 +317,29 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
             aqm2 = aqm1;
             aqm1 = aq;
             const double pq = n + nu - 2 * q, p1 = pq - 2 * m, p2 = pq + 2 * m;
-            const double al1 = alpha_pnnu(pq + 1, n, nu), al2 = alpha_pnnu(pq + 2, n, nu);
-            const double al3 = have_al ? al_next3 : alpha_pnnu(pq + 3, n, nu);
-            const double al4 = have_al ? al_next4 : alpha_pnnu(pq + 4, n, nu);
+            const AlphaND al1 = alpha_nd(pq + 1, s2, d2), al2 = alpha_nd(pq + 2, s2, d2);
+            const AlphaND al3 = have_al ? al_next3 : alpha_nd(pq + 3, s2, d2);
+            const AlphaND al4 = have_al ? al_next4 : alpha_nd(pq + 4, s2, d2);
             al_next3 = al1;
             al_next4 = al2;
             have_al = true;
             if (Ap != 0) {
-                const double c0 = (pq + 2) * (pq + 3) * (p1 + 1) * (p1 + 2) * Ap * al1;
-                const double c1 = Ap * (Ap * Ap + (pq + 1) * (pq + 3) * (p1 + 2) * (p2 + 2) * al2
-                                  + (pq + 2) * (pq + 4) * (p1 + 3) * (p2 + 3) * al3);
-                const double c2 = -(pq + 2) * (pq + 3) * (p2 + 3) * (p2 + 4) * Ap * al4;
-                aq = (c1 * aqm1 + c2 * aqm2) / c0;
+                // a_q = (c1 a_{q-1} + c2 a_{q-2}) / c0 of eq. (35) with
+                //   c0 = (p+2)(p+3)(p1+1)(p1+2) Ap α1,
+                //   c1 = Ap [Ap² + (p+1)(p+3)(p1+2)(p2+2) α2 + (p+2)(p+4)(p1+3)(p2+3) α3],
+                //   c2 = -(p+2)(p+3)(p2+3)(p2+4) Ap α4;
+                // Ap cancels, and numerator and denominator are multiplied by
+                // Dα1 Dα2 Dα3 Dα4
+                const double k2 = (pq + 1) * (pq + 3) * (p1 + 2) * (p2 + 2);
+                const double k3 = (pq + 2) * (pq + 4) * (p1 + 3) * (p2 + 3);
+                const double k4 = (pq + 2) * (pq + 3) * (p2 + 3) * (p2 + 4);
+                const double d23 = al2.D * al3.D;
+                const double num1 = (Ap * Ap * d23 + k2 * al2.N * al3.D + k3 * al3.N * al2.D) * al4.D;
+                const double num2 = k4 * al4.N * d23;
+                const double den = (pq + 2) * (pq + 3) * (p1 + 1) * (p1 + 2) * al1.N * d23 * al4.D;
+                aq = (num1 * aqm1 - num2 * aqm2) * al1.D / den;
             } else {
-                aq = (pq + 1) * (p2 + 2) * al2 * aqm1 / ((pq + 2) * (p1 + 1) * al1);
+                aq = (pq + 1) * (p2 + 2) * al2.N * al1.D * aqm1 / ((pq + 2) * (p1 + 1) * al1.N * al2.D);
             }
             const double fa = fabs(aq);
             if (fa > 0x1p332) {
