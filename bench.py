@@ -111,7 +111,10 @@ def mixed_aspect(ca, steps):
     if os.path.exists(ref):
         with open(ref) as f:
             rows = json.load(f)["data"]
-        err = max(abs(v - r["logdetD"]) / abs(r["logdetD"]) for v, r in zip(out, rows))
+        err = {"max_abs": max(abs(v - r["logdetD"]) for v, r in zip(out, rows)),
+               "max_rel": max(abs(v - r["logdetD"]) / abs(r["logdetD"]) for v, r in zip(out, rows)),
+               "max_vs_tolerance": max(abs(v - r["logdetD"]) / (1e-9 * abs(r["logdetD"]) + 1e-12)
+                                       for v, r in zip(out, rows))}
     print("bench: mixed-aspect batch %.3f s for %d blocks" % (dt, len(blocks)), file=sys.stderr, flush=True)
     return {"workload": "64 perfect-reflector blocks, R/L log-U[50,500], ldim=ceil(6 R/L), xi_ U[0.1,300], "
                         "m U[0,ldim/4], seed 0, one batch",
@@ -120,7 +123,35 @@ def mixed_aspect(ca, steps):
             "stage_ms_per_batch": {k: v / steps for k, v in st["stage_ms"].items()},
             "lu_tflops": st["lu_flops"] / lu_s / 1e12 if lu_s > 0 else None,
             "blocks_factored": st["blocks_factored"] / steps,
-            "max_rel_err_vs_reference": err}
+            "error_vs_reference": err}
+
+
+def hbm_probe(nbytes=1 << 30, reps=3):
+    """A plain copy of known size: per dispatch it reads and writes nbytes.
+    Profiled with FETCH_SIZE / WRITE_SIZE it gives the factor between those
+    counters and bytes."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"hbm_probe_bytes_read_per_dispatch": nbytes, "bytes_written_per_dispatch": nbytes,
+                      "dispatches": reps}))
+
+
+def gemm_traffic():
+    """HBM bytes per k_lu_gemm launch measured with FETCH_SIZE and WRITE_SIZE
+    (profiles/k_lu_gemm_c2_traffic.txt, scaled by the copy calibration), or
+    None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_lu_gemm_c2_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch"):
+                return float(line.split("=")[1])
+    return None
 
 
 def main():
@@ -132,7 +163,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy three times (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.hbm_probe:
+        hbm_probe()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (every (xi, m) block fits one MI355X); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -185,10 +221,12 @@ def main():
     gemm_s = prof["gemm_ms"] / 1e3
     achieved = prof["gemm_flops"] / gemm_s / 1e12 if gemm_s > 0 else None
     roof = {"kernel": "k_lu_gemm", "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS if achieved else None, "traffic": None,
+            "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS if achieved else None, "traffic": gemm_traffic(),
             "launches": prof["gemm_launches"],
             "avg_launch_us": 1e3 * prof["gemm_ms"] / max(1, prof["gemm_launches"]),
-            "flops_per_launch": prof["gemm_flops"] / max(1, prof["gemm_launches"])}
+            "flops_per_launch": prof["gemm_flops"] / max(1, prof["gemm_launches"]),
+            "algorithmic_bytes_per_launch": prof["gemm_bytes"] / max(1, prof["gemm_launches"]),
+            "arithmetic_intensity_flop_per_byte": prof["gemm_flops"] / max(1.0, prof["gemm_bytes"])}
     lu_s = st["stage_ms"]["lu"] / 1e3
     mixed = None if args.no_mixed else mixed_aspect(caps_amd, args.steps)
     cpu = None if args.no_cpu else cpu_baseline()

@@ -152,14 +152,14 @@ class Caps:
 
     def stats(self, reset=False):
         """Accumulated stage times and work counters of this object's batches."""
-        a = np.zeros(14)
+        a = np.zeros(16)
         lib().capsx_stats(self.ptr, _dptr(a))
         if reset:
             lib().capsx_stats_reset(self.ptr)
         names = ["mie", "K", "I", "assemble", "lu", "total"]
         return {"stage_ms": dict(zip(names, a[:6])), "blocks": int(a[6]), "blocks_factored": int(a[7]),
                 "lu_flops": a[8], "batches": int(a[9]), "I_integrals": a[10],
-                "gemm_ms": a[11], "gemm_flops": a[12], "gemm_launches": int(a[13])}
+                "gemm_ms": a[11], "gemm_flops": a[12], "gemm_launches": int(a[13]), "gemm_bytes": a[14]}
 
     def profile(self, on=True):
         """Time every launch of the LU's trailing-update kernel with HIP events."""
@@ -211,10 +211,12 @@ class Caps:
 
     def free_energy(self, T, cutoff=1e-9, epsrel=1e-6, mode="batched", drude_xi0=True):
         """F (L+R)/(ħc) at temperature T with the reference driver's sums
-        (src/caps.c:307-396, 722-850); returns (F, determinants computed,
-        determinants entering the sums).  mode: "plain" (one block at a
-        time), "batched" (planned from the first two Matsubara terms) or
-        "surrogate" (planned by an attached caps_amd.surrogate.MLP)."""
+        (src/caps.c:235-396, 722-850); T = 0 integrates over ξ with the
+        Fourier-Chebyshev rule on [0, ∞) to relative accuracy epsrel.
+        Returns (F, determinants computed, determinants entering the sums).
+        mode: "plain" (one block at a time), "batched" (planned from the
+        first Matsubara terms, or one batch per quadrature level) or
+        "surrogate" (planned by an attached caps_amd.surrogate.MLP; T > 0)."""
         nb, nu = ctypes.c_long(), ctypes.c_long()
         F = lib().caps_free_energy(self.ptr, T, cutoff, epsrel, {"plain": 0, "batched": 1, "surrogate": 2}[mode],
                                    1 if drude_xi0 else 0, ctypes.byref(nb), ctypes.byref(nu))

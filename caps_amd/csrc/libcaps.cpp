@@ -34,12 +34,12 @@ void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int
 void capsd_launch_trace(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
 void capsd_launch_mie_points(int, const double*, const int*, double, const double*, double*, hipStream_t);
 void capsd_launch_fresnel_points(int, const double*, const double*, const double*, double*, hipStream_t);
-int capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t, int, hipEvent_t*, int*);
+int capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t, int, int, hipEvent_t*,
+                    int*);
 int capsd_lu_max_updates(int);
 int capsd_lu_init(void);
 int capsd_lu_max_dim(void);
 int capsd_lu_nb(void);
-int capsd_lu_maxmove(void);
 void capsd_launch_mfma_probe(const double*, const double*, double*, hipStream_t);
 }
 
@@ -109,8 +109,9 @@ struct Ctx {
     int* h_status = nullptr;
     double timing[6] = {0, 0, 0, 0, 0, 0};
     // accumulated since capsx_stats_reset: stage ms [0..5], blocks, blocks
-    // factored (not skipped), algorithmic LU flops, batches, I pairs
-    double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // factored (not skipped), algorithmic LU flops, batches, I pairs, and
+    // with profiling on the trailing update's ms, flops, launches, bytes
+    double acc[16] = {};
     // per-launch timing of the LU's trailing-update kernel (capsx_profile)
     int prof = 0;
     std::vector<hipEvent_t> gemm_ev;
@@ -207,6 +208,10 @@ double env_double(const char* name, double dflt)
 // LU schedule: 1 = 64-column steps, 2 = 128-column steps of two panels with
 // one K = 128 trailing update (CAPS_AMD_LU_LEVELS, default 2).
 int lu_levels() { return env_double("CAPS_AMD_LU_LEVELS", 2) >= 2 ? 2 : 1; }
+// Trailing-update tile order: 1 = all tiles of a block on one XCD, 0 =
+// block-major (CAPS_AMD_GEMM_XCD, default 0: with blocks of unequal size the
+// per-XCD order unbalances the XCDs).
+int gemm_xcd() { return env_double("CAPS_AMD_GEMM_XCD", 0) != 0 ? 1 : 0; }
 
 // Quadrature grid of the K integrals of one block (see k_K_partial):
 // x(v) = 1 + A² softplus(v)², A = 0.5/sqrt(α), step h, from v0 to the end of
@@ -245,7 +250,7 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
 {
     BatchPlan p;
     p.n = n;
-    const int maxmove = capsd_lu_maxmove(), nb = capsd_lu_nb();
+    const int nb = capsd_lu_nb();
     int64_t off = 0;
     double last_xi = NAN, ep = 0, es = 0;
     caps_t* last_obj = nullptr;
@@ -302,7 +307,7 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
             s.off_logs = off;
             off = align256(off + (int64_t)N * 8);
             s.off_moved = off;
-            off = align256(off + 2 * (1 + 2 * (int64_t)maxmove) * 4);
+            off = align256(off + 2 * (int64_t)nb * 4);
         } else
             s.off_panel = s.off_logs = s.off_moved = 0;
         p.maxN = std::max(p.maxN, N);
@@ -362,17 +367,17 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     int nlaunched = 0;
     if (mode == 0) {
         // profiled runs keep one group so each update launch is timed alone
-        int groups = c->prof ? 1 : (int)env_double("CAPS_AMD_LU_STREAMS", 3);
+        int groups = c->prof ? 1 : (int)env_double("CAPS_AMD_LU_STREAMS", 1);
         groups = std::max(1, std::min(groups, std::min(MAX_LU_STREAMS, n / 4)));
         if (groups == 1) {
-            nlaunched = capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, p.maxN, s, levels,
+            nlaunched = capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, p.maxN, s, levels, gemm_xcd(),
                                         c->prof ? c->gemm_ev.data() : nullptr, c->prof ? c->gemm_info.data() : nullptr);
         } else {
             for (int g = 0; g < groups; g++) {
                 const int b0 = (int)((int64_t)n * g / groups), b1 = (int)((int64_t)n * (g + 1) / groups);
                 HIPCHECK(hipStreamWaitEvent(c->lu_stream[g], c->ev[4], 0));
                 capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, b1 - b0, p.maxN,
-                                c->lu_stream[g], levels, nullptr, nullptr);
+                                c->lu_stream[g], levels, gemm_xcd(), nullptr, nullptr);
                 HIPCHECK(hipEventRecord(c->lu_done[g], c->lu_stream[g]));
                 HIPCHECK(hipStreamWaitEvent(s, c->lu_done[g], 0));
             }
@@ -404,9 +409,15 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
             for (int b = 0; b < n; b++) {
                 if (c->h_out[b].skip) continue;
                 const double r1 = c->h_desc[b].N - k - nb, r2 = r1 - nb;
-                if (umode == 0 && r1 > 0) c->acc[12] += 2.0 * r1 * r1 * nb;
-                if (umode == 1 && r1 > 0) c->acc[12] += 2.0 * r1 * std::min(r1, nb) * nb;
-                if (umode == 2 && r2 > 0) c->acc[12] += 2.0 * r2 * r2 * 2 * nb;
+                // region rows x cols, depth kk: flops 2 rows cols kk; HBM
+                // bytes: C read and written once, L and U read once
+                double rows = 0, cols = 0, kk = 0;
+                if (umode == 0 && r1 > 0) rows = cols = r1, kk = nb;
+                if (umode == 1 && r1 > 0) rows = r1, cols = std::min(r1, nb), kk = nb;
+                if (umode == 2 && r2 > 0) rows = cols = r2, kk = 2 * nb;
+                if (umode == 3 && r2 > 0) rows = nb, cols = r2, kk = nb;
+                c->acc[12] += 2.0 * rows * cols * kk;
+                c->acc[14] += 16.0 * rows * cols + 8.0 * (rows + cols) * kk;
             }
         }
     }
@@ -830,7 +841,7 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     s.off_logs = off;
     off = align256(off + (int64_t)dim * 8);
     s.off_moved = off;
-    off = align256(off + 2 * (1 + 2 * (int64_t)capsd_lu_maxmove()) * 4);
+    off = align256(off + 2 * (int64_t)capsd_lu_nb() * 4);
     ensure_arena(c, off);
     BlockOut o;
     memset(&o, 0, sizeof(o));
@@ -838,7 +849,8 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     HIPCHECK(hipMemcpyAsync(c->d_side, &s, sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->d_out, &o, sizeof(BlockOut), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->arena, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, 1, dim, c->stream, lu_levels(), nullptr, nullptr);
+    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, 1, dim, c->stream, lu_levels(), gemm_xcd(), nullptr,
+                    nullptr);
     HIPCHECK(hipMemcpyAsync(&o, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     caps_free(tmp);
@@ -995,12 +1007,129 @@ void run_requests(caps_t* self, std::vector<Term>& terms, const std::vector<std:
     for (size_t i = 0; i < xs.size(); i++) terms[owner[i]].v.push_back(out[i]);
 }
 
+// Fourier-Chebyshev quadrature on [0, ∞) (Boyd, J. Sci. Comput. 2 (1987),
+// eq. (3.2e); the reference's fcqs_semiinf, src/fcqs.c:107-175, boost L = 1):
+// with M = N+1 and t_i = π i/M, i = 1..N,
+//   x_i = cot²(t_i/2),
+//   w_i = 8 sin t_i / ((1 - cos t_i)² M) Σ_{j odd <= N} sin(j t_i)/j.
+// The nodes of M are a subset of those of 2M.
+double fcqs_weight(double t, int N)
+{
+    double s = 0;
+    for (int j = 1; j <= N; j += 2) s += sin(j * t) / j;
+    const double omc = 1 - cos(t);
+    return 8 * sin(t) / (omc * omc) / (N + 1) * s;
+}
+
+// T = 0: F (L+R)/(ħc) = ∫_0^∞ F_ξ(x/α) dx / (α π), α = 2L/(L+R), where
+// F_ξ is the m-sum of one frequency (src/caps.c:235-266, 722-748).  For
+// R/L > 10 the integrand is taken as 0 above the frequency where the PFA
+// estimate of logdet falls below 1e-100 (src/caps.c:256-263).  The rule's
+// levels M = 5, 10, 20, ... 2560 are refined until two successive levels
+// agree to epsrel; every level's new nodes form one batch (modes 1, 2) or
+// run one block at a time (mode 0).
+double free_energy_T0(caps_t* self, double cutoff, double epsrel, int mode, long* nblocks, long* nused)
+{
+    const int MMIN = 5, MMAX = 2560;
+    const double LbyR = self->LbyR;
+    const double alpha = 2 * LbyR / (1 + LbyR);
+    const double xi_cut = LbyR < 0.1 ? -(1 + 1 / LbyR) * log(2 * LbyR * 1e-100) / 2 : INFINITY;
+    long computed = 0, used = 0;
+    std::vector<double> fcache(MMAX + 1, NAN);
+    std::vector<int> mstop_at(MMAX + 1, -1);
+    Ctx* cx = ctx_of(self);
+    double Ilast = NAN;
+    for (int M = MMIN; M <= MMAX; M *= 2) {
+        const int ratio = MMAX / M, N = M - 1;
+        // new nodes of this level
+        std::vector<Term> terms;
+        std::vector<int> slot;
+        for (int i = 1; i <= N; i++) {
+            const int idx = ratio * i;
+            if (!std::isnan(fcache[idx])) continue;
+            const double t = M_PI * i / M;
+            const double c = cos(t / 2) / sin(t / 2);
+            const double xi = c * c / alpha;
+            if (xi > xi_cut) {
+                fcache[idx] = 0;
+                mstop_at[idx] = -1;
+                continue;
+            }
+            Term tm;
+            tm.xi = xi;
+            terms.push_back(tm);
+            slot.push_back(idx);
+        }
+        if (mode == 0) {
+            for (Term& t : terms)
+                for (int m = 0;; m++) {
+                    TERMINATE(m >= 100000, "sum over m did not converge at xi_=%g", t.xi);
+                    t.v.push_back(caps_logdetD(self, t.xi, m));
+                    computed++;
+                    if (msum(t.v, cutoff, &t.value, &t.mstop)) break;
+                }
+        } else if (!terms.empty()) {
+            // first guess of each node's m-range: the m-cutoff of the
+            // nearest known node at a lower frequency (m-cutoffs fall with
+            // ξ) plus a margin, else 64; then every node is extended until
+            // its own cutoff is reached
+            std::vector<std::pair<int, int>> req;
+            for (size_t q = 0; q < terms.size(); q++) {
+                int guess = -1;
+                double best = -1;
+                for (int idx = 1; idx <= MMAX; idx++) {
+                    if (mstop_at[idx] < 0) continue;
+                    const double t = M_PI * idx / MMAX, c = cos(t / 2) / sin(t / 2);
+                    const double xk = c * c / alpha;
+                    if (xk <= terms[q].xi && xk > best) {
+                        best = xk;
+                        guess = mstop_at[idx];
+                    }
+                }
+                req.push_back({(int)q, guess >= 0 ? guess + 8 + guess / 8 : 64});
+            }
+            run_requests(self, terms, req, &computed);
+            cx->plan_batches++;
+            for (size_t q = 0; q < terms.size(); q++) {
+                Term& t = terms[q];
+                int upto = (int)t.v.size();
+                while (!msum(t.v, cutoff, &t.value, &t.mstop)) {
+                    TERMINATE(upto >= 100000, "sum over m did not converge at xi_=%g", t.xi);
+                    upto += 16;
+                    run_requests(self, terms, {{(int)q, upto}}, &computed);
+                    cx->plan_batches++;
+                }
+            }
+        }
+        for (size_t q = 0; q < terms.size(); q++) {
+            fcache[slot[q]] = terms[q].value;
+            mstop_at[slot[q]] = terms[q].mstop;
+            used += terms[q].mstop + 1;
+        }
+        double I = 0;
+        for (int i = 1; i <= N; i++) {
+            const double f = fcache[ratio * i];
+            TERMINATE(!std::isfinite(f), "integrand not finite at node %d of %d", i, N);
+            I += fcqs_weight(M_PI * i / M, N) * f;
+        }
+        if (!std::isnan(Ilast) && fabs(1 - Ilast / I) < epsrel) {
+            if (nblocks) *nblocks = computed;
+            if (nused) *nused = used;
+            return I / alpha / M_PI;
+        }
+        Ilast = I;
+    }
+    TERMINATE(true, "the xi-integral did not reach epsrel=%g with %d nodes", epsrel, MMAX - 1);
+    return NAN;
+}
+
 }  // namespace
 
 double caps_free_energy(caps_t* self, double T, double cutoff, double epsrel, int mode, int drude_xi0,
                         long* nblocks, long* nused)
 {
-    TERMINATE(!(T > 0), "caps_free_energy needs T > 0 (got %g)", T);
+    TERMINATE(!(T >= 0), "caps_free_energy needs T >= 0 (got %g)", T);
+    if (T == 0) return free_energy_T0(self, cutoff, epsrel, mode, nblocks, nused);
     TERMINATE(!drude_xi0, "the xi = 0 term is implemented for the Drude model only");
     const double T_scaled = 2 * M_PI * CAPS_kB * (self->R + self->L) * T / (CAPS_hbar * CAPS_c);
     long computed = 0, used = 0;
@@ -1170,7 +1299,7 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
         s.off_logs = off;
         off = align256(off + (int64_t)dim * 8);
         s.off_moved = off;
-        off = align256(off + 2 * (1 + 2 * (int64_t)capsd_lu_maxmove()) * 4);
+        off = align256(off + 2 * (int64_t)capsd_lu_nb() * 4);
     }
     ensure_arena(c, off);
     std::vector<BlockOut> o(n);
@@ -1182,7 +1311,8 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
         HIPCHECK(hipMemcpyAsync(c->arena + c->h_desc[b].off_mat, A + (size_t)b * dim * dim,
                                 (size_t)dim * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, dim, c->stream, lu_levels(), nullptr, nullptr);
+    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, dim, c->stream, lu_levels(), gemm_xcd(), nullptr,
+                    nullptr);
     HIPCHECK(hipEventRecord(c->ev[1], c->stream));
     HIPCHECK(hipMemcpyAsync(o.data(), c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1254,17 +1384,19 @@ int capsx_M_copy(caps_M_t* self, double* out)
 // out[9] batches, out[10] (l1 >= l2, polarization) I integrals.
 // With capsx_profile on: out[11] milliseconds in the LU's trailing-update
 // kernel (k_lu_gemm, HIP events around each launch), out[12] its algorithmic
-// flops Σ 2 (N-k-64)² 64 over factored blocks, out[13] its launches.
-void capsx_stats(caps_t* self, double out[14])
+// flops (2 rows cols depth per updated region, over factored blocks),
+// out[13] its launches, out[14] its algorithmic HBM bytes (C read and
+// written once, both operands read once).
+void capsx_stats(caps_t* self, double out[16])
 {
     Ctx* c = ctx_of(self);
-    for (int i = 0; i < 14; i++) out[i] = c->acc[i];
+    for (int i = 0; i < 16; i++) out[i] = c->acc[i];
 }
 
 void capsx_stats_reset(caps_t* self)
 {
     Ctx* c = ctx_of(self);
-    for (int i = 0; i < 14; i++) c->acc[i] = 0;
+    for (int i = 0; i < 16; i++) c->acc[i] = 0;
 }
 
 // Extension: per-launch events around the LU's trailing-update kernel.

@@ -4,8 +4,8 @@
 //
 // Per panel of NB = 64 columns, three batched kernels (or, with two levels,
 // per step of 128 columns: panel, swaps + TRSM and a narrow update of the
-// second panel's columns, panel, swaps + a 128-row TRSM, one K = 128
-// update of the trailing matrix):
+// second panel's columns, panel, swaps + TRSM, a 64-row update, TRSM, one
+// K = 128 update of the trailing matrix):
 //   k_lu_panel      one workgroup per matrix factors the N-k x 64 panel:
 //                   left-looking over leaves of 2..8 columns, each leaf held
 //                   in registers (R rows per lane; 256, 512 or 1024 lanes by
@@ -13,10 +13,9 @@
 //                   rows swapped through a logical->physical row map in LDS
 //                   (no row moves in HBM during the panel); writes the
 //                   factored panel in pivoted order to a side buffer, the
-//                   list of moved rows and log|u_ii|;
-//   k_lu_swap_trsm  applies the row permutation to the trailing columns and
-//                   solves U12 = L11^{-1} A12 (one lane per column, the
-//                   64 x 64 tile staged through LDS);
+//                   pivots (LAPACK ipiv convention) and log|u_ii|;
+//   k_lu_swap_trsm  applies the row interchanges to the trailing columns
+//                   and solves U12 = L11^{-1} A12 (one lane per column);
 //   k_lu_gemm       A22 -= L21 U12 on FP64 MFMA (v_mfma_f64_16x16x4f64).
 // k_lu_finalize sums log|u_ii| with Kahan compensation.
 // Matrices are column-major with leading dimension lda.
@@ -29,7 +28,6 @@
 namespace capsd {
 
 constexpr int NB = 64;       // panel width
-constexpr int MAXMOVE = 2 * NB;
 
 template <typename T>
 __device__ __forceinline__ T* at(char* base, int64_t off) { return reinterpret_cast<T*>(base + off); }
@@ -41,9 +39,12 @@ struct LuSide {
     int64_t off_panel;   // 2 x lda x NB doubles: the two factored panels of a
                          // 2 NB-column step, pivoted order
     int64_t off_logs;    // N doubles: log|u_ii|
-    int64_t off_moved;   // 2 x (1 + 2*MAXMOVE) ints: per panel count, (dst, src) pairs
+    int64_t off_moved;   // 2 x NB ints: the row interchanges of each of the two
+                         // panels, LAPACK ipiv convention relative to the
+                         // panel's first row (at column c rows c and ipiv[c]
+                         // were exchanged)
 };
-constexpr int MOVED_STRIDE = 1 + 2 * MAXMOVE;
+constexpr int MOVED_STRIDE = NB;
 
 __device__ __forceinline__ double* panel_buf(char* arena, const LuSide& sd, int64_t lda, int pb)
 {
@@ -84,7 +85,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
     double* __restrict__ A = at<double>(arena, d.off_mat);
     double* __restrict__ P = panel_buf(arena, side[b], lda, pb);
     double* __restrict__ logs = at<double>(arena, side[b].off_logs);
-    int* __restrict__ moved = moved_buf(arena, side[b], pb);
+    int* __restrict__ ipiv = moved_buf(arena, side[b], pb);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 
     extern __shared__ int perm[];   // R * NT
@@ -93,10 +94,8 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
     __shared__ double prow2[2][LF], xrow2[2][LF];
     __shared__ double Uc[2][LF * LF];   // solved U rows of one chunk, [row][col]
     __shared__ double pivv[NB];         // |pivot| per panel column
-    __shared__ int nmoved;
 
     for (int r = tid; r < np; r += NT) perm[r] = r;
-    if (tid == 0) nmoved = 0;
     __syncthreads();
 
     double a[R][LF];
@@ -220,6 +219,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
                     perm[p] = t;
                 }
                 pivv[c] = gv;
+                ipiv[c] = p;
             }
             __syncthreads();
             const double u = prow[j];
@@ -259,8 +259,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
         }
         __syncthreads();
     }
-    // log|u_ii|, the panel in pivoted (logical) order, and the list of moved
-    // rows
+    // log|u_ii| and the panel in pivoted (logical) order
     if (tid < nbk) logs[k + tid] = log(pivv[tid]);
     for (int r = tid; r < np; r += NT) {
         const int64_t pr = k + perm[r];
@@ -273,16 +272,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
             for (int j = 0; j < 16; j++)
                 if (j0 + j < nbk) P[r + (j0 + j) * lda] = v[j];
         }
-        if (perm[r] != r) {
-            const int slot = atomicAdd(&nmoved, 1);
-            if (slot < MAXMOVE) {
-                moved[1 + 2 * slot] = r;
-                moved[2 + 2 * slot] = perm[r];
-            }
-        }
     }
-    __syncthreads();
-    if (tid == 0) moved[0] = min(nmoved, MAXMOVE);
 }
 
 
@@ -296,64 +286,56 @@ k_lu_panel(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const L
     lu_panel_body<R, LF, NT>(desc, arena, side, out, k, pb);
 }
 
-// Row interchanges of one column (lane) through LDS: every source row is
-// read before any row is written.  moved: count, then (dst, src) pairs of
-// rows relative to r0.
-__device__ __forceinline__ void swap_rows_col(double* __restrict__ Ac, const int* __restrict__ moved, int r0,
-                                              double* tmpS, int lane)
+// Row interchanges of one column (lane), in place: the panel's pivots as
+// nbk sequential exchanges of rows r0+c and r0+ipiv[c] (uniform over the
+// wavefront).
+__device__ __forceinline__ void swap_rows_ipiv(double* __restrict__ Ac, const int* __restrict__ ipiv, int nbk, int r0)
 {
-    const int nm = moved[0];
-#pragma unroll 8
-    for (int t = 0; t < nm; t++) tmpS[t * 64 + lane] = Ac[r0 + moved[2 + 2 * t]];
-#pragma unroll 8
-    for (int t = 0; t < nm; t++) Ac[r0 + moved[1 + 2 * t]] = tmpS[t * 64 + lane];
+    for (int c = 0; c < nbk; c++) {
+        const int p = ipiv[c];
+        if (p != c) {
+            const double t = Ac[r0 + c];
+            Ac[r0 + c] = Ac[r0 + p];
+            Ac[r0 + p] = t;
+        }
+    }
 }
 
-// Row permutation of trailing columns and U12 = L11^{-1} A12.  One lane per
-// column, 64 columns per workgroup, columns [k+NB, min(N, k+NB+climit)).
-// Rows moved by the panel's pivoting are exchanged first; the 64 x 64 tile
-// A12 (rows k..k+63) then goes through LDS: coalesced loads (lane = row), a
-// forward substitution per column with the unit lower L11 of panel buffer
-// pb (uniform loads), coalesced stores.  Dynamic LDS: MAXMOVE x 64 doubles.
+// Row interchanges and U12 = L11^{-1} A12 for the 64-row tile of rows
+// [r0, r0+64), columns [r0+cs_off, min(N, r0+cs_off+climit)), one lane per
+// column.  swaps bit 0: exchange rows by the pivots of panel 0 (relative to
+// r0); bit 1: then by those of panel 1 (relative to r0+64).  Then the
+// column's 64 entries are solved in registers with the unit lower L11 of
+// panel buffer pb (uniform loads along the columns of L11).
 __global__ void __launch_bounds__(64) k_lu_swap_trsm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                      const LuSide* __restrict__ side,
-                                                     const BlockOut* __restrict__ out, int k, int pb, int climit)
+                                                     const BlockOut* __restrict__ out, int r0, int pb, int cs_off,
+                                                     int climit, int swaps)
 {
     const int b = blockIdx.y;
     const BlockDesc& d = desc[b];
-    if (out[b].skip || k + NB >= d.N) return;
     const int N = d.N;
-    const int cend = min(N, k + NB + climit);
+    if (out[b].skip || r0 + cs_off >= N) return;
+    const int cend = min(N, r0 + cs_off + climit);
     const int64_t lda = d.lda;
-    const int c0 = k + NB + blockIdx.x * 64;
-    if (c0 >= cend) return;
-    const int lane = threadIdx.x;
-    const int col = c0 + lane;
-    const int ncol = min(64, cend - c0);
-    double* A = at<double>(arena, d.off_mat);
+    const int col = r0 + cs_off + blockIdx.x * 64 + threadIdx.x;
+    if (col >= cend) return;
+    double* Ac = at<double>(arena, d.off_mat) + (int64_t)col * lda;
     const double* P = panel_buf(arena, side[b], lda, pb);
-    extern __shared__ double tmpS[];   // [MAXMOVE][64]; then the tile [col][65]
-    if (col < cend) swap_rows_col(A + (int64_t)col * lda, moved_buf(arena, side[b], pb), k, tmpS, lane);
-    __syncthreads();
-    double* Tt = tmpS;
-#pragma unroll 16
-    for (int c = 0; c < ncol; c++) Tt[c * 65 + lane] = A[(k + lane) + (int64_t)(c0 + c) * lda];
-    __syncthreads();
-    if (col < cend) {
-        double x[NB];
+    // pivots of the panel at r0 cover min(NB, N-r0) columns; those of the
+    // panel at r0+NB, min(NB, N-r0-NB)
+    if (swaps & 1) swap_rows_ipiv(Ac, moved_buf(arena, side[b], 0), min(NB, N - r0), r0);
+    if (swaps & 2) swap_rows_ipiv(Ac, moved_buf(arena, side[b], 1), min(NB, N - r0 - NB), r0 + NB);
+    double x[NB];
 #pragma unroll
-        for (int i = 0; i < NB; i++) x[i] = Tt[lane * 65 + i];
+    for (int i = 0; i < NB; i++) x[i] = Ac[r0 + i];
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
+    for (int i = 0; i < NB; i++) {
 #pragma unroll
-            for (int j = i + 1; j < NB; j++) x[j] -= P[j + (int64_t)i * lda] * x[i];
-        }
-#pragma unroll
-        for (int i = 0; i < NB; i++) Tt[lane * 65 + i] = x[i];
+        for (int j = i + 1; j < NB; j++) x[j] -= P[j + (int64_t)i * lda] * x[i];
     }
-    __syncthreads();
-#pragma unroll 16
-    for (int c = 0; c < ncol; c++) A[(k + lane) + (int64_t)(c0 + c) * lda] = Tt[c * 65 + lane];
+#pragma unroll
+    for (int i = 0; i < NB; i++) Ac[r0 + i] = x[i];
 }
 
 // Second panel's row interchanges applied to the rows of the first panel's
@@ -366,88 +348,8 @@ __global__ void __launch_bounds__(64) k_lu_swap_panel(const BlockDesc* __restric
     const int b = blockIdx.x;
     const BlockDesc& d = desc[b];
     if (out[b].skip || k + 2 * NB >= d.N) return;
-    extern __shared__ double tmpS[];   // [MAXMOVE][64]
     double* P1 = panel_buf(arena, side[b], d.lda, 0);
-    swap_rows_col(P1 + (int64_t)threadIdx.x * d.lda, moved_buf(arena, side[b], 1), NB, tmpS, threadIdx.x);
-}
-
-// Trailing columns [k+2NB, N) of a 2NB-column step: the row interchanges of
-// both panels (the first relative to row k, then the second relative to
-// k+NB), then U12 = L11^{-1} A12 with the 2NB x 2NB unit lower
-//   L11 = [ L11(1)            0      ]
-//         [ L21(1)[0:NB)    L11(2)   ]
-// by blocks: X1 = L11(1)^{-1} B1, B2 -= L21(1) X1, X2 = L11(2)^{-1} B2.
-// The 128 x 64 tile goes through LDS as in k_lu_swap_trsm.  Dynamic LDS:
-// max(MAXMOVE x 64, 64 x 129) doubles.
-constexpr int T2P = 2 * NB + 1;
-__global__ void __launch_bounds__(64) k_lu_swap_trsm2(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                                      const LuSide* __restrict__ side,
-                                                      const BlockOut* __restrict__ out, int k)
-{
-    const int b = blockIdx.y;
-    const BlockDesc& d = desc[b];
-    if (out[b].skip || k + 2 * NB >= d.N) return;
-    const int N = d.N;
-    const int64_t lda = d.lda;
-    const int c0 = k + 2 * NB + blockIdx.x * 64;
-    if (c0 >= N) return;
-    const int lane = threadIdx.x;
-    const int col = c0 + lane;
-    const int ncol = min(64, N - c0);
-    double* A = at<double>(arena, d.off_mat);
-    const double* P1 = panel_buf(arena, side[b], lda, 0);
-    const double* P2 = panel_buf(arena, side[b], lda, 1);
-    extern __shared__ double tmpS[];
-    if (col < N) {
-        double* Ac = A + (int64_t)col * lda;
-        swap_rows_col(Ac, moved_buf(arena, side[b], 0), k, tmpS, lane);
-        swap_rows_col(Ac, moved_buf(arena, side[b], 1), k + NB, tmpS, lane);
-    }
-    __syncthreads();
-    double* Tt = tmpS;
-#pragma unroll 8
-    for (int c = 0; c < ncol; c++) {
-        Tt[c * T2P + lane] = A[(k + lane) + (int64_t)(c0 + c) * lda];
-        Tt[c * T2P + NB + lane] = A[(k + NB + lane) + (int64_t)(c0 + c) * lda];
-    }
-    __syncthreads();
-    if (col < N) {
-        double* T = Tt + lane * T2P;
-        {
-            double x[NB];
-#pragma unroll
-            for (int i = 0; i < NB; i++) x[i] = T[i];
-#pragma unroll
-            for (int i = 0; i < NB; i++) {
-#pragma unroll
-                for (int j = i + 1; j < NB; j++) x[j] -= P1[j + (int64_t)i * lda] * x[i];
-            }
-#pragma unroll
-            for (int i = 0; i < NB; i++) T[i] = x[i];
-            for (int i2 = 0; i2 < NB; i2++) {
-                double acc = T[NB + i2];
-#pragma unroll
-                for (int j = 0; j < NB; j++) acc -= P1[(NB + i2) + (int64_t)j * lda] * x[j];
-                T[NB + i2] = acc;
-            }
-        }
-        double x[NB];
-#pragma unroll
-        for (int i = 0; i < NB; i++) x[i] = T[NB + i];
-#pragma unroll
-        for (int i = 0; i < NB; i++) {
-#pragma unroll
-            for (int j = i + 1; j < NB; j++) x[j] -= P2[j + (int64_t)i * lda] * x[i];
-        }
-#pragma unroll
-        for (int i = 0; i < NB; i++) T[NB + i] = x[i];
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int c = 0; c < ncol; c++) {
-        A[(k + lane) + (int64_t)(c0 + c) * lda] = Tt[c * T2P + lane];
-        A[(k + NB + lane) + (int64_t)(c0 + c) * lda] = Tt[c * T2P + NB + lane];
-    }
+    swap_rows_ipiv(P1 + (int64_t)threadIdx.x * d.lda, moved_buf(arena, side[b], 1), min(NB, d.N - k - NB), NB);
 }
 
 // A22 -= L21 U12 with FP64 MFMA.  Workgroup tile 128x128 of A22, 4 waves
@@ -463,33 +365,54 @@ constexpr int GTP = GT + 16;     // padded LDS row
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                  const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
-                                                 int k, int mode)
+                                                 int k, int mode, int gx, int gy, int nblocks, int xcd_order)
 {
-    const int b = blockIdx.z;
+    // XCD-aware order (xcd_order = 1): workgroup i runs on XCD i % 8, so all
+    // tiles of block b go to XCD b % 8 and share its L2 for the L and U
+    // operands; xcd_order = 0: block-major order
+    const int wg = blockIdx.x, tiles = gx * gy;
+    int b, tile;
+    if (xcd_order) {
+        const int xcd = wg & 7, local = wg >> 3;
+        b = (local / tiles) * 8 + xcd;
+        tile = local % tiles;
+    } else {
+        b = wg / tiles;
+        tile = wg % tiles;
+    }
+    if (b >= nblocks) return;
     const BlockDesc& d = desc[b];
     if (out[b].skip) return;
     const int N = d.N;
     // the updated region [r0, rend) x [c0, cend) and the depth KK:
     //   mode 0: one panel at k, all of A22;
     //   mode 1: first panel of a 2NB step, the second panel's columns only;
-    //   mode 2: both panels of a 2NB step, A22 below and right of them
-    int r0, c0, cend, KK;
+    //   mode 2: both panels of a 2NB step, A22 below and right of them;
+    //   mode 3: first panel of a 2NB step, the second panel's U rows right
+    //           of the second panel
+    int r0, rend, c0, cend, KK;
     if (mode == 0) {
         r0 = c0 = k + NB;
-        cend = N;
+        rend = cend = N;
         KK = NB;
     } else if (mode == 1) {
         r0 = c0 = k + NB;
+        rend = N;
         cend = min(N, k + 2 * NB);
         KK = NB;
-    } else {
+    } else if (mode == 2) {
         r0 = c0 = k + 2 * NB;
-        cend = N;
+        rend = cend = N;
         KK = 2 * NB;
+    } else {
+        r0 = k + NB;
+        rend = min(N, k + 2 * NB);
+        c0 = k + 2 * NB;
+        cend = N;
+        KK = NB;
     }
-    const int rend = N;
-    const int m0 = r0 + blockIdx.x * GT;   // rows of the update
-    const int n0 = c0 + blockIdx.y * GT;   // columns of the update
+    const int m0 = r0 + (tile % gx) * GT;   // rows of the update
+    const int n0 = c0 + (tile / gx) * GT;   // columns of the update
     if (m0 >= rend || n0 >= cend) return;
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
@@ -627,7 +550,6 @@ extern "C" {
 
 int capsd_lu_max_dim(void) { return 24 * 1024; }
 int capsd_lu_nb(void) { return NB; }
-int capsd_lu_maxmove(void) { return MAXMOVE; }
 
 static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
                          int k, int pb, hipStream_t s)
@@ -644,43 +566,35 @@ static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuS
 #undef PANEL
 }
 
-constexpr size_t SWAP_LDS = MAXMOVE * 64 * sizeof(double);
-constexpr size_t SWAP2_LDS = (MAXMOVE * 64 > 64 * T2P ? MAXMOVE * 64 : 64 * T2P) * sizeof(double);
-
 int capsd_lu_init(void)
 {
-    hipError_t e = hipFuncSetAttribute((const void*)k_lu_swap_trsm, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       SWAP_LDS);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_lu_swap_panel, hipFuncAttributeMaxDynamicSharedMemorySize, SWAP_LDS);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_lu_swap_trsm2, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                SWAP2_LDS);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_lu_panel<24, 2, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                24 * 1024 * sizeof(int));
-    return (int)e;
+    return (int)hipFuncSetAttribute((const void*)k_lu_panel<24, 2, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    24 * 1024 * sizeof(int));
 }
 
 // Upper bound of the number of trailing-update launches of one LU.
-int capsd_lu_max_updates(int maxN) { return (maxN + NB - 1) / NB + 2; }
+int capsd_lu_max_updates(int maxN) { return 3 * ((maxN + 2 * NB - 1) / (2 * NB)) + 2; }
 
 // Whole LU of a batch (host loop over panels; all launches on stream s).
 // levels 1: NB-column steps (panel, swaps + TRSM, K = NB update);
-// levels 2: 2NB-column steps, the second panel factored after a narrow
-// update of its columns, then one K = 2NB update of the trailing matrix.
+// levels 2: 2NB-column steps: panel, swaps + TRSM and a narrow update of
+// the second panel's columns, panel, swaps of both panels + TRSM of the
+// first 64 U rows, a K = NB update of the second 64 U rows, their TRSM, one
+// K = 2NB update of the trailing matrix.
 // gemm_ev: NULL, or 2 events per update launch recorded around it;
 // gemm_info receives (k, mode) of each update launch.  Returns the number
 // of update launches.
 int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockOut* out, int nblocks, int maxN,
-                    hipStream_t s, int levels, hipEvent_t* gemm_ev, int* gemm_info)
+                    hipStream_t s, int levels, int xcd_order, hipEvent_t* gemm_ev, int* gemm_info)
 {
     const LuSide* sd = (const LuSide*)side;
+    const int nb8 = xcd_order ? (nblocks + 7) & ~7 : nblocks;
     int p = 0;
     auto update = [&](int k, int mode, int rows, int cols) {
+        const int gx = (rows + GT - 1) / GT, gy = (cols + GT - 1) / GT;
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p], s);
-        hipLaunchKernelGGL(k_lu_gemm, dim3((rows + GT - 1) / GT, (cols + GT - 1) / GT, nblocks), dim3(256), 0, s,
-                           desc, arena, sd, out, k, mode);
+        hipLaunchKernelGGL(k_lu_gemm, dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k, mode, gx, gy,
+                           nblocks, xcd_order);
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
         if (gemm_info) {
             gemm_info[2 * p] = k;
@@ -688,13 +602,16 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         }
         p++;
     };
+    auto trsm = [&](int r0, int pb, int cs_off, int cols, int swaps) {
+        hipLaunchKernelGGL(k_lu_swap_trsm, dim3((cols + 63) / 64, nblocks), dim3(64), 0, s, desc, arena, sd, out, r0,
+                           pb, cs_off, cols, swaps);
+    };
     if (levels <= 1) {
         for (int k = 0; k < maxN; k += NB) {
             launch_panel(maxN - k, desc, arena, sd, out, nblocks, k, 0, s);
             const int rest = maxN - k - NB;
             if (rest > 0) {
-                hipLaunchKernelGGL(k_lu_swap_trsm, dim3((rest + 63) / 64, nblocks), dim3(64), SWAP_LDS, s, desc,
-                                   arena, sd, out, k, 0, rest);
+                trsm(k, 0, NB, rest, 1);
                 update(k, 0, rest, rest);
             }
         }
@@ -703,15 +620,15 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
             launch_panel(maxN - k, desc, arena, sd, out, nblocks, k, 0, s);
             const int rest = maxN - k - NB;
             if (rest <= 0) continue;
-            hipLaunchKernelGGL(k_lu_swap_trsm, dim3(1, nblocks), dim3(64), SWAP_LDS, s, desc, arena, sd, out, k, 0,
-                               NB);
+            trsm(k, 0, NB, NB, 1);
             update(k, 1, rest, NB);
             launch_panel(rest, desc, arena, sd, out, nblocks, k + NB, 1, s);
             const int rest2 = rest - NB;
             if (rest2 <= 0) continue;
-            hipLaunchKernelGGL(k_lu_swap_panel, dim3(nblocks), dim3(64), SWAP_LDS, s, desc, arena, sd, out, k);
-            hipLaunchKernelGGL(k_lu_swap_trsm2, dim3((rest2 + 63) / 64, nblocks), dim3(64), SWAP2_LDS, s, desc,
-                               arena, sd, out, k);
+            hipLaunchKernelGGL(k_lu_swap_panel, dim3(nblocks), dim3(64), 0, s, desc, arena, sd, out, k);
+            trsm(k, 0, 2 * NB, rest2, 3);
+            update(k, 3, NB, rest2);
+            trsm(k + NB, 1, NB, rest2, 0);
             update(k, 2, rest2, rest2);
         }
     }

@@ -193,19 +193,27 @@ int capsx_logdetD_multi(caps_t *const *objs, const double *xi_, const int *m, in
  * (HIP events on the library's stream). */
 void caps_last_timing(caps_t *self, double ms[6]);
 
-/* Extension: free energy of the plane-sphere system at temperature T (K)
- * in units of k_B T for T > 0, with the Matsubara sum and m-sum of the
- * reference driver (src/caps.c:307-396, 722-850: m-sum stops at the first
- * m with logdetD/logdetD(m=0) < cutoff, Matsubara sum stops at
- * |v_n/v_0| < epsrel, m = 0 and n = 0 halved).  mode 0 = plain (one block
- * at a time, in F_xi order), mode 1 = batched (all m of a Matsubara term in
- * one launch, verified against the same cutoff rule).  drude_xi0: nonzero
- * adds the analytic ξ = 0 Drude term (caps_ht_drude), as the reference
- * driver does for finite ωp and γ > 0 (required: other ξ = 0 terms abort).
- * nblocks (may be NULL) receives the number of determinants computed, nused
- * (may be NULL) the number that enter the sums.  Returns E·(L+R)/(ħc) =
- * T_scaled/π Σ'_n v_n, T_scaled = 2π k_B (L+R) T/(ħc), the value the
- * reference driver prints. */
+/* Extension: free energy of the plane-sphere system at temperature T (K),
+ * with the sums of the reference driver (src/caps.c:235-396, 722-850).
+ * Every frequency's m-sum stops at the first m with logdetD/logdetD(m=0) <
+ * cutoff (that m included, m = 0 halved).
+ *   T > 0: Matsubara sum, stopped at |v_n/v_0| < epsrel, n = 0 halved;
+ *          drude_xi0 nonzero adds the analytic ξ = 0 Drude term
+ *          (caps_ht_drude), as the reference driver does for finite ωp and
+ *          γ > 0 (required: other ξ = 0 terms abort).  Returns T_scaled/π
+ *          Σ'_n v_n, T_scaled = 2π k_B (L+R) T/(ħc).
+ *   T = 0: ∫_0^∞ F_ξ(x/α) dx/(απ), α = 2L/(L+R), by the Fourier-Chebyshev
+ *          rule on [0, ∞) (the reference's --fcqs option, src/fcqs.c:107)
+ *          with levels 5, 10, ..., 2560 refined until two successive
+ *          levels agree to epsrel; the integrand is 0 above the PFA
+ *          frequency cutoff for R/L > 10 (src/caps.c:256-263).
+ * mode 0 = plain (one block at a time, in the driver's order), mode 1 =
+ * batched (all m of a frequency, or of a quadrature level, in one launch
+ * sequence, verified against the same cutoff rule), mode 2 = batched with the
+ * blocks planned by the surrogate (capsx_set_surrogate; T > 0).  nblocks
+ * (may be NULL) receives the number of determinants computed, nused (may be
+ * NULL) the number that enter the sums.  Returns E·(L+R)/(ħc), the value
+ * the reference driver prints. */
 double caps_free_energy(caps_t *self, double T, double cutoff, double epsrel, int mode,
                         int drude_xi0, long *nblocks, long *nused);
 
