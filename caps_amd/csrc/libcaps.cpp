@@ -7,6 +7,7 @@
 // context's stream, and only the per-block scalars come back.  The
 // determinant shortcuts of the reference (src/matrix.c:78-82, 144-145,
 // 494-505) are applied to those scalars on the host.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "blocks.h"
@@ -34,8 +36,9 @@ void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int
 void capsd_launch_trace(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
 void capsd_launch_mie_points(int, const double*, const int*, double, const double*, double*, hipStream_t);
 void capsd_launch_fresnel_points(int, const double*, const double*, const double*, double*, hipStream_t);
-int capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t, int, int, hipEvent_t*,
-                    int*);
+int capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t, int, int, int,
+                    hipEvent_t*, int*);
+void capsd_launch_lu_fill(const BlockDesc*, char*, int, unsigned long long, hipStream_t);
 int capsd_lu_max_updates(int);
 int capsd_lu_init(void);
 int capsd_lu_max_dim(void);
@@ -205,13 +208,98 @@ double env_double(const char* name, double dflt)
     return s ? atof(s) : dflt;
 }
 
-// LU schedule: 1 = 64-column steps, 2 = 128-column steps of two panels with
-// one K = 128 trailing update (CAPS_AMD_LU_LEVELS, default 2).
-int lu_levels() { return env_double("CAPS_AMD_LU_LEVELS", 2) >= 2 ? 2 : 1; }
-// Trailing-update tile order: 1 = all tiles of a block on one XCD, 0 =
-// block-major (CAPS_AMD_GEMM_XCD, default 0: with blocks of unequal size the
-// per-XCD order unbalances the XCDs).
-int gemm_xcd() { return env_double("CAPS_AMD_GEMM_XCD", 0) != 0 ? 1 : 0; }
+// ---- LU schedule configuration ---------------------------------------------
+// levels: 1 = 64-column steps, 2 = 128-column steps of two panels with one
+// K = 128 trailing update; streams: LU groups of blocks on separate streams;
+// xcd: trailing-update tile order (1 = all tiles of a block on one XCD);
+// tile: trailing-update workgroup tile, 128 or 64.
+struct LuCfg {
+    int levels = 2, streams = 1, xcd = 0, tile = 128;
+};
+struct TuneEntry {
+    int maxN, n;
+    LuCfg cfg;
+};
+std::mutex g_cfg_mu;
+LuCfg g_override;                 // used when g_have_override
+bool g_have_override = false;
+std::vector<TuneEntry> g_tune;    // autotuned table (lu_tuning.txt)
+std::string g_tune_path;
+
+// Table lines: "maxN n levels streams xcd tile" ('#' starts a comment).
+// A missing file empties the table (the defaults then apply).
+int load_tuning(const char* path)
+{
+    FILE* f = fopen(path, "r");
+    if (!f) {
+        std::lock_guard<std::mutex> lock(g_cfg_mu);
+        g_tune.clear();
+        g_tune_path = path;
+        return 0;
+    }
+    std::vector<TuneEntry> t;
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        TuneEntry e;
+        if (sscanf(line, "%d %d %d %d %d %d", &e.maxN, &e.n, &e.cfg.levels, &e.cfg.streams, &e.cfg.xcd,
+                   &e.cfg.tile) == 6 &&
+            e.maxN > 0 && e.n > 0 && (e.cfg.levels == 1 || e.cfg.levels == 2) && e.cfg.streams >= 1 &&
+            (e.cfg.tile == 64 || e.cfg.tile == 128))
+            t.push_back(e);
+    }
+    fclose(f);
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    g_tune = t;
+    g_tune_path = path;
+    return (int)t.size();
+}
+
+// The table next to the library (or CAPS_AMD_TUNING), read once.
+void load_default_tuning()
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char* env = getenv("CAPS_AMD_TUNING");
+        if (env) {
+            load_tuning(env);
+            return;
+        }
+        Dl_info info;
+        if (dladdr((void*)&load_tuning, &info) && info.dli_fname) {
+            std::string p(info.dli_fname);
+            const size_t slash = p.rfind('/');
+            p = (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) + "/lu_tuning.txt";
+            load_tuning(p.c_str());
+        }
+    });
+}
+
+// Configuration for a batch of n blocks of largest dimension maxN: the
+// override if set, else environment variables (CAPS_AMD_LU_LEVELS,
+// CAPS_AMD_LU_STREAMS, CAPS_AMD_GEMM_XCD, CAPS_AMD_GEMM_TILE) over the
+// nearest autotuned shape in (log maxN, log n), else the defaults.
+LuCfg lu_config(int maxN, int n)
+{
+    load_default_tuning();
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    if (g_have_override) return g_override;
+    LuCfg c;
+    double best = INFINITY;
+    for (const TuneEntry& e : g_tune) {
+        const double dx = log((double)maxN / e.maxN), dy = 0.5 * log((double)std::max(n, 1) / e.n);
+        const double d2 = dx * dx + dy * dy;
+        if (d2 < best) {
+            best = d2;
+            c = e.cfg;
+        }
+    }
+    if (getenv("CAPS_AMD_LU_LEVELS")) c.levels = env_double("CAPS_AMD_LU_LEVELS", 2) >= 2 ? 2 : 1;
+    if (getenv("CAPS_AMD_LU_STREAMS")) c.streams = std::max(1, (int)env_double("CAPS_AMD_LU_STREAMS", 1));
+    if (getenv("CAPS_AMD_GEMM_XCD")) c.xcd = env_double("CAPS_AMD_GEMM_XCD", 0) != 0 ? 1 : 0;
+    if (getenv("CAPS_AMD_GEMM_TILE")) c.tile = env_double("CAPS_AMD_GEMM_TILE", 128) <= 64 ? 64 : 128;
+    return c;
+}
 
 // Quadrature grid of the K integrals of one block (see k_K_partial):
 // x(v) = 1 + A² softplus(v)², A = 0.5/sqrt(α), step h, from v0 to the end of
@@ -337,6 +425,30 @@ size_t block_bytes_estimate(caps_t* self, int m)
            (size_t)N * 32 + 65536;
 }
 
+// The LU of blocks [0, n) (descriptors already on the device) after event
+// `ready` on the library's stream, in cfg.streams groups of blocks on their
+// own streams; the library's stream waits for all groups.  With prof, HIP
+// events around every trailing-update launch (one group).  Returns the
+// number of update launches of the first group.
+int launch_lu_groups(Ctx* c, int n, int maxN, const LuCfg& cfg, hipEvent_t ready, bool prof)
+{
+    const int groups = std::max(1, std::min(cfg.streams, std::min(MAX_LU_STREAMS, n / 4)));
+    if (groups == 1)
+        return capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, maxN, c->stream, cfg.levels, cfg.xcd,
+                               cfg.tile, prof ? c->gemm_ev.data() : nullptr, prof ? c->gemm_info.data() : nullptr);
+    int launched = 0;
+    for (int g = 0; g < groups; g++) {
+        const int b0 = (int)((int64_t)n * g / groups), b1 = (int)((int64_t)n * (g + 1) / groups);
+        HIPCHECK(hipStreamWaitEvent(c->lu_stream[g], ready, 0));
+        const int k = capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, b1 - b0, maxN,
+                                      c->lu_stream[g], cfg.levels, cfg.xcd, cfg.tile, nullptr, nullptr);
+        if (g == 0) launched = k;
+        HIPCHECK(hipEventRecord(c->lu_done[g], c->lu_stream[g]));
+        HIPCHECK(hipStreamWaitEvent(c->stream, c->lu_done[g], 0));
+    }
+    return launched;
+}
+
 // Run the pipeline for n blocks already planned; mode 0: LU path, 1: M only.
 void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts)
 {
@@ -363,25 +475,12 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
         c->gemm_info.resize(2 * nupd);
         for (size_t i = old; i < c->gemm_ev.size(); i++) HIPCHECK(hipEventCreate(&c->gemm_ev[i]));
     }
-    const int levels = lu_levels();
     int nlaunched = 0;
     if (mode == 0) {
+        LuCfg cfg = lu_config(p.maxN, n);
         // profiled runs keep one group so each update launch is timed alone
-        int groups = c->prof ? 1 : (int)env_double("CAPS_AMD_LU_STREAMS", 1);
-        groups = std::max(1, std::min(groups, std::min(MAX_LU_STREAMS, n / 4)));
-        if (groups == 1) {
-            nlaunched = capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, p.maxN, s, levels, gemm_xcd(),
-                                        c->prof ? c->gemm_ev.data() : nullptr, c->prof ? c->gemm_info.data() : nullptr);
-        } else {
-            for (int g = 0; g < groups; g++) {
-                const int b0 = (int)((int64_t)n * g / groups), b1 = (int)((int64_t)n * (g + 1) / groups);
-                HIPCHECK(hipStreamWaitEvent(c->lu_stream[g], c->ev[4], 0));
-                capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, b1 - b0, p.maxN,
-                                c->lu_stream[g], levels, gemm_xcd(), nullptr, nullptr);
-                HIPCHECK(hipEventRecord(c->lu_done[g], c->lu_stream[g]));
-                HIPCHECK(hipStreamWaitEvent(s, c->lu_done[g], 0));
-            }
-        }
+        if (c->prof) cfg.streams = 1;
+        nlaunched = launch_lu_groups(c, n, p.maxN, cfg, c->ev[4], c->prof);
     }
     HIPCHECK(hipEventRecord(c->ev[5], s));
     HIPCHECK(hipGetLastError());
@@ -849,8 +948,10 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     HIPCHECK(hipMemcpyAsync(c->d_side, &s, sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->d_out, &o, sizeof(BlockOut), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->arena, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, 1, dim, c->stream, lu_levels(), gemm_xcd(), nullptr,
-                    nullptr);
+    {
+        HIPCHECK(hipEventRecord(c->ev[6], c->stream));
+        launch_lu_groups(c, 1, dim, lu_config(dim, 1), c->ev[6], false);
+    }
     HIPCHECK(hipMemcpyAsync(&o, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     caps_free(tmp);
@@ -1311,8 +1412,7 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
         HIPCHECK(hipMemcpyAsync(c->arena + c->h_desc[b].off_mat, A + (size_t)b * dim * dim,
                                 (size_t)dim * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-    capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, dim, c->stream, lu_levels(), gemm_xcd(), nullptr,
-                    nullptr);
+    launch_lu_groups(c, n, dim, lu_config(dim, n), c->ev[0], false);
     HIPCHECK(hipEventRecord(c->ev[1], c->stream));
     HIPCHECK(hipMemcpyAsync(o.data(), c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1321,6 +1421,100 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
     for (int b = 0; b < n; b++) out[b] = o[b].logdet;
     caps_free(tmp);
     return 0;
+}
+
+// Extension (autotuning): milliseconds of one batched LU of n synthetic
+// dim x dim matrices (1 - 0.3 X/sqrt(dim), X uniform in [-1, 1)) with the
+// given schedule (a value < 0 keeps the configured one), averaged over reps;
+// the matrices are regenerated before each repetition, outside the timed
+// region.  Returns 0 on success.
+int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, int tile, double* ms_out)
+{
+    if (dim <= 0 || dim > capsd_lu_max_dim() || n <= 0 || reps <= 0) return 1;
+    caps_t* tmp = caps_init(1, 1);
+    Ctx* c = ctx_of(tmp);
+    ensure_blocks(c, n);
+    const int lda = (dim + 15) & ~15, nb = capsd_lu_nb();
+    int64_t off = 0;
+    for (int b = 0; b < n; b++) {
+        BlockDesc& d = c->h_desc[b];
+        memset(&d, 0, sizeof(d));
+        d.N = dim;
+        d.ldim = (dim + 1) / 2;
+        d.lda = lda;
+        d.off_mat = off;
+        off = align256(off + (int64_t)lda * dim * 8);
+        LuSide& sd = c->h_side[b];
+        sd.off_panel = off;
+        off = align256(off + 2 * (int64_t)lda * nb * 8);
+        sd.off_logs = off;
+        off = align256(off + (int64_t)dim * 8);
+        sd.off_moved = off;
+        off = align256(off + 2 * (int64_t)nb * 4);
+    }
+    ensure_arena(c, off);
+    HIPCHECK(hipMemcpyAsync(c->d_desc, c->h_desc, n * sizeof(BlockDesc), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->d_side, c->h_side, n * sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
+    LuCfg cfg = lu_config(dim, n);
+    if (levels > 0) cfg.levels = levels >= 2 ? 2 : 1;
+    if (streams > 0) cfg.streams = streams;
+    if (xcd >= 0) cfg.xcd = xcd ? 1 : 0;
+    if (tile > 0) cfg.tile = tile <= 64 ? 64 : 128;
+    double tot = 0;
+    for (int r = 0; r < reps; r++) {
+        HIPCHECK(hipMemsetAsync(c->d_out, 0, n * sizeof(BlockOut), c->stream));
+        capsd_launch_lu_fill(c->d_desc, c->arena, n, 1234ull + r, c->stream);
+        HIPCHECK(hipEventRecord(c->ev[0], c->stream));
+        launch_lu_groups(c, n, dim, cfg, c->ev[0], false);
+        HIPCHECK(hipEventRecord(c->ev[1], c->stream));
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        HIPCHECK(hipGetLastError());
+        float ms;
+        HIPCHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        tot += ms;
+    }
+    *ms_out = tot / reps;
+    caps_free(tmp);
+    return 0;
+}
+
+// Extension: fix the LU schedule of all later batches (levels 1|2, streams
+// >= 1, xcd 0|1, tile 64|128); levels = 0 returns to the autotuned table.
+// Returns 0 on success, 1 on invalid values.
+int capsx_lu_config(int levels, int streams, int xcd, int tile)
+{
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    if (levels == 0) {
+        g_have_override = false;
+        return 0;
+    }
+    if ((levels != 1 && levels != 2) || streams < 1 || (xcd != 0 && xcd != 1) || (tile != 64 && tile != 128))
+        return 1;
+    g_override.levels = levels;
+    g_override.streams = streams;
+    g_override.xcd = xcd;
+    g_override.tile = tile;
+    g_have_override = true;
+    return 0;
+}
+
+// Extension: replace the autotuned table by the one in `path` (lines "maxN n
+// levels streams xcd tile", '#' comments).  Returns the entries read.
+int capsx_lu_tuning_load(const char* path)
+{
+    load_default_tuning();
+    return load_tuning(path);
+}
+
+// Extension: the schedule used for n blocks of largest dimension maxN:
+// cfg = {levels, streams, xcd, tile}.
+void capsx_lu_config_get(int maxN, int n, int cfg[4])
+{
+    const LuCfg c = lu_config(maxN, n);
+    cfg[0] = c.levels;
+    cfg[1] = c.streams;
+    cfg[2] = c.xcd;
+    cfg[3] = c.tile;
 }
 
 // Extension (test hook): D = A(16x4) B(4x16) through one v_mfma_f64_16x16x4f64.

@@ -358,15 +358,16 @@ __global__ void __launch_bounds__(64) k_lu_swap_panel(const BlockDesc* __restric
 // the four k-rows a wavefront reads in one instruction fall on disjoint bank
 // halves.
 typedef double double4_t __attribute__((ext_vector_type(4)));
-constexpr int GT = 128;          // workgroup tile
 constexpr int KS = 16;           // K per LDS stage
-constexpr int GTP = GT + 16;     // padded LDS row
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                                 const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
-                                                 int k, int mode, int gx, int gy, int nblocks, int xcd_order)
+// WM x WN wavefronts of 64 x 64 each: workgroup tile (64 WM) x (64 WN).
+template <int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
+          const BlockOut* __restrict__ out, int k, int mode, int gx, int gy, int nblocks, int xcd_order)
 {
+    constexpr int NT = 64 * WM * WN, GTM = 64 * WM, GTN = 64 * WN;
+    constexpr int GTPM = GTM + 16, GTPN = GTN + 16;   // padded LDS rows
     // XCD-aware order (xcd_order = 1): workgroup i runs on XCD i % 8, so all
     // tiles of block b go to XCD b % 8 and share its L2 for the L and U
     // operands; xcd_order = 0: block-major order
@@ -411,20 +412,22 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
         cend = N;
         KK = NB;
     }
-    const int m0 = r0 + (tile % gx) * GT;   // rows of the update
-    const int n0 = c0 + (tile / gx) * GT;   // columns of the update
+    const int m0 = r0 + (tile % gx) * GTM;   // rows of the update
+    const int n0 = c0 + (tile / gx) * GTN;   // columns of the update
     if (m0 >= rend || n0 >= cend) return;
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
     const double* P1 = panel_buf(arena, side[b], lda, 0);
     const double* P2 = panel_buf(arena, side[b], lda, 1);
     // one LDS buffer: the A and B stages, reused for the C strips at the end
-    __shared__ double smem[2 * KS * GTP];
-    double(*As)[GTP] = reinterpret_cast<double(*)[GTP]>(smem);            // [kk][row]
-    double(*Bs)[GTP] = reinterpret_cast<double(*)[GTP]>(smem + KS * GTP);  // [kk][col]
+    constexpr int SMEM = KS * (GTPM + GTPN);
+    static_assert(SMEM >= WM * WN * (16 * 64 + 4), "LDS holds the epilogue strips");
+    __shared__ double smem[SMEM];
+    double(*As)[GTPM] = reinterpret_cast<double(*)[GTPM]>(smem);             // [kk][row]
+    double(*Bs)[GTPN] = reinterpret_cast<double(*)[GTPN]>(smem + KS * GTPM);  // [kk][col]
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
-    const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
+    const int wr = (wv % WM) * 64, wc = (wv / WM) * 64;
     double4_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -433,17 +436,21 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
     const int li = lane & 15, lk = lane >> 4;
     // the next stage's operands are fetched into registers while the
     // current stage runs on the MFMA units
-    constexpr int PER = (KS * GT) / 256;
-    double pa[PER], pb[PER];
+    constexpr int PERA = (KS * GTM) / NT, PERB = (KS * GTN) / NT;
+    double pa[PERA], pb[PERB];
     auto fetch = [&](int ks) {
         // L columns [ks, ks+KS) (of panel 0 or 1), in the panels' logical rows
         const double* Lp = ks < NB ? P1 + (int64_t)ks * lda - k : P2 + (int64_t)(ks - NB) * lda - (k + NB);
 #pragma unroll
-        for (int t = 0; t < PER; t++) {
-            const int idx = tid + 256 * t;
-            const int kk = idx / GT, r = idx % GT;
+        for (int t = 0; t < PERA; t++) {
+            const int idx = tid + NT * t;
+            const int kk = idx / GTM, r = idx % GTM;
             const int row = m0 + r;
             pa[t] = row < rend ? Lp[row + (int64_t)kk * lda] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < PERB; t++) {
+            const int idx = tid + NT * t;
             const int kk2 = idx % KS, c = idx / KS;
             const int colg = n0 + c;
             pb[t] = colg < cend ? A[(k + ks + kk2) + (int64_t)colg * lda] : 0.0;
@@ -453,9 +460,13 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
     for (int ks = 0; ks < KK; ks += KS) {
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < PER; t++) {
-            const int idx = tid + 256 * t;
-            As[idx / GT][idx % GT] = pa[t];
+        for (int t = 0; t < PERA; t++) {
+            const int idx = tid + NT * t;
+            As[idx / GTM][idx % GTM] = pa[t];
+        }
+#pragma unroll
+        for (int t = 0; t < PERB; t++) {
+            const int idx = tid + NT * t;
             Bs[idx % KS][idx / KS] = pb[t];
         }
         __syncthreads();
@@ -529,6 +540,26 @@ __global__ void k_lu_finalize(const BlockDesc* __restrict__ desc, char* __restri
     out[b].logdet = s;
 }
 
+// Synthetic LU input (autotuning): A = 1 - 0.3 X / sqrt(N) with X_ij uniform
+// in [-1, 1) from a counter-based hash of (seed, block, i, j), column-major.
+__global__ void k_lu_fill(const BlockDesc* __restrict__ desc, char* __restrict__ arena, unsigned long long seed)
+{
+    const BlockDesc& d = desc[blockIdx.y];
+    const int N = d.N;
+    double* A = at<double>(arena, d.off_mat);
+    const double sc = 0.3 / sqrt((double)N);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)N * N;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % N), j = (int)(e / N);
+        unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (e + 1 + ((unsigned long long)blockIdx.y << 40));
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double x = (double)(z >> 11) * 0x1p-53 * 2 - 1;
+        A[i + (int64_t)j * d.lda] = (i == j ? 1.0 : 0.0) - sc * x;
+    }
+}
+
 // MFMA operand-layout probe: D = A(16x4) B(4x16) for one wave.  A is
 // row-major 16x4, B row-major 4x16, D row-major 16x16.
 __global__ void k_mfma_probe(const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ D)
@@ -581,20 +612,28 @@ int capsd_lu_max_updates(int maxN) { return 3 * ((maxN + 2 * NB - 1) / (2 * NB))
 // the second panel's columns, panel, swaps of both panels + TRSM of the
 // first 64 U rows, a K = NB update of the second 64 U rows, their TRSM, one
 // K = 2NB update of the trailing matrix.
+// tile: 128 (update workgroups of 128 x 128, 4 wavefronts) or 64 (64 x 64,
+// one wavefront); xcd_order: tile order of the update (k_lu_gemm).
 // gemm_ev: NULL, or 2 events per update launch recorded around it;
 // gemm_info receives (k, mode) of each update launch.  Returns the number
 // of update launches.
 int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockOut* out, int nblocks, int maxN,
-                    hipStream_t s, int levels, int xcd_order, hipEvent_t* gemm_ev, int* gemm_info)
+                    hipStream_t s, int levels, int xcd_order, int tile, hipEvent_t* gemm_ev, int* gemm_info)
 {
     const LuSide* sd = (const LuSide*)side;
     const int nb8 = xcd_order ? (nblocks + 7) & ~7 : nblocks;
     int p = 0;
     auto update = [&](int k, int mode, int rows, int cols) {
-        const int gx = (rows + GT - 1) / GT, gy = (cols + GT - 1) / GT;
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p], s);
-        hipLaunchKernelGGL(k_lu_gemm, dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k, mode, gx, gy,
-                           nblocks, xcd_order);
+        if (tile == 64) {
+            const int gx = (rows + 63) / 64, gy = (cols + 63) / 64;
+            hipLaunchKernelGGL((k_lu_gemm<1, 1>), dim3(gx * gy * nb8), dim3(64), 0, s, desc, arena, sd, out, k, mode,
+                               gx, gy, nblocks, xcd_order);
+        } else {
+            const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
+            hipLaunchKernelGGL((k_lu_gemm<2, 2>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k,
+                               mode, gx, gy, nblocks, xcd_order);
+        }
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
         if (gemm_info) {
             gemm_info[2 * p] = k;
@@ -634,6 +673,11 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
     }
     hipLaunchKernelGGL(k_lu_finalize, dim3((nblocks + 63) / 64), dim3(64), 0, s, desc, arena, sd, out, nblocks);
     return p;
+}
+
+void capsd_launch_lu_fill(const BlockDesc* desc, char* arena, int nblocks, unsigned long long seed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lu_fill, dim3(256, nblocks), dim3(256), 0, s, desc, arena, seed);
 }
 
 void capsd_launch_mfma_probe(const double* A, const double* B, double* D, hipStream_t s)

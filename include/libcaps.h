@@ -188,6 +188,29 @@ int caps_logdetD_batch(caps_t *self, const double *xi_, const int *m, int n, dou
  * build one caps object per geometry, src/caps.c:869-957, at 913). */
 int capsx_logdetD_multi(caps_t *const *objs, const double *xi_, const int *m, int n, double *out);
 
+/* Extension: LU schedule of the determinant stage.  levels 1 = 64-column
+ * steps, 2 = 128-column steps of two panels with one K = 128 trailing
+ * update; streams = groups of blocks factored on separate HIP streams; xcd =
+ * trailing-update tile order (1: all tiles of a block on one XCD); tile =
+ * trailing-update workgroup tile (128 or 64).  By default the library picks,
+ * per batch, the nearest shape (largest dimension, number of blocks) of the
+ * autotuned table lu_tuning.txt next to the library (or the file named by
+ * CAPS_AMD_TUNING), written by caps_amd/autotune.py from capsx_lu_time.
+ * No counterpart in the reference (which calls LAPACK dgetrf_,
+ * src/matrix.c:555-577). */
+/* Fix the schedule for all later batches; levels = 0 returns to the table.
+ * Returns 0 on success, 1 on invalid values. */
+int capsx_lu_config(int levels, int streams, int xcd, int tile);
+/* Schedule used for n blocks of largest dimension maxN: {levels, streams,
+ * xcd, tile}. */
+void capsx_lu_config_get(int maxN, int n, int cfg[4]);
+/* Replace the table by the file at path; returns the entries read. */
+int capsx_lu_tuning_load(const char *path);
+/* Milliseconds of one batched LU of n synthetic dim x dim matrices with the
+ * given schedule (values < 0 keep the configured ones), averaged over reps.
+ * Returns 0 on success. */
+int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, int tile, double *ms);
+
 /* Extension: per-stage timing of the last batch on this caps object, in
  * milliseconds: [0] Mie, [1] K, [2] I, [3] assembly, [4] LU, [5] total
  * (HIP events on the library's stream). */

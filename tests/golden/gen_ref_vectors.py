@@ -9,6 +9,7 @@ machines without the reference.
     python tests/golden/gen_ref_vectors.py tables      # K, I and M elements
     python tests/golden/gen_ref_vectors.py free_energy # C2 free energy (~10 min, 8 cores)
     python tests/golden/gen_ref_vectors.py mixed       # 64 mixed-aspect PR blocks
+    python tests/golden/gen_ref_vectors.py t0          # T = 0 free energies (dqagi)
 """
 import ctypes
 import json
@@ -228,6 +229,56 @@ def gen_mixed(pool):
     return rows
 
 
+def gen_t0(pool):
+    """T = 0 free energies F (L+R)/(ħc) = ∫_0^∞ F_ξ(x/α) dx/(απ), α = 2L/(L+R)
+    (src/caps.c:235-266, 722-748), with the reference's own dqagi (cquadpack,
+    compiled into oracle/_ref) at epsrel 1e-10 and F_ξ the driver's m-sum
+    with sequential semantics (first m with v/v0 < cutoff included, m = 0
+    halved), the blocks evaluated 8 at a time by the reference's
+    caps_logdetD.  R/L = 10 (no PFA frequency cutoff, which needs R/L > 10)."""
+    L_ = lib()
+    L_.dqagi.restype = ctypes.c_double
+    FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
+    L_.dqagi.argtypes = [FN, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int),
+                         ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]
+    out = []
+    for tag, R, L, ldim, drude, cutoff in [("pr", 10e-6, 1e-6, 60, None, 1e-12),
+                                           ("drude", 10e-6, 1e-6, 60, drude_c2(), 1e-12)]:
+        alpha = 2 * L / (L + R)
+        nodes = []
+        t0 = time.time()
+
+        def F_xi(xi):
+            terms, m0 = [], 0
+            while True:
+                chunk = list(range(m0, m0 + 8))
+                res = pool.map(_block, [(R, L, ldim, drude, xi, m) for m in chunk], chunksize=1)
+                for m, (val, dt) in zip(chunk, res):
+                    terms.append(val)
+                    if val == 0 or val / terms[0] < cutoff:
+                        t = [terms[0] / 2] + terms[1:]
+                        return math.fsum(t), m
+                m0 += 8
+
+        def integrand(x, _):
+            v, mstop = F_xi(x / alpha)
+            nodes.append((x / alpha, mstop, v))
+            return v
+
+        cb = FN(integrand)
+        abserr, neval, ier = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        I = L_.dqagi(cb, 0.0, 1, 0.0, 1e-10, ctypes.byref(abserr), ctypes.byref(neval), ctypes.byref(ier), None)
+        F = I / alpha / math.pi
+        print("t0 %s: F=%.16g ier=%d neval=%d abserr/I=%.2e (%.0fs)" % (tag, F, ier.value, neval.value,
+                                                                     abserr.value / abs(I), time.time() - t0))
+        out.append({"kind": tag, "R": R, "L": L, "ldim": ldim, "omegap": None if drude is None else drude[0],
+                    "gamma": None if drude is None else drude[1], "cutoff": cutoff, "epsrel": 1e-10,
+                    "F": F, "integral": I, "alpha": alpha, "ier": ier.value, "neval": neval.value,
+                    "abserr": abserr.value, "samples": nodes[::max(1, len(nodes) // 40)]})
+    return out
+
+
 def main():
     what = sys.argv[1] if len(sys.argv) > 1 else "blocks"
     if what == "tables":
@@ -244,6 +295,9 @@ def main():
             elif what == "mixed":
                 data = gen_mixed(pool)
                 name = "ref_mixed_aspect.json"
+            elif what == "t0":
+                data = gen_t0(pool)
+                name = "ref_free_energy_t0.json"
             else:
                 sys.exit("unknown target " + what)
     with open(os.path.join(HERE, name), "w") as f:

@@ -266,7 +266,7 @@ def test_reference_cli_relinked(args, expect):
 
 
 def test_lu_big_panel_path(ca):
-    """N above the 256-lane panel's 6144 rows: the 1024-lane panel kernel."""
+    """N above the 512-lane panel's 6144 rows: the 1024-lane panel kernel."""
     rng = np.random.default_rng(11)
     N = 6600
     X = rng.normal(size=(N, N)) / math.sqrt(N)
@@ -285,3 +285,71 @@ def test_logdetD_reference_goldens_ldim10000(ca, row):
     c = ca.Caps(row["R"], row["L"], ldim=row["ldim"])
     v = c.logdetD(row["xi_"], row["m"])
     assert relerr(v, row["expect"]) < row["tol"], (v, row["expect"])
+
+
+T0_CASES = _ref("ref_free_energy_t0.json") if os.path.exists(os.path.join(GOLDEN, "ref_free_energy_t0.json")) else []
+
+
+@pytest.mark.parametrize("case", T0_CASES, ids=lambda c: "T0_%s" % c["kind"])
+def test_free_energy_T0_vs_reference(ca, case):
+    """T = 0 free energy at R/L = 10, ldim = 60 against the reference driver's
+    integrand integrated by the reference's dqagi at epsrel 1e-10: the
+    Fourier-Chebyshev levels of the batched form converge to the same
+    integral within 1e-8.  The integrand itself is checked at the frequencies
+    dqagi sampled (m-sum of the GPU blocks vs the reference's)."""
+    c = make(ca, case["R"], case["L"], case["ldim"], case["omegap"], case["gamma"])
+    F, computed, used = c.free_energy(0.0, cutoff=case["cutoff"], epsrel=1e-10, mode="batched")
+    print("T=0 %s: F=%.16g reference %.16g (rel %.2e), %d blocks computed, %d used" % (
+        case["kind"], F, case["F"], abs(F / case["F"] - 1), computed, used))
+    assert abs(F / case["F"] - 1) < 1e-8, (F, case["F"])
+    for xi, mstop, v in case["samples"][::4]:
+        vals = c.logdetD_batch(np.full(mstop + 1, xi), np.arange(mstop + 1))
+        got = math.fsum([vals[0] / 2] + list(vals[1:]))
+        assert abs(got - v) <= 1e-9 * abs(v) + 1e-12, (xi, got, v)
+
+
+def test_free_energy_T0_plain_equals_batched(ca):
+    c = make(ca, 10e-6, 1e-6, 40)
+    Fb, cb, ub = c.free_energy(0.0, cutoff=1e-9, epsrel=1e-6, mode="batched")
+    Fp, cp, up = c.free_energy(0.0, cutoff=1e-9, epsrel=1e-6, mode="plain")
+    assert ub == up and cp == up
+    assert abs(Fb - Fp) <= 1e-12 * abs(Fp), (Fb, Fp)
+
+
+def test_lu_schedules_agree(ca):
+    """Every LU schedule (levels, streams, tile order, tile) gives the same
+    determinants, with and without row interchanges."""
+    import itertools
+    from caps_amd import autotune as at
+    rng = np.random.default_rng(5)
+    N = 300
+    mats = [np.eye(N) - 0.3 * rng.normal(size=(N, N)) / math.sqrt(N) for _ in range(7)]
+    A = rng.normal(size=(N, N))
+    A[np.arange(N), np.arange(N)] = 0.0
+    mats.append(A)
+    mats = np.array(mats)
+    ref = [np.linalg.slogdet(M)[1] for M in mats]
+    try:
+        for cfg in itertools.product((1, 2), (1, 2), (0, 1), (64, 128)):
+            at.set_config(*cfg)
+            out = ca.lu_logdet_batch(mats)
+            for v, r in zip(out, ref):
+                assert abs(v - r) < 1e-10 * N, (cfg, v, r)
+    finally:
+        at.set_config(0)
+
+
+def test_autotune_table_and_timer(ca, tmp_path):
+    from caps_amd import autotune as at
+    p = tmp_path / "lu_tuning.txt"
+    p.write_text("# test table\n600 16 1 2 1 64\n2400 640 2 1 0 128\n")
+    try:
+        assert at.load_table(str(p)) == 2
+        assert at.config_for(600, 16) == (1, 2, 1, 64)
+        assert at.config_for(500, 12) == (1, 2, 1, 64)
+        assert at.config_for(2400, 700) == (2, 1, 0, 128)
+        ms = at.lu_time(240, 8, (2, 2, 0, 64), reps=1)
+        assert 0 < ms < 10000
+    finally:
+        shipped = os.path.join(os.path.dirname(GOLDEN), "..", "caps_amd", "lu_tuning.txt")
+        at.load_table(shipped if os.path.exists(shipped) else str(tmp_path / "none.txt"))
