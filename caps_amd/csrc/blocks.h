@@ -1,8 +1,9 @@
 // Layout of one batch of (ξ, m) round-trip-matrix blocks in HBM.
 //
-// A batch is a set of blocks of one caps object.  The host fills one
-// BlockDesc per block; every kernel takes the descriptor array and the arena
-// base pointer and finds its block through blockIdx (the batch dimension).
+// A batch is a set of blocks (of one or several caps objects).  The host
+// fills one BlockDesc per block; every kernel takes the descriptor array and
+// the arena base pointer and finds its block through blockIdx (the batch
+// dimension).
 // All per-block tables live in one device arena, at the offsets recorded in
 // the descriptor (in bytes, 256-byte aligned).
 #pragma once
@@ -38,6 +39,14 @@ struct BlockDesc {
     int64_t off_mat;    // lda * N doubles (column-major)
     int64_t off_diag;   // N doubles: diagonal of M
     int64_t off_piv;    // N ints: LU pivots
+};
+
+// Blocks whose I tables share the coefficients a_q: same m, ℓ-window and
+// polarizations (the blocks of one m at several frequencies).
+constexpr int IG_MAX = 8;
+struct IGroup {
+    int count;
+    int blk[IG_MAX];
 };
 
 // Partial K sums of one wave of quadrature nodes for one ν.

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "blocks.h"
@@ -25,13 +26,15 @@
 using capsd::BlockDesc;
 using capsd::BlockOut;
 using capsd::KPart;
+using capsd::IG_MAX;
+using capsd::IGroup;
 
 struct capsx_mlp;
 extern "C" {
 void capsx_mlp_predict(capsx_mlp* h, const float* X, int n, float* Y);
 void capsd_launch_mie(const BlockDesc*, char*, const double*, int, int, hipStream_t);
 void capsd_launch_K(const BlockDesc*, char*, const double*, int, int, int, int*, hipStream_t);
-void capsd_launch_I(const BlockDesc*, char*, const double*, int, int64_t, int*, hipStream_t);
+void capsd_launch_I(const BlockDesc*, char*, const void*, const int*, const double*, int64_t, int*, hipStream_t);
 void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int, hipStream_t);
 void capsd_launch_trace(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
 void capsd_launch_mie_points(int, const double*, const int*, double, const double*, double*, hipStream_t);
@@ -105,6 +108,8 @@ struct Ctx {
     LuSide* d_side = nullptr;
     BlockOut* d_out = nullptr;
     int* d_status = nullptr;
+    IGroup* d_groups = nullptr;
+    IGroup* h_groups = nullptr;
     int cap_blocks = 0;
     BlockDesc* h_desc = nullptr;  // pinned
     LuSide* h_side = nullptr;
@@ -158,7 +163,10 @@ void ctx_free(Ctx* c)
     if (c->h_side) hipHostFree(c->h_side);
     if (c->h_out) hipHostFree(c->h_out);
     if (c->h_status) hipHostFree(c->h_status);
+    if (c->d_groups) hipFree(c->d_groups);
+    if (c->h_groups) hipHostFree(c->h_groups);
     for (auto& e : c->ev) hipEventDestroy(e);
+    for (auto& e : c->gemm_ev) hipEventDestroy(e);
     for (int g = 0; g < MAX_LU_STREAMS; g++) {
         hipStreamDestroy(c->lu_stream[g]);
         hipEventDestroy(c->lu_done[g]);
@@ -179,8 +187,12 @@ void ensure_blocks(Ctx* c, int n)
         hipHostFree(c->h_desc);
         hipHostFree(c->h_side);
         hipHostFree(c->h_out);
+        hipFree(c->d_groups);
+        hipHostFree(c->h_groups);
     }
     HIPCHECK(hipMalloc(&c->d_desc, cap * sizeof(BlockDesc)));
+    HIPCHECK(hipMalloc(&c->d_groups, cap * sizeof(IGroup)));
+    HIPCHECK(hipHostMalloc((void**)&c->h_groups, cap * sizeof(IGroup), hipHostMallocDefault));
     HIPCHECK(hipMalloc(&c->d_side, cap * sizeof(LuSide)));
     HIPCHECK(hipMalloc(&c->d_out, cap * sizeof(BlockOut)));
     HIPCHECK(hipHostMalloc((void**)&c->h_desc, cap * sizeof(BlockDesc), hipHostMallocDefault));
@@ -331,7 +343,43 @@ struct BatchPlan {
     int n = 0, maxN = 0, max_ldim = 0, max_waves = 0, max_nK = 0;
     int64_t max_ntri = 0;
     size_t bytes = 0;
+    std::vector<IGroup> groups;   // ordered by class (see build_igroups)
+    int class_counts[8] = {};
 };
+
+// Groups of blocks that share the I coefficients a_q: same caps object, m,
+// ℓ-window and polarizations, at most IG_MAX per group.
+void build_igroups(caps_t* const* objs, const BlockDesc* desc, int n, BatchPlan& p)
+{
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; i++) idx[i] = i;
+    auto key = [&](int i) {
+        const BlockDesc& d = desc[i];
+        return std::make_tuple(1 - d.is_pr, (uintptr_t)objs[i], d.m, d.lmin, d.ldim);
+    };
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return key(x) < key(y); });
+    // classes: perfect reflector (TM only) or both polarizations, times at
+    // most 1, 2, 4 or 8 blocks; the kernel is instantiated per class
+    std::vector<IGroup> cls[8];
+    for (int i = 0; i < n;) {
+        IGroup g;
+        memset(&g, 0, sizeof(g));
+        int j = i;
+        while (j < n && j - i < IG_MAX && key(idx[j]) == key(idx[i])) {
+            g.blk[j - i] = idx[j];
+            j++;
+        }
+        g.count = j - i;
+        const int sc = g.count <= 1 ? 0 : g.count <= 2 ? 1 : g.count <= 4 ? 2 : 3;
+        cls[(desc[idx[i]].is_pr ? 0 : 4) + sc].push_back(g);
+        i = j;
+    }
+    p.groups.clear();
+    for (int k = 0; k < 8; k++) {
+        p.class_counts[k] = (int)cls[k].size();
+        p.groups.insert(p.groups.end(), cls[k].begin(), cls[k].end());
+    }
+}
 
 // Fill descriptors for blocks [0, n) and their arena offsets.
 BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const int* m, int n, int with_lu)
@@ -405,6 +453,7 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         p.max_ntri = std::max(p.max_ntri, (int64_t)d.nl * (d.nl + 1) / 2);
     }
     p.bytes = (size_t)off;
+    build_igroups(objs, c->h_desc, n, p);
     return p;
 }
 
@@ -463,7 +512,10 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     HIPCHECK(hipEventRecord(c->ev[1], s));
     capsd_launch_K(c->d_desc, c->arena, g_lfac_dev, n, p.max_waves, p.max_nK, c->d_status, s);
     HIPCHECK(hipEventRecord(c->ev[2], s));
-    capsd_launch_I(c->d_desc, c->arena, g_lfac_dev, n, p.max_ntri, c->d_status, s);
+    const int ng = (int)p.groups.size();
+    memcpy(c->h_groups, p.groups.data(), ng * sizeof(IGroup));
+    HIPCHECK(hipMemcpyAsync(c->d_groups, c->h_groups, ng * sizeof(IGroup), hipMemcpyHostToDevice, s));
+    capsd_launch_I(c->d_desc, c->arena, c->d_groups, p.class_counts, g_lfac_dev, p.max_ntri, c->d_status, s);
     HIPCHECK(hipEventRecord(c->ev[3], s));
     capsd_launch_assemble(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, mode, s);
     if (mode == 0) capsd_launch_trace(c->d_desc, c->arena, c->d_out, n, lu_mode_shortcuts, s);

@@ -235,10 +235,17 @@ __device__ __forceinline__ AlphaND alpha_nd(double p, double s2, double d2)
     return {(p2 - s2) * (p2 - d2), 4 * p2 - 1};
 }
 
+// One group: up to GM blocks that share (m, ℓ-window, polarizations) — in
+// a free energy the blocks of one m at different frequencies — so the a_q of
+// a pair are computed once and applied to every block's K table.
+template <int NPOL, int GM>
 __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                           const double* __restrict__ lfac_tab, int* __restrict__ status)
+                                           const IGroup* __restrict__ groups, const double* __restrict__ lfac_tab,
+                                           int* __restrict__ status)
 {
-    const BlockDesc& d = desc[blockIdx.y];
+    const IGroup& grp = groups[blockIdx.y];
+    const int G = grp.count;
+    const BlockDesc& d = desc[grp.blk[0]];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int nl = d.nl;
     const int64_t ntri = (int64_t)nl * (nl + 1) / 2;
@@ -249,28 +256,27 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     while ((int64_t)(i1 + 1) * (i1 + 2) / 2 <= t) i1++;
     const int i2 = (int)(t - (int64_t)i1 * (i1 + 1) / 2);
     const int l1 = d.lo + i1, l2 = d.lo + i2;
-
-    double* logI = at<double>(arena, d.off_i);                           // [2][nl*nl]
-    int8_t* sgnI = at<int8_t>(arena, d.off_i + 16 * (int64_t)nl * nl);   // [2][nl*nl]
     const int64_t a = (int64_t)i1 * nl + i2, b = (int64_t)i2 * nl + i1;
+    const int64_t nl2 = (int64_t)nl * nl;
 
     const int m_ = d.m > 0 ? d.m : 1;
     if (l2 < m_ || l2 < d.m) {
-        for (int pol = 0; pol < 2; pol++) {
-            logI[pol * nl * nl + a] = -INFINITY;
-            logI[pol * nl * nl + b] = -INFINITY;
-            sgnI[pol * nl * nl + a] = 0;
-            sgnI[pol * nl * nl + b] = 0;
+        for (int g = 0; g < G; g++) {
+            const BlockDesc& dg = desc[grp.blk[g]];
+            double* logI = at<double>(arena, dg.off_i);
+            int8_t* sgnI = at<int8_t>(arena, dg.off_i + 16 * nl2);
+            for (int pol = 0; pol < 2; pol++) {
+                logI[pol * nl2 + a] = -INFINITY;
+                logI[pol * nl2 + b] = -INFINITY;
+                sgnI[pol * nl2 + a] = 0;
+                sgnI[pol * nl2 + b] = 0;
+            }
         }
         return;
     }
 
-    const double* kmant = at<double>(arena, d.off_k);
-    const int* kexp = at<int>(arena, d.off_k + 16 * (int64_t)d.nK);
     const int mu = d.mu;
-    const int npol = d.is_pr ? 1 : 2;   // pol index 1 = TM; PR: TM only
-    const int pol0 = d.is_pr ? 1 : 0;
-
+    constexpr int POL0 = NPOL == 1 ? 1 : 0;   // pol index 1 = TM; perfect reflector: TM only
     const double n = l1, nu = l2, m = m_;
     const double n4 = l1 + l2 - 2 * m_;
     const int l12 = l1 + l2;
@@ -281,16 +287,38 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                         + lfac(lfac_tab, l12) - lfac(lfac_tab, 2 * l12) + lfac(lfac_tab, l12 - 2 * m_)
                         - lfac(lfac_tab, l1 - m_) - lfac(lfac_tab, l2 - m_);
 
+    // per block and polarization: K table, running sum (exponent-aligned to
+    // the q = 0 term), |q = 0 term|, its exponent, early-exit count
     const double thr = 8.7565107626965203e-27;  // e^{-60}
-    double sum[2] = {0, 0}, t0abs[2] = {0, 0};
-    int E0[2] = {0, 0}, cnt[2] = {0, 0};
-    bool done[2] = {false, false};
-    for (int pp = 0; pp < npol; pp++) {
-        const int pol = pol0 + pp;
-        const int j = l12 - mu;
-        E0[pol] = kexp[pol * d.nK + j];
-        sum[pol] = kmant[pol * d.nK + j];
-        t0abs[pol] = fabs(sum[pol]);
+    const double* km[GM][NPOL];
+    const int* ke[GM][NPOL];
+    double sum[GM][NPOL], t0abs[GM][NPOL];
+    int E0[GM][NPOL], cnt[GM][NPOL];
+    int ndone = 0;
+#pragma unroll
+    for (int g = 0; g < GM; g++) {
+#pragma unroll
+        for (int pp = 0; pp < NPOL; pp++) {
+            const int pol = POL0 + pp;
+            if (g < G) {
+                const BlockDesc& dg = desc[grp.blk[g]];
+                km[g][pp] = at<double>(arena, dg.off_k) + pol * dg.nK;
+                ke[g][pp] = at<int>(arena, dg.off_k + 16 * (int64_t)dg.nK) + pol * dg.nK;
+                const int j = l12 - mu;
+                E0[g][pp] = ke[g][pp][j];
+                sum[g][pp] = km[g][pp][j];
+                t0abs[g][pp] = fabs(sum[g][pp]);
+                cnt[g][pp] = 0;
+            } else {
+                km[g][pp] = nullptr;
+                ke[g][pp] = nullptr;
+                E0[g][pp] = 0;
+                sum[g][pp] = 0;
+                t0abs[g][pp] = 0;
+                cnt[g][pp] = 3;   // absent: counts as done
+                ndone++;
+            }
+        }
     }
 
     double aqm2 = 1, aqm1 = 0, aq = 1;  // a_{q-2}, a_{q-1}, a_q
@@ -299,7 +327,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     const double s2 = (n + nu + 1) * (n + nu + 1), d2 = (n - nu) * (n - nu);
     AlphaND al_next3 = {0, 1}, al_next4 = {0, 1};
     bool have_al = false;
-    for (int q = 1; q <= qmax; q++) {
+    for (int q = 1; q <= qmax && ndone < GM * NPOL; q++) {
         if (q == 1) {
             aq = (n + nu - 1.5) * (1 - (2 * n + 2 * nu - 1) / (n4 * (n4 - 1))
                  * ((m - n) * (m - n + 1) / (2 * n - 1) + (m - nu) * (m - nu + 1) / (2 * nu - 1)));
@@ -342,12 +370,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 aq = (pq + 1) * (p2 + 2) * al2.N * al1.D * aqm1 / ((pq + 2) * (p1 + 1) * al1.N * al2.D);
             }
             const double fa = fabs(aq);
-            if (fa > 0x1p332) {
-                const int e = ilogb(aq);
-                aq = ldexp(aq, -e);
-                aqm1 = ldexp(aqm1, -e);
-                S += e;
-            } else if (fa < 0x1p-332 && fa > 0) {
+            if (fa > 0x1p332 || (fa < 0x1p-332 && fa > 0)) {
                 const int e = ilogb(aq);
                 aq = ldexp(aq, -e);
                 aqm1 = ldexp(aqm1, -e);
@@ -355,39 +378,47 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
             }
         }
         const int j = l12 - 2 * q - mu;
-        bool all_done = true;
-        for (int pp = 0; pp < npol; pp++) {
-            const int pol = pol0 + pp;
-            if (done[pol]) continue;
-            const double term = ldexp(aq * kmant[pol * d.nK + j], S + kexp[pol * d.nK + j] - E0[pol]);
-            sum[pol] += term;
-            if (q >= 3) {
-                if (fabs(term) < thr * t0abs[pol]) {
-                    if (++cnt[pol] >= 3) done[pol] = true;
-                } else
-                    cnt[pol] = 0;
+#pragma unroll
+        for (int g = 0; g < GM; g++) {
+#pragma unroll
+            for (int pp = 0; pp < NPOL; pp++) {
+                if (cnt[g][pp] >= 3) continue;
+                const double term = ldexp(aq * km[g][pp][j], S + ke[g][pp][j] - E0[g][pp]);
+                sum[g][pp] += term;
+                if (q >= 3) {
+                    if (fabs(term) < thr * t0abs[g][pp]) {
+                        if (++cnt[g][pp] >= 3) ndone++;
+                    } else
+                        cnt[g][pp] = 0;
+                }
             }
-            all_done = all_done && done[pol];
         }
-        if (all_done) break;
     }
 
-    for (int pp = 0; pp < npol; pp++) {
-        const int pol = pol0 + pp;
-        const double s = sum[pol];
-        const double li = log_a0 + log(fabs(s)) + (double)E0[pol] * LOG2;
-        if (!isfinite(li)) atomicOr(status, 2);
-        const int8_t sg = (s > 0) - (s < 0);
-        logI[pol * nl * nl + a] = li;
-        logI[pol * nl * nl + b] = li;
-        sgnI[pol * nl * nl + a] = sg;
-        sgnI[pol * nl * nl + b] = sg;
-    }
-    if (d.is_pr) {
-        logI[a] = logI[nl * nl + a];
-        logI[b] = logI[nl * nl + b];
-        sgnI[a] = -1;
-        sgnI[b] = -1;
+#pragma unroll
+    for (int g = 0; g < GM; g++) {
+        if (g >= G) continue;
+        const BlockDesc& dg = desc[grp.blk[g]];
+        double* logI = at<double>(arena, dg.off_i);                     // [2][nl*nl]
+        int8_t* sgnI = at<int8_t>(arena, dg.off_i + 16 * nl2);         // [2][nl*nl]
+#pragma unroll
+        for (int pp = 0; pp < NPOL; pp++) {
+            const int pol = POL0 + pp;
+            const double sv = sum[g][pp];
+            const double li = log_a0 + log(fabs(sv)) + (double)E0[g][pp] * LOG2;
+            if (!isfinite(li)) atomicOr(status, 2);
+            const int8_t sg = (sv > 0) - (sv < 0);
+            logI[pol * nl2 + a] = li;
+            logI[pol * nl2 + b] = li;
+            sgnI[pol * nl2 + a] = sg;
+            sgnI[pol * nl2 + b] = sg;
+            if (NPOL == 1) {
+                logI[a] = li;
+                logI[b] = li;
+                sgnI[a] = -1;
+                sgnI[b] = -1;
+            }
+        }
     }
 }
 
@@ -575,11 +606,23 @@ void capsd_launch_K(const BlockDesc* desc, char* arena, const double* lfac_tab, 
     hipLaunchKernelGGL(k_K_combine, dim3((max_nK + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, status);
 }
 
-void capsd_launch_I(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int64_t max_ntri,
-                    int* status, hipStream_t s)
+// I tables of the batch's groups, ordered by class: perfect reflectors
+// (TM only) with at most 1, 2, 4, 8 blocks per group, then both
+// polarizations likewise; counts[8] holds the groups per class.
+void capsd_launch_I(const BlockDesc* desc, char* arena, const void* groups, const int* counts,
+                    const double* lfac_tab, int64_t max_ntri, int* status, hipStream_t s)
 {
-    dim3 g((unsigned)((max_ntri + 255) / 256), nblocks);
-    hipLaunchKernelGGL(k_I, g, dim3(256), 0, s, desc, arena, lfac_tab, status);
+    const IGroup* gr = (const IGroup*)groups;
+    const unsigned gx = (unsigned)((max_ntri + 255) / 256);
+    int off = 0;
+#define KI(NP, GMV, CLS)                                                                                    \
+    if (counts[CLS] > 0)                                                                                   \
+        hipLaunchKernelGGL((k_I<NP, GMV>), dim3(gx, counts[CLS]), dim3(256), 0, s, desc, arena, gr + off,  \
+                           lfac_tab, status);                                                              \
+    off += counts[CLS];
+    KI(1, 1, 0) KI(1, 2, 1) KI(1, 4, 2) KI(1, 8, 3)
+    KI(2, 1, 4) KI(2, 2, 5) KI(2, 4, 6) KI(2, 8, 7)
+#undef KI
 }
 
 void capsd_launch_assemble(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int max_ldim,
