@@ -106,6 +106,16 @@ def mixed_aspect(ca, steps):
     dt = (time.perf_counter() - t0) / steps
     st = objs[0].stats(reset=True)
     lu_s = st["stage_ms"]["lu"] / 1e3
+    # the same batch captured once into a HIP graph and replayed
+    objs[0].set_graph(True)
+    out_g = ca.logdetD_multi(objs, xs, ms)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out_g = ca.logdetD_multi(objs, xs, ms)
+    dt_graph = (time.perf_counter() - t0) / steps
+    objs[0].set_graph(False)
+    objs[0].reset_stats()
+    same = bool((out_g == out).all())
     err = None
     ref = os.path.join(ROOT, "tests", "golden", "ref_mixed_aspect.json")
     if os.path.exists(ref):
@@ -119,6 +129,8 @@ def mixed_aspect(ca, steps):
     return {"workload": "64 perfect-reflector blocks, R/L log-U[50,500], ldim=ceil(6 R/L), xi_ U[0.1,300], "
                         "m U[0,ldim/4], seed 0, one batch",
             "blocks_per_s": len(blocks) / dt, "seconds_per_batch": dt,
+            "graph_replay": {"blocks_per_s": len(blocks) / dt_graph, "seconds_per_batch": dt_graph,
+                             "identical_to_eager": same},
             "dims": [min(2 * b["ldim"] for b in blocks), max(2 * b["ldim"] for b in blocks)],
             "stage_ms_per_batch": {k: v / steps for k, v in st["stage_ms"].items()},
             "lu_tflops": st["lu_flops"] / lu_s / 1e12 if lu_s > 0 else None,

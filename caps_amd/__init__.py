@@ -69,6 +69,7 @@ def lib():
             "capsx_M_copy": (i, [p, dp]),
             "capsx_stats": (None, [p, dp]), "capsx_stats_reset": (None, [p]),
             "capsx_profile": (None, [p, i]),
+            "capsx_set_graph": (None, [p, i]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -164,6 +165,11 @@ class Caps:
     def profile(self, on=True):
         """Time every launch of the LU's trailing-update kernel with HIP events."""
         lib().capsx_profile(self.ptr, 1 if on else 0)
+
+    def set_graph(self, on=True):
+        """Capture each batch shape into a HIP graph and replay it (on), or
+        launch eagerly (off, the default)."""
+        lib().capsx_set_graph(self.ptr, 1 if on else 0)
 
     def reset_stats(self):
         lib().capsx_stats_reset(self.ptr)

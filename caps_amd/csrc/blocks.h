@@ -32,10 +32,12 @@ struct BlockDesc {
     int lda;
     int pad_;
     // byte offsets into the arena
-    int64_t off_mie;    // 4 * ldim doubles: log|a_l|, log|b_l|, ea(l), eb(l)
+    int64_t off_mie;    // 4 * ldim doubles: log|a_l|, log|b_l|, then the mantissas
+                        // of e^{ea(l)}, e^{eb(l)}; then 2 * ldim int exponents
     int64_t off_kpart;  // n_waves * nK * KPart
     int64_t off_k;      // 2 pols * nK * (double mant, int exp, double lnK)
-    int64_t off_i;      // 2 pols * nl*nl doubles, then 2 pols * nl*nl int8 signs
+    int64_t off_i;      // I = mant * 2^exp: 2 pols * nl*nl signed double mantissas,
+                        // then 2 pols * nl*nl int exponents
     int64_t off_mat;    // lda * N doubles (column-major)
     int64_t off_diag;   // N doubles: diagonal of M
     int64_t off_piv;    // N ints: LU pivots

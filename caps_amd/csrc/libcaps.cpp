@@ -122,6 +122,9 @@ struct Ctx {
     double acc[16] = {};
     // per-launch timing of the LU's trailing-update kernel (capsx_profile)
     int prof = 0;
+    // captured launch sequences of recent batch shapes (capsx_set_graph)
+    int graph = 0;
+    std::vector<std::pair<std::vector<char>, hipGraphExec_t>> graphs;
     std::vector<hipEvent_t> gemm_ev;
     std::vector<int> gemm_info;
     // planner surrogate (capsx_set_surrogate) and the material features it
@@ -167,6 +170,7 @@ void ctx_free(Ctx* c)
     if (c->h_groups) hipHostFree(c->h_groups);
     for (auto& e : c->ev) hipEventDestroy(e);
     for (auto& e : c->gemm_ev) hipEventDestroy(e);
+    for (auto& g : c->graphs) hipGraphExecDestroy(g.second);
     for (int g = 0; g < MAX_LU_STREAMS; g++) {
         hipStreamDestroy(c->lu_stream[g]);
         hipEventDestroy(c->lu_done[g]);
@@ -423,13 +427,13 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         d.nK = d.nu_max - d.mu + 1;
         grid_for(d);
         d.off_mie = off;
-        off = align256(off + 4 * (int64_t)ldim * 8);
+        off = align256(off + 4 * (int64_t)ldim * 8 + 2 * (int64_t)ldim * 4);
         d.off_kpart = off;
         off = align256(off + (int64_t)d.n_waves * d.nK * sizeof(KPart));
         d.off_k = off;
         off = align256(off + 2 * (int64_t)d.nK * 24);
         d.off_i = off;
-        off = align256(off + 2 * (int64_t)d.nl * d.nl * 9);
+        off = align256(off + 2 * (int64_t)d.nl * d.nl * 12);
         d.off_mat = off;
         off = align256(off + (int64_t)lda * N * 8);
         d.off_diag = off;
@@ -470,7 +474,7 @@ size_t block_bytes_estimate(caps_t* self, int m)
     size_t lmin, lmax;
     caps_estimate_lminmax(self, m, &lmin, &lmax);
     const size_t nK = 2 * (lmin + ldim) + 4;
-    return (size_t)2 * (ldim + 2) * (ldim + 2) * 9 + lda * N * 8 + lda * 128 * 8 + nK * (24 * 2 + 24 * 64) +
+    return (size_t)2 * (ldim + 2) * (ldim + 2) * 12 + lda * N * 8 + lda * 128 * 8 + nK * (24 * 2 + 24 * 64) +
            (size_t)N * 32 + 65536;
 }
 
@@ -499,6 +503,73 @@ int launch_lu_groups(Ctx* c, int n, int maxN, const LuCfg& cfg, hipEvent_t ready
 }
 
 // Run the pipeline for n blocks already planned; mode 0: LU path, 1: M only.
+// The kernels of one planned batch on the library's stream (and the LU
+// streams); stage events between the stages when `events`.  Returns the
+// number of trailing-update launches of the first LU group.
+int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, bool events)
+{
+    const int n = p.n;
+    hipStream_t s = c->stream;
+    if (events) HIPCHECK(hipEventRecord(c->ev[0], s));
+    capsd_launch_mie(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, s);
+    if (events) HIPCHECK(hipEventRecord(c->ev[1], s));
+    capsd_launch_K(c->d_desc, c->arena, g_lfac_dev, n, p.max_waves, p.max_nK, c->d_status, s);
+    if (events) HIPCHECK(hipEventRecord(c->ev[2], s));
+    capsd_launch_I(c->d_desc, c->arena, c->d_groups, p.class_counts, g_lfac_dev, p.max_ntri, c->d_status, s);
+    if (events) HIPCHECK(hipEventRecord(c->ev[3], s));
+    capsd_launch_assemble(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, mode, s);
+    if (mode == 0) capsd_launch_trace(c->d_desc, c->arena, c->d_out, n, lu_mode_shortcuts, s);
+    // also the fork point of the LU stream groups
+    HIPCHECK(hipEventRecord(c->ev[4], s));
+    int nlaunched = 0;
+    if (mode == 0) {
+        LuCfg cfg = lu_config(p.maxN, n);
+        // profiled runs keep one group so each update launch is timed alone
+        if (c->prof) cfg.streams = 1;
+        nlaunched = launch_lu_groups(c, n, p.maxN, cfg, c->ev[4], c->prof);
+    }
+    return nlaunched;
+}
+
+// Identity of a captured batch: everything the launch sequence depends on
+// (grid shapes, group classes, LU schedule, buffers).  Descriptor contents
+// are read by the kernels at run time and are not part of it.
+std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts)
+{
+    const LuCfg cfg = lu_config(p.maxN, p.n);
+    struct {
+        int n, maxN, max_ldim, max_waves, max_nK, mode, shortcuts;
+        int64_t max_ntri;
+        int cls[8];
+        int levels, streams, xcd, tile;
+        const void *arena, *desc, *side, *out, *groups;
+    } k;
+    memset(&k, 0, sizeof(k));
+    k.n = p.n;
+    k.maxN = p.maxN;
+    k.max_ldim = p.max_ldim;
+    k.max_waves = p.max_waves;
+    k.max_nK = p.max_nK;
+    k.mode = mode;
+    k.shortcuts = lu_mode_shortcuts;
+    k.max_ntri = p.max_ntri;
+    memcpy(k.cls, p.class_counts, sizeof(k.cls));
+    k.levels = cfg.levels;
+    k.streams = cfg.streams;
+    k.xcd = cfg.xcd;
+    k.tile = cfg.tile;
+    k.arena = c->arena;
+    k.desc = c->d_desc;
+    k.side = c->d_side;
+    k.out = c->d_out;
+    k.groups = c->d_groups;
+    const char* b = reinterpret_cast<const char*>(&k);
+    return std::vector<char>(b, b + sizeof(k));
+}
+
+// Run the pipeline for n blocks already planned; mode 0: LU path, 1: M only.
+// With graph mode on, each batch shape is captured once into a HIP graph and
+// replayed; descriptor uploads and the status reset stay outside the graph.
 void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts)
 {
     const int n = p.n;
@@ -506,20 +577,10 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     ensure_arena(c, p.bytes);
     HIPCHECK(hipMemcpyAsync(c->d_desc, c->h_desc, n * sizeof(BlockDesc), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemcpyAsync(c->d_side, c->h_side, n * sizeof(LuSide), hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemsetAsync(c->d_status, 0, sizeof(int), s));
-    HIPCHECK(hipEventRecord(c->ev[0], s));
-    capsd_launch_mie(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, s);
-    HIPCHECK(hipEventRecord(c->ev[1], s));
-    capsd_launch_K(c->d_desc, c->arena, g_lfac_dev, n, p.max_waves, p.max_nK, c->d_status, s);
-    HIPCHECK(hipEventRecord(c->ev[2], s));
     const int ng = (int)p.groups.size();
     memcpy(c->h_groups, p.groups.data(), ng * sizeof(IGroup));
     HIPCHECK(hipMemcpyAsync(c->d_groups, c->h_groups, ng * sizeof(IGroup), hipMemcpyHostToDevice, s));
-    capsd_launch_I(c->d_desc, c->arena, c->d_groups, p.class_counts, g_lfac_dev, p.max_ntri, c->d_status, s);
-    HIPCHECK(hipEventRecord(c->ev[3], s));
-    capsd_launch_assemble(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, mode, s);
-    if (mode == 0) capsd_launch_trace(c->d_desc, c->arena, c->d_out, n, lu_mode_shortcuts, s);
-    HIPCHECK(hipEventRecord(c->ev[4], s));
+    HIPCHECK(hipMemsetAsync(c->d_status, 0, sizeof(int), s));
     const int nupd = capsd_lu_max_updates(p.maxN);
     if (c->prof && (int)c->gemm_ev.size() < 2 * nupd) {
         const size_t old = c->gemm_ev.size();
@@ -528,11 +589,29 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
         for (size_t i = old; i < c->gemm_ev.size(); i++) HIPCHECK(hipEventCreate(&c->gemm_ev[i]));
     }
     int nlaunched = 0;
-    if (mode == 0) {
-        LuCfg cfg = lu_config(p.maxN, n);
-        // profiled runs keep one group so each update launch is timed alone
-        if (c->prof) cfg.streams = 1;
-        nlaunched = launch_lu_groups(c, n, p.maxN, cfg, c->ev[4], c->prof);
+    const bool use_graph = c->graph && !c->prof;
+    if (use_graph) {
+        const std::vector<char> key = graph_key(c, p, mode, lu_mode_shortcuts);
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : c->graphs)
+            if (g.first == key) exec = g.second;
+        if (!exec) {
+            hipGraph_t graph;
+            HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            enqueue_batch(c, p, mode, lu_mode_shortcuts, false);
+            HIPCHECK(hipStreamEndCapture(s, &graph));
+            HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            HIPCHECK(hipGraphDestroy(graph));
+            if (c->graphs.size() >= 8) {
+                HIPCHECK(hipGraphExecDestroy(c->graphs.front().second));
+                c->graphs.erase(c->graphs.begin());
+            }
+            c->graphs.push_back({key, exec});
+        }
+        HIPCHECK(hipEventRecord(c->ev[0], s));
+        HIPCHECK(hipGraphLaunch(exec, s));
+    } else {
+        nlaunched = enqueue_batch(c, p, mode, lu_mode_shortcuts, true);
     }
     HIPCHECK(hipEventRecord(c->ev[5], s));
     HIPCHECK(hipGetLastError());
@@ -540,9 +619,14 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     HIPCHECK(hipMemcpyAsync(c->h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
     float ms;
-    for (int i = 0; i < 5; i++) {
-        HIPCHECK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
-        c->timing[i] = ms;
+    if (use_graph) {
+        // a replayed graph is timed as a whole (no stage events inside)
+        for (int i = 0; i < 5; i++) c->timing[i] = 0;
+    } else {
+        for (int i = 0; i < 5; i++) {
+            HIPCHECK(hipEventElapsedTime(&ms, c->ev[i], c->ev[i + 1]));
+            c->timing[i] = ms;
+        }
     }
     HIPCHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[5]));
     c->timing[5] = ms;
@@ -1530,6 +1614,21 @@ int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, in
     return 0;
 }
 
+// Extension: capture each batch shape of this object into a HIP graph and
+// replay it for later batches of the same shape (on != 0), or launch
+// eagerly (0, the default).  Stage timings of replayed batches are 0 (the
+// total is measured).
+void capsx_set_graph(caps_t* self, int on)
+{
+    Ctx* c = ctx_of(self);
+    std::lock_guard<std::mutex> lock(c->mu);
+    c->graph = on ? 1 : 0;
+    if (!on) {
+        for (auto& g : c->graphs) HIPCHECK(hipGraphExecDestroy(g.second));
+        c->graphs.clear();
+    }
+}
+
 // Extension: fix the LU schedule of all later batches (levels 1|2, streams
 // >= 1, xcd 0|1, tile 64|128); levels = 0 returns to the autotuned table.
 // Returns 0 on success, 1 on invalid values.
@@ -1589,7 +1688,7 @@ int capsx_mfma_probe(const double* A, const double* B, double* D)
 }
 
 // Extension (test hook): the K table (log|K_ν^p|, ν = mu..nu_max) and the I
-// table (log|I|, sign) of one block, as the kernels compute them.
+// table (log|I|, sign) of one block, from the values the kernels compute.
 int capsx_block_tables(caps_t* self, double xi_, int m, int* mu, int* nK, double* lnK /* 2*nK */, int* lo,
                        int* nl, double* logI /* 2*nl*nl */, signed char* sgnI /* 2*nl*nl */, int cap_nK, int cap_nl)
 {
@@ -1609,9 +1708,17 @@ int capsx_block_tables(caps_t* self, double xi_, int m, int* mu, int* nK, double
                            hipMemcpyDeviceToHost));
     }
     if (logI) {
-        HIPCHECK(hipMemcpy(logI, c->arena + d.off_i, 2 * (size_t)d.nl * d.nl * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(sgnI, c->arena + d.off_i + 16 * (int64_t)d.nl * d.nl, 2 * (size_t)d.nl * d.nl,
+        // the table holds I = mant 2^exp; returned as log|I| and sign
+        const size_t nn = 2 * (size_t)d.nl * d.nl;
+        std::vector<int> ex(nn);
+        HIPCHECK(hipMemcpy(logI, c->arena + d.off_i, nn * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(ex.data(), c->arena + d.off_i + 16 * (int64_t)d.nl * d.nl, nn * sizeof(int),
                            hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nn; i++) {
+            const double mt = logI[i];
+            sgnI[i] = (signed char)((mt > 0) - (mt < 0));
+            logI[i] = mt == 0 ? -INFINITY : log(fabs(mt)) + ex[i] * M_LN2;
+        }
     }
     return 0;
 }

@@ -25,7 +25,8 @@ __device__ __forceinline__ T* at(char* base, int64_t off) { return reinterpret_c
 // prefactors: log Λ_{l1 l2} = (g(l1) + g(l2))/2 with
 // g(l) = log(2l+1) - log l - log(l+1) + log(l-m)! - log(l+m)! (the
 // separable form of src/libcaps.c:45-48), so ea(l) = (log|a_l| + g(l))/2 and
-// eb(l) = (log|b_l| + g(l))/2 give every element prefactor as a sum of two.
+// eb(l) = (log|b_l| + g(l))/2 give every element prefactor as a product of
+// two factors e^{ea}, e^{eb}, stored as mantissa × 2^exponent.
 __global__ void __launch_bounds__(256) k_mie(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                              const double* __restrict__ lfac_tab)
 {
@@ -39,8 +40,14 @@ __global__ void __launch_bounds__(256) k_mie(const BlockDesc* __restrict__ desc,
     const double g = logi(2 * l + 1) - logi(l) - logi(l + 1) + lfac(lfac_tab, l - m) - lfac(lfac_tab, l + m);
     al[i] = lna;
     al[d.ldim + i] = lnb;
-    al[2 * d.ldim + i] = (lna + g) / 2;
-    al[3 * d.ldim + i] = (lnb + g) / 2;
+    int* ex = reinterpret_cast<int*>(al + 4 * d.ldim);
+    // e^{ea}, e^{eb} as mantissa in [1, 2) times 2^exponent
+    const double xa = (lna + g) / 2 * (1 / LOG2), xb = (lnb + g) / 2 * (1 / LOG2);
+    const double fa = floor(xa), fb = floor(xb);
+    al[2 * d.ldim + i] = exp2(xa - fa);
+    al[3 * d.ldim + i] = exp2(xb - fb);
+    ex[i] = (int)fa;
+    ex[d.ldim + i] = (int)fb;
 }
 
 // Mie coefficients at arbitrary (ξ, l) points (scalar API caps_mie).
@@ -263,13 +270,13 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     if (l2 < m_ || l2 < d.m) {
         for (int g = 0; g < G; g++) {
             const BlockDesc& dg = desc[grp.blk[g]];
-            double* logI = at<double>(arena, dg.off_i);
-            int8_t* sgnI = at<int8_t>(arena, dg.off_i + 16 * nl2);
+            double* Im = at<double>(arena, dg.off_i);
+            int* Ie = at<int>(arena, dg.off_i + 16 * nl2);
             for (int pol = 0; pol < 2; pol++) {
-                logI[pol * nl2 + a] = -INFINITY;
-                logI[pol * nl2 + b] = -INFINITY;
-                sgnI[pol * nl2 + a] = 0;
-                sgnI[pol * nl2 + b] = 0;
+                Im[pol * nl2 + a] = 0;
+                Im[pol * nl2 + b] = 0;
+                Ie[pol * nl2 + a] = 0;
+                Ie[pol * nl2 + b] = 0;
             }
         }
         return;
@@ -395,113 +402,146 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
         }
     }
 
+    // I = e^{log a0} · sum · 2^{E0}, stored as a signed mantissa times 2^exp
+    const double la2 = log_a0 * (1 / LOG2);
+    const double fl = floor(la2);
+    const double a0m = exp2(la2 - fl);
+    const int a0e = (int)fl;
 #pragma unroll
     for (int g = 0; g < GM; g++) {
         if (g >= G) continue;
         const BlockDesc& dg = desc[grp.blk[g]];
-        double* logI = at<double>(arena, dg.off_i);                     // [2][nl*nl]
-        int8_t* sgnI = at<int8_t>(arena, dg.off_i + 16 * nl2);         // [2][nl*nl]
+        double* Im = at<double>(arena, dg.off_i);                 // [2][nl*nl]
+        int* Ie = at<int>(arena, dg.off_i + 16 * nl2);            // [2][nl*nl]
 #pragma unroll
         for (int pp = 0; pp < NPOL; pp++) {
             const int pol = POL0 + pp;
             const double sv = sum[g][pp];
-            const double li = log_a0 + log(fabs(sv)) + (double)E0[g][pp] * LOG2;
-            if (!isfinite(li)) atomicOr(status, 2);
-            const int8_t sg = (sv > 0) - (sv < 0);
-            logI[pol * nl2 + a] = li;
-            logI[pol * nl2 + b] = li;
-            sgnI[pol * nl2 + a] = sg;
-            sgnI[pol * nl2 + b] = sg;
+            if (!(sv != 0) || !isfinite(sv)) atomicOr(status, 2);
+            int e2;
+            const double mt = frexp(sv * a0m, &e2);
+            const int ex = e2 + a0e + E0[g][pp];
+            Im[pol * nl2 + a] = mt;
+            Im[pol * nl2 + b] = mt;
+            Ie[pol * nl2 + a] = ex;
+            Ie[pol * nl2 + b] = ex;
             if (NPOL == 1) {
-                logI[a] = li;
-                logI[b] = li;
-                sgnI[a] = -1;
-                sgnI[b] = -1;
+                Im[a] = -fabs(mt);
+                Im[b] = -fabs(mt);
+                Ie[a] = ex;
+                Ie[b] = ex;
             }
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Matrix assembly: one lane per 2x2 (E,M) tile (l1, l2).  Writes
+// Matrix assembly: one lane per 2x2 (E,M) tile (l1, l2), 16 x 16 tiles per
+// workgroup.  Writes
 // mode 0: 1 - M (the matrix the LU factors), mode 1: M, column-major with
 // leading dimension lda; row i = 2(l1-lmin) + p1, column j = 2(l2-lmin) + p2
 // (the interleaved index map of src/libcaps.c:717-743).  Elements follow
 // src/libcaps.c:761-831 and A/B/C/D src/integration.c:762-888.
 
-struct LogVal {
-    double v;
-    int s;
+// I_{l1 l2}^pol as mantissa × 2^exponent (0 below l = m).  The table is
+// symmetric; it is read as I[l2][l1] so that lanes with consecutive l1 read
+// consecutive addresses.
+struct LinVal {
+    double m;
+    int e;
 };
 
-__device__ __forceinline__ LogVal getI(const double* logI, const int8_t* sgnI, const BlockDesc& d, int pol, int l1, int l2)
+__device__ __forceinline__ LinVal getI(const double* Im, const int* Ie, const BlockDesc& d, int pol, int l1, int l2)
 {
-    if (l1 < d.m || l2 < d.m) return {-INFINITY, 0};
-    const int64_t idx = (int64_t)pol * d.nl * d.nl + (int64_t)(l1 - d.lo) * d.nl + (l2 - d.lo);
-    return {logI[idx], (int)sgnI[idx]};
+    if (l1 < d.m || l2 < d.m) return {0.0, 0};
+    const int64_t idx = (int64_t)pol * d.nl * d.nl + (int64_t)(l2 - d.lo) * d.nl + (l1 - d.lo);
+    return {Im[idx], Ie[idx]};
 }
 
-// Element contributions in linear form: for a prefactor p (log of the Mie
-// and Λ factors), X·e^p is summed directly from the I values,
-//   A·e^p = m² s e^{I + p},
-//   B·e^p = Σ_k c_k s_k e^{I_k + p}   (4 terms; m = 0: I itself),
-//   C·e^p = m [-(l2+1)(l2+m)/(2l2+1) s_1 e^{I_1+p} + l2(l2-m+1)/(2l2+1) s_2 e^{I_2+p}],
+// Element contributions in linear form: with the prefactor e^p = pm · 2^pe
+// (Mie and Λ factors), X·e^p is summed directly from the I values,
+//   A·e^p = m² I e^p,
+//   B·e^p = Σ_k c_k I_k e^p   (4 terms; m = 0: I itself),
+//   C·e^p = m [-(l2+1)(l2+m)/(2l2+1) I_1 e^p + l2(l2-m+1)/(2l2+1) I_2 e^p],
 // the same quantities as src/integration.c:762-888 followed by the exp of
-// src/libcaps.c:784-829, without the intermediate log.  Terms below e^{-745}
+// src/libcaps.c:784-829, without the intermediate log: each term is a
+// product of mantissas and one ldexp.  Terms below the smallest double
 // vanish, as they do there.
-__device__ __forceinline__ double ex(LogVal t, double p) { return t.s == 0 ? 0.0 : t.s * exp(t.v + p); }
+__device__ __forceinline__ double ex(LinVal t, double pm, int pe) { return ldexp(t.m * pm, t.e + pe); }
 
-__device__ double linA(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol, double p)
+// Coefficients of B and C, separable in l1 and l2:
+//   u(l) = (l+1)(l+m)/(2l+1),  v(l) = l(l-m+1)/(2l+1),
+//   B = u1 u2 I(l1-1,l2-1) - v1 u2 I(l1+1,l2-1) - u1 v2 I(l1-1,l2+1) + v1 v2 I(l1+1,l2+1),
+//   C(l1,l2) = m [-u2 I(l1,l2-1) + v2 I(l1,l2+1)].
+struct UV {
+    double u, v;
+};
+
+__device__ __forceinline__ UV uv(int l, int m)
+{
+    const double r = 1.0 / (2 * l + 1);
+    return {(l + 1.) * (l + m) * r, l * (l - m + 1.) * r};
+}
+
+__device__ __forceinline__ double linA(const double* Im, const int* Ie, const BlockDesc& d, int l1, int l2, int pol,
+                                       double pm, int pe)
 {
     if (d.m == 0) return 0.0;
-    return (double)d.m * d.m * ex(getI(lI, sI, d, pol, l1, l2), p);
+    return (double)d.m * d.m * ex(getI(Im, Ie, d, pol, l1, l2), pm, pe);
 }
 
-__device__ double linB(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol, double p)
+__device__ __forceinline__ double linB(const double* Im, const int* Ie, const BlockDesc& d, int l1, int l2, UV c1,
+                                       UV c2, int pol, double pm, int pe)
 {
-    const int m = d.m;
-    if (m == 0) return ex(getI(lI, sI, d, pol, l1, l2), p);
-    const double den = (2 * l1 + 1.) * (2 * l2 + 1.);
-    double r = (l1 + 1.) * (l1 + m) * (l2 + 1.) * (l2 + m) / den * ex(getI(lI, sI, d, pol, l1 - 1, l2 - 1), p);
-    r -= l1 * (l1 - m + 1.) * (l2 + 1.) * (l2 + m) / den * ex(getI(lI, sI, d, pol, l1 + 1, l2 - 1), p);
-    r -= (l1 + 1.) * (l1 + m) * l2 * (l2 - m + 1.) / den * ex(getI(lI, sI, d, pol, l1 - 1, l2 + 1), p);
-    r += l1 * (l1 - m + 1.) * l2 * (l2 - m + 1.) / den * ex(getI(lI, sI, d, pol, l1 + 1, l2 + 1), p);
+    if (d.m == 0) return ex(getI(Im, Ie, d, pol, l1, l2), pm, pe);
+    double r = c1.u * c2.u * ex(getI(Im, Ie, d, pol, l1 - 1, l2 - 1), pm, pe);
+    r -= c1.v * c2.u * ex(getI(Im, Ie, d, pol, l1 + 1, l2 - 1), pm, pe);
+    r -= c1.u * c2.v * ex(getI(Im, Ie, d, pol, l1 - 1, l2 + 1), pm, pe);
+    r += c1.v * c2.v * ex(getI(Im, Ie, d, pol, l1 + 1, l2 + 1), pm, pe);
     return r;
 }
 
-__device__ double linC(const double* lI, const int8_t* sI, const BlockDesc& d, int l1, int l2, int pol, double p)
+// C(l1, l2) with c2 = uv(l2)
+__device__ __forceinline__ double linC(const double* Im, const int* Ie, const BlockDesc& d, int l1, int l2, UV c2,
+                                       int pol, double pm, int pe)
 {
     const int m = d.m;
     if (m == 0) return 0.0;
-    const double den = 2 * l2 + 1;
-    return m * (-(l2 + 1.) * (l2 + m) / den * ex(getI(lI, sI, d, pol, l1, l2 - 1), p)
-                + l2 * (l2 - m + 1.) / den * ex(getI(lI, sI, d, pol, l1, l2 + 1), p));
+    return m * (-c2.u * ex(getI(Im, Ie, d, pol, l1, l2 - 1), pm, pe) + c2.v * ex(getI(Im, Ie, d, pol, l1, l2 + 1), pm, pe));
 }
 
 __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                   const double* __restrict__ lfac_tab, int mode)
 {
     const BlockDesc& d = desc[blockIdx.z];
+    // 16 x 16 tiles: the I values of neighbouring (l1, l2) are reused from
+    // cache; 16 consecutive l1 per row of lanes read consecutive addresses
     const int i2 = blockIdx.x * 16 + (threadIdx.x & 15);
     const int j2 = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (i2 >= d.ldim || j2 >= d.ldim) return;
     const int l1 = d.lmin + i2, l2 = d.lmin + j2, m = d.m;
-    const double* lI = at<double>(arena, d.off_i);
-    const int8_t* sI = at<int8_t>(arena, d.off_i + 16 * (int64_t)d.nl * d.nl);
-    const double* ea = at<double>(arena, d.off_mie) + 2 * d.ldim;
-    const double* eb = ea + d.ldim;
-    const double ea1 = ea[i2], eb1 = eb[i2], ea2 = ea[j2], eb2 = eb[j2];
+    const double* Im = at<double>(arena, d.off_i);
+    const int* Ie = at<int>(arena, d.off_i + 16 * (int64_t)d.nl * d.nl);
+    const double* ma = at<double>(arena, d.off_mie) + 2 * d.ldim;   // mantissas of e^{ea}, e^{eb}
+    const double* mb = ma + d.ldim;
+    const int* xa = reinterpret_cast<const int*>(ma + 2 * d.ldim);  // their exponents
+    const int* xb = xa + d.ldim;
+    const double ma1 = ma[i2], mb1 = mb[i2], ma2 = ma[j2], mb2 = mb[j2];
+    const int xa1 = xa[i2], xb1 = xb[i2], xa2 = xa[j2], xb2 = xb[j2];
     const int TEp = 0, TMp = 1;
     // EE: √(a_l1 a_l2) Λ (B_TM - A_TE);  MM: √(b_l1 b_l2) Λ (A_TM - B_TE)
-    const double pEE = ea1 + ea2, pMM = eb1 + eb2;
-    const double EE = linB(lI, sI, d, l1, l2, TMp, pEE) - linA(lI, sI, d, l1, l2, TEp, pEE);
-    const double MM = linA(lI, sI, d, l1, l2, TMp, pMM) - linB(lI, sI, d, l1, l2, TEp, pMM);
+    const double pEE = ma1 * ma2, pMM = mb1 * mb2;
+    const int eEE = xa1 + xa2, eMM = xb1 + xb2;
+    const UV c1 = uv(l1, m), c2 = uv(l2, m);
+    const double EE = linB(Im, Ie, d, l1, l2, c1, c2, TMp, pEE, eEE) - linA(Im, Ie, d, l1, l2, TEp, pEE, eEE);
+    const double MM = linA(Im, Ie, d, l1, l2, TMp, pMM, eMM) - linB(Im, Ie, d, l1, l2, c1, c2, TEp, pMM, eMM);
     double EM = 0, ME = 0;
     if (m != 0) {
         // EM: D_TM - C_TE with (a_l1 + b_l2)/2;  ME: C_TM - D_TE with (b_l1 + a_l2)/2;  D(l1,l2) = C(l2,l1)
-        const double pEM = ea1 + eb2, pME = eb1 + ea2;
-        EM = linC(lI, sI, d, l2, l1, TMp, pEM) - linC(lI, sI, d, l1, l2, TEp, pEM);
-        ME = linC(lI, sI, d, l1, l2, TMp, pME) - linC(lI, sI, d, l2, l1, TEp, pME);
+        const double pEM = ma1 * mb2, pME = mb1 * ma2;
+        const int eEM = xa1 + xb2, eME = xb1 + xa2;
+        EM = linC(Im, Ie, d, l2, l1, c1, TMp, pEM, eEM) - linC(Im, Ie, d, l1, l2, c2, TEp, pEM, eEM);
+        ME = linC(Im, Ie, d, l1, l2, c2, TMp, pME, eME) - linC(Im, Ie, d, l2, l1, c1, TEp, pME, eME);
     }
     double* A = at<double>(arena, d.off_mat);
     const int64_t r = 2 * i2, c = 2 * j2, lda = d.lda;

@@ -198,6 +198,12 @@ int capsx_logdetD_multi(caps_t *const *objs, const double *xi_, const int *m, in
  * CAPS_AMD_TUNING), written by caps_amd/autotune.py from capsx_lu_time.
  * No counterpart in the reference (which calls LAPACK dgetrf_,
  * src/matrix.c:555-577). */
+/* Extension: capture each batch shape of this object into a HIP graph once
+ * and replay it for later batches of the same shape (on != 0); 0 (default)
+ * launches eagerly.  Descriptor uploads and the status reset stay outside
+ * the graph.  Stage timings of replayed batches read 0 (the total is
+ * measured).  No counterpart in the reference. */
+void capsx_set_graph(caps_t *self, int on);
 /* Fix the schedule for all later batches; levels = 0 returns to the table.
  * Returns 0 on success, 1 on invalid values. */
 int capsx_lu_config(int levels, int streams, int xcd, int tile);
