@@ -352,13 +352,10 @@ __global__ void __launch_bounds__(64) k_lu_swap_panel(const BlockDesc* __restric
     swap_rows_ipiv(P1 + (int64_t)threadIdx.x * d.lda, moved_buf(arena, side[b], 1), min(NB, d.N - k - NB), NB);
 }
 
-// A22 -= L21 U12 with FP64 MFMA.  Workgroup tile 128x128 of A22, 4 waves
-// of 64x64 (4x4 tiles of 16x16 per wave, 16 accumulators), K = 64 or 128
-// staged through LDS 16 at a time.  LDS rows are padded to 144 doubles so
-// the four k-rows a wavefront reads in one instruction fall on disjoint bank
-// halves.
+// A22 -= L21 U12 with FP64 MFMA.  Workgroup tile 128x128 (or 64x64) of A22,
+// waves of 64x64 (4x4 tiles of 16x16 per wave, 16 accumulators), K = 64 or
+// 128 staged through LDS 16 (one-wave tiles: 8) at a time.
 typedef double double4_t __attribute__((ext_vector_type(4)));
-constexpr int KS = 16;           // K per LDS stage
 
 // WM x WN wavefronts of 64 x 64 each: workgroup tile (64 WM) x (64 WN).
 template <int WM, int WN>
@@ -367,7 +364,11 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
           const BlockOut* __restrict__ out, int k, int mode, int gx, int gy, int nblocks, int xcd_order)
 {
     constexpr int NT = 64 * WM * WN, GTM = 64 * WM, GTN = 64 * WN;
-    constexpr int GTPM = GTM + 16, GTPN = GTN + 16;   // padded LDS rows
+    constexpr int KS = NT >= 256 ? 16 : 8;   // K per LDS stage (prefetch registers)
+    // padded LDS rows: GTPM keeps the A rows 128-byte aligned (its stores and
+    // reads run along rows); GTPN is odd, so the B stores, which run down a
+    // column (16 consecutive k), fall on distinct banks
+    constexpr int GTPM = GTM + 16, GTPN = GTN + 17;
     // XCD-aware order (xcd_order = 1): workgroup i runs on XCD i % 8, so all
     // tiles of block b go to XCD b % 8 and share its L2 for the L and U
     // operands; xcd_order = 0: block-major order
@@ -421,7 +422,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     const double* P2 = panel_buf(arena, side[b], lda, 1);
     // one LDS buffer: the A and B stages, reused for the C strips at the end
     constexpr int SMEM = KS * (GTPM + GTPN);
-    static_assert(SMEM >= WM * WN * (16 * 64 + 4), "LDS holds the epilogue strips");
+    static_assert(SMEM >= WM * WN * (64 * 17 + 4), "LDS holds the epilogue strips");
     __shared__ double smem[SMEM];
     double(*As)[GTPM] = reinterpret_cast<double(*)[GTPM]>(smem);             // [kk][row]
     double(*Bs)[GTPN] = reinterpret_cast<double(*)[GTPN]>(smem + KS * GTPM);  // [kk][col]
@@ -491,7 +492,9 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     // read-modify-write of C touches whole 128-byte column segments: lane
     // (r = l%16, c = l/16) updates rows r of columns c, c+4, ..., c+60, all
     // loads of a strip issued before its stores.
-    double* Ts = smem + wv * (16 * 64 + 4);        // per-wave strip [col][row], 16 x 64
+    // per-wave strip [col][row], 64 columns of 16 rows at a pitch of 17 so
+    // that the 16 lanes writing one row fall on distinct banks
+    double* Ts = smem + wv * (64 * 17 + 4);
     __syncthreads();                                // As/Bs are reused
     const int rr = lane & 15, cc = lane >> 4;
 #pragma unroll
@@ -499,7 +502,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
 #pragma unroll
         for (int j = 0; j < 4; j++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) Ts[(16 * j + li) * 16 + (lk + 4 * q)] = acc[i][j][q];
+            for (int q = 0; q < 4; q++) Ts[(16 * j + li) * 17 + (lk + 4 * q)] = acc[i][j][q];
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -513,7 +516,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
 #pragma unroll
         for (int t = 0; t < 16; t++) {
             const int col = n0 + wc + cc + 4 * t;
-            if (row < rend && col < cend) A[row + (int64_t)col * lda] = cv[t] - Ts[(cc + 4 * t) * 16 + rr];
+            if (row < rend && col < cend) A[row + (int64_t)col * lda] = cv[t] - Ts[(cc + 4 * t) * 17 + rr];
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
