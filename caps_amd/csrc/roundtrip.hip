@@ -443,23 +443,9 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
 // (the interleaved index map of src/libcaps.c:717-743).  Elements follow
 // src/libcaps.c:761-831 and A/B/C/D src/integration.c:762-888.
 
-// I_{l1 l2}^pol as mantissa × 2^exponent (0 below l = m).  The table is
-// symmetric; it is read as I[l2][l1] so that lanes with consecutive l1 read
-// consecutive addresses.
-struct LinVal {
-    double m;
-    int e;
-};
-
-__device__ __forceinline__ LinVal getI(const double* Im, const int* Ie, const BlockDesc& d, int pol, int l1, int l2)
-{
-    if (l1 < d.m || l2 < d.m) return {0.0, 0};
-    const int64_t idx = (int64_t)pol * d.nl * d.nl + (int64_t)(l2 - d.lo) * d.nl + (l1 - d.lo);
-    return {Im[idx], Ie[idx]};
-}
-
 // Element contributions in linear form: with the prefactor e^p = pm · 2^pe
-// (Mie and Λ factors), X·e^p is summed directly from the I values,
+// (Mie and Λ factors), X·e^p is summed directly from the I values (stored as
+// mantissa × 2^exponent),
 //   A·e^p = m² I e^p,
 //   B·e^p = Σ_k c_k I_k e^p   (4 terms; m = 0: I itself),
 //   C·e^p = m [-(l2+1)(l2+m)/(2l2+1) I_1 e^p + l2(l2-m+1)/(2l2+1) I_2 e^p],
@@ -467,12 +453,9 @@ __device__ __forceinline__ LinVal getI(const double* Im, const int* Ie, const Bl
 // src/libcaps.c:784-829, without the intermediate log: each term is a
 // product of mantissas and one ldexp.  Terms below the smallest double
 // vanish, as they do there.
-__device__ __forceinline__ double ex(LinVal t, double pm, int pe) { return ldexp(t.m * pm, t.e + pe); }
 
 // Coefficients of B and C, separable in l1 and l2:
-//   u(l) = (l+1)(l+m)/(2l+1),  v(l) = l(l-m+1)/(2l+1),
-//   B = u1 u2 I(l1-1,l2-1) - v1 u2 I(l1+1,l2-1) - u1 v2 I(l1-1,l2+1) + v1 v2 I(l1+1,l2+1),
-//   C(l1,l2) = m [-u2 I(l1,l2-1) + v2 I(l1,l2+1)].
+//   u(l) = (l+1)(l+m)/(2l+1),  v(l) = l(l-m+1)/(2l+1).
 struct UV {
     double u, v;
 };
@@ -483,74 +466,84 @@ __device__ __forceinline__ UV uv(int l, int m)
     return {(l + 1.) * (l + m) * r, l * (l - m + 1.) * r};
 }
 
-__device__ __forceinline__ double linA(const double* Im, const int* Ie, const BlockDesc& d, int l1, int l2, int pol,
-                                       double pm, int pe)
-{
-    if (d.m == 0) return 0.0;
-    return (double)d.m * d.m * ex(getI(Im, Ie, d, pol, l1, l2), pm, pe);
-}
-
-__device__ __forceinline__ double linB(const double* Im, const int* Ie, const BlockDesc& d, int l1, int l2, UV c1,
-                                       UV c2, int pol, double pm, int pe)
-{
-    if (d.m == 0) return ex(getI(Im, Ie, d, pol, l1, l2), pm, pe);
-    double r = c1.u * c2.u * ex(getI(Im, Ie, d, pol, l1 - 1, l2 - 1), pm, pe);
-    r -= c1.v * c2.u * ex(getI(Im, Ie, d, pol, l1 + 1, l2 - 1), pm, pe);
-    r -= c1.u * c2.v * ex(getI(Im, Ie, d, pol, l1 - 1, l2 + 1), pm, pe);
-    r += c1.v * c2.v * ex(getI(Im, Ie, d, pol, l1 + 1, l2 + 1), pm, pe);
-    return r;
-}
-
-// C(l1, l2) with c2 = uv(l2)
-__device__ __forceinline__ double linC(const double* Im, const int* Ie, const BlockDesc& d, int l1, int l2, UV c2,
-                                       int pol, double pm, int pe)
-{
-    const int m = d.m;
-    if (m == 0) return 0.0;
-    return m * (-c2.u * ex(getI(Im, Ie, d, pol, l1, l2 - 1), pm, pe) + c2.v * ex(getI(Im, Ie, d, pol, l1, l2 + 1), pm, pe));
-}
-
 __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                   const double* __restrict__ lfac_tab, int mode)
 {
     const BlockDesc& d = desc[blockIdx.z];
-    // 16 x 16 tiles: the I values of neighbouring (l1, l2) are reused from
-    // cache; 16 consecutive l1 per row of lanes read consecutive addresses
-    const int i2 = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int j2 = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int L1 = d.lmin + blockIdx.x * 16, L2 = d.lmin + blockIdx.y * 16;
+    if (L1 >= d.lmin + d.ldim || L2 >= d.lmin + d.ldim) return;   // uniform over the workgroup
+    const int m = d.m;
+    // the I values of the workgroup's 16 x 16 tiles and their ±1 halo, both
+    // polarizations, loaded once into LDS with coalesced reads; I is
+    // symmetric, the window holds I(l1, l2) at [pol][l2 - L2 + 1][l1 - L1 + 1]
+    __shared__ double wm[2][18][18];
+    __shared__ int we[2][18][18];
+    {
+        const double* Im = at<double>(arena, d.off_i);
+        const int* Ie = at<int>(arena, d.off_i + 16 * (int64_t)d.nl * d.nl);
+        const int64_t nl2 = (int64_t)d.nl * d.nl;
+        for (int t = threadIdx.x; t < 2 * 18 * 18; t += 256) {
+            const int pol = t / 324, r = (t / 18) % 18, c = t % 18;
+            const int l1 = L1 - 1 + c, l2 = L2 - 1 + r;
+            const bool ok = l1 >= d.lo && l2 >= d.lo && l1 < d.lo + d.nl && l2 < d.lo + d.nl && l1 >= m && l2 >= m;
+            const int64_t idx = pol * nl2 + (int64_t)(l2 - d.lo) * d.nl + (l1 - d.lo);
+            wm[pol][r][c] = ok ? Im[idx] : 0.0;
+            we[pol][r][c] = ok ? Ie[idx] : 0;
+        }
+    }
+    __syncthreads();
+    const int c0 = threadIdx.x & 15, r0 = threadIdx.x >> 4;
+    const int i2 = blockIdx.x * 16 + c0, j2 = blockIdx.y * 16 + r0;
     if (i2 >= d.ldim || j2 >= d.ldim) return;
-    const int l1 = d.lmin + i2, l2 = d.lmin + j2, m = d.m;
-    const double* Im = at<double>(arena, d.off_i);
-    const int* Ie = at<int>(arena, d.off_i + 16 * (int64_t)d.nl * d.nl);
+    const int l1 = d.lmin + i2, l2 = d.lmin + j2;
+    // I(l1 + a, l2 + b) of polarization pol times the prefactor pm 2^pe
+    auto X = [&](int pol, int a, int b, double pm, int pe) {
+        return ldexp(wm[pol][r0 + 1 + b][c0 + 1 + a] * pm, we[pol][r0 + 1 + b][c0 + 1 + a] + pe);
+    };
     const double* ma = at<double>(arena, d.off_mie) + 2 * d.ldim;   // mantissas of e^{ea}, e^{eb}
     const double* mb = ma + d.ldim;
     const int* xa = reinterpret_cast<const int*>(ma + 2 * d.ldim);  // their exponents
     const int* xb = xa + d.ldim;
     const double ma1 = ma[i2], mb1 = mb[i2], ma2 = ma[j2], mb2 = mb[j2];
     const int xa1 = xa[i2], xb1 = xb[i2], xa2 = xa[j2], xb2 = xb[j2];
-    const int TEp = 0, TMp = 1;
+    const int TE = 0, TM = 1;
+    const UV c1 = uv(l1, m), c2 = uv(l2, m);
+    // A = m² I;  B = u1 u2 I(-1,-1) - v1 u2 I(+1,-1) - u1 v2 I(-1,+1) + v1 v2 I(+1,+1)
+    // (m = 0: B = I);  C(l1,l2) = m [-u2 I(0,-1) + v2 I(0,+1)];
+    // D(l1,l2) = C(l2,l1) = m [-u1 I(-1,0) + v1 I(+1,0)]   (I symmetric)
+    auto A = [&](int pol, double pm, int pe) { return m == 0 ? 0.0 : (double)m * m * X(pol, 0, 0, pm, pe); };
+    auto B = [&](int pol, double pm, int pe) {
+        if (m == 0) return X(pol, 0, 0, pm, pe);
+        return c1.u * c2.u * X(pol, -1, -1, pm, pe) - c1.v * c2.u * X(pol, 1, -1, pm, pe)
+             - c1.u * c2.v * X(pol, -1, 1, pm, pe) + c1.v * c2.v * X(pol, 1, 1, pm, pe);
+    };
+    auto C = [&](int pol, double pm, int pe) {
+        return m * (-c2.u * X(pol, 0, -1, pm, pe) + c2.v * X(pol, 0, 1, pm, pe));
+    };
+    auto D = [&](int pol, double pm, int pe) {
+        return m * (-c1.u * X(pol, -1, 0, pm, pe) + c1.v * X(pol, 1, 0, pm, pe));
+    };
     // EE: √(a_l1 a_l2) Λ (B_TM - A_TE);  MM: √(b_l1 b_l2) Λ (A_TM - B_TE)
     const double pEE = ma1 * ma2, pMM = mb1 * mb2;
     const int eEE = xa1 + xa2, eMM = xb1 + xb2;
-    const UV c1 = uv(l1, m), c2 = uv(l2, m);
-    const double EE = linB(Im, Ie, d, l1, l2, c1, c2, TMp, pEE, eEE) - linA(Im, Ie, d, l1, l2, TEp, pEE, eEE);
-    const double MM = linA(Im, Ie, d, l1, l2, TMp, pMM, eMM) - linB(Im, Ie, d, l1, l2, c1, c2, TEp, pMM, eMM);
+    const double EE = B(TM, pEE, eEE) - A(TE, pEE, eEE);
+    const double MM = A(TM, pMM, eMM) - B(TE, pMM, eMM);
     double EM = 0, ME = 0;
     if (m != 0) {
-        // EM: D_TM - C_TE with (a_l1 + b_l2)/2;  ME: C_TM - D_TE with (b_l1 + a_l2)/2;  D(l1,l2) = C(l2,l1)
+        // EM: D_TM - C_TE with (a_l1 + b_l2)/2;  ME: C_TM - D_TE with (b_l1 + a_l2)/2
         const double pEM = ma1 * mb2, pME = mb1 * ma2;
         const int eEM = xa1 + xb2, eME = xb1 + xa2;
-        EM = linC(Im, Ie, d, l2, l1, c1, TMp, pEM, eEM) - linC(Im, Ie, d, l1, l2, c2, TEp, pEM, eEM);
-        ME = linC(Im, Ie, d, l1, l2, c2, TMp, pME, eME) - linC(Im, Ie, d, l2, l1, c1, TEp, pME, eME);
+        EM = D(TM, pEM, eEM) - C(TE, pEM, eEM);
+        ME = C(TM, pME, eME) - D(TE, pME, eME);
     }
-    double* A = at<double>(arena, d.off_mat);
+    double* Am = at<double>(arena, d.off_mat);
     const int64_t r = 2 * i2, c = 2 * j2, lda = d.lda;
     const double sg = mode == 0 ? -1.0 : 1.0;
     const double one = mode == 0 ? 1.0 : 0.0;
-    A[r + c * lda] = sg * EE + (i2 == j2 ? one : 0.0);
-    A[r + (c + 1) * lda] = sg * EM;
-    A[r + 1 + c * lda] = sg * ME;
-    A[r + 1 + (c + 1) * lda] = sg * MM + (i2 == j2 ? one : 0.0);
+    Am[r + c * lda] = sg * EE + (i2 == j2 ? one : 0.0);
+    Am[r + (c + 1) * lda] = sg * EM;
+    Am[r + 1 + c * lda] = sg * ME;
+    Am[r + 1 + (c + 1) * lda] = sg * MM + (i2 == j2 ? one : 0.0);
     if (i2 == j2) {
         double* dg = at<double>(arena, d.off_diag);
         dg[r] = EE;
