@@ -33,7 +33,8 @@ struct BlockDesc {
     int pad_;
     // byte offsets into the arena
     int64_t off_mie;    // 4 * ldim doubles: log|a_l|, log|b_l|, then the mantissas
-                        // of e^{ea(l)}, e^{eb(l)}; then 2 * ldim int exponents
+                        // of e^{ea(l)}, e^{eb(l)}; then 2 * ldim int exponents;
+                        // then nl + (2 nl - 1) doubles: parts of log a0 of I
     int64_t off_kpart;  // n_waves * nK * KPart
     int64_t off_k;      // 2 pols * nK * (double mant, int exp, double lnK)
     int64_t off_i;      // I = mant * 2^exp: 2 pols * nl*nl signed double mantissas,

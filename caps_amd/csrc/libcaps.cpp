@@ -427,7 +427,7 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         d.nK = d.nu_max - d.mu + 1;
         grid_for(d);
         d.off_mie = off;
-        off = align256(off + 4 * (int64_t)ldim * 8 + 2 * (int64_t)ldim * 4);
+        off = align256(off + (5 * (int64_t)ldim + 3 * (int64_t)(ldim + 2)) * 8);
         d.off_kpart = off;
         off = align256(off + (int64_t)d.n_waves * d.nK * sizeof(KPart));
         d.off_k = off;

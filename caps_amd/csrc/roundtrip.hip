@@ -32,8 +32,25 @@ __global__ void __launch_bounds__(256) k_mie(const BlockDesc* __restrict__ desc,
 {
     const BlockDesc& d = desc[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.ldim) return;
     double* al = at<double>(arena, d.off_mie);   // [a | b | ea | eb], ldim each
+    {
+        // separable parts of log a0 of the I integrals:
+        //   log a0(l1, l2) = g1(l1) + g1(l2) + g2(l1 + l2),
+        //   g1(l) = log(2l)! - log l! - log(l-m)!,
+        //   g2(s) = log s! - log(2s)! + log(s-2m)!      (m -> max(m, 1))
+        const int m_ = d.m > 0 ? d.m : 1;
+        double* g1 = al + 5 * d.ldim;
+        double* g2 = g1 + d.nl;
+        if (i < d.nl) {
+            const int l = d.lo + i;
+            g1[i] = l >= m_ ? lfac(lfac_tab, 2 * l) - lfac(lfac_tab, l) - lfac(lfac_tab, l - m_) : 0.0;
+        }
+        if (i < 2 * d.nl - 1) {
+            const int sl = 2 * d.lo + i;
+            g2[i] = sl >= 2 * m_ ? lfac(lfac_tab, sl) - lfac(lfac_tab, 2 * sl) + lfac(lfac_tab, sl - 2 * m_) : 0.0;
+        }
+    }
+    if (i >= d.ldim) return;
     double lna, lnb;
     const int l = d.lmin + i, m = d.m;
     mie(d.xi_, d.LbyR, d.epsm1_sphere, l, lna, lnb);
@@ -290,9 +307,9 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     int qmax = (l12 - 2 * m_) / 2;
     qmax = min(qmax, min(l1, l2));
     const double Ap = -2 * m * (n - nu) * (n + nu + 1);
-    const double log_a0 = lfac(lfac_tab, 2 * l1) - lfac(lfac_tab, l1) + lfac(lfac_tab, 2 * l2) - lfac(lfac_tab, l2)
-                        + lfac(lfac_tab, l12) - lfac(lfac_tab, 2 * l12) + lfac(lfac_tab, l12 - 2 * m_)
-                        - lfac(lfac_tab, l1 - m_) - lfac(lfac_tab, l2 - m_);
+    const double* g1 = at<double>(arena, d.off_mie) + 5 * d.ldim;   // separable parts of log a0 (k_mie)
+    const double* g2 = g1 + nl;
+    const double log_a0 = g1[i1] + g1[i2] + g2[i1 + i2];
 
     // per block and polarization: K table, running sum (exponent-aligned to
     // the q = 0 term), |q = 0 term|, its exponent, early-exit count
@@ -300,7 +317,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     const double* km[GM][NPOL];
     const int* ke[GM][NPOL];
     double sum[GM][NPOL], t0abs[GM][NPOL];
-    int E0[GM][NPOL], cnt[GM][NPOL];
+    int E0[GM][NPOL], cnt[GM][NPOL], cs[GM][NPOL];
     int ndone = 0;
 #pragma unroll
     for (int g = 0; g < GM; g++) {
@@ -316,6 +333,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 sum[g][pp] = km[g][pp][j];
                 t0abs[g][pp] = fabs(sum[g][pp]);
                 cnt[g][pp] = 0;
+                cs[g][pp] = 0;
             } else {
                 km[g][pp] = nullptr;
                 ke[g][pp] = nullptr;
@@ -323,12 +341,14 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 sum[g][pp] = 0;
                 t0abs[g][pp] = 0;
                 cnt[g][pp] = 3;   // absent: counts as done
+                cs[g][pp] = 0;
                 ndone++;
             }
         }
     }
 
     double aqm2 = 1, aqm1 = 0, aq = 1;  // a_{q-2}, a_{q-1}, a_q
+    double rn1 = 0, rv1 = 0, r3 = 0, tn = 0, tv = 0;
     int S = 0;                           // a_q carries a factor 2^S
     // α(p+3), α(p+4) of a step are α(p+1), α(p+2) of the step before
     const double s2 = (n + nu + 1) * (n + nu + 1), d2 = (n - nu) * (n - nu);
@@ -336,18 +356,22 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     bool have_al = false;
     for (int q = 1; q <= qmax && ndone < GM * NPOL; q++) {
         if (q == 1) {
-            aq = (n + nu - 1.5) * (1 - (2 * n + 2 * nu - 1) / (n4 * (n4 - 1))
-                 * ((m - n) * (m - n + 1) / (2 * n - 1) + (m - nu) * (m - nu + 1) / (2 * nu - 1)));
+            // eqs. (26)-(30) with the reciprocals shared between q = 1 and 2
+            rn1 = 1 / (2 * n - 1);
+            rv1 = 1 / (2 * nu - 1);
+            r3 = 1 / (n4 * (n4 - 1));
+            tn = (m - n) * (m - n + 1) * rn1;
+            tv = (m - nu) * (m - nu + 1) * rv1;
+            aq = (n + nu - 1.5) * (1 - (2 * n + 2 * nu - 1) * r3 * (tn + tv));
             aqm1 = 1;
         } else if (q == 2) {
             aqm2 = aqm1;
             aqm1 = aq;
-            aq = (2 * n + 2 * nu - 1) * (2 * n + 2 * nu - 7) / 4 * ((2 * n + 2 * nu - 3) / (n4 * (n4 - 1))
-                 * ((2 * n + 2 * nu - 5) / (2 * (n4 - 2) * (n4 - 3))
-                 * ((m - n) * (m - n + 1) * (m - n + 2) * (m - n + 3) / (2 * n - 1) / (2 * n - 3)
-                 + 2 * (m - n) * (m - n + 1) * (m - nu) * (m - nu + 1) / ((2 * n - 1) * (2 * nu - 1))
-                 + (m - nu) * (m - nu + 1) * (m - nu + 2) * (m - nu + 3) / (2 * nu - 1) / (2 * nu - 3))
-                 - (m - n) * (m - n + 1) / (2 * n - 1) - (m - nu) * (m - nu + 1) / (2 * nu - 1)) + 0.5);
+            const double rn3 = 1 / (2 * n - 3), rv3 = 1 / (2 * nu - 3), r6 = 1 / (2 * (n4 - 2) * (n4 - 3));
+            aq = (2 * n + 2 * nu - 1) * (2 * n + 2 * nu - 7) / 4 * ((2 * n + 2 * nu - 3) * r3
+                 * ((2 * n + 2 * nu - 5) * r6
+                 * (tn * (m - n + 2) * (m - n + 3) * rn3 + 2 * tn * tv + tv * (m - nu + 2) * (m - nu + 3) * rv3)
+                 - tn - tv) + 0.5);
         } else {
             aqm2 = aqm1;
             aqm1 = aq;
@@ -393,10 +417,17 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 const double term = ldexp(aq * km[g][pp][j], S + ke[g][pp][j] - E0[g][pp]);
                 sum[g][pp] += term;
                 if (q >= 3) {
-                    if (fabs(term) < thr * t0abs[g][pp]) {
-                        if (++cnt[g][pp] >= 3) ndone++;
-                    } else
-                        cnt[g][pp] = 0;
+                    // the reference's rule (three terms below e^-60 of the
+                    // first), or three terms below 2^-56 of the running sum:
+                    // such terms are below half an ulp of the sum and leave
+                    // it unchanged
+                    const double aterm = fabs(term);
+                    cnt[g][pp] = aterm < thr * t0abs[g][pp] ? cnt[g][pp] + 1 : 0;
+                    cs[g][pp] = aterm < ldexp(fabs(sum[g][pp]), -56) ? cs[g][pp] + 1 : 0;
+                    if (cnt[g][pp] >= 3 || cs[g][pp] >= 3) {
+                        cnt[g][pp] = 3;
+                        ndone++;
+                    }
                 }
             }
         }
@@ -628,7 +659,7 @@ extern "C" {
 void capsd_launch_mie(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int max_ldim,
                       hipStream_t s)
 {
-    dim3 g((max_ldim + 255) / 256, nblocks);
+    dim3 g((2 * (max_ldim + 2) + 255) / 256, nblocks);
     hipLaunchKernelGGL(k_mie, g, dim3(256), 0, s, desc, arena, lfac_tab);
 }
 
