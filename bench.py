@@ -166,6 +166,32 @@ def gemm_traffic():
     return None
 
 
+def large_aspect(ca):
+    """Eight perfect-reflector blocks at R/L = 2000, ldim = 12000 (N = 24000,
+    the large-aspect configuration) in one batch: blocks/s, LU TF/s and the
+    largest deviation from the reference values in
+    tests/golden/ref_c5_blocks.json."""
+    path = os.path.join(ROOT, "tests", "golden", "ref_c5_blocks.json")
+    with open(path) as f:
+        rows = json.load(f)["data"]
+    c = ca.Caps(2e-3, 1e-6, ldim=12000)
+    xs = [r["xi_"] for r in rows]
+    ms = [r["m"] for r in rows]
+    c.reset_stats()
+    t0 = time.perf_counter()
+    out = c.logdetD_batch(xs, ms)
+    dt = time.perf_counter() - t0
+    st = c.stats(reset=True)
+    lu_s = st["stage_ms"]["lu"] / 1e3
+    err = max(abs(v - r["logdetD"]) / abs(r["logdetD"]) for v, r in zip(out, rows))
+    print("bench: large-aspect batch %.2f s for %d blocks" % (dt, len(rows)), file=sys.stderr, flush=True)
+    return {"workload": "8 perfect-reflector blocks, R/L=2000 (R=2mm, L=1um), ldim=12000 (N=24000), one batch",
+            "blocks_per_s": len(rows) / dt, "seconds": dt,
+            "stage_ms": {k: v for k, v in st["stage_ms"].items()},
+            "lu_tflops": st["lu_flops"] / lu_s / 1e12 if lu_s > 0 else None,
+            "max_rel_err_vs_reference": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +201,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy three times (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -241,6 +268,7 @@ def main():
             "arithmetic_intensity_flop_per_byte": prof["gemm_flops"] / max(1.0, prof["gemm_bytes"])}
     lu_s = st["stage_ms"]["lu"] / 1e3
     mixed = None if args.no_mixed else mixed_aspect(caps_amd, args.steps)
+    large = None if args.no_large else large_aspect(caps_amd)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "logdetD blocks/s @R/L=200 FP64 (batched+surrogate form; plain form and LU FP64 TF/s alongside)",
@@ -257,6 +285,7 @@ def main():
         "blocks_factored_per_step": st["blocks_factored"] / args.steps,
         "roofline": roof,
         "mixed_aspect_batch": mixed,
+        "large_aspect_batch": large,
         "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
     }
     print(json.dumps(line))

@@ -70,6 +70,7 @@ def lib():
             "capsx_stats": (None, [p, dp]), "capsx_stats_reset": (None, [p]),
             "capsx_profile": (None, [p, i]),
             "capsx_set_graph": (None, [p, i]),
+            "capsx_lu_grid_panel_rows": (i, [i]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)

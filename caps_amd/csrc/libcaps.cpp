@@ -43,6 +43,7 @@ int capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, h
                     hipEvent_t*, int*);
 void capsd_launch_lu_fill(const BlockDesc*, char*, int, unsigned long long, hipStream_t);
 int capsd_lu_max_updates(int);
+int capsd_lu_grid_panel_rows(int);
 int capsd_lu_init(void);
 int capsd_lu_max_dim(void);
 int capsd_lu_nb(void);
@@ -1655,6 +1656,15 @@ int capsx_lu_tuning_load(const char* path)
 {
     load_default_tuning();
     return load_tuning(path);
+}
+
+// Extension: panels taller than `rows` are factored column by column by
+// grid-wide kernels instead of one workgroup per matrix (default 6144);
+// rows <= 0 only queries.  Returns the previous value.
+int capsx_lu_grid_panel_rows(int rows)
+{
+    device_init();
+    return capsd_lu_grid_panel_rows(rows);
 }
 
 // Extension: the schedule used for n blocks of largest dimension maxN:

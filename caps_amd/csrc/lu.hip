@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "blocks.h"
 
@@ -275,6 +276,144 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
     }
 }
 
+
+// Tall panels (more rows than one workgroup holds in registers): the panel
+// is factored one column at a time by grid-wide kernels, 256 rows per
+// workgroup; rows are exchanged in place within the panel's columns.  The
+// panel buffer serves as scratch for the per-workgroup pivot candidates
+// (value, row) until the panel is copied into it at the end.
+constexpr int GP_ROWS = 256;
+
+__device__ __forceinline__ void block_argmax_store(double v, int r, double* scratch, int slot)
+{
+    __shared__ double sv[GP_ROWS / 64];
+    __shared__ int si[GP_ROWS / 64];
+    wave_argmax(v, r);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        sv[wid] = v;
+        si[wid] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double bv = sv[0];
+        int bi = si[0];
+        for (int q = 1; q < GP_ROWS / 64; q++)
+            if (sv[q] > bv || (sv[q] == bv && si[q] < bi)) {
+                bv = sv[q];
+                bi = si[q];
+            }
+        scratch[2 * slot] = bv;
+        scratch[2 * slot + 1] = (double)bi;
+    }
+}
+
+// pivot candidates of column 0 of the panel at k
+__global__ void __launch_bounds__(GP_ROWS) k_gp_max(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                    const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
+                                                    int k, int pb)
+{
+    const int b = blockIdx.y;
+    const BlockDesc& d = desc[b];
+    const int N = d.N;
+    if (out[b].skip || k >= N || k + (int)blockIdx.x * GP_ROWS >= N) return;
+    const int64_t lda = d.lda;
+    const double* A = at<double>(arena, d.off_mat);
+    const int r = k + blockIdx.x * GP_ROWS + threadIdx.x;
+    const double v = r < N ? fabs(A[r + (int64_t)k * lda]) : -1.0;
+    block_argmax_store(v, r < N ? r : 0x7fffffff, panel_buf(arena, side[b], lda, pb), blockIdx.x);
+}
+
+// pivot of column c: reduce the candidates, exchange rows k+c and p within
+// the panel, record ipiv and log|u_cc|
+__global__ void __launch_bounds__(64) k_gp_pivot(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                 const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
+                                                 int k, int pb, int c)
+{
+    const int b = blockIdx.x;
+    const BlockDesc& d = desc[b];
+    const int N = d.N;
+    if (out[b].skip || k >= N) return;
+    const int nbk = min(NB, N - k);
+    if (c >= nbk) return;
+    const int64_t lda = d.lda;
+    double* A = at<double>(arena, d.off_mat);
+    const double* sc = panel_buf(arena, side[b], lda, pb);
+    const int nch = (N - k + GP_ROWS - 1) / GP_ROWS;
+    double v = -1.0;
+    int p = 0x7fffffff;
+    for (int q = threadIdx.x; q < nch; q += 64) {
+        const double qv = sc[2 * q];
+        const int qi = (int)sc[2 * q + 1];
+        if (qv > v || (qv == v && qi < p)) {
+            v = qv;
+            p = qi;
+        }
+    }
+    wave_argmax(v, p);
+    const int j = threadIdx.x;
+    if (j < nbk && p != k + c) {
+        double* col = A + (int64_t)(k + j) * lda;
+        const double t = col[k + c];
+        col[k + c] = col[p];
+        col[p] = t;
+    }
+    if (j == 0) {
+        moved_buf(arena, side[b], pb)[c] = p - k;
+        at<double>(arena, side[b].off_logs)[k + c] = log(v);
+    }
+}
+
+// column c: l = a/u below the pivot, rank-1 update of the panel columns
+// c+1.., and the pivot candidates of column c+1
+__global__ void __launch_bounds__(GP_ROWS) k_gp_update(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                       const LuSide* __restrict__ side,
+                                                       const BlockOut* __restrict__ out, int k, int pb, int c)
+{
+    const int b = blockIdx.y;
+    const BlockDesc& d = desc[b];
+    const int N = d.N;
+    if (out[b].skip || k >= N || k + (int)blockIdx.x * GP_ROWS >= N) return;
+    const int nbk = min(NB, N - k);
+    if (c >= nbk) return;
+    const int64_t lda = d.lda;
+    double* A = at<double>(arena, d.off_mat);
+    const int r = k + blockIdx.x * GP_ROWS + threadIdx.x;
+    const int pr = k + c;
+    double vnext = -1.0;
+    if (r > pr && r < N) {
+        const double u = A[pr + (int64_t)pr * lda];
+        const double l = A[r + (int64_t)pr * lda] / u;
+        A[r + (int64_t)pr * lda] = l;
+        for (int j = c + 1; j < nbk; j++) {
+            const int64_t cj = (int64_t)(k + j) * lda;
+            const double x = A[r + cj] - l * A[pr + cj];
+            A[r + cj] = x;
+            if (j == c + 1) vnext = fabs(x);
+        }
+    }
+    if (c + 1 < nbk)
+        block_argmax_store(vnext, (r > pr && r < N) ? r : 0x7fffffff, panel_buf(arena, side[b], lda, pb),
+                           blockIdx.x);
+}
+
+// the factored panel in (now physical = logical) row order into its buffer
+__global__ void __launch_bounds__(GP_ROWS) k_gp_copy(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                     const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
+                                                     int k, int pb)
+{
+    const int b = blockIdx.y;
+    const BlockDesc& d = desc[b];
+    const int N = d.N;
+    if (out[b].skip || k >= N) return;
+    const int nbk = min(NB, N - k);
+    const int64_t lda = d.lda;
+    const double* A = at<double>(arena, d.off_mat);
+    double* P = panel_buf(arena, side[b], lda, pb);
+    const int r = k + blockIdx.x * GP_ROWS + threadIdx.x;
+    if (r >= N) return;
+    for (int j = 0; j < nbk; j++) P[(r - k) + (int64_t)j * lda] = A[r + (int64_t)(k + j) * lda];
+}
 
 // R rows per lane, leaves of LF columns, NT lanes (NT/256 waves per SIMD,
 // so 512/(NT/256) registers per lane hold the leaf and one chunk of L).
@@ -585,13 +724,26 @@ extern "C" {
 int capsd_lu_max_dim(void) { return 24 * 1024; }
 int capsd_lu_nb(void) { return NB; }
 
+// Panels taller than grid_panel_rows use the column-at-a-time grid kernels
+// (CAPS_AMD_GRID_PANEL_ROWS, default 6144: above it the one-workgroup panel
+// needs 24 rows per lane).
+static int grid_panel_rows = 6144;
+
 static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
                          int k, int pb, hipStream_t s)
 {
 #define PANEL(RR, LF, NT) \
     hipLaunchKernelGGL((k_lu_panel<RR, LF, NT>), dim3(nb), dim3(NT), (RR) * (NT) * sizeof(int), s, desc, arena, side, \
                        out, k, pb)
-    if (rows <= 2 * 256) PANEL(2, 8, 256);
+    if (rows > grid_panel_rows) {
+        const int nch = (rows + GP_ROWS - 1) / GP_ROWS;
+        hipLaunchKernelGGL(k_gp_max, dim3(nch, nb), dim3(GP_ROWS), 0, s, desc, arena, side, out, k, pb);
+        for (int c = 0; c < NB; c++) {
+            hipLaunchKernelGGL(k_gp_pivot, dim3(nb), dim3(64), 0, s, desc, arena, side, out, k, pb, c);
+            hipLaunchKernelGGL(k_gp_update, dim3(nch, nb), dim3(GP_ROWS), 0, s, desc, arena, side, out, k, pb, c);
+        }
+        hipLaunchKernelGGL(k_gp_copy, dim3(nch, nb), dim3(GP_ROWS), 0, s, desc, arena, side, out, k, pb);
+    } else if (rows <= 2 * 256) PANEL(2, 8, 256);
     else if (rows <= 4 * 256) PANEL(4, 8, 256);
     else if (rows <= 8 * 256) PANEL(8, 8, 256);
     else if (rows <= 6 * 512) PANEL(6, 4, 512);
@@ -602,8 +754,18 @@ static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuS
 
 int capsd_lu_init(void)
 {
+    if (const char* e = getenv("CAPS_AMD_GRID_PANEL_ROWS")) grid_panel_rows = atoi(e);
     return (int)hipFuncSetAttribute((const void*)k_lu_panel<24, 2, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     24 * 1024 * sizeof(int));
+}
+
+// Panel height above which the grid-wide panel kernels run; returns the
+// previous value (rows <= 0: query only).
+int capsd_lu_grid_panel_rows(int rows)
+{
+    const int old = grid_panel_rows;
+    if (rows > 0) grid_panel_rows = rows;
+    return old;
 }
 
 // Upper bound of the number of trailing-update launches of one LU.

@@ -212,6 +212,10 @@ int capsx_lu_config(int levels, int streams, int xcd, int tile);
 void capsx_lu_config_get(int maxN, int n, int cfg[4]);
 /* Replace the table by the file at path; returns the entries read. */
 int capsx_lu_tuning_load(const char *path);
+/* Panels taller than rows are factored column by column by grid-wide
+ * kernels (default 6144, above which one workgroup would hold more than 12
+ * rows per lane); rows <= 0 only queries.  Returns the previous value. */
+int capsx_lu_grid_panel_rows(int rows);
 /* Milliseconds of one batched LU of n synthetic dim x dim matrices with the
  * given schedule (values < 0 keep the configured ones), averaged over reps.
  * Returns 0 on success. */

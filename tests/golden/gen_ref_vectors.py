@@ -10,6 +10,7 @@ machines without the reference.
     python tests/golden/gen_ref_vectors.py free_energy # C2 free energy (~10 min, 8 cores)
     python tests/golden/gen_ref_vectors.py mixed       # 64 mixed-aspect PR blocks
     python tests/golden/gen_ref_vectors.py t0          # T = 0 free energies (dqagi)
+    python tests/golden/gen_ref_vectors.py c5          # R/L = 2000, ldim = 12000 blocks
 """
 import ctypes
 import json
@@ -279,6 +280,19 @@ def gen_t0(pool):
     return out
 
 
+def gen_c5(pool):
+    """Perfect-reflector blocks at R/L = 2000 (R = 2 mm, L = 1 µm), ldim =
+    12000 (N = 24000): the large-aspect configuration."""
+    jobs = [(xi, m) for xi, m in [(0.5, 0), (1.0, 0), (1.0, 3), (3.0, 1), (10.0, 5), (30.0, 20), (3.0, 40),
+                                    (100.0, 60)]]
+    t0 = time.time()
+    res = pool.map(_block, [(2e-3, 1e-6, 12000, None, xi, m) for xi, m in jobs], chunksize=1)
+    rows = [{"R": 2e-3, "L": 1e-6, "ldim": 12000, "xi_": xi, "m": m, "logdetD": v, "ref_seconds": dt}
+            for (xi, m), (v, dt) in zip(jobs, res)]
+    print("c5: %d blocks in %.0fs (8 workers)" % (len(rows), time.time() - t0))
+    return rows
+
+
 def main():
     what = sys.argv[1] if len(sys.argv) > 1 else "blocks"
     if what == "tables":
@@ -298,6 +312,9 @@ def main():
             elif what == "t0":
                 data = gen_t0(pool)
                 name = "ref_free_energy_t0.json"
+            elif what == "c5":
+                data = gen_c5(pool)
+                name = "ref_c5_blocks.json"
             else:
                 sys.exit("unknown target " + what)
     with open(os.path.join(HERE, name), "w") as f:

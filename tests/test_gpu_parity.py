@@ -381,3 +381,41 @@ def test_graph_replay_matches_eager(ca, streams):
         assert np.array_equal(g.logdetD_batch(xs, ms), ref1)
     finally:
         at.set_config(0)
+
+
+def test_lu_grid_panel_small_threshold(ca):
+    """The grid-wide column panel (used for tall panels) on small matrices,
+    with and without row interchanges, both LU schedules."""
+    from caps_amd import autotune as at
+    rng = np.random.default_rng(9)
+    N = 300
+    mats = [np.eye(N) - 0.3 * rng.normal(size=(N, N)) / math.sqrt(N) for _ in range(3)]
+    A = rng.normal(size=(N, N))
+    A[np.arange(N), np.arange(N)] = 0.0
+    mats.append(A)
+    mats = np.array(mats)
+    ref = [np.linalg.slogdet(M)[1] for M in mats]
+    old = ca.lib().capsx_lu_grid_panel_rows(100)
+    try:
+        for levels in (1, 2):
+            at.set_config(levels, 1, 0, 128)
+            out = ca.lu_logdet_batch(mats)
+            for v, r in zip(out, ref):
+                assert abs(v - r) < 1e-10 * N, (levels, v, r)
+    finally:
+        ca.lib().capsx_lu_grid_panel_rows(old)
+        at.set_config(0)
+
+
+def test_c5_blocks_vs_reference(ca):
+    """R/L = 2000, ldim = 12000 (N = 24000, 4.6 GB per matrix): eight blocks
+    in one batch against the reference (its default HODLR route) on the same
+    inputs; the LU's tall panels take the grid-wide panel kernels."""
+    rows = _ref("ref_c5_blocks.json")
+    c = ca.Caps(2e-3, 1e-6, ldim=12000)
+    out = c.logdetD_batch([r["xi_"] for r in rows], [r["m"] for r in rows])
+    t = c.last_timing()
+    print("C5: 8 blocks N=24000 in %.2f s (LU %.2f s), max rel err %.2e" % (
+        t["total"] / 1e3, t["lu"] / 1e3, max(abs(v / r["logdetD"] - 1) for v, r in zip(out, rows))))
+    bad = [(r["xi_"], r["m"], v, r["logdetD"]) for v, r in zip(out, rows) if not close_block(v, r["logdetD"])]
+    assert not bad, bad
