@@ -415,10 +415,11 @@ __global__ void __launch_bounds__(GP_ROWS) k_gp_copy(const BlockDesc* __restrict
     for (int j = 0; j < nbk; j++) P[(r - k) + (int64_t)j * lda] = A[r + (int64_t)(k + j) * lda];
 }
 
-// R rows per lane, leaves of LF columns, NT lanes (NT/256 waves per SIMD,
-// so 512/(NT/256) registers per lane hold the leaf and one chunk of L).
-template <int R, int LF, int NT>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT / 256, NT / 256)))
+// R rows per lane, leaves of LF columns, NT lanes; OCC waves per SIMD (OCC
+// NT/256 workgroups per CU share the registers that hold the leaf and one
+// chunk of L).
+template <int R, int LF, int NT, int OCC>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_lu_panel(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
            const BlockOut* __restrict__ out, int k, int pb)
 {
@@ -728,13 +729,16 @@ int capsd_lu_nb(void) { return NB; }
 // (CAPS_AMD_GRID_PANEL_ROWS, default 6144: above it the one-workgroup panel
 // needs 24 rows per lane).
 static int grid_panel_rows = 6144;
+// one-workgroup panel variants: 0 = one workgroup per CU up to 2048 rows, 1
+// = two (CAPS_AMD_PANEL_SET)
+static int panel_set = 0;
 
 static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
                          int k, int pb, hipStream_t s)
 {
-#define PANEL(RR, LF, NT) \
-    hipLaunchKernelGGL((k_lu_panel<RR, LF, NT>), dim3(nb), dim3(NT), (RR) * (NT) * sizeof(int), s, desc, arena, side, \
-                       out, k, pb)
+#define PANEL(RR, LF, NT, OCC)                                                                              \
+    hipLaunchKernelGGL((k_lu_panel<RR, LF, NT, OCC>), dim3(nb), dim3(NT), (RR) * (NT) * sizeof(int), s, desc, arena, \
+                       side, out, k, pb)
     if (rows > grid_panel_rows) {
         const int nch = (rows + GP_ROWS - 1) / GP_ROWS;
         hipLaunchKernelGGL(k_gp_max, dim3(nch, nb), dim3(GP_ROWS), 0, s, desc, arena, side, out, k, pb);
@@ -743,19 +747,30 @@ static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuS
             hipLaunchKernelGGL(k_gp_update, dim3(nch, nb), dim3(GP_ROWS), 0, s, desc, arena, side, out, k, pb, c);
         }
         hipLaunchKernelGGL(k_gp_copy, dim3(nch, nb), dim3(GP_ROWS), 0, s, desc, arena, side, out, k, pb);
-    } else if (rows <= 2 * 256) PANEL(2, 8, 256);
-    else if (rows <= 4 * 256) PANEL(4, 8, 256);
-    else if (rows <= 8 * 256) PANEL(8, 8, 256);
-    else if (rows <= 6 * 512) PANEL(6, 4, 512);
-    else if (rows <= 12 * 512) PANEL(12, 4, 512);
-    else PANEL(24, 2, 1024);
+    } else if (panel_set == 1) {
+        // two workgroups per CU: more matrices' panels in flight at once
+        if (rows <= 2 * 256) PANEL(2, 8, 256, 2);
+        else if (rows <= 4 * 256) PANEL(4, 4, 256, 2);
+        else if (rows <= 8 * 256) PANEL(8, 4, 256, 2);
+        else if (rows <= 6 * 512) PANEL(6, 4, 512, 2);
+        else if (rows <= 12 * 512) PANEL(12, 4, 512, 2);
+        else PANEL(24, 2, 1024, 4);
+    } else {
+        if (rows <= 2 * 256) PANEL(2, 8, 256, 1);
+        else if (rows <= 4 * 256) PANEL(4, 8, 256, 1);
+        else if (rows <= 8 * 256) PANEL(8, 8, 256, 1);
+        else if (rows <= 6 * 512) PANEL(6, 4, 512, 2);
+        else if (rows <= 12 * 512) PANEL(12, 4, 512, 2);
+        else PANEL(24, 2, 1024, 4);
+    }
 #undef PANEL
 }
 
 int capsd_lu_init(void)
 {
     if (const char* e = getenv("CAPS_AMD_GRID_PANEL_ROWS")) grid_panel_rows = atoi(e);
-    return (int)hipFuncSetAttribute((const void*)k_lu_panel<24, 2, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (const char* e = getenv("CAPS_AMD_PANEL_SET")) panel_set = atoi(e);
+    return (int)hipFuncSetAttribute((const void*)k_lu_panel<24, 2, 1024, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     24 * 1024 * sizeof(int));
 }
 

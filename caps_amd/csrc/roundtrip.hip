@@ -347,67 +347,11 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
         }
     }
 
-    double aqm2 = 1, aqm1 = 0, aq = 1;  // a_{q-2}, a_{q-1}, a_q
-    double rn1 = 0, rv1 = 0, r3 = 0, tn = 0, tv = 0;
-    int S = 0;                           // a_q carries a factor 2^S
-    // α(p+3), α(p+4) of a step are α(p+1), α(p+2) of the step before
-    const double s2 = (n + nu + 1) * (n + nu + 1), d2 = (n - nu) * (n - nu);
-    AlphaND al_next3 = {0, 1}, al_next4 = {0, 1};
-    bool have_al = false;
-    for (int q = 1; q <= qmax && ndone < GM * NPOL; q++) {
-        if (q == 1) {
-            // eqs. (26)-(30) with the reciprocals shared between q = 1 and 2
-            rn1 = 1 / (2 * n - 1);
-            rv1 = 1 / (2 * nu - 1);
-            r3 = 1 / (n4 * (n4 - 1));
-            tn = (m - n) * (m - n + 1) * rn1;
-            tv = (m - nu) * (m - nu + 1) * rv1;
-            aq = (n + nu - 1.5) * (1 - (2 * n + 2 * nu - 1) * r3 * (tn + tv));
-            aqm1 = 1;
-        } else if (q == 2) {
-            aqm2 = aqm1;
-            aqm1 = aq;
-            const double rn3 = 1 / (2 * n - 3), rv3 = 1 / (2 * nu - 3), r6 = 1 / (2 * (n4 - 2) * (n4 - 3));
-            aq = (2 * n + 2 * nu - 1) * (2 * n + 2 * nu - 7) / 4 * ((2 * n + 2 * nu - 3) * r3
-                 * ((2 * n + 2 * nu - 5) * r6
-                 * (tn * (m - n + 2) * (m - n + 3) * rn3 + 2 * tn * tv + tv * (m - nu + 2) * (m - nu + 3) * rv3)
-                 - tn - tv) + 0.5);
-        } else {
-            aqm2 = aqm1;
-            aqm1 = aq;
-            const double pq = n + nu - 2 * q, p1 = pq - 2 * m, p2 = pq + 2 * m;
-            const AlphaND al1 = alpha_nd(pq + 1, s2, d2), al2 = alpha_nd(pq + 2, s2, d2);
-            const AlphaND al3 = have_al ? al_next3 : alpha_nd(pq + 3, s2, d2);
-            const AlphaND al4 = have_al ? al_next4 : alpha_nd(pq + 4, s2, d2);
-            al_next3 = al1;
-            al_next4 = al2;
-            have_al = true;
-            if (Ap != 0) {
-                // a_q = (c1 a_{q-1} + c2 a_{q-2}) / c0 of eq. (35) with
-                //   c0 = (p+2)(p+3)(p1+1)(p1+2) Ap α1,
-                //   c1 = Ap [Ap² + (p+1)(p+3)(p1+2)(p2+2) α2 + (p+2)(p+4)(p1+3)(p2+3) α3],
-                //   c2 = -(p+2)(p+3)(p2+3)(p2+4) Ap α4;
-                // Ap cancels, and numerator and denominator are multiplied by
-                // Dα1 Dα2 Dα3 Dα4
-                const double k2 = (pq + 1) * (pq + 3) * (p1 + 2) * (p2 + 2);
-                const double k3 = (pq + 2) * (pq + 4) * (p1 + 3) * (p2 + 3);
-                const double k4 = (pq + 2) * (pq + 3) * (p2 + 3) * (p2 + 4);
-                const double d23 = al2.D * al3.D;
-                const double num1 = (Ap * Ap * d23 + k2 * al2.N * al3.D + k3 * al3.N * al2.D) * al4.D;
-                const double num2 = k4 * al4.N * d23;
-                const double den = (pq + 2) * (pq + 3) * (p1 + 1) * (p1 + 2) * al1.N * d23 * al4.D;
-                aq = (num1 * aqm1 - num2 * aqm2) * al1.D / den;
-            } else {
-                aq = (pq + 1) * (p2 + 2) * al2.N * al1.D * aqm1 / ((pq + 2) * (p1 + 1) * al1.N * al2.D);
-            }
-            const double fa = fabs(aq);
-            if (fa > 0x1p332 || (fa < 0x1p-332 && fa > 0)) {
-                const int e = ilogb(aq);
-                aq = ldexp(aq, -e);
-                aqm1 = ldexp(aqm1, -e);
-                S += e;
-            }
-        }
+    // add a_q K_{l1+l2-2q} (a_q = aq 2^S) to every block's sum; from q = 3 on
+    // with the exit rules: the reference's (three terms below e^-60 of the
+    // first), or three terms below 2^-56 of the running sum (below half an
+    // ulp of the sum, they leave it unchanged)
+    auto accumulate = [&](int q, double aq, int S, bool check) {
         const int j = l12 - 2 * q - mu;
 #pragma unroll
         for (int g = 0; g < GM; g++) {
@@ -416,11 +360,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 if (cnt[g][pp] >= 3) continue;
                 const double term = ldexp(aq * km[g][pp][j], S + ke[g][pp][j] - E0[g][pp]);
                 sum[g][pp] += term;
-                if (q >= 3) {
-                    // the reference's rule (three terms below e^-60 of the
-                    // first), or three terms below 2^-56 of the running sum:
-                    // such terms are below half an ulp of the sum and leave
-                    // it unchanged
+                if (check) {
                     const double aterm = fabs(term);
                     cnt[g][pp] = aterm < thr * t0abs[g][pp] ? cnt[g][pp] + 1 : 0;
                     cs[g][pp] = aterm < ldexp(fabs(sum[g][pp]), -56) ? cs[g][pp] + 1 : 0;
@@ -431,6 +371,69 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 }
             }
         }
+    };
+
+    double aqm2 = 1, aqm1 = 1, aq = 1;  // a_{q-2}, a_{q-1}, a_q
+    double tn = 0, tv = 0, r3 = 0, rn1 = 0, rv1 = 0;
+    if (qmax >= 1) {
+        // eqs. (26)-(30) with the reciprocals shared between q = 1 and 2
+        rn1 = 1 / (2 * n - 1);
+        rv1 = 1 / (2 * nu - 1);
+        r3 = 1 / (n4 * (n4 - 1));
+        tn = (m - n) * (m - n + 1) * rn1;
+        tv = (m - nu) * (m - nu + 1) * rv1;
+        aq = (n + nu - 1.5) * (1 - (2 * n + 2 * nu - 1) * r3 * (tn + tv));
+        accumulate(1, aq, 0, false);
+    }
+    if (qmax >= 2) {
+        aqm2 = 1;
+        aqm1 = aq;
+        const double rn3 = 1 / (2 * n - 3), rv3 = 1 / (2 * nu - 3), r6 = 1 / (2 * (n4 - 2) * (n4 - 3));
+        aq = (2 * n + 2 * nu - 1) * (2 * n + 2 * nu - 7) / 4 * ((2 * n + 2 * nu - 3) * r3
+             * ((2 * n + 2 * nu - 5) * r6
+             * (tn * (m - n + 2) * (m - n + 3) * rn3 + 2 * tn * tv + tv * (m - nu + 2) * (m - nu + 3) * rv3)
+             - tn - tv) + 0.5);
+        accumulate(2, aq, 0, false);
+    }
+    // q >= 3: eq. (35).  α(p+3), α(p+4) of a step are α(p+1), α(p+2) of the
+    // step before (p = l1 + l2 - 2q).
+    int S = 0;                           // a_q carries a factor 2^S
+    const double s2 = (n + nu + 1) * (n + nu + 1), d2 = (n - nu) * (n - nu);
+    AlphaND al3 = alpha_nd(n + nu - 3, s2, d2), al4 = alpha_nd(n + nu - 2, s2, d2);
+#pragma unroll 2
+    for (int q = 3; q <= qmax && ndone < GM * NPOL; q++) {
+        aqm2 = aqm1;
+        aqm1 = aq;
+        const double pq = n + nu - 2 * q, p1 = pq - 2 * m, p2 = pq + 2 * m;
+        const AlphaND al1 = alpha_nd(pq + 1, s2, d2), al2 = alpha_nd(pq + 2, s2, d2);
+        if (Ap != 0) {
+            // a_q = (c1 a_{q-1} + c2 a_{q-2}) / c0 of eq. (35) with
+            //   c0 = (p+2)(p+3)(p1+1)(p1+2) Ap α1,
+            //   c1 = Ap [Ap² + (p+1)(p+3)(p1+2)(p2+2) α2 + (p+2)(p+4)(p1+3)(p2+3) α3],
+            //   c2 = -(p+2)(p+3)(p2+3)(p2+4) Ap α4;
+            // Ap cancels, and numerator and denominator are multiplied by
+            // Dα1 Dα2 Dα3 Dα4
+            const double k2 = (pq + 1) * (pq + 3) * (p1 + 2) * (p2 + 2);
+            const double k3 = (pq + 2) * (pq + 4) * (p1 + 3) * (p2 + 3);
+            const double k4 = (pq + 2) * (pq + 3) * (p2 + 3) * (p2 + 4);
+            const double d23 = al2.D * al3.D;
+            const double num1 = (Ap * Ap * d23 + k2 * al2.N * al3.D + k3 * al3.N * al2.D) * al4.D;
+            const double num2 = k4 * al4.N * d23;
+            const double den = (pq + 2) * (pq + 3) * (p1 + 1) * (p1 + 2) * al1.N * d23 * al4.D;
+            aq = (num1 * aqm1 - num2 * aqm2) * al1.D / den;
+        } else {
+            aq = (pq + 1) * (p2 + 2) * al2.N * al1.D * aqm1 / ((pq + 2) * (p1 + 1) * al1.N * al2.D);
+        }
+        al3 = al1;
+        al4 = al2;
+        const double fa = fabs(aq);
+        if (fa > 0x1p332 || (fa < 0x1p-332 && fa > 0)) {
+            const int e = ilogb(aq);
+            aq = ldexp(aq, -e);
+            aqm1 = ldexp(aqm1, -e);
+            S += e;
+        }
+        accumulate(q, aq, S, true);
     }
 
     // I = e^{log a0} · sum · 2^{E0}, stored as a signed mantissa times 2^exp
