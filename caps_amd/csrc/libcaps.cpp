@@ -229,7 +229,7 @@ double env_double(const char* name, double dflt)
 // levels: 1 = 64-column steps, 2 = 128-column steps of two panels with one
 // K = 128 trailing update; streams: LU groups of blocks on separate streams;
 // xcd: trailing-update tile order (1 = all tiles of a block on one XCD);
-// tile: trailing-update workgroup tile, 128 or 64.
+// tile: trailing-update workgroup tile, 128 (x 128), 256 (x 128) or 64 (x 64).
 struct LuCfg {
     int levels = 2, streams = 1, xcd = 0, tile = 128;
 };
@@ -262,7 +262,7 @@ int load_tuning(const char* path)
         if (sscanf(line, "%d %d %d %d %d %d", &e.maxN, &e.n, &e.cfg.levels, &e.cfg.streams, &e.cfg.xcd,
                    &e.cfg.tile) == 6 &&
             e.maxN > 0 && e.n > 0 && (e.cfg.levels == 1 || e.cfg.levels == 2) && e.cfg.streams >= 1 &&
-            (e.cfg.tile == 64 || e.cfg.tile == 128))
+            (e.cfg.tile == 64 || e.cfg.tile == 128 || e.cfg.tile == 256))
             t.push_back(e);
     }
     fclose(f);
@@ -314,7 +314,10 @@ LuCfg lu_config(int maxN, int n)
     if (getenv("CAPS_AMD_LU_LEVELS")) c.levels = env_double("CAPS_AMD_LU_LEVELS", 2) >= 2 ? 2 : 1;
     if (getenv("CAPS_AMD_LU_STREAMS")) c.streams = std::max(1, (int)env_double("CAPS_AMD_LU_STREAMS", 1));
     if (getenv("CAPS_AMD_GEMM_XCD")) c.xcd = env_double("CAPS_AMD_GEMM_XCD", 0) != 0 ? 1 : 0;
-    if (getenv("CAPS_AMD_GEMM_TILE")) c.tile = env_double("CAPS_AMD_GEMM_TILE", 128) <= 64 ? 64 : 128;
+    if (getenv("CAPS_AMD_GEMM_TILE")) {
+        const double t = env_double("CAPS_AMD_GEMM_TILE", 128);
+        c.tile = t <= 64 ? 64 : t >= 256 ? 256 : 128;
+    }
     return c;
 }
 
@@ -1596,7 +1599,7 @@ int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, in
     if (levels > 0) cfg.levels = levels >= 2 ? 2 : 1;
     if (streams > 0) cfg.streams = streams;
     if (xcd >= 0) cfg.xcd = xcd ? 1 : 0;
-    if (tile > 0) cfg.tile = tile <= 64 ? 64 : 128;
+    if (tile > 0) cfg.tile = tile <= 64 ? 64 : tile >= 256 ? 256 : 128;
     double tot = 0;
     for (int r = 0; r < reps; r++) {
         HIPCHECK(hipMemsetAsync(c->d_out, 0, n * sizeof(BlockOut), c->stream));
@@ -1631,7 +1634,7 @@ void capsx_set_graph(caps_t* self, int on)
 }
 
 // Extension: fix the LU schedule of all later batches (levels 1|2, streams
-// >= 1, xcd 0|1, tile 64|128); levels = 0 returns to the autotuned table.
+// >= 1, xcd 0|1, tile 64|128|256); levels = 0 returns to the autotuned table.
 // Returns 0 on success, 1 on invalid values.
 int capsx_lu_config(int levels, int streams, int xcd, int tile)
 {
@@ -1640,7 +1643,8 @@ int capsx_lu_config(int levels, int streams, int xcd, int tile)
         g_have_override = false;
         return 0;
     }
-    if ((levels != 1 && levels != 2) || streams < 1 || (xcd != 0 && xcd != 1) || (tile != 64 && tile != 128))
+    if ((levels != 1 && levels != 2) || streams < 1 || (xcd != 0 && xcd != 1) ||
+        (tile != 64 && tile != 128 && tile != 256))
         return 1;
     g_override.levels = levels;
     g_override.streams = streams;

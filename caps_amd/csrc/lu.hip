@@ -561,8 +561,8 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     const double* P1 = panel_buf(arena, side[b], lda, 0);
     const double* P2 = panel_buf(arena, side[b], lda, 1);
     // one LDS buffer: the A and B stages, reused for the C strips at the end
-    constexpr int SMEM = KS * (GTPM + GTPN);
-    static_assert(SMEM >= WM * WN * (64 * 17 + 4), "LDS holds the epilogue strips");
+    constexpr int STG = KS * (GTPM + GTPN), STRIPS = WM * WN * (64 * 17 + 4);
+    constexpr int SMEM = STG > STRIPS ? STG : STRIPS;   // stages, then the epilogue strips
     __shared__ double smem[SMEM];
     double(*As)[GTPM] = reinterpret_cast<double(*)[GTPM]>(smem);             // [kk][row]
     double(*Bs)[GTPN] = reinterpret_cast<double(*)[GTPN]>(smem + KS * GTPM);  // [kk][col]
@@ -729,9 +729,10 @@ int capsd_lu_nb(void) { return NB; }
 // (CAPS_AMD_GRID_PANEL_ROWS, default 6144: above it the one-workgroup panel
 // needs 24 rows per lane).
 static int grid_panel_rows = 6144;
-// one-workgroup panel variants: 0 = one workgroup per CU up to 2048 rows, 1
-// = two (CAPS_AMD_PANEL_SET)
-static int panel_set = 0;
+// one-workgroup panel variants: 1 = two workgroups per CU (leaves of 4
+// columns above 512 rows), 0 = one workgroup per CU with leaves of 8 up to
+// 2048 rows (CAPS_AMD_PANEL_SET, default 1)
+static int panel_set = 1;
 
 static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
                          int k, int pb, hipStream_t s)
@@ -792,8 +793,9 @@ int capsd_lu_max_updates(int maxN) { return 3 * ((maxN + 2 * NB - 1) / (2 * NB))
 // the second panel's columns, panel, swaps of both panels + TRSM of the
 // first 64 U rows, a K = NB update of the second 64 U rows, their TRSM, one
 // K = 2NB update of the trailing matrix.
-// tile: 128 (update workgroups of 128 x 128, 4 wavefronts) or 64 (64 x 64,
-// one wavefront); xcd_order: tile order of the update (k_lu_gemm).
+// tile: 128 (update workgroups of 128 x 128, 4 wavefronts), 256 (256 x 128,
+// 8 wavefronts) or 64 (64 x 64, one wavefront); xcd_order: tile order of the
+// update (k_lu_gemm).
 // gemm_ev: NULL, or 2 events per update launch recorded around it;
 // gemm_info receives (k, mode) of each update launch.  Returns the number
 // of update launches.
@@ -809,6 +811,10 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
             const int gx = (rows + 63) / 64, gy = (cols + 63) / 64;
             hipLaunchKernelGGL((k_lu_gemm<1, 1>), dim3(gx * gy * nb8), dim3(64), 0, s, desc, arena, sd, out, k, mode,
                                gx, gy, nblocks, xcd_order);
+        } else if (tile == 256) {
+            const int gx = (rows + 255) / 256, gy = (cols + 127) / 128;
+            hipLaunchKernelGGL((k_lu_gemm<4, 2>), dim3(gx * gy * nb8), dim3(512), 0, s, desc, arena, sd, out, k,
+                               mode, gx, gy, nblocks, xcd_order);
         } else {
             const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
             hipLaunchKernelGGL((k_lu_gemm<2, 2>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k,

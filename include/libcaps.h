@@ -192,7 +192,7 @@ int capsx_logdetD_multi(caps_t *const *objs, const double *xi_, const int *m, in
  * steps, 2 = 128-column steps of two panels with one K = 128 trailing
  * update; streams = groups of blocks factored on separate HIP streams; xcd =
  * trailing-update tile order (1: all tiles of a block on one XCD); tile =
- * trailing-update workgroup tile (128 or 64).  By default the library picks,
+ * trailing-update workgroup tile (128, 256 or 64 rows).  By default the library picks,
  * per batch, the nearest shape (largest dimension, number of blocks) of the
  * autotuned table lu_tuning.txt next to the library (or the file named by
  * CAPS_AMD_TUNING), written by caps_amd/autotune.py from capsx_lu_time.

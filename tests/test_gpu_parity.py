@@ -330,7 +330,7 @@ def test_lu_schedules_agree(ca):
     mats = np.array(mats)
     ref = [np.linalg.slogdet(M)[1] for M in mats]
     try:
-        for cfg in itertools.product((1, 2), (1, 2), (0, 1), (64, 128)):
+        for cfg in itertools.product((1, 2), (1, 2), (0, 1), (64, 128, 256)):
             at.set_config(*cfg)
             out = ca.lu_logdet_batch(mats)
             for v, r in zip(out, ref):
