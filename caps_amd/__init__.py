@@ -71,6 +71,7 @@ def lib():
             "capsx_profile": (None, [p, i]),
             "capsx_set_graph": (None, [p, i]),
             "capsx_lu_grid_panel_rows": (i, [i]),
+            "capsx_surrogate_report": (None, [p, dp, i]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -223,11 +224,21 @@ class Caps:
         Returns (F, determinants computed, determinants entering the sums).
         mode: "plain" (one block at a time), "batched" (planned from the
         first Matsubara terms, or one batch per quadrature level) or
-        "surrogate" (planned by an attached caps_amd.surrogate.MLP; T > 0)."""
+        "surrogate" (planned by an attached caps_amd.surrogate.MLP: at T > 0
+        the m-cutoffs and end of the Matsubara sum, at T = 0 the m-cutoff of
+        every quadrature node)."""
         nb, nu = ctypes.c_long(), ctypes.c_long()
         F = lib().caps_free_energy(self.ptr, T, cutoff, epsrel, {"plain": 0, "batched": 1, "surrogate": 2}[mode],
                                    1 if drude_xi0 else 0, ctypes.byref(nb), ctypes.byref(nu))
         return F, nb.value, nu.value
+
+    def surrogate_report(self, reset=True):
+        """Surrogate accuracy on the blocks computed for its plans (mode
+        "surrogate") since the last reset."""
+        out = np.zeros(6)
+        lib().capsx_surrogate_report(self.ptr, _dptr(out), 1 if reset else 0)
+        return {"blocks": int(out[0]), "rms_log_error": out[1], "max_log_error": out[2],
+                "frequencies": int(out[3]), "mean_mcut_error": out[4], "max_mcut_error": int(out[5])}
 
     def logdetD0(self, m):
         """ξ = 0 EE and MM blocks (perfect reflector / Drude)."""

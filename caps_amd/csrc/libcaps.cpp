@@ -132,6 +132,10 @@ struct Ctx {
     // is evaluated with
     struct capsx_mlp* sur = nullptr;
     float sur_feat[2] = {0, 0};
+    // surrogate against the computed blocks of its plans since the last
+    // capsx_surrogate_report reset: blocks, sum of squared and largest
+    // |log error|, frequencies, sum and largest |m-cutoff error|
+    double sur_acc[6] = {};
     long plan_batches = 0;
     std::mutex mu;
 };
@@ -1227,6 +1231,55 @@ struct Term {
     double value = 0;
 };
 
+// Surrogate predictions of log|logdetD(ξ_, m)| for m = 0..mcap at each of
+// the frequencies xs (row q of P holds frequency q).
+void sur_predict(caps_t* self, const std::vector<double>& xs, int mcap, std::vector<float>& P)
+{
+    Ctx* cx = ctx_of(self);
+    const size_t per = (size_t)mcap + 1;
+    std::vector<float> X(xs.size() * per * 5);
+    P.assign(xs.size() * per, 0.f);
+    for (size_t q = 0; q < xs.size(); q++)
+        for (int m = 0; m <= mcap; m++) {
+            float* x = &X[(q * per + m) * 5];
+            x[0] = (float)log(1 / self->LbyR);
+            x[1] = (float)log(xs[q]);
+            x[2] = (float)m / self->ldim;
+            x[3] = cx->sur_feat[0];
+            x[4] = cx->sur_feat[1];
+        }
+    if (!xs.empty()) capsx_mlp_predict(cx->sur, X.data(), (int)(xs.size() * per), P.data());
+}
+
+// Predicted m-cutoff of one frequency: the first m whose predicted ratio
+// logdetD(m)/logdetD(0) falls below the cutoff (mcap if none does).
+int sur_mcut(const float* pr, int mcap, double cutoff)
+{
+    const double lcut = log(cutoff);
+    for (int m = 1; m <= mcap; m++)
+        if (pr[m] - pr[0] < lcut) return m;
+    return mcap;
+}
+
+// Score a plan against the computed terms: log errors of every computed
+// block with a prediction, m-cutoff error of the finished term.
+void sur_score(Ctx* cx, const float* pr, int mcap, int mpred, const Term& t)
+{
+    for (size_t m = 0; m < t.v.size() && (int)m <= mcap; m++) {
+        if (t.v[m] == 0 || !std::isfinite(t.v[m])) continue;
+        const double e = fabs((double)pr[m] - log(fabs(t.v[m])));
+        cx->sur_acc[0] += 1;
+        cx->sur_acc[1] += e * e;
+        cx->sur_acc[2] = std::max(cx->sur_acc[2], e);
+    }
+    if (t.mstop >= 0) {
+        const double d = fabs((double)(mpred - t.mstop));
+        cx->sur_acc[3] += 1;
+        cx->sur_acc[4] += d;
+        cx->sur_acc[5] = std::max(cx->sur_acc[5], d);
+    }
+}
+
 // Evaluate the listed (term, m-range) requests in one batch.
 void run_requests(caps_t* self, std::vector<Term>& terms, const std::vector<std::pair<int, int>>& req,
                   long* computed)
@@ -1310,12 +1363,27 @@ double free_energy_T0(caps_t* self, double cutoff, double epsrel, int mode, long
                     if (msum(t.v, cutoff, &t.value, &t.mstop)) break;
                 }
         } else if (!terms.empty()) {
-            // first guess of each node's m-range: the m-cutoff of the
-            // nearest known node at a lower frequency (m-cutoffs fall with
-            // ξ) plus a margin, else 64; then every node is extended until
-            // its own cutoff is reached
+            // first guess of each node's m-range: with the surrogate (mode
+            // 2) its predicted m-cutoff plus a margin, else the m-cutoff of
+            // the nearest known node at a lower frequency (m-cutoffs fall
+            // with ξ at large ξ) plus a margin, else 64; then every node is
+            // extended until its own cutoff is reached
             std::vector<std::pair<int, int>> req;
-            for (size_t q = 0; q < terms.size(); q++) {
+            const int MCAP = std::min(self->ldim - 1, 4000), per = MCAP + 1;
+            std::vector<float> P;
+            std::vector<int> mpred;
+            if (mode == 2) {
+                TERMINATE(!cx->sur, "caps_free_energy mode 2 needs a surrogate (capsx_set_surrogate)");
+                std::vector<double> xs;
+                for (const Term& t : terms) xs.push_back(t.xi);
+                sur_predict(self, xs, MCAP, P);
+                for (size_t q = 0; q < terms.size(); q++) {
+                    const int ms = sur_mcut(&P[q * per], MCAP, cutoff);
+                    mpred.push_back(ms);
+                    req.push_back({(int)q, std::min(self->ldim * 8, ms + std::max(4, ms / 25) + 1)});
+                }
+            }
+            for (size_t q = 0; q < terms.size() && mode != 2; q++) {
                 int guess = -1;
                 double best = -1;
                 for (int idx = 1; idx <= MMAX; idx++) {
@@ -1340,6 +1408,7 @@ double free_energy_T0(caps_t* self, double cutoff, double epsrel, int mode, long
                     run_requests(self, terms, {{(int)q, upto}}, &computed);
                     cx->plan_batches++;
                 }
+                if (mode == 2) sur_score(cx, &P[q * per], MCAP, mpred[q], t);
             }
         }
         for (size_t q = 0; q < terms.size(); q++) {
@@ -1437,27 +1506,16 @@ double caps_free_energy(caps_t* self, double T, double cutoff, double epsrel, in
             const int NCAP = 48;
             const int MCAP = std::min(self->ldim - 1, 4000);
             const int per = MCAP + 1;
-            std::vector<float> X((size_t)NCAP * per * 5), P((size_t)NCAP * per);
-            for (int j = 1; j <= NCAP; j++)
-                for (int m = 0; m <= MCAP; m++) {
-                    float* x = &X[((size_t)(j - 1) * per + m) * 5];
-                    x[0] = (float)log(1 / self->LbyR);
-                    x[1] = (float)log(j * T_scaled);
-                    x[2] = (float)m / self->ldim;
-                    x[3] = cx->sur_feat[0];
-                    x[4] = cx->sur_feat[1];
-                }
-            capsx_mlp_predict(cx->sur, X.data(), NCAP * per, P.data());
+            std::vector<double> xs;
+            for (int j = 1; j <= NCAP; j++) xs.push_back(j * T_scaled);
+            std::vector<float> P;
+            sur_predict(self, xs, MCAP, P);
             std::vector<std::pair<int, int>> req;
-            const double lcut = log(cutoff);
+            std::vector<int> mpred(NCAP + 1, 0);
             for (int j = 1; j <= NCAP; j++) {
                 const float* pr = &P[(size_t)(j - 1) * per];
-                int ms = MCAP;
-                for (int m = 1; m <= MCAP; m++)
-                    if (pr[m] - pr[0] < lcut) {
-                        ms = m;
-                        break;
-                    }
+                const int ms = sur_mcut(pr, MCAP, cutoff);
+                mpred[j] = ms;
                 double vj = exp((double)pr[0]) / 2;
                 for (int m = 1; m <= ms; m++) vj += exp((double)pr[m]);
                 const int margin = std::max(4, ms / 25);
@@ -1471,6 +1529,7 @@ double caps_free_energy(caps_t* self, double T, double cutoff, double epsrel, in
                 const int j = req[r].first;
                 Term& t = term(j);
                 if (!msum(t.v, cutoff, &t.value, &t.mstop)) finish_term(j, (int)t.v.size() + 16, 16);
+                sur_score(cx, &P[(size_t)(j - 1) * per], MCAP, mpred[j], t);
                 accept(j);
             }
         }
@@ -1782,6 +1841,24 @@ void capsx_set_surrogate(caps_t* self, struct capsx_mlp* mlp, float f_omegap, fl
     c->sur = mlp;
     c->sur_feat[0] = f_omegap;
     c->sur_feat[1] = f_gamma;
+}
+
+// Extension: accuracy of the planner surrogate against the blocks computed
+// for its plans since the last reset (resets when reset != 0):
+// out = {blocks compared, RMS and largest |log|logdetD| error|, frequencies
+// compared, mean and largest |predicted - actual m-cutoff|}.
+void capsx_surrogate_report(caps_t* self, double* out, int reset)
+{
+    Ctx* c = ctx_of(self);
+    const double* a = c->sur_acc;
+    out[0] = a[0];
+    out[1] = a[0] > 0 ? sqrt(a[1] / a[0]) : 0;
+    out[2] = a[2];
+    out[3] = a[3];
+    out[4] = a[3] > 0 ? a[4] / a[3] : 0;
+    out[5] = a[5];
+    if (reset)
+        for (double& x : c->sur_acc) x = 0;
 }
 
 // Extension: launch sequences (batches) the free-energy driver used since

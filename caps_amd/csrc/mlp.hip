@@ -347,7 +347,8 @@ void capsx_mlp_grad(capsx_mlp* h, const float* Xn, const float* Tn, int n, float
 }
 
 // Train on raw (X, Y) with Adam, minibatches of `batch` samples, the
-// sample order shuffled each epoch (SplitMix64(seed)).  Sets the
+// sample order shuffled each epoch (SplitMix64(seed)), the step size
+// cosine-annealed from lr in the first epoch to lr/20 in the last.  Sets the
 // standardisation from the data.  Returns the final standardised MSE over
 // the whole set; loss_curve (may be NULL) receives one value per epoch.
 float capsx_mlp_train(capsx_mlp* h, const float* X, const float* Y, int n, int epochs, int batch, float lr,
@@ -398,6 +399,8 @@ float capsx_mlp_train(capsx_mlp* h, const float* X, const float* Y, int n, int e
             std::swap(perm[i], perm[j]);
         }
         MCHECK(hipMemcpyAsync(dI, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, h->s));
+        const float lr_ep =
+            lr / 20 + (lr - lr / 20) * 0.5f * (1 + cosf(3.14159265f * ep / std::max(1, epochs - 1)));
         for (int s0 = 0; s0 + batch <= n || s0 == 0; s0 += batch) {
             const int nb = std::min(batch, n - s0);
             const int nt = (nb + TILE - 1) / TILE;
@@ -406,7 +409,7 @@ float capsx_mlp_train(capsx_mlp* h, const float* X, const float* Y, int n, int e
             hipLaunchKernelGGL(k_mlp_reduce, dim3((NPAR + 255) / 256), dim3(256), 0, h->s, dG, nt, dg);
             h->step++;
             const float c1 = 1 - powf(b1, (float)h->step), c2 = 1 - powf(b2, (float)h->step);
-            hipLaunchKernelGGL(k_adam, dim3((NPAR + 255) / 256), dim3(256), 0, h->s, h->P, dg, h->M, h->V, lr, b1,
+            hipLaunchKernelGGL(k_adam, dim3((NPAR + 255) / 256), dim3(256), 0, h->s, h->P, dg, h->M, h->V, lr_ep, b1,
                                b2, eps, c1, c2);
             if (s0 + batch >= n) break;
         }

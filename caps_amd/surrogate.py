@@ -2,11 +2,12 @@
 log|logdetD| from (log R/L, log ξ(L+R)/c, m/ldim, log ωp/eV, log γ/eV).
 
 Training data are blocks computed by the GPU path itself (caps_logdetD_batch)
-on configurations sampled from seed 1; the network is trained by the
-library's HIP forward/backward/Adam kernels (caps_amd/csrc/mlp.hip).  The
-free-energy driver uses it only to plan which blocks to batch (mode
-"surrogate"); every planned Matsubara term is verified against the cutoff
-rules, so the energy equals the plain form's.
+on 300 configurations sampled from seed 1 (R/L log-uniform in [20, 1000],
+ldim = 6 R/L, about 2e5 blocks); the network is trained by the library's
+HIP forward/backward/Adam kernels (caps_amd/csrc/mlp.hip).  The free-energy
+driver uses it only to plan which blocks to batch (mode "surrogate"); every
+planned frequency is verified against the cutoff rules, so the energy equals
+the plain form's.
 
 Material encoding: perfect reflector -> (log ωp, log γ) = (PR_OMEGAP,
 NO_GAMMA); plasma (γ = 0) -> (log ωp, NO_GAMMA); Drude -> (log ωp, log γ).
@@ -142,65 +143,90 @@ class MLP:
         caps._surrogate = self
 
 
-def sample_configs(n_configs=30, seed=1):
-    """(R/L, material, T) triples from seed 1: R/L log-uniform in [20, 500],
-    material perfect reflector / plasma / Drude with ωp log-U[1, 20] eV and
-    γ log-U[0.005, 0.1] eV, temperature log-U[50, 600] K."""
+def sample_configs(n_configs=300, seed=1):
+    """(R/L, material) pairs from seed 1: R/L log-uniform in [20, 1000];
+    perfect reflector, plasma and Drude in turn, with ωp log-U[1, 20] eV and
+    γ log-U[0.005, 0.1] eV."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n_configs):
-        RbyL = float(math.exp(rng.uniform(math.log(20), math.log(500))))
+        RbyL = float(math.exp(rng.uniform(math.log(20), math.log(1000))))
         kind = ["pr", "plasma", "drude"][k % 3]
         wp = float(math.exp(rng.uniform(0, math.log(20))))
         gm = float(math.exp(rng.uniform(math.log(0.005), math.log(0.1))))
-        T = float(math.exp(rng.uniform(math.log(50), math.log(600))))
         out.append({"RbyL": RbyL, "kind": kind, "omegap": None if kind == "pr" else wp,
-                    "gamma": gm if kind == "drude" else None, "T": T})
+                    "gamma": gm if kind == "drude" else None})
     return out
 
 
-def generate(n_configs=30, xi_per_config=6, m_per_xi=40, seed=1, ldim_factor=6, log=print):
-    """Blocks of the GPU path on sampled configurations; returns X (n x 5)
-    and Y = log|logdetD|."""
+def blocks_per_config(RbyL, full=1100, knee=150.0, floor=40):
+    """Blocks drawn for one configuration: `full` up to R/L = knee, then
+    (knee/(R/L))^3 of that (a block costs ~ ldim^3), at least `floor`."""
+    return max(floor, int(full * min(1.0, (knee / RbyL) ** 3)))
+
+
+def block_sample(RbyL, ldim, nblocks, rng):
+    """(ξ(L+R)/c, m) of the blocks of one configuration: ξ(L+R)/c
+    log-uniform in [0.02, 40 (1 + R/L)] (where the Matsubara terms and most
+    of the T = 0 integral lie; beyond it logdetD(m = 0) < e^-80) for 70% of
+    the blocks, in [1e-4, 120 (1 + R/L)] (the T = 0 quadrature nodes up to
+    the frequency cutoff) for the rest; m uniform in [0, 16 sqrt(R/L) + 16]
+    (about twice the m-cutoff 1e-9)."""
+    wide = rng.random(nblocks) < 0.3
+    lo = np.where(wide, math.log(1e-4), math.log(0.02))
+    hi = np.where(wide, math.log(120 * (1 + RbyL)), math.log(40 * (1 + RbyL)))
+    x = np.exp(lo + (hi - lo) * rng.random(nblocks))
+    mhi = min(ldim - 1, int(16 * math.sqrt(RbyL)) + 16)
+    m = rng.integers(0, mhi + 1, size=nblocks)
+    return x, m
+
+
+def generate(n_configs=300, seed=1, ldim_factor=6, full=1100, log=print):
+    """Blocks of the GPU path (caps_logdetD_batch, one batch per
+    configuration) on the sampled configurations with ldim = 6 R/L; returns
+    X (n x 5) and Y = log|logdetD| (blocks with logdetD = 0 dropped)."""
     rng = np.random.default_rng(seed + 1000)
     X, Y = [], []
     t0 = time.time()
-    for cfg in sample_configs(n_configs, seed):
+    for k, cfg in enumerate(sample_configs(n_configs, seed)):
         L_ = 1e-6
         R_ = cfg["RbyL"] * L_
         ldim = max(20, int(math.ceil(ldim_factor * cfg["RbyL"])))
         c = caps_amd.Caps(R_, L_, ldim=ldim)
         if cfg["omegap"] is not None:
             c.set_drude(cfg["omegap"], cfg["gamma"] or 0.0)
-        Ts = caps_amd.T_scaled(R_, L_, cfg["T"])
-        ns = sorted(set(int(x) for x in rng.integers(1, 13, size=xi_per_config)))
-        mmax = max(8, int(0.2 * ldim))
-        xs, ms = [], []
-        for n in ns:
-            mm = np.unique(np.concatenate([[0], rng.integers(0, mmax, size=m_per_xi)]))
-            xs += [n * Ts] * len(mm)
-            ms += list(mm)
-        vals = c.logdetD_batch(np.array(xs), np.array(ms))
+        xs, ms = block_sample(cfg["RbyL"], ldim, blocks_per_config(cfg["RbyL"], full), rng)
+        vals = c.logdetD_batch(xs, ms)
         fw, fg = material_features(cfg["omegap"], cfg["gamma"])
-        for x, m, v in zip(xs, ms, vals):
-            if v == 0 or not np.isfinite(v):
-                continue
-            X.append([math.log(cfg["RbyL"]), math.log(x), m / ldim, fw, fg])
-            Y.append(math.log(abs(v)))
-        log("surrogate data: R/L=%.1f %s T=%.0fK ldim=%d blocks=%d (%.0fs)" % (
-            cfg["RbyL"], cfg["kind"], cfg["T"], ldim, len(xs), time.time() - t0))
-    return np.array(X, dtype=np.float32), np.array(Y, dtype=np.float32)
+        keep = (vals != 0) & np.isfinite(vals)
+        n = int(keep.sum())
+        X.append(np.column_stack([np.full(n, math.log(cfg["RbyL"])), np.log(xs[keep]), ms[keep] / ldim,
+                                  np.full(n, fw), np.full(n, fg)]))
+        Y.append(np.log(np.abs(vals[keep])))
+        if k % 10 == 0 or k == n_configs - 1:
+            log("surrogate data: config %d/%d R/L=%.1f %s ldim=%d blocks=%d (total %d, %.0fs)" % (
+                k + 1, n_configs, cfg["RbyL"], cfg["kind"], ldim, len(xs), sum(len(y) for y in Y),
+                time.time() - t0), flush=True)
+        del c
+    return np.concatenate(X).astype(np.float32), np.concatenate(Y).astype(np.float32)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
-    ap.add_argument("--configs", type=int, default=30)
-    ap.add_argument("--epochs", type=int, default=400)
+    ap.add_argument("--configs", type=int, default=300)
+    ap.add_argument("--blocks", type=int, default=1100, help="blocks per configuration up to R/L = 150")
+    ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--data", default=None, help="also save the data set (npz)")
+    ap.add_argument("--load-data", default=None, help="train on a saved data set instead of generating one")
     args = ap.parse_args()
     caps_amd.lib()
-    X, Y = generate(args.configs)
+    if args.load_data:
+        d = np.load(args.load_data)
+        X, Y = d["X"], d["Y"]
+    else:
+        X, Y = generate(args.configs, full=args.blocks)
     if args.data:
         np.savez(args.data, X=X, Y=Y)
     rng = np.random.default_rng(7)
@@ -209,11 +235,19 @@ def main():
     val, tr = idx[:nval], idx[nval:]
     net = MLP(seed=1)
     t0 = time.time()
-    loss, curve = net.train(X[tr], Y[tr], epochs=args.epochs)
+    loss, curve = net.train(X[tr], Y[tr], epochs=args.epochs, batch=args.batch)
     pv = net.predict(X[val])
-    rmse = float(np.sqrt(np.mean((pv - Y[val]) ** 2)))
+    err = pv - Y[val]
+    rmse = float(np.sqrt(np.mean(err ** 2)))
     print("surrogate: %d samples, train loss %.4g (standardised), validation RMSE %.3f in log|logdetD|, "
           "training %.1fs" % (len(X), loss, rmse, time.time() - t0), flush=True)
+    rl = np.exp(X[val, 0])
+    for lo, hi in [(20, 50), (50, 150), (150, 400), (400, 1000)]:
+        sel = (rl >= lo) & (rl < hi)
+        if sel.any():
+            print("surrogate: validation R/L in [%d, %d): %d samples, RMSE %.3f, max |error| %.3f" % (
+                lo, hi, int(sel.sum()), float(np.sqrt(np.mean(err[sel] ** 2))), float(np.max(np.abs(err[sel])))),
+                flush=True)
     net.save(args.out)
     print("wrote", args.out)
 

@@ -243,12 +243,48 @@ void caps_last_timing(caps_t *self, double ms[6]);
  * mode 0 = plain (one block at a time, in the driver's order), mode 1 =
  * batched (all m of a frequency, or of a quadrature level, in one launch
  * sequence, verified against the same cutoff rule), mode 2 = batched with the
- * blocks planned by the surrogate (capsx_set_surrogate; T > 0).  nblocks
+ * blocks planned by the surrogate (capsx_set_surrogate).  nblocks
  * (may be NULL) receives the number of determinants computed, nused (may be
  * NULL) the number that enter the sums.  Returns E·(L+R)/(ħc), the value
  * the reference driver prints. */
 double caps_free_energy(caps_t *self, double T, double cutoff, double epsrel, int mode,
                         int drude_xi0, long *nblocks, long *nused);
+
+/* ---- planner surrogate (no counterpart in the reference) ---------------
+ * An MLP 5 -> 64 -> 64 -> 1 (tanh, FP32) predicting log|logdetD| from
+ * (log R/L, log ξ(L+R)/c, m/ldim, log ωp/eV, log γ/eV), trained and
+ * evaluated by the library's HIP kernels.  Parameter layout W1[64][5] b1[64]
+ * W2[64][64] b2[64] W3[64] b3. */
+struct capsx_mlp;
+int capsx_mlp_nparams(void);
+/* New network, Glorot-uniform weights from SplitMix64(seed), zero biases. */
+struct capsx_mlp *capsx_mlp_new(unsigned long long seed);
+void capsx_mlp_free(struct capsx_mlp *h);
+void capsx_mlp_get_params(struct capsx_mlp *h, float *out);
+void capsx_mlp_set_params(struct capsx_mlp *h, const float *in);
+/* Standardisation x' = (x - mean)/std per feature, y' = (y - ymean)/ystd. */
+void capsx_mlp_set_norm(struct capsx_mlp *h, const float *xmean, const float *xstd, float ymean, float ystd);
+void capsx_mlp_get_norm(struct capsx_mlp *h, float *xmean, float *xstd, float *ymean, float *ystd);
+/* Predictions in raw units for raw features X[n][5]. */
+void capsx_mlp_predict(struct capsx_mlp *h, const float *X, int n, float *Y);
+/* Gradient of the standardised MSE over n standardised samples. */
+void capsx_mlp_grad(struct capsx_mlp *h, const float *Xn, const float *Tn, int n, float *grad);
+/* Adam on raw (X, Y): minibatches of `batch`, order shuffled per epoch,
+ * step size cosine-annealed from lr to lr/20; sets the standardisation from
+ * the data; returns the final standardised MSE (loss_curve: one value per
+ * epoch, may be NULL). */
+float capsx_mlp_train(struct capsx_mlp *h, const float *X, const float *Y, int n, int epochs, int batch,
+                      float lr, unsigned long long seed, float *loss_curve);
+/* Attach a network (borrowed) for caps_free_energy mode 2, with the
+ * material features it is evaluated at. */
+void capsx_set_surrogate(caps_t *self, struct capsx_mlp *mlp, float f_omegap, float f_gamma);
+/* Launch sequences used by the free-energy driver since the last reset
+ * (resets when reset != 0). */
+long capsx_plan_batches(caps_t *self, int reset);
+/* Surrogate accuracy on the blocks computed for its plans since the last
+ * reset: out = {blocks compared, RMS and largest |log|logdetD| error|,
+ * frequencies compared, mean and largest |predicted - actual m-cutoff|}. */
+void capsx_surrogate_report(caps_t *self, double out[6], int reset);
 
 #ifdef __cplusplus
 }

@@ -118,7 +118,43 @@ def test_surrogate_planned_free_energy_c2(sur):
     net.attach(c, 9.0, 0.035)
     F2, comp2, used2 = c.free_energy(300.0, mode="surrogate")
     b2 = L.capsx_plan_batches(c.ptr, 1)
+    rep = c.surrogate_report()
     assert used1 == used2 == ref["determinants"]
-    assert abs(F2 / ref["F"] - 1) < 1e-8 and F1 == F2
+    assert abs(F2 / ref["F"] - 1) < 1e-8 and abs(F1 - F2) <= 1e-13 * abs(F1)
     print("without surrogate: %d launch sequences, %d blocks computed; with: %d, %d" % (b1, comp1, b2, comp2))
+    print("surrogate vs computed blocks:", rep)
     assert b2 < b1
+    # every planned frequency is scored, and the m-cutoff is predicted to a
+    # few m (the plan adds a margin of max(4, m/25))
+    assert rep["frequencies"] >= 5 and rep["blocks"] >= used2
+    assert rep["rms_log_error"] < 1.0 and rep["mean_mcut_error"] < 8
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained surrogate weights committed")
+@pytest.mark.parametrize("RbyL,ldim,material,epsrel", [(50, 300, None, 1e-8), (100, 600, (9.0, 0.035), 1e-6)],
+                         ids=["pr_RbyL50", "drude_RbyL100"])
+def test_surrogate_planned_free_energy_T0(sur, RbyL, ldim, material, epsrel):
+    """T = 0 free energy with every quadrature node's m-range planned by the
+    surrogate: the same energy and the same blocks in the sums as the batched
+    driver that guesses m-ranges from neighbouring nodes (which itself equals
+    the plain form, test_free_energy_T0_plain_equals_batched), in no more
+    launch sequences."""
+    import caps_amd
+    c = caps_amd.Caps(RbyL * 1e-6, 1e-6, ldim=ldim)
+    if material:
+        c.set_drude(*material)
+    L = sur._sig()
+    L.capsx_plan_batches(c.ptr, 1)
+    F1, comp1, used1 = c.free_energy(0.0, cutoff=1e-9, epsrel=epsrel, mode="batched")
+    b1 = L.capsx_plan_batches(c.ptr, 1)
+    net = sur.MLP.load(WEIGHTS)
+    net.attach(c, *(material or (None, None)))
+    c.surrogate_report(reset=True)
+    F2, comp2, used2 = c.free_energy(0.0, cutoff=1e-9, epsrel=epsrel, mode="surrogate")
+    b2 = L.capsx_plan_batches(c.ptr, 1)
+    rep = c.surrogate_report()
+    print("T=0 R/L=%d: F %.16g / %.16g; without surrogate %d launch sequences, %d blocks; with %d, %d; %s" % (
+        RbyL, F1, F2, b1, comp1, b2, comp2, rep))
+    assert used1 == used2 and abs(F1 - F2) <= 1e-13 * abs(F1)
+    assert rep["frequencies"] > 0 and rep["rms_log_error"] < 1.0
+    assert b2 <= b1
