@@ -236,6 +236,7 @@ def main():
     st = c.stats(reset=True)
     total = sum(times)
     value = nb * args.steps / total
+    sur_acc = c.surrogate_report(reset=True) if form == "surrogate" else None
 
     # one more step with HIP events around every launch of the dominant
     # kernel (the LU's trailing update); its launches then run one at a time
@@ -280,6 +281,7 @@ def main():
                    "blocks_per_step": nb, "blocks_computed_per_step": ncomp, "F": F, "form": form},
         "plain": plain,
         "batched_without_surrogate": heuristic,
+        "surrogate_vs_computed": sur_acc,
         "stage_ms_per_step": {k: v / args.steps for k, v in st["stage_ms"].items()},
         "lu_tflops": st["lu_flops"] / lu_s / 1e12 if lu_s > 0 else None,
         "blocks_factored_per_step": st["blocks_factored"] / args.steps,

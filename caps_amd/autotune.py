@@ -13,7 +13,7 @@ first (levels, tile) with one stream, then (streams, xcd) for the best of
 those, and writes the fastest schedule per shape to lu_tuning.txt, which the
 library reads at start-up (nearest shape in log dimension and log count).
 
-    python -m caps_amd.autotune [--out caps_amd/lu_tuning.txt] [--max-gb 48]
+    python -m caps_amd.autotune [--out caps_amd/lu_tuning.txt] [--max-gb 64]
 """
 import argparse
 import ctypes
@@ -28,7 +28,7 @@ import caps_amd  # noqa: E402
 
 DEFAULT_OUT = os.path.join(HERE, "lu_tuning.txt")
 DIMS = (240, 600, 1200, 2400, 4800, 9600, 24000)
-COUNTS = (1, 16, 128, 640)
+COUNTS = (1, 4, 16, 128, 640)
 
 
 def _sig():
@@ -97,7 +97,7 @@ def tune_shape(dim, n, log=print):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_OUT)
-    ap.add_argument("--max-gb", type=float, default=48.0, help="largest batch of matrices to time (GB)")
+    ap.add_argument("--max-gb", type=float, default=64.0, help="largest batch of matrices to time (GB)")
     args = ap.parse_args()
     set_config(0)
     lines = ["# LU schedule per batch shape, written by caps_amd/autotune.py on %s" % time.strftime("%Y-%m-%d"),
