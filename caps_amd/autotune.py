@@ -4,7 +4,8 @@ For a grid of batch shapes (matrix dimension, number of matrices) it times
 the library's batched LU (capsx_lu_time: synthetic matrices 1 - 0.3 X/sqrt(N)
 generated on the GPU, events around the LU only) under the schedules
 
-    levels  1 (64-column steps) or 2 (128-column steps, one K = 128 update)
+    levels  1 (64-column steps), 2 (128-column steps, one K = 128 update) or
+            3 (256-column steps, one K = 256 update)
     tile    trailing-update workgroup tile 128 x 128, 256 x 128 or 64 x 64
     streams 1, 2 or 3 groups of matrices on separate HIP streams
     xcd     trailing-update tile order (1: all tiles of a matrix on one XCD)
@@ -78,7 +79,7 @@ def tune_shape(dim, n, log=print):
     # the warm-up run of each shape allocates its arena outside the timings
     lu_time(dim, n, (2, 1, 0, 128), reps=1)
     results = {}
-    for levels, tile in itertools.product((1, 2), (128, 256, 64)):
+    for levels, tile in itertools.product((1, 2, 3), (128, 256, 64)):
         cfg = (levels, 1, 0, tile)
         results[cfg] = lu_time(dim, n, cfg)
     lv, _, _, tl = min(results, key=results.get)

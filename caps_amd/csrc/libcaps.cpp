@@ -47,6 +47,7 @@ int capsd_lu_grid_panel_rows(int);
 int capsd_lu_init(void);
 int capsd_lu_max_dim(void);
 int capsd_lu_nb(void);
+int capsd_lu_npanels(void);
 void capsd_launch_mfma_probe(const double*, const double*, double*, hipStream_t);
 }
 
@@ -231,7 +232,8 @@ double env_double(const char* name, double dflt)
 
 // ---- LU schedule configuration ---------------------------------------------
 // levels: 1 = 64-column steps, 2 = 128-column steps of two panels with one
-// K = 128 trailing update; streams: LU groups of blocks on separate streams;
+// K = 128 trailing update, 3 = 256-column steps of four panels with one
+// K = 256 trailing update; streams: LU groups of blocks on separate streams;
 // xcd: trailing-update tile order (1 = all tiles of a block on one XCD);
 // tile: trailing-update workgroup tile, 128 (x 128), 256 (x 128) or 64 (x 64).
 struct LuCfg {
@@ -265,7 +267,7 @@ int load_tuning(const char* path)
         TuneEntry e;
         if (sscanf(line, "%d %d %d %d %d %d", &e.maxN, &e.n, &e.cfg.levels, &e.cfg.streams, &e.cfg.xcd,
                    &e.cfg.tile) == 6 &&
-            e.maxN > 0 && e.n > 0 && (e.cfg.levels == 1 || e.cfg.levels == 2) && e.cfg.streams >= 1 &&
+            e.maxN > 0 && e.n > 0 && e.cfg.levels >= 1 && e.cfg.levels <= 3 && e.cfg.streams >= 1 &&
             (e.cfg.tile == 64 || e.cfg.tile == 128 || e.cfg.tile == 256))
             t.push_back(e);
     }
@@ -315,7 +317,7 @@ LuCfg lu_config(int maxN, int n)
             c = e.cfg;
         }
     }
-    if (getenv("CAPS_AMD_LU_LEVELS")) c.levels = env_double("CAPS_AMD_LU_LEVELS", 2) >= 2 ? 2 : 1;
+    if (getenv("CAPS_AMD_LU_LEVELS")) c.levels = std::max(1, std::min(3, (int)env_double("CAPS_AMD_LU_LEVELS", 2)));
     if (getenv("CAPS_AMD_LU_STREAMS")) c.streams = std::max(1, (int)env_double("CAPS_AMD_LU_STREAMS", 1));
     if (getenv("CAPS_AMD_GEMM_XCD")) c.xcd = env_double("CAPS_AMD_GEMM_XCD", 0) != 0 ? 1 : 0;
     if (getenv("CAPS_AMD_GEMM_TILE")) {
@@ -451,11 +453,11 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         LuSide& s = c->h_side[b];
         if (with_lu) {
             s.off_panel = off;
-            off = align256(off + 2 * (int64_t)lda * nb * 8);
+            off = align256(off + capsd_lu_npanels() * (int64_t)lda * nb * 8);
             s.off_logs = off;
             off = align256(off + (int64_t)N * 8);
             s.off_moved = off;
-            off = align256(off + 2 * (int64_t)nb * 4);
+            off = align256(off + capsd_lu_npanels() * (int64_t)nb * 4);
         } else
             s.off_panel = s.off_logs = s.off_moved = 0;
         p.maxN = std::max(p.maxN, N);
@@ -593,7 +595,7 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     if (c->prof && (int)c->gemm_ev.size() < 2 * nupd) {
         const size_t old = c->gemm_ev.size();
         c->gemm_ev.resize(2 * nupd);
-        c->gemm_info.resize(2 * nupd);
+        c->gemm_info.resize(6 * nupd);
         for (size_t i = old; i < c->gemm_ev.size(); i++) HIPCHECK(hipEventCreate(&c->gemm_ev[i]));
     }
     int nlaunched = 0;
@@ -643,22 +645,20 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     for (int i = 0; i < 6; i++) c->acc[i] += c->timing[i];
     if (c->prof && mode == 0) {
         // algorithmic flops of each update launch: 2 x rows x columns x depth
-        const double nb = capsd_lu_nb();
         for (int pi = 0; pi < nlaunched; pi++) {
-            const int k = c->gemm_info[2 * pi], umode = c->gemm_info[2 * pi + 1];
+            const int* g = &c->gemm_info[6 * pi];   // k, r_off, r_lim, c_off, c_lim, KK
             HIPCHECK(hipEventElapsedTime(&ms, c->gemm_ev[2 * pi], c->gemm_ev[2 * pi + 1]));
             c->acc[11] += ms;
             c->acc[13] += 1;
             for (int b = 0; b < n; b++) {
                 if (c->h_out[b].skip) continue;
-                const double r1 = c->h_desc[b].N - k - nb, r2 = r1 - nb;
+                const double N = c->h_desc[b].N;
+                const double r0 = g[0] + g[1], c0 = g[0] + g[3];
                 // region rows x cols, depth kk: flops 2 rows cols kk; HBM
                 // bytes: C read and written once, L and U read once
-                double rows = 0, cols = 0, kk = 0;
-                if (umode == 0 && r1 > 0) rows = cols = r1, kk = nb;
-                if (umode == 1 && r1 > 0) rows = r1, cols = std::min(r1, nb), kk = nb;
-                if (umode == 2 && r2 > 0) rows = cols = r2, kk = 2 * nb;
-                if (umode == 3 && r2 > 0) rows = nb, cols = r2, kk = nb;
+                const double rows = std::max(0.0, std::min(N, r0 + g[2]) - r0);
+                const double cols = std::max(0.0, std::min(N, c0 + g[4]) - c0);
+                const double kk = rows > 0 && cols > 0 ? g[5] : 0;
                 c->acc[12] += 2.0 * rows * cols * kk;
                 c->acc[14] += 16.0 * rows * cols + 8.0 * (rows + cols) * kk;
             }
@@ -1080,11 +1080,11 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     off = align256(off + (int64_t)lda * dim * 8);
     LuSide& s = c->h_side[0];
     s.off_panel = off;
-    off = align256(off + 2 * (int64_t)lda * capsd_lu_nb() * 8);
+    off = align256(off + capsd_lu_npanels() * (int64_t)lda * capsd_lu_nb() * 8);
     s.off_logs = off;
     off = align256(off + (int64_t)dim * 8);
     s.off_moved = off;
-    off = align256(off + 2 * (int64_t)capsd_lu_nb() * 4);
+    off = align256(off + capsd_lu_npanels() * (int64_t)capsd_lu_nb() * 4);
     ensure_arena(c, off);
     BlockOut o;
     memset(&o, 0, sizeof(o));
@@ -1595,11 +1595,11 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
         off = align256(off + (int64_t)lda * dim * 8);
         LuSide& s = c->h_side[b];
         s.off_panel = off;
-        off = align256(off + 2 * (int64_t)lda * capsd_lu_nb() * 8);
+        off = align256(off + capsd_lu_npanels() * (int64_t)lda * capsd_lu_nb() * 8);
         s.off_logs = off;
         off = align256(off + (int64_t)dim * 8);
         s.off_moved = off;
-        off = align256(off + 2 * (int64_t)capsd_lu_nb() * 4);
+        off = align256(off + capsd_lu_npanels() * (int64_t)capsd_lu_nb() * 4);
     }
     ensure_arena(c, off);
     std::vector<BlockOut> o(n);
@@ -1645,17 +1645,17 @@ int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, in
         off = align256(off + (int64_t)lda * dim * 8);
         LuSide& sd = c->h_side[b];
         sd.off_panel = off;
-        off = align256(off + 2 * (int64_t)lda * nb * 8);
+        off = align256(off + capsd_lu_npanels() * (int64_t)lda * nb * 8);
         sd.off_logs = off;
         off = align256(off + (int64_t)dim * 8);
         sd.off_moved = off;
-        off = align256(off + 2 * (int64_t)nb * 4);
+        off = align256(off + capsd_lu_npanels() * (int64_t)nb * 4);
     }
     ensure_arena(c, off);
     HIPCHECK(hipMemcpyAsync(c->d_desc, c->h_desc, n * sizeof(BlockDesc), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->d_side, c->h_side, n * sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
     LuCfg cfg = lu_config(dim, n);
-    if (levels > 0) cfg.levels = levels >= 2 ? 2 : 1;
+    if (levels > 0) cfg.levels = std::min(levels, 3);
     if (streams > 0) cfg.streams = streams;
     if (xcd >= 0) cfg.xcd = xcd ? 1 : 0;
     if (tile > 0) cfg.tile = tile <= 64 ? 64 : tile >= 256 ? 256 : 128;
@@ -1692,7 +1692,7 @@ void capsx_set_graph(caps_t* self, int on)
     }
 }
 
-// Extension: fix the LU schedule of all later batches (levels 1|2, streams
+// Extension: fix the LU schedule of all later batches (levels 1|2|3, streams
 // >= 1, xcd 0|1, tile 64|128|256); levels = 0 returns to the autotuned table.
 // Returns 0 on success, 1 on invalid values.
 int capsx_lu_config(int levels, int streams, int xcd, int tile)
@@ -1702,7 +1702,7 @@ int capsx_lu_config(int levels, int streams, int xcd, int tile)
         g_have_override = false;
         return 0;
     }
-    if ((levels != 1 && levels != 2) || streams < 1 || (xcd != 0 && xcd != 1) ||
+    if (levels < 1 || levels > 3 || streams < 1 || (xcd != 0 && xcd != 1) ||
         (tile != 64 && tile != 128 && tile != 256))
         return 1;
     g_override.levels = levels;

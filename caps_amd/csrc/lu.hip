@@ -2,10 +2,10 @@
 // partial pivoting (the dgetrf_ route of src/matrix.c:555-577 and the Kahan
 // sum of log|u_ii| of src/matrix.c:462-471), on gfx950.
 //
-// Per panel of NB = 64 columns, three batched kernels (or, with two levels,
-// per step of 128 columns: panel, swaps + TRSM and a narrow update of the
-// second panel's columns, panel, swaps + TRSM, a 64-row update, TRSM, one
-// K = 128 update of the trailing matrix):
+// Per panel of NB = 64 columns, three batched kernels; with two (three)
+// levels, steps of 128 (256) columns whose panels are factored with updates
+// restricted to the step's own columns, followed by one K = 128 (K = 256)
+// update of the trailing matrix (schedules at capsd_launch_lu):
 //   k_lu_panel      one workgroup per matrix factors the N-k x 64 panel:
 //                   left-looking over leaves of 2..8 columns, each leaf held
 //                   in registers (R rows per lane; 256, 512 or 1024 lanes by
@@ -37,23 +37,24 @@ __device__ __forceinline__ T* at(char* base, int64_t off) { return reinterpret_c
 //   off_piv:              N ints (unused by the logdet; kept for the layout)
 // and, appended by the host after the matrix:
 struct LuSide {
-    int64_t off_panel;   // 2 x lda x NB doubles: the two factored panels of a
-                         // 2 NB-column step, pivoted order
+    int64_t off_panel;   // NPANEL x lda x NB doubles: the factored panels of a
+                         // step (panel q at column k + q NB), rows relative to
+                         // the panel's first row, pivoted order
     int64_t off_logs;    // N doubles: log|u_ii|
-    int64_t off_moved;   // 2 x NB ints: the row interchanges of each of the two
-                         // panels, LAPACK ipiv convention relative to the
+    int64_t off_moved;   // NPANEL x NB ints: the row interchanges of each
+                         // panel, LAPACK ipiv convention relative to the
                          // panel's first row (at column c rows c and ipiv[c]
                          // were exchanged)
 };
-constexpr int MOVED_STRIDE = NB;
+constexpr int NPANEL = 4;    // panels per step of the three-level schedule
 
 __device__ __forceinline__ double* panel_buf(char* arena, const LuSide& sd, int64_t lda, int pb)
 {
-    return at<double>(arena, sd.off_panel) + (pb ? lda * NB : 0);
+    return at<double>(arena, sd.off_panel) + (int64_t)pb * lda * NB;
 }
 __device__ __forceinline__ int* moved_buf(char* arena, const LuSide& sd, int pb)
 {
-    return at<int>(arena, sd.off_moved) + (pb ? MOVED_STRIDE : 0);
+    return at<int>(arena, sd.off_moved) + pb * NB;
 }
 
 __device__ __forceinline__ void wave_argmax(double& v, int& idx)
@@ -442,19 +443,20 @@ __device__ __forceinline__ void swap_rows_ipiv(double* __restrict__ Ac, const in
 }
 
 // Row interchanges and U12 = L11^{-1} A12 for the 64-row tile of rows
-// [r0, r0+64), columns [r0+cs_off, min(N, r0+cs_off+climit)), one lane per
-// column.  swaps bit 0: exchange rows by the pivots of panel 0 (relative to
-// r0); bit 1: then by those of panel 1 (relative to r0+64).  Then the
-// column's 64 entries are solved in registers with the unit lower L11 of
-// panel buffer pb (uniform loads along the columns of L11).
+// [r0, r0+64), r0 = k + pb NB, columns [r0+cs_off, min(N, r0+cs_off+climit)),
+// one lane per column.  For each set bit q of swaps (in increasing q) the
+// rows are exchanged by the pivots of panel q (relative to row k + q NB);
+// then the column's 64 entries are solved in registers with the unit lower
+// L11 of panel buffer pb (uniform loads along the columns of L11).
 __global__ void __launch_bounds__(64) k_lu_swap_trsm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                      const LuSide* __restrict__ side,
-                                                     const BlockOut* __restrict__ out, int r0, int pb, int cs_off,
+                                                     const BlockOut* __restrict__ out, int k, int pb, int cs_off,
                                                      int climit, int swaps)
 {
     const int b = blockIdx.y;
     const BlockDesc& d = desc[b];
     const int N = d.N;
+    const int r0 = k + pb * NB;
     if (out[b].skip || r0 + cs_off >= N) return;
     const int cend = min(N, r0 + cs_off + climit);
     const int64_t lda = d.lda;
@@ -462,10 +464,10 @@ __global__ void __launch_bounds__(64) k_lu_swap_trsm(const BlockDesc* __restrict
     if (col >= cend) return;
     double* Ac = at<double>(arena, d.off_mat) + (int64_t)col * lda;
     const double* P = panel_buf(arena, side[b], lda, pb);
-    // pivots of the panel at r0 cover min(NB, N-r0) columns; those of the
-    // panel at r0+NB, min(NB, N-r0-NB)
-    if (swaps & 1) swap_rows_ipiv(Ac, moved_buf(arena, side[b], 0), min(NB, N - r0), r0);
-    if (swaps & 2) swap_rows_ipiv(Ac, moved_buf(arena, side[b], 1), min(NB, N - r0 - NB), r0 + NB);
+    // the pivots of the panel at row k + q NB cover min(NB, N - k - q NB)
+    // columns
+    for (int q = 0; q < NPANEL; q++)
+        if ((swaps >> q) & 1) swap_rows_ipiv(Ac, moved_buf(arena, side[b], q), min(NB, N - k - q * NB), k + q * NB);
     double x[NB];
 #pragma unroll
     for (int i = 0; i < NB; i++) x[i] = Ac[r0 + i];
@@ -478,18 +480,22 @@ __global__ void __launch_bounds__(64) k_lu_swap_trsm(const BlockDesc* __restrict
     for (int i = 0; i < NB; i++) Ac[r0 + i] = x[i];
 }
 
-// Second panel's row interchanges applied to the rows of the first panel's
-// L below it: rows 64.. of panel buffer 0 (matrix rows k+64..) follow the
-// pivoting of the panel at k+64.  One lane per column of the panel.
-__global__ void __launch_bounds__(64) k_lu_swap_panel(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                                      const LuSide* __restrict__ side,
-                                                      const BlockOut* __restrict__ out, int k)
+// Later panels' row interchanges applied to the L rows of earlier panels of
+// the step: wavefront w handles buffer q = qbegin + w and applies the pivots
+// of panels max(pfirst, q+1) .. plast in order to its rows below them (panel
+// p's rows start (p - q) NB into buffer q).  One lane per column of the
+// panel.
+__global__ void __launch_bounds__(64 * (NPANEL - 1))
+    k_lu_swap_panel(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
+                    const BlockOut* __restrict__ out, int k, int qbegin, int pfirst, int plast)
 {
     const int b = blockIdx.x;
     const BlockDesc& d = desc[b];
-    if (out[b].skip || k + 2 * NB >= d.N) return;
-    double* P1 = panel_buf(arena, side[b], d.lda, 0);
-    swap_rows_ipiv(P1 + (int64_t)threadIdx.x * d.lda, moved_buf(arena, side[b], 1), min(NB, d.N - k - NB), NB);
+    const int q = qbegin + (threadIdx.x >> 6);
+    if (out[b].skip) return;
+    double* Pq = panel_buf(arena, side[b], d.lda, q) + (int64_t)(threadIdx.x & 63) * d.lda;
+    for (int p = max(pfirst, q + 1); p <= plast; p++)
+        if (k + p * NB < d.N) swap_rows_ipiv(Pq, moved_buf(arena, side[b], p), min(NB, d.N - k - p * NB), (p - q) * NB);
 }
 
 // A22 -= L21 U12 with FP64 MFMA.  Workgroup tile 128x128 (or 64x64) of A22,
@@ -501,7 +507,8 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 template <int WM, int WN>
 __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
-          const BlockOut* __restrict__ out, int k, int mode, int gx, int gy, int nblocks, int xcd_order)
+          const BlockOut* __restrict__ out, int k, int r_off, int r_lim, int c_off, int c_lim, int kk0, int KK,
+          int gx, int gy, int nblocks, int xcd_order)
 {
     constexpr int NT = 64 * WM * WN, GTM = 64 * WM, GTN = 64 * WN;
     constexpr int KS = NT >= 256 ? 16 : 8;   // K per LDS stage (prefetch registers)
@@ -526,40 +533,18 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     const BlockDesc& d = desc[b];
     if (out[b].skip) return;
     const int N = d.N;
-    // the updated region [r0, rend) x [c0, cend) and the depth KK:
-    //   mode 0: one panel at k, all of A22;
-    //   mode 1: first panel of a 2NB step, the second panel's columns only;
-    //   mode 2: both panels of a 2NB step, A22 below and right of them;
-    //   mode 3: first panel of a 2NB step, the second panel's U rows right
-    //           of the second panel
-    int r0, rend, c0, cend, KK;
-    if (mode == 0) {
-        r0 = c0 = k + NB;
-        rend = cend = N;
-        KK = NB;
-    } else if (mode == 1) {
-        r0 = c0 = k + NB;
-        rend = N;
-        cend = min(N, k + 2 * NB);
-        KK = NB;
-    } else if (mode == 2) {
-        r0 = c0 = k + 2 * NB;
-        rend = cend = N;
-        KK = 2 * NB;
-    } else {
-        r0 = k + NB;
-        rend = min(N, k + 2 * NB);
-        c0 = k + 2 * NB;
-        cend = N;
-        KK = NB;
-    }
+    // the updated region: rows [k + r_off, k + r_off + r_lim) and columns
+    // [k + c_off, k + c_off + c_lim), both cut at N; depth: the step's L
+    // columns and U rows k + kk0 .. k + kk0 + KK (the caller keeps
+    // kk0 + KK <= r_off, so they exist whenever the region does)
+    const int r0 = k + r_off, rend = min(N, r0 + r_lim);
+    const int c0 = k + c_off, cend = min(N, c0 + c_lim);
     const int m0 = r0 + (tile % gx) * GTM;   // rows of the update
     const int n0 = c0 + (tile / gx) * GTN;   // columns of the update
     if (m0 >= rend || n0 >= cend) return;
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
-    const double* P1 = panel_buf(arena, side[b], lda, 0);
-    const double* P2 = panel_buf(arena, side[b], lda, 1);
+    const double* Pb = panel_buf(arena, side[b], lda, 0);
     // one LDS buffer: the A and B stages, reused for the C strips at the end
     constexpr int STG = KS * (GTPM + GTPN), STRIPS = WM * WN * (64 * 17 + 4);
     constexpr int SMEM = STG > STRIPS ? STG : STRIPS;   // stages, then the epilogue strips
@@ -580,8 +565,10 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     constexpr int PERA = (KS * GTM) / NT, PERB = (KS * GTN) / NT;
     double pa[PERA], pb[PERB];
     auto fetch = [&](int ks) {
-        // L columns [ks, ks+KS) (of panel 0 or 1), in the panels' logical rows
-        const double* Lp = ks < NB ? P1 + (int64_t)ks * lda - k : P2 + (int64_t)(ks - NB) * lda - (k + NB);
+        // L columns k + kk0 + ks .. +KS, all in panel q = (kk0 + ks) / NB,
+        // whose buffer rows start at matrix row k + q NB
+        const int j = kk0 + ks, q = j / NB;
+        const double* Lp = Pb + (int64_t)q * lda * NB + (int64_t)(j % NB) * lda - (k + q * NB);
 #pragma unroll
         for (int t = 0; t < PERA; t++) {
             const int idx = tid + NT * t;
@@ -594,7 +581,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
             const int idx = tid + NT * t;
             const int kk2 = idx % KS, c = idx / KS;
             const int colg = n0 + c;
-            pb[t] = colg < cend ? A[(k + ks + kk2) + (int64_t)colg * lda] : 0.0;
+            pb[t] = colg < cend ? A[(k + kk0 + ks + kk2) + (int64_t)colg * lda] : 0.0;
         }
     };
     fetch(0);
@@ -784,77 +771,120 @@ int capsd_lu_grid_panel_rows(int rows)
     return old;
 }
 
-// Upper bound of the number of trailing-update launches of one LU.
-int capsd_lu_max_updates(int maxN) { return 3 * ((maxN + 2 * NB - 1) / (2 * NB)) + 2; }
+// Upper bound of the number of trailing-update launches of one LU (the
+// three-level schedule has 8 per 256 columns, the others fewer).
+int capsd_lu_max_updates(int maxN) { return 8 * ((maxN + 4 * NB - 1) / (4 * NB)) + 2; }
 
-// Whole LU of a batch (host loop over panels; all launches on stream s).
-// levels 1: NB-column steps (panel, swaps + TRSM, K = NB update);
-// levels 2: 2NB-column steps: panel, swaps + TRSM and a narrow update of
-// the second panel's columns, panel, swaps of both panels + TRSM of the
-// first 64 U rows, a K = NB update of the second 64 U rows, their TRSM, one
-// K = 2NB update of the trailing matrix.
+// Panel buffers per block (LuSide::off_panel holds this many lda x NB panels).
+int capsd_lu_npanels(void) { return NPANEL; }
+
+// Whole LU of a batch (host loop over steps; all launches on stream s).
+//   levels 1: NB-column steps: panel, swaps + TRSM, one K = NB update.
+//   levels 2: 2NB-column steps: panel 0, its swaps + TRSM of the second
+//     panel's columns and their update, panel 1, panel 1's pivots into
+//     panel 0's L rows, swaps of both panels + TRSM of the first 64 U rows
+//     of the trailing columns, a K = NB update of the next 64, their TRSM,
+//     one K = 2NB update of the trailing matrix.
+//   levels 3: 4NB-column steps: the levels-2 step on the step's own 256
+//     columns (panels 0, 1 and the K = 2NB update of columns 128..256), the
+//     same for panels 2 and 3 on columns 128..256, the later panels' pivots
+//     into the earlier panels' L rows; for the trailing columns all four
+//     panels' swaps and the 256-row TRSM as four 64-row TRSMs with three
+//     strip updates between them; one K = 4NB update of the trailing matrix
+//     (half the read-modify-write of C per flop of levels 2).
 // tile: 128 (update workgroups of 128 x 128, 4 wavefronts), 256 (256 x 128,
 // 8 wavefronts) or 64 (64 x 64, one wavefront); xcd_order: tile order of the
 // update (k_lu_gemm).
 // gemm_ev: NULL, or 2 events per update launch recorded around it;
-// gemm_info receives (k, mode) of each update launch.  Returns the number
-// of update launches.
+// gemm_info receives (k, r_off, r_lim, c_off, c_lim, KK) of each update
+// launch.  Returns the number of update launches.
 int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockOut* out, int nblocks, int maxN,
                     hipStream_t s, int levels, int xcd_order, int tile, hipEvent_t* gemm_ev, int* gemm_info)
 {
     const LuSide* sd = (const LuSide*)side;
     const int nb8 = xcd_order ? (nblocks + 7) & ~7 : nblocks;
+    constexpr int ALL = 1 << 30;   // region limit: up to N
     int p = 0;
-    auto update = [&](int k, int mode, int rows, int cols) {
+    // A[k+r_off .., k+c_off ..] -= L[.., k+kk0 ..] U[k+kk0 .., ..], depth KK
+    auto update = [&](int k, int r_off, int r_lim, int c_off, int c_lim, int kk0, int KK) {
+        const int rows = min(maxN, k + r_off + min(r_lim, maxN)) - (k + r_off);
+        const int cols = min(maxN, k + c_off + min(c_lim, maxN)) - (k + c_off);
+        if (rows <= 0 || cols <= 0) return;
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p], s);
         if (tile == 64) {
             const int gx = (rows + 63) / 64, gy = (cols + 63) / 64;
-            hipLaunchKernelGGL((k_lu_gemm<1, 1>), dim3(gx * gy * nb8), dim3(64), 0, s, desc, arena, sd, out, k, mode,
-                               gx, gy, nblocks, xcd_order);
+            hipLaunchKernelGGL((k_lu_gemm<1, 1>), dim3(gx * gy * nb8), dim3(64), 0, s, desc, arena, sd, out, k, r_off,
+                               r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
         } else if (tile == 256) {
             const int gx = (rows + 255) / 256, gy = (cols + 127) / 128;
             hipLaunchKernelGGL((k_lu_gemm<4, 2>), dim3(gx * gy * nb8), dim3(512), 0, s, desc, arena, sd, out, k,
-                               mode, gx, gy, nblocks, xcd_order);
+                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
         } else {
             const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
             hipLaunchKernelGGL((k_lu_gemm<2, 2>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k,
-                               mode, gx, gy, nblocks, xcd_order);
+                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
         }
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
         if (gemm_info) {
-            gemm_info[2 * p] = k;
-            gemm_info[2 * p + 1] = mode;
+            int* g = gemm_info + 6 * p;
+            g[0] = k, g[1] = r_off, g[2] = r_lim, g[3] = c_off, g[4] = c_lim, g[5] = KK;
         }
         p++;
     };
-    auto trsm = [&](int r0, int pb, int cs_off, int cols, int swaps) {
-        hipLaunchKernelGGL(k_lu_swap_trsm, dim3((cols + 63) / 64, nblocks), dim3(64), 0, s, desc, arena, sd, out, r0,
+    // swaps (bit q: panel q's pivots) and TRSM of rows k + pb NB .. +NB in
+    // the columns [k + pb NB + cs_off, +cols)
+    auto trsm = [&](int k, int pb, int cs_off, int cols, int swaps) {
+        const int c0 = k + pb * NB + cs_off;
+        const int n = min(maxN - c0, cols);
+        if (n <= 0) return;
+        hipLaunchKernelGGL(k_lu_swap_trsm, dim3((n + 63) / 64, nblocks), dim3(64), 0, s, desc, arena, sd, out, k,
                            pb, cs_off, cols, swaps);
+    };
+    auto panel = [&](int k, int pb) {
+        if (k + pb * NB < maxN) launch_panel(maxN - k - pb * NB, desc, arena, sd, out, nblocks, k + pb * NB, pb, s);
+    };
+    // buffers [qbegin, qend) take the pivots of the later panels pfirst..plast
+    auto swap_panels = [&](int k, int qbegin, int qend, int pfirst, int plast) {
+        if (k + pfirst * NB < maxN)
+            hipLaunchKernelGGL(k_lu_swap_panel, dim3(nblocks), dim3(64 * (qend - qbegin)), 0, s, desc, arena, sd, out,
+                               k, qbegin, pfirst, plast);
+    };
+    // the two-panel step at column k0 = k + h NB (panels h, h+1) applied to
+    // the columns up to k0 + 2NB + tail: its own second panel and `tail`
+    // columns beyond it
+    auto step2 = [&](int k, int h, int tail) {
+        const int o = h * NB;
+        panel(k, h);
+        trsm(k, h, NB, NB, 1 << h);
+        update(k, o + NB, ALL, o + NB, NB, o, NB);
+        panel(k, h + 1);
+        swap_panels(k, h, h + 1, h + 1, h + 1);
+        trsm(k, h, 2 * NB, tail, 3 << h);
+        update(k, o + NB, NB, o + 2 * NB, tail, o, NB);
+        trsm(k, h + 1, NB, tail, 0);
+        update(k, o + 2 * NB, ALL, o + 2 * NB, tail, o, 2 * NB);
     };
     if (levels <= 1) {
         for (int k = 0; k < maxN; k += NB) {
-            launch_panel(maxN - k, desc, arena, sd, out, nblocks, k, 0, s);
-            const int rest = maxN - k - NB;
-            if (rest > 0) {
-                trsm(k, 0, NB, rest, 1);
-                update(k, 0, rest, rest);
-            }
+            panel(k, 0);
+            trsm(k, 0, NB, ALL, 1);
+            update(k, NB, ALL, NB, ALL, 0, NB);
         }
+    } else if (levels == 2) {
+        for (int k = 0; k < maxN; k += 2 * NB) step2(k, 0, ALL);
     } else {
-        for (int k = 0; k < maxN; k += 2 * NB) {
-            launch_panel(maxN - k, desc, arena, sd, out, nblocks, k, 0, s);
-            const int rest = maxN - k - NB;
-            if (rest <= 0) continue;
-            trsm(k, 0, NB, NB, 1);
-            update(k, 1, rest, NB);
-            launch_panel(rest, desc, arena, sd, out, nblocks, k + NB, 1, s);
-            const int rest2 = rest - NB;
-            if (rest2 <= 0) continue;
-            hipLaunchKernelGGL(k_lu_swap_panel, dim3(nblocks), dim3(64), 0, s, desc, arena, sd, out, k);
-            trsm(k, 0, 2 * NB, rest2, 3);
-            update(k, 3, NB, rest2);
-            trsm(k + NB, 1, NB, rest2, 0);
-            update(k, 2, rest2, rest2);
+        for (int k = 0; k < maxN; k += 4 * NB) {
+            step2(k, 0, 2 * NB);
+            step2(k, 2, 0);
+            swap_panels(k, 0, 2, 2, 3);
+            trsm(k, 0, 4 * NB, ALL, 15);
+            update(k, NB, NB, 4 * NB, ALL, 0, NB);
+            trsm(k, 1, 3 * NB, ALL, 0);
+            update(k, 2 * NB, 2 * NB, 4 * NB, ALL, 0, 2 * NB);
+            trsm(k, 2, 2 * NB, ALL, 0);
+            update(k, 3 * NB, NB, 4 * NB, ALL, 2 * NB, NB);
+            trsm(k, 3, NB, ALL, 0);
+            update(k, 4 * NB, ALL, 4 * NB, ALL, 0, 4 * NB);
         }
     }
     hipLaunchKernelGGL(k_lu_finalize, dim3((nblocks + 63) / 64), dim3(64), 0, s, desc, arena, sd, out, nblocks);

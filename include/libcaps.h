@@ -190,7 +190,8 @@ int capsx_logdetD_multi(caps_t *const *objs, const double *xi_, const int *m, in
 
 /* Extension: LU schedule of the determinant stage.  levels 1 = 64-column
  * steps, 2 = 128-column steps of two panels with one K = 128 trailing
- * update; streams = groups of blocks factored on separate HIP streams; xcd =
+ * update, 3 = 256-column steps of four panels with one K = 256 update;
+ * streams = groups of blocks factored on separate HIP streams; xcd =
  * trailing-update tile order (1: all tiles of a block on one XCD); tile =
  * trailing-update workgroup tile (128, 256 or 64 rows).  By default the library picks,
  * per batch, the nearest shape (largest dimension, number of blocks) of the

@@ -180,10 +180,13 @@ def test_logdetD_C1_batch_equals_single(ca):
         assert c.logdetD(xs[i], int(ms[i])) == vb[i]
 
 
-def test_mixed_aspect_batch_vs_reference(ca):
+@pytest.mark.parametrize("schedule", [None, (3, 1, 0, 128), (1, 1, 0, 64)], ids=["tuned", "levels3", "levels1"])
+def test_mixed_aspect_batch_vs_reference(ca, schedule):
     """64 perfect-reflector blocks of 64 geometries (R/L 50..500, ldim =
     ceil(6 R/L), dimensions ~600..5500) in one batch, against the reference
-    run block by block on the same inputs."""
+    run block by block on the same inputs; with the tuned LU schedule and
+    with fixed ones."""
+    from caps_amd import autotune as at
     from caps_amd.workloads import mixed_aspect_blocks, make_objects
     rows = _ref("ref_mixed_aspect.json")
     blocks = mixed_aspect_blocks()
@@ -193,7 +196,12 @@ def test_mixed_aspect_batch_vs_reference(ca):
     objs = make_objects(blocks)
     xs = [b["xi_"] for b in blocks]
     ms = [b["m"] for b in blocks]
-    out = ca.logdetD_multi(objs, xs, ms)
+    try:
+        if schedule:
+            at.set_config(*schedule)
+        out = ca.logdetD_multi(objs, xs, ms)
+    finally:
+        at.set_config(0)
     errs = [abs(v - r["logdetD"]) / max(abs(r["logdetD"]), 1e-300) for v, r in zip(out, rows)]
     print("mixed-aspect batch: max rel err vs reference %.2e, dims %d..%d" % (
         max(errs), min(2 * b["ldim"] for b in blocks), max(2 * b["ldim"] for b in blocks)))
@@ -322,21 +330,23 @@ def test_lu_schedules_agree(ca):
     import itertools
     from caps_amd import autotune as at
     rng = np.random.default_rng(5)
-    N = 300
-    mats = [np.eye(N) - 0.3 * rng.normal(size=(N, N)) / math.sqrt(N) for _ in range(7)]
-    A = rng.normal(size=(N, N))
-    A[np.arange(N), np.arange(N)] = 0.0
-    mats.append(A)
-    mats = np.array(mats)
-    ref = [np.linalg.slogdet(M)[1] for M in mats]
-    try:
-        for cfg in itertools.product((1, 2), (1, 2), (0, 1), (64, 128, 256)):
-            at.set_config(*cfg)
-            out = ca.lu_logdet_batch(mats)
-            for v, r in zip(out, ref):
-                assert abs(v - r) < 1e-10 * N, (cfg, v, r)
-    finally:
-        at.set_config(0)
+    # 300: a partial second 256-column step; 517: two full steps and 5
+    # columns; 129: one column past a 128-column step
+    for N in (300, 517, 129):
+        mats = [np.eye(N) - 0.3 * rng.normal(size=(N, N)) / math.sqrt(N) for _ in range(7)]
+        A = rng.normal(size=(N, N))
+        A[np.arange(N), np.arange(N)] = 0.0
+        mats.append(A)
+        mats = np.array(mats)
+        ref = [np.linalg.slogdet(M)[1] for M in mats]
+        try:
+            for cfg in itertools.product((1, 2, 3), (1, 2), (0, 1), (64, 128, 256)):
+                at.set_config(*cfg)
+                out = ca.lu_logdet_batch(mats)
+                for v, r in zip(out, ref):
+                    assert abs(v - r) < 1e-10 * N, (N, cfg, v, r)
+        finally:
+            at.set_config(0)
 
 
 def test_autotune_table_and_timer(ca, tmp_path):
