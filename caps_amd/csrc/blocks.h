@@ -36,7 +36,8 @@ struct BlockDesc {
                         // of e^{ea(l)}, e^{eb(l)}; then 2 * ldim int exponents;
                         // then nl + (2 nl - 1) doubles: parts of log a0 of I
     int64_t off_kpart;  // n_waves * nK * KPart
-    int64_t off_k;      // 2 pols * nK * (double mant, int exp, double lnK)
+    int64_t off_k;      // 2 pols * nK * KEnt (K = m * 2^e), then 2 pols * nK
+                        // doubles log|K|
     int64_t off_i;      // I = mant * 2^exp: 2 pols * nl*nl signed double mantissas,
                         // then 2 pols * nl*nl int exponents
     int64_t off_mat;    // lda * N doubles (column-major)
@@ -55,6 +56,14 @@ struct IGroup {
 // Partial K sums of one wave of quadrature nodes for one ν.
 struct KPart {
     double s_te, s_tm;
+    int e;
+    int pad_;
+};
+
+// One K_ν value as signed mantissa times 2^e, read by the I recurrence with
+// a single 16-byte load.
+struct KEnt {
+    double m;
     int e;
     int pad_;
 };

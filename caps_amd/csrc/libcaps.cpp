@@ -379,7 +379,10 @@ void build_igroups(caps_t* const* objs, const BlockDesc* desc, int n, BatchPlan&
         IGroup g;
         memset(&g, 0, sizeof(g));
         int j = i;
-        while (j < n && j - i < IG_MAX && key(idx[j]) == key(idx[i])) {
+        // groups of at most 4 by default: the 8-block instantiation needs 231
+        // VGPRs (2 waves per SIMD) and ran the C2 I stage 25% slower
+        static const int gmax = std::max(1, std::min(IG_MAX, (int)env_double("CAPS_AMD_IGROUP_MAX", 4)));
+        while (j < n && j - i < gmax && key(idx[j]) == key(idx[i])) {
             g.blk[j - i] = idx[j];
             j++;
         }
@@ -441,7 +444,7 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         d.off_kpart = off;
         off = align256(off + (int64_t)d.n_waves * d.nK * sizeof(KPart));
         d.off_k = off;
-        off = align256(off + 2 * (int64_t)d.nK * 24);
+        off = align256(off + 2 * (int64_t)d.nK * (sizeof(capsd::KEnt) + 8));
         d.off_i = off;
         off = align256(off + 2 * (int64_t)d.nl * d.nl * 12);
         d.off_mat = off;
@@ -1777,7 +1780,7 @@ int capsx_block_tables(caps_t* self, double xi_, int m, int* mu, int* nK, double
     *nl = d.nl;
     if (d.nK > cap_nK || d.nl > cap_nl) return 1;
     if (lnK) {
-        HIPCHECK(hipMemcpy(lnK, c->arena + d.off_k + 24 * (int64_t)d.nK, 2 * d.nK * sizeof(double),
+        HIPCHECK(hipMemcpy(lnK, c->arena + d.off_k + 2 * sizeof(capsd::KEnt) * (int64_t)d.nK, 2 * d.nK * sizeof(double),
                            hipMemcpyDeviceToHost));
     }
     if (logI) {

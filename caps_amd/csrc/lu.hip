@@ -29,6 +29,12 @@
 namespace capsd {
 
 constexpr int NB = 64;       // panel width
+#ifndef LU_EPI_DIRECT
+#define LU_EPI_DIRECT 1
+#endif
+#ifndef LU_KS
+#define LU_KS 16
+#endif
 
 template <typename T>
 __device__ __forceinline__ T* at(char* base, int64_t off) { return reinterpret_cast<T*>(base + off); }
@@ -511,7 +517,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
           int gx, int gy, int nblocks, int xcd_order)
 {
     constexpr int NT = 64 * WM * WN, GTM = 64 * WM, GTN = 64 * WN;
-    constexpr int KS = NT >= 256 ? 16 : 8;   // K per LDS stage (prefetch registers)
+    constexpr int KS = NT >= 256 ? LU_KS : 8;   // K per LDS stage (prefetch registers)
     // padded LDS rows: GTPM keeps the A rows 128-byte aligned (its stores and
     // reads run along rows); GTPN is odd, so the B stores, which run down a
     // column (16 consecutive k), fall on distinct banks
@@ -547,7 +553,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     const double* Pb = panel_buf(arena, side[b], lda, 0);
     // one LDS buffer: the A and B stages, reused for the C strips at the end
     constexpr int STG = KS * (GTPM + GTPN), STRIPS = WM * WN * (64 * 17 + 4);
-    constexpr int SMEM = STG > STRIPS ? STG : STRIPS;   // stages, then the epilogue strips
+    constexpr int SMEM = (LU_EPI_DIRECT || STG > STRIPS) ? STG : STRIPS;   // stages (then the epilogue strips)
     __shared__ double smem[SMEM];
     double(*As)[GTPM] = reinterpret_cast<double(*)[GTPM]>(smem);             // [kk][row]
     double(*Bs)[GTPN] = reinterpret_cast<double(*)[GTPN]>(smem + KS * GTPM);  // [kk][col]
@@ -610,9 +616,44 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
+#if LU_EPI_DIRECT
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+#else
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+#endif
         }
     }
+#if LU_EPI_DIRECT
+    // With the operands swapped the MFMA yields the transposed tile: lane l,
+    // register q of acc[i][j] holds C row wr + 16i + l%16, column
+    // wc + 16j + l/16 + 4q, so each register's 64 lanes cover four whole
+    // 128-byte column segments and C is updated straight from the
+    // accumulators, one row band i at a time (loads of the band before its
+    // stores).
+    {
+        const int rr = lane & 15, cq = lane >> 4;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = m0 + wr + 16 * i + rr;
+            const bool rok = row < rend;
+            double cv[4][4];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int col = n0 + wc + 16 * j + cq + 4 * q;
+                    cv[j][q] = (rok && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
+                }
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int col = n0 + wc + 16 * j + cq + 4 * q;
+                    if (rok && col < cend) A[row + (int64_t)col * lda] = cv[j][q] - acc[i][j][q];
+                }
+        }
+    }
+#else
     // D layout (measured with k_mfma_probe): lane l, register q holds row
     // l/16 + 4q, column l%16 of the 16x16 tile.  Each wave passes its
     // 64x64 result through LDS one 16-row strip at a time so that the
@@ -649,6 +690,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+#endif
 }
 
 // Kahan sum of log|u_ii| (src/matrix.c:462-471).

@@ -212,23 +212,26 @@ __global__ void __launch_bounds__(256) k_K_combine(const BlockDesc* __restrict__
         ste += ldexp(part[w].s_te, part[w].e - F);
         stm += ldexp(part[w].s_tm, part[w].e - F);
     }
-    double* kmant = at<double>(arena, d.off_k);          // [2][nK]
-    int* kexp = at<int>(arena, d.off_k + 16 * (int64_t)d.nK);  // [2][nK]
-    double* lnk = at<double>(arena, d.off_k + 24 * (int64_t)d.nK);  // [2][nK]
+    KEnt* kt = at<KEnt>(arena, d.off_k);                                     // [2][nK]
+    double* lnk = at<double>(arena, d.off_k + 2 * sizeof(KEnt) * (int64_t)d.nK);  // [2][nK]
     const double s[2] = {ste, stm};
     for (int pol = 0; pol < 2; pol++) {
         const double v = s[pol];
+        KEnt k;
+        k.pad_ = 0;
         if (!(v > 0) || !isfinite(v)) {
             if (!(pol == 0 && d.is_pr)) atomicOr(status, 1);
-            kmant[pol * d.nK + j] = 0;
-            kexp[pol * d.nK + j] = 0;
+            k.m = 0;
+            k.e = 0;
+            kt[pol * d.nK + j] = k;
             lnk[pol * d.nK + j] = -INFINITY;
             continue;
         }
         const int e = ilogb(v);
         const double mnt = ldexp(v, -e);
-        kmant[pol * d.nK + j] = pol == 0 ? -mnt : mnt;
-        kexp[pol * d.nK + j] = F + e;
+        k.m = pol == 0 ? -mnt : mnt;
+        k.e = F + e;
+        kt[pol * d.nK + j] = k;
         lnk[pol * d.nK + j] = log(v) + (double)F * LOG2;
     }
 }
@@ -314,32 +317,32 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     // per block and polarization: K table, running sum (exponent-aligned to
     // the q = 0 term), |q = 0 term|, its exponent, early-exit count
     const double thr = 8.7565107626965203e-27;  // e^{-60}
-    const double* km[GM][NPOL];
-    const int* ke[GM][NPOL];
-    double sum[GM][NPOL], t0abs[GM][NPOL];
-    int E0[GM][NPOL], cnt[GM][NPOL], cs[GM][NPOL];
+    // the group's K tables (same nK: same m and ℓ-window); polarization pol
+    // of block g at kb[g] + pol nK
+    const KEnt* kb[GM];
+    const int nK = d.nK;
+    double sum[GM][NPOL], tthr[GM][NPOL];
+    int E0[GM][NPOL], sh[GM][NPOL], cnt[GM][NPOL], cs[GM][NPOL];
     int ndone = 0;
 #pragma unroll
     for (int g = 0; g < GM; g++) {
+        kb[g] = g < G ? at<KEnt>(arena, desc[grp.blk[g]].off_k) : nullptr;
 #pragma unroll
         for (int pp = 0; pp < NPOL; pp++) {
             const int pol = POL0 + pp;
             if (g < G) {
-                const BlockDesc& dg = desc[grp.blk[g]];
-                km[g][pp] = at<double>(arena, dg.off_k) + pol * dg.nK;
-                ke[g][pp] = at<int>(arena, dg.off_k + 16 * (int64_t)dg.nK) + pol * dg.nK;
-                const int j = l12 - mu;
-                E0[g][pp] = ke[g][pp][j];
-                sum[g][pp] = km[g][pp][j];
-                t0abs[g][pp] = fabs(sum[g][pp]);
+                const KEnt k0 = kb[g][pol * nK + l12 - mu];
+                E0[g][pp] = k0.e;
+                sh[g][pp] = -k0.e;   // S - E0, S = 0
+                sum[g][pp] = k0.m;
+                tthr[g][pp] = thr * fabs(k0.m);
                 cnt[g][pp] = 0;
                 cs[g][pp] = 0;
             } else {
-                km[g][pp] = nullptr;
-                ke[g][pp] = nullptr;
                 E0[g][pp] = 0;
+                sh[g][pp] = 0;
                 sum[g][pp] = 0;
-                t0abs[g][pp] = 0;
+                tthr[g][pp] = 0;
                 cnt[g][pp] = 3;   // absent: counts as done
                 cs[g][pp] = 0;
                 ndone++;
@@ -347,27 +350,39 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
         }
     }
 
-    // add a_q K_{l1+l2-2q} (a_q = aq 2^S) to every block's sum; from q = 3 on
-    // with the exit rules: the reference's (three terms below e^-60 of the
-    // first), or three terms below 2^-56 of the running sum (below half an
-    // ulp of the sum, they leave it unchanged)
-    auto accumulate = [&](int q, double aq, int S, bool check) {
+    // add a_q K_{l1+l2-2q} (a_q = aq 2^S, sh = S - E0) to every block's sum;
+    // from q = 3 on with the exit rules: the reference's (three terms below
+    // e^-60 of the first), or three terms below 2^-56 of the running sum
+    // (below half an ulp of the sum, they leave it unchanged)
+    // All of a step's K loads are issued together, before the step's a_q is
+    // computed (one memory latency per step, hidden by the recurrence, not
+    // one per block and polarization); the updates are selects, so finished
+    // sums stay unchanged without divergent branches.
+    auto load_k = [&](int q, KEnt (&kv)[GM][NPOL]) {
         const int j = l12 - 2 * q - mu;
 #pragma unroll
+        for (int g = 0; g < GM; g++)
+#pragma unroll
+            for (int pp = 0; pp < NPOL; pp++)
+                if (g < G) kv[g][pp] = kb[g][(POL0 + pp) * nK + j];   // g < G: uniform
+    };
+    auto accumulate = [&](const KEnt (&kv)[GM][NPOL], double aq, bool check) {
+#pragma unroll
         for (int g = 0; g < GM; g++) {
+            if (g >= G) continue;
 #pragma unroll
             for (int pp = 0; pp < NPOL; pp++) {
-                if (cnt[g][pp] >= 3) continue;
-                const double term = ldexp(aq * km[g][pp][j], S + ke[g][pp][j] - E0[g][pp]);
-                sum[g][pp] += term;
+                const bool act = cnt[g][pp] < 3;
+                const double term = ldexp(aq * kv[g][pp].m, kv[g][pp].e + sh[g][pp]);
+                sum[g][pp] += act ? term : 0.0;
                 if (check) {
                     const double aterm = fabs(term);
-                    cnt[g][pp] = aterm < thr * t0abs[g][pp] ? cnt[g][pp] + 1 : 0;
-                    cs[g][pp] = aterm < ldexp(fabs(sum[g][pp]), -56) ? cs[g][pp] + 1 : 0;
-                    if (cnt[g][pp] >= 3 || cs[g][pp] >= 3) {
-                        cnt[g][pp] = 3;
-                        ndone++;
-                    }
+                    const int c1 = aterm < tthr[g][pp] ? cnt[g][pp] + 1 : 0;
+                    const int c2 = aterm * 0x1p56 < fabs(sum[g][pp]) ? cs[g][pp] + 1 : 0;
+                    const bool fin = act && (c1 >= 3 || c2 >= 3);
+                    cnt[g][pp] = act ? (fin ? 3 : c1) : cnt[g][pp];
+                    cs[g][pp] = act ? c2 : cs[g][pp];
+                    ndone += fin ? 1 : 0;
                 }
             }
         }
@@ -383,7 +398,9 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
         tn = (m - n) * (m - n + 1) * rn1;
         tv = (m - nu) * (m - nu + 1) * rv1;
         aq = (n + nu - 1.5) * (1 - (2 * n + 2 * nu - 1) * r3 * (tn + tv));
-        accumulate(1, aq, 0, false);
+        KEnt kv[GM][NPOL];
+        load_k(1, kv);
+        accumulate(kv, aq, false);
     }
     if (qmax >= 2) {
         aqm2 = 1;
@@ -393,15 +410,20 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
              * ((2 * n + 2 * nu - 5) * r6
              * (tn * (m - n + 2) * (m - n + 3) * rn3 + 2 * tn * tv + tv * (m - nu + 2) * (m - nu + 3) * rv3)
              - tn - tv) + 0.5);
-        accumulate(2, aq, 0, false);
+        KEnt kv[GM][NPOL];
+        load_k(2, kv);
+        accumulate(kv, aq, false);
     }
     // q >= 3: eq. (35).  α(p+3), α(p+4) of a step are α(p+1), α(p+2) of the
     // step before (p = l1 + l2 - 2q).
-    int S = 0;                           // a_q carries a factor 2^S
     const double s2 = (n + nu + 1) * (n + nu + 1), d2 = (n - nu) * (n - nu);
     AlphaND al3 = alpha_nd(n + nu - 3, s2, d2), al4 = alpha_nd(n + nu - 2, s2, d2);
 #pragma unroll 2
     for (int q = 3; q <= qmax && ndone < GM * NPOL; q++) {
+        // this step's K values are fetched first; the recurrence below
+        // covers their latency
+        KEnt kv[GM][NPOL];
+        load_k(q, kv);
         aqm2 = aqm1;
         aqm1 = aq;
         const double pq = n + nu - 2 * q, p1 = pq - 2 * m, p2 = pq + 2 * m;
@@ -431,9 +453,12 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
             const int e = ilogb(aq);
             aq = ldexp(aq, -e);
             aqm1 = ldexp(aqm1, -e);
-            S += e;
+#pragma unroll
+            for (int g = 0; g < GM; g++)
+#pragma unroll
+                for (int pp = 0; pp < NPOL; pp++) sh[g][pp] += e;
         }
-        accumulate(q, aq, S, true);
+        accumulate(kv, aq, true);
     }
 
     // I = e^{log a0} · sum · 2^{E0}, stored as a signed mantissa times 2^exp
