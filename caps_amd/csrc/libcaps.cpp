@@ -502,14 +502,21 @@ int launch_lu_groups(Ctx* c, int n, int maxN, const LuCfg& cfg, hipEvent_t ready
     if (groups == 1)
         return capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, maxN, c->stream, cfg.levels, cfg.xcd,
                                cfg.tile, prof ? c->gemm_ev.data() : nullptr, prof ? c->gemm_info.data() : nullptr);
+    // group 0 runs on the library's stream itself, groups 1.. on their own
+    // streams, forked at `ready` and joined back: HIP put two fresh streams
+    // on one hardware queue (the trace showed both groups serialised), while
+    // the library's stream holds a queue of its own
     int launched = 0;
-    for (int g = 0; g < groups; g++) {
+    for (int g = groups - 1; g >= 0; g--) {
         const int b0 = (int)((int64_t)n * g / groups), b1 = (int)((int64_t)n * (g + 1) / groups);
-        HIPCHECK(hipStreamWaitEvent(c->lu_stream[g], ready, 0));
-        const int k = capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, b1 - b0, maxN,
-                                      c->lu_stream[g], cfg.levels, cfg.xcd, cfg.tile, nullptr, nullptr);
+        hipStream_t sg = g == 0 ? c->stream : c->lu_stream[g - 1];
+        if (g > 0) HIPCHECK(hipStreamWaitEvent(sg, ready, 0));
+        const int k = capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, b1 - b0, maxN, sg,
+                                      cfg.levels, cfg.xcd, cfg.tile, nullptr, nullptr);
         if (g == 0) launched = k;
-        HIPCHECK(hipEventRecord(c->lu_done[g], c->lu_stream[g]));
+    }
+    for (int g = 1; g < groups; g++) {
+        HIPCHECK(hipEventRecord(c->lu_done[g], c->lu_stream[g - 1]));
         HIPCHECK(hipStreamWaitEvent(c->stream, c->lu_done[g], 0));
     }
     return launched;
