@@ -96,6 +96,7 @@ void device_init()
 }
 
 constexpr int MAX_LU_STREAMS = 4;
+constexpr int MAX_PARTS = MAX_LU_STREAMS;   // parts of a batch (BatchPlan)
 
 struct Ctx {
     hipStream_t stream = nullptr;
@@ -104,6 +105,7 @@ struct Ctx {
     // that one group's panel factorizations overlap another's updates
     hipStream_t lu_stream[MAX_LU_STREAMS];
     hipEvent_t lu_done[MAX_LU_STREAMS];
+    hipEvent_t part_i_done[MAX_PARTS];   // part g's I finished
     char* arena = nullptr;
     size_t arena_size = 0;
     BlockDesc* d_desc = nullptr;
@@ -152,6 +154,7 @@ Ctx* ctx_of(caps_t* self)
             HIPCHECK(hipStreamCreateWithFlags(&c->lu_stream[g], hipStreamNonBlocking));
             HIPCHECK(hipEventCreateWithFlags(&c->lu_done[g], hipEventDisableTiming));
         }
+        for (auto& e : c->part_i_done) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HIPCHECK(hipMalloc(&c->d_status, sizeof(int)));
         HIPCHECK(hipHostMalloc((void**)&c->h_status, sizeof(int), hipHostMallocDefault));
         self->priv = c;
@@ -181,6 +184,7 @@ void ctx_free(Ctx* c)
         hipStreamDestroy(c->lu_stream[g]);
         hipEventDestroy(c->lu_done[g]);
     }
+    for (auto& e : c->part_i_done) hipEventDestroy(e);
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -353,20 +357,33 @@ void grid_for(BlockDesc& d)
     d.n_waves = (d.n_nodes + 63) / 64;
 }
 
+// A batch runs as `parts` parts, contiguous block ranges [part_b[g],
+// part_b[g+1]) (the LU stream groups): part g's I, assembly and LU run in
+// order on stream g, and part g's I starts when part g-1's I is done, so
+// the I of one part runs beside the assembly and LU of the part before it
+// (measured on the C2 batch: 2 parts 0.295 s per step, against 0.302 s
+// with the parts' I run together; one stream computing every part's I
+// while a second factored the parts in turn was slower, 0.31-0.44 s for
+// 2-8 parts).
 struct BatchPlan {
     int n = 0, maxN = 0, max_ldim = 0, max_waves = 0, max_nK = 0;
     int64_t max_ntri = 0;
     size_t bytes = 0;
-    std::vector<IGroup> groups;   // ordered by class (see build_igroups)
-    int class_counts[8] = {};
+    std::vector<IGroup> groups;   // per part, ordered by class (see build_igroups)
+    int parts = 1;
+    int part_b[MAX_PARTS + 1] = {};
+    int part_goff[MAX_PARTS] = {};      // first I group of each part
+    int part_cls[MAX_PARTS][8] = {};    // I groups per class of each part
 };
 
-// Groups of blocks that share the I coefficients a_q: same caps object, m,
-// ℓ-window and polarizations, at most IG_MAX per group.
-void build_igroups(caps_t* const* objs, const BlockDesc* desc, int n, BatchPlan& p)
+// Groups of blocks [b0, b1) that share the I coefficients a_q: same caps
+// object, m, ℓ-window and polarizations, at most IG_MAX per group; appended
+// to p.groups by class, the class sizes in cls.
+void build_igroups(caps_t* const* objs, const BlockDesc* desc, int b0, int b1, BatchPlan& p, int* cls_count)
 {
+    const int n = b1 - b0;
     std::vector<int> idx(n);
-    for (int i = 0; i < n; i++) idx[i] = i;
+    for (int i = 0; i < n; i++) idx[i] = b0 + i;
     auto key = [&](int i) {
         const BlockDesc& d = desc[i];
         return std::make_tuple(1 - d.is_pr, (uintptr_t)objs[i], d.m, d.lmin, d.ldim);
@@ -391,9 +408,8 @@ void build_igroups(caps_t* const* objs, const BlockDesc* desc, int n, BatchPlan&
         cls[(desc[idx[i]].is_pr ? 0 : 4) + sc].push_back(g);
         i = j;
     }
-    p.groups.clear();
     for (int k = 0; k < 8; k++) {
-        p.class_counts[k] = (int)cls[k].size();
+        cls_count[k] = (int)cls[k].size();
         p.groups.insert(p.groups.end(), cls[k].begin(), cls[k].end());
     }
 }
@@ -470,7 +486,17 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         p.max_ntri = std::max(p.max_ntri, (int64_t)d.nl * (d.nl + 1) / 2);
     }
     p.bytes = (size_t)off;
-    build_igroups(objs, c->h_desc, n, p);
+    // parts = LU stream groups of the tuned schedule (one without an LU,
+    // and in profiled runs, which time each update launch alone)
+    p.parts = 1;
+    if (with_lu && !c->prof)
+        p.parts = std::max(1, std::min(lu_config(p.maxN, n).streams, std::min(MAX_PARTS, n / 4)));
+    p.groups.clear();
+    for (int g = 0; g <= p.parts; g++) p.part_b[g] = (int)((int64_t)n * g / p.parts);
+    for (int g = 0; g < p.parts; g++) {
+        p.part_goff[g] = (int)p.groups.size();
+        build_igroups(objs, c->h_desc, p.part_b[g], p.part_b[g + 1], p, p.part_cls[g]);
+    }
     return p;
 }
 
@@ -535,18 +561,33 @@ int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, b
     if (events) HIPCHECK(hipEventRecord(c->ev[1], s));
     capsd_launch_K(c->d_desc, c->arena, g_lfac_dev, n, p.max_waves, p.max_nK, c->d_status, s);
     if (events) HIPCHECK(hipEventRecord(c->ev[2], s));
-    capsd_launch_I(c->d_desc, c->arena, c->d_groups, p.class_counts, g_lfac_dev, p.max_ntri, c->d_status, s);
-    if (events) HIPCHECK(hipEventRecord(c->ev[3], s));
-    capsd_launch_assemble(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, mode, s);
-    if (mode == 0) capsd_launch_trace(c->d_desc, c->arena, c->d_out, n, lu_mode_shortcuts, s);
-    // also the fork point of the LU stream groups
-    HIPCHECK(hipEventRecord(c->ev[4], s));
     int nlaunched = 0;
-    if (mode == 0) {
-        LuCfg cfg = lu_config(p.maxN, n);
-        // profiled runs keep one group so each update launch is timed alone
-        if (c->prof) cfg.streams = 1;
-        nlaunched = launch_lu_groups(c, n, p.maxN, cfg, c->ev[4], c->prof);
+    // part g on stream g (part 0 on the library's stream); its I waits for
+    // part g-1's I; stage events: I and assembly end with the last part's,
+    // the LU with the join of all parts
+    for (int g = 0; g < p.parts; g++) {
+        hipStream_t sg = g == 0 ? s : c->lu_stream[g - 1];
+        const int b0 = p.part_b[g], nb = p.part_b[g + 1] - b0;
+        const bool last = g == p.parts - 1;
+        if (g > 0) HIPCHECK(hipStreamWaitEvent(sg, c->part_i_done[g - 1], 0));
+        capsd_launch_I(c->d_desc, c->arena, c->d_groups + p.part_goff[g], p.part_cls[g], g_lfac_dev, p.max_ntri,
+                       c->d_status, sg);
+        if (!last) HIPCHECK(hipEventRecord(c->part_i_done[g], sg));
+        if (events && last) HIPCHECK(hipEventRecord(c->ev[3], sg));
+        capsd_launch_assemble(c->d_desc + b0, c->arena, g_lfac_dev, nb, p.max_ldim, mode, sg);
+        if (mode == 0) capsd_launch_trace(c->d_desc + b0, c->arena, c->d_out + b0, nb, lu_mode_shortcuts, sg);
+        if (events && last) HIPCHECK(hipEventRecord(c->ev[4], sg));
+        if (mode != 0) continue;
+        const bool prof = c->prof && p.parts == 1;
+        const LuCfg cfg = lu_config(p.maxN, n);
+        const int k = capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, nb, p.maxN, sg,
+                                      cfg.levels, cfg.xcd, cfg.tile, prof ? c->gemm_ev.data() : nullptr,
+                                      prof ? c->gemm_info.data() : nullptr);
+        if (g == 0) nlaunched = k;
+    }
+    for (int g = 1; g < p.parts; g++) {
+        HIPCHECK(hipEventRecord(c->lu_done[g], c->lu_stream[g - 1]));
+        HIPCHECK(hipStreamWaitEvent(s, c->lu_done[g], 0));
     }
     return nlaunched;
 }
@@ -560,7 +601,7 @@ std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_sh
     struct {
         int n, maxN, max_ldim, max_waves, max_nK, mode, shortcuts;
         int64_t max_ntri;
-        int cls[8];
+        int parts, part_b[MAX_PARTS + 1], part_goff[MAX_PARTS], cls[MAX_PARTS][8];
         int levels, streams, xcd, tile;
         const void *arena, *desc, *side, *out, *groups;
     } k;
@@ -573,7 +614,10 @@ std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_sh
     k.mode = mode;
     k.shortcuts = lu_mode_shortcuts;
     k.max_ntri = p.max_ntri;
-    memcpy(k.cls, p.class_counts, sizeof(k.cls));
+    k.parts = p.parts;
+    memcpy(k.part_b, p.part_b, sizeof(k.part_b));
+    memcpy(k.part_goff, p.part_goff, sizeof(k.part_goff));
+    memcpy(k.cls, p.part_cls, sizeof(k.cls));
     k.levels = cfg.levels;
     k.streams = cfg.streams;
     k.xcd = cfg.xcd;
