@@ -72,6 +72,10 @@ def lib():
             "capsx_set_graph": (None, [p, i]),
             "capsx_lu_grid_panel_rows": (i, [i]),
             "capsx_surrogate_report": (None, [p, dp, i]),
+            "caps_ht_plasma": (d, [p, d, d]),
+            "caps_integrate_plasma_init": (p, [p, d, d]),
+            "caps_integrate_plasma": (d, [p, i, i, i, dp, dp]),
+            "caps_integrate_plasma_free": (None, [p]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -240,14 +244,30 @@ class Caps:
         return {"blocks": int(out[0]), "rms_log_error": out[1], "max_log_error": out[2],
                 "frequencies": int(out[3]), "mean_mcut_error": out[4], "max_mcut_error": int(out[5])}
 
-    def logdetD0(self, m):
-        """ξ = 0 EE and MM blocks (perfect reflector / Drude)."""
-        ee, mm = ctypes.c_double(), ctypes.c_double()
-        lib().caps_logdetD0(self.ptr, m, 0.0, ctypes.byref(ee), ctypes.byref(mm), None)
-        return ee.value, mm.value
+    def logdetD0(self, m, omegap=None):
+        """ξ = 0 EE and MM blocks (perfect reflector / Drude); with omegap
+        (rad/s) also the plasma-model MM block: (EE, MM, MM_plasma)."""
+        ee, mm, mp = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        lib().caps_logdetD0(self.ptr, m, omegap or 0.0, ctypes.byref(ee), ctypes.byref(mm),
+                            ctypes.byref(mp) if omegap else None)
+        return (ee.value, mm.value, mp.value) if omegap else (ee.value, mm.value)
 
     def ht_perf(self, eps=1e-13):
         return lib().caps_ht_perf(self.ptr, eps)
+
+    def ht_plasma(self, omegap, eps=1e-13):
+        """ξ = 0 plasma-model free energy (k_B T units), omegap in rad/s."""
+        return lib().caps_ht_plasma(self.ptr, omegap, eps)
+
+    def plasma_integrals(self, omegap, l1, l2, m=0):
+        """(I(l1 + l2), I_{l1-1/2}/I_{l1+1/2}, I_{l2-1/2}/I_{l2+1/2}) at α =
+        ωp (L+R)/c/(1 + L/R) (host computation)."""
+        L = lib()
+        h = L.caps_integrate_plasma_init(self.ptr, omegap, 1e-8)
+        r1, r2 = ctypes.c_double(), ctypes.c_double()
+        v = L.caps_integrate_plasma(h, l1, l2, m, ctypes.byref(r1), ctypes.byref(r2))
+        L.caps_integrate_plasma_free(h)
+        return v, r1.value, r2.value
 
 
 class _MObj:

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -37,6 +38,7 @@ void capsd_launch_K(const BlockDesc*, char*, const double*, int, int, int, int*,
 void capsd_launch_I(const BlockDesc*, char*, const void*, const int*, const double*, int64_t, int*, hipStream_t);
 void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int, hipStream_t);
 void capsd_launch_trace(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
+void capsd_launch_fill_xi0(const BlockDesc*, char*, int64_t, int, hipStream_t);
 void capsd_launch_mie_points(int, const double*, const int*, double, const double*, double*, hipStream_t);
 void capsd_launch_fresnel_points(int, const double*, const double*, const double*, double*, hipStream_t);
 int capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t, int, int, int,
@@ -1199,25 +1201,239 @@ double caps_kernel_M0_MM(int i, int j, void* args_)
            * sqrt(((double)l1 * (double)l2) / ((l1 + 1.) * (l2 + 1.)));
 }
 
-// log det of the ξ = 0 EE and MM blocks (ldim x ldim, dense LU on the GPU
-// through kernel_logdet).  The plasma-model MM block needs the ξ = 0 plasma
-// integrals (src/integration.c:897-1031), which this library does not
-// provide: a non-NULL MM_plasma aborts with a message.
-// Reference: src/libcaps.c:984-1017.
+namespace {
+
+// log det(1 - M) of the n x n block M_ij = exp(h_i + h_j + T_{i+j}) (the
+// ξ = 0 kernels): filled, traced and factored on the GPU on this object's
+// context, with kernel_logdet's shortcut rules (src/matrix.c:57-149).
+double sep_logdet(caps_t* self, const std::vector<double>& h, const std::vector<double>& T)
+{
+    const int n = (int)h.size();
+    TERMINATE(n <= 0 || n > capsd_lu_max_dim() || (int)T.size() != 2 * n - 1, "xi = 0 block of dimension %d", n);
+    Ctx* c = ctx_of(self);
+    std::lock_guard<std::mutex> lock(c->mu);
+    hipStream_t s = c->stream;
+    ensure_blocks(c, 1);
+    const int lda = (n + 15) & ~15;
+    BlockDesc& d = c->h_desc[0];
+    memset(&d, 0, sizeof(d));
+    d.N = n;
+    d.ldim = (n + 1) / 2;
+    d.lda = lda;
+    int64_t off = 0;
+    d.off_mat = off;
+    off = align256(off + (int64_t)lda * n * 8);
+    d.off_diag = off;
+    off = align256(off + (int64_t)n * 8);
+    const int64_t off_tab = off;
+    off = align256(off + (int64_t)(3 * n - 1) * 8);
+    LuSide& sd = c->h_side[0];
+    sd.off_panel = off;
+    off = align256(off + capsd_lu_npanels() * (int64_t)lda * capsd_lu_nb() * 8);
+    sd.off_logs = off;
+    off = align256(off + (int64_t)n * 8);
+    sd.off_moved = off;
+    off = align256(off + capsd_lu_npanels() * (int64_t)capsd_lu_nb() * 4);
+    ensure_arena(c, off);
+    std::vector<double> tab(h);
+    tab.insert(tab.end(), T.begin(), T.end());
+    HIPCHECK(hipMemcpyAsync(c->d_desc, &d, sizeof(BlockDesc), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(c->d_side, &sd, sizeof(LuSide), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemsetAsync(c->d_out, 0, sizeof(BlockOut), s));
+    HIPCHECK(hipMemcpyAsync(c->arena + off_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    const int lu_mode = is_lu_mode(self->detalg);
+    capsd_launch_fill_xi0(c->d_desc, c->arena, off_tab, n, s);
+    capsd_launch_trace(c->d_desc, c->arena, c->d_out, 1, lu_mode, s);
+    HIPCHECK(hipEventRecord(c->ev[6], s));
+    launch_lu_groups(c, 1, n, lu_config(n, 1), c->ev[6], false);
+    BlockOut o;
+    HIPCHECK(hipMemcpyAsync(&o, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    HIPCHECK(hipGetLastError());
+    return finish_block(o, lu_mode);
+}
+
+// I_ν(x)/I_{ν+1}(x) by the continued fraction of DLMF 10.33.1,
+// 2(ν+1)/x + 1/(2(ν+2)/x + 1/(...)), evaluated with the modified Lentz
+// method to full precision (the reference's bessel_ratioI, src/bessel.c:
+// 635-660, evaluates the same fraction; its length grows like x).
+double ratio_I(double nu, double x)
+{
+    const double tiny = 1e-300;
+    double f = 2 * (nu + 1) / x, C = f, D = 0;
+    for (int k = 2; k < 100000000; k++) {
+        const double b = 2 * (nu + k) / x;
+        D = b + D;
+        C = b + 1 / C;
+        if (D == 0) D = tiny;
+        if (C == 0) C = tiny;
+        D = 1 / D;
+        const double delta = C * D;
+        f *= delta;
+        if (fabs(delta - 1) < 1e-16) return f;
+    }
+    return f;
+}
+
+// (1/ν!) ∫_0^∞ -r_TE(t) e^{-t} t^ν dt, -r_TE = (sqrt(t² + β²) - t)/(sqrt(t² + β²) + t),
+// β = 2ω: with t = e^u the integrand w(e^u) e^{(ν+1)u - e^u}/ν! is smooth,
+// unimodal (peak at u = log(ν+1), width 1/sqrt(ν+1)) and analytic in the
+// strip |Im u| < π/2 (w has its branch points at t = ±iβ), so the trapezoid
+// rule with step h converges like e^{-π²/h}; h = min(0.2, width/4) and the
+// sum runs out from the peak until the terms fall below 1e-20 of it.  The
+// reference integrates the same function with QUADPACK (src/integration.c:
+// 924-1019).
+double plasma_integral(int nu, double omega)
+{
+    const double u0 = log(nu + 1.0), h = std::min(0.2, 0.25 / sqrt(nu + 1.0));
+    const double lf = host_lfac((unsigned)nu);
+    auto f = [&](double u) {
+        const double t = exp(u), q = 2 * omega / t;
+        const double bm1 = q * q / (sqrt(1 + q * q) + 1);   // sqrt(1 + q²) - 1 without cancellation
+        return bm1 / (2 + bm1) * exp((nu + 1) * u - t - lf);
+    };
+    const double f0 = f(u0);
+    double s = f0;
+    for (int dir = -1; dir <= 1; dir += 2)
+        for (int k = 1;; k++) {
+            const double v = f(u0 + dir * k * h);
+            s += v;
+            if (v < 1e-20 * f0 && k > 4) break;
+        }
+    return s * h;
+}
+
+}  // namespace
+
+// Plasma-model ξ = 0 integrals (the reference's integration_plasma_t):
+// ω = ωp (L+R)/c, α = ω/(1 + L/R), cached I(ν) and Bessel ratios.
+struct caps_block_state_plasma {
+    double omega, alpha;
+    std::vector<double> I, ratio;   // by ν and by l, NaN = not computed
+    std::mutex mu;
+};
+
+extern "C" {
+
+// Reference: src/integration.c:897-915.
+integration_plasma_t* caps_integrate_plasma_init(caps_t* caps, double omegap, double epsrel)
+{
+    (void)epsrel;   // the trapezoid rule above converges to full precision
+    integration_plasma_t* p = new integration_plasma_t();
+    p->omega = omegap * caps->calL / CAPS_c;
+    p->alpha = p->omega / (1 + caps->LbyR);
+    return p;
+}
+
+// I(l1 + l2) and the ratios I_{l-1/2}(α)/I_{l+1/2}(α) for l1 and l2.
+// Reference: src/integration.c:955-1019.
+double caps_integrate_plasma(integration_plasma_t* self, int l1, int l2, int m, double* ratio1, double* ratio2)
+{
+    (void)m;
+    std::lock_guard<std::mutex> lock(self->mu);
+    auto cached = [](std::vector<double>& v, int k, const std::function<double()>& fn) {
+        if ((int)v.size() <= k) v.resize(k + 1, NAN);
+        if (std::isnan(v[k])) v[k] = fn();
+        return v[k];
+    };
+    const double a = self->alpha, w = self->omega;
+    *ratio1 = cached(self->ratio, l1, [&] { return ratio_I(l1 - 0.5, a); });
+    *ratio2 = cached(self->ratio, l2, [&] { return ratio_I(l2 - 0.5, a); });
+    return cached(self->I, l1 + l2, [&] { return plasma_integral(l1 + l2, w); });
+}
+
+void caps_integrate_plasma_free(integration_plasma_t* self) { delete self; }
+
+}  // extern "C"
+
+namespace {
+
+// ln of the common ξ = 0 element factor, split as h(l1) + h(l2) + T(l1+l2)
+// with h(l) = -½ (log (l+m)! + log (l-m)!) + extra(l) and
+// T(ν) = (ν + 1) y + log ν! + extra_T(ν) (src/libcaps.c:860-906).
+void xi0_tables(caps_t* self, int m, int lmin, std::vector<double>& h, std::vector<double>& T)
+{
+    const int n = self->ldim;
+    h.resize(n);
+    T.resize(2 * n - 1);
+    for (int i = 0; i < n; i++) {
+        const int l = lmin + i;
+        h[i] = -0.5 * (host_lfac(l + m) + host_lfac(l - m));
+    }
+    for (int k = 0; k < 2 * n - 1; k++) {
+        const int nu = 2 * lmin + k;
+        T[k] = (nu + 1) * self->y + host_lfac(nu);
+    }
+}
+
+}  // namespace
+
+// Plasma-model MM kernel.  Reference: src/libcaps.c:882-906.
+double caps_kernel_M0_MM_plasma(int i, int j, void* args_)
+{
+    caps_M_t* a = (caps_M_t*)args_;
+    const double y = a->caps->y;
+    const int l1 = i + a->lmin, l2 = j + a->lmin, m = a->m;
+    double r1, r2;
+    const double I = caps_integrate_plasma(a->integration_plasma, l1, l2, m, &r1, &r2);
+    const double alpha = a->integration_plasma->alpha;
+    const double f1 = 1 - (2 * l1 + 1.) / (alpha * r1), f2 = 1 - (2 * l2 + 1.) / (alpha * r2);
+    const double factor = sqrt(l1 * f1 / (l1 + 1.) * l2 * f2 / (l2 + 1.));
+    return factor * exp(host_lfac(l1 + l2)
+                        - 0.5 * (host_lfac(l1 + m) + host_lfac(l1 - m) + host_lfac(l2 + m) + host_lfac(l2 - m))
+                        + (l1 + l2 + 1) * y) * I;
+}
+
+// log det of the ξ = 0 blocks EE, MM and the plasma-model MM (omegap in
+// rad/s), each ldim x ldim, filled and factored on the GPU; NULL outputs
+// are skipped.  Reference: src/libcaps.c:984-1017.
 void caps_logdetD0(caps_t* self, int m, double omegap, double* EE, double* MM, double* MM_plasma)
 {
-    (void)omegap;
-    TERMINATE(MM_plasma != NULL, "caps_logdetD0: the plasma-model MM block (xi = 0) is not implemented");
-    size_t lmin, lmax;
-    caps_estimate_lminmax(self, m, &lmin, &lmax);
-    caps_M_t args;
-    memset(&args, 0, sizeof(args));
-    args.caps = self;
-    args.m = m;
-    args.lmin = (int)lmin;
-    args.ldim = self->ldim;
-    if (EE) *EE = kernel_logdet(self->ldim, caps_kernel_M0_EE, &args, 2, self->detalg);
-    if (MM) *MM = kernel_logdet(self->ldim, caps_kernel_M0_MM, &args, 2, self->detalg);
+    size_t lmin_, lmax_;
+    caps_estimate_lminmax(self, m, &lmin_, &lmax_);
+    const int lmin = (int)lmin_, n = self->ldim;
+    std::vector<double> h0, T0;
+    xi0_tables(self, m, lmin, h0, T0);
+    if (EE) *EE = sep_logdet(self, h0, T0);
+    if (MM) {
+        std::vector<double> h(h0);
+        for (int i = 0; i < n; i++) h[i] += 0.5 * log((lmin + i) / (lmin + i + 1.0));
+        *MM = sep_logdet(self, h, T0);
+    }
+    if (MM_plasma) {
+        integration_plasma_t* p = caps_integrate_plasma_init(self, omegap, self->epsrel);
+        std::vector<double> h(h0), T(T0);
+        for (int i = 0; i < n; i++) {
+            const int l = lmin + i;
+            double r1, r2;
+            caps_integrate_plasma(p, l, l, m, &r1, &r2);
+            const double f = 1 - (2 * l + 1.) / (p->alpha * r1);
+            TERMINATE(!(f > 0), "plasma xi = 0 block: factor %g at l = %d (omegap = %g)", f, l, omegap);
+            h[i] += 0.5 * log(l * f / (l + 1.));
+        }
+        for (int k = 0; k < 2 * n - 1; k++) {
+            double r1, r2;
+            const int nu = 2 * lmin + k;
+            T[k] += log(caps_integrate_plasma(p, nu / 2, nu - nu / 2, m, &r1, &r2));
+        }
+        caps_integrate_plasma_free(p);
+        *MM_plasma = sep_logdet(self, h, T);
+    }
+}
+
+// ξ = 0 free energy of the plasma model in units of k_B T: the Drude EE
+// term plus Σ'_m MM_plasma (m = 0 halved) until |v/Σ| < eps.  Reference:
+// src/libcaps.c:1127-1145.
+double caps_ht_plasma(caps_t* caps, double omegap, double eps)
+{
+    const double drude = caps_ht_drude(caps);
+    double s = 0;
+    for (int m = 0;; m++) {
+        double v;
+        caps_logdetD0(caps, m, omegap, NULL, NULL, &v);
+        s += m == 0 ? v / 2 : v;
+        if (fabs(v / s) < eps) return drude + s;
+    }
 }
 
 // ξ = 0 free energy of perfect reflectors: Drude EE term plus the MM blocks,

@@ -610,6 +610,25 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     }
 }
 
+// ξ = 0 blocks (EE, MM, plasma MM; src/libcaps.c:860-941): elements
+// M_ij = exp(h_i + h_j + T_{i+j}) with per-row terms h (n doubles) and
+// per-(l1+l2) terms T (2n - 1 doubles), both at arena offset off_tab.
+// Writes 1 - M column-major (leading dimension lda) and the diagonal of M.
+// Lanes run down the columns (coalesced stores), 64 x 4 per workgroup.
+__global__ void __launch_bounds__(256) k_fill_xi0(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                  int64_t off_tab)
+{
+    const BlockDesc& d = desc[0];
+    const int n = d.N;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= n || j >= n) return;
+    const double* h = at<double>(arena, off_tab);
+    const double* T = h + n;
+    const double v = exp(h[i] + h[j] + T[i + j]);
+    at<double>(arena, d.off_mat)[i + (int64_t)j * d.lda] = (i == j ? 1.0 : 0.0) - v;
+    if (i == j) at<double>(arena, d.off_diag)[i] = v;
+}
+
 // Trace of M (Kahan, in index order as src/matrix.c:63-68) and the trace
 // shortcut flag |tr M| < 1e-8 (src/matrix.c:78-82).  One lane per block.
 __global__ void k_trace(const BlockDesc* __restrict__ desc, char* __restrict__ arena, BlockOut* __restrict__ out,
@@ -722,6 +741,11 @@ void capsd_launch_assemble(const BlockDesc* desc, char* arena, const double* lfa
 {
     dim3 g((max_ldim + 15) / 16, (max_ldim + 15) / 16, nblocks);
     hipLaunchKernelGGL(k_assemble, g, dim3(256), 0, s, desc, arena, lfac_tab, mode);
+}
+
+void capsd_launch_fill_xi0(const BlockDesc* desc, char* arena, int64_t off_tab, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_xi0, dim3((n + 63) / 64, (n + 3) / 4), dim3(256), 0, s, desc, arena, off_tab);
 }
 
 void capsd_launch_trace(const BlockDesc* desc, char* arena, BlockOut* out, int nblocks, int frobenius, hipStream_t s)

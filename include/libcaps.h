@@ -162,12 +162,26 @@ double caps_ht_drude(caps_t *caps);
  * Reference: src/libcaps.c:860-873, 925-941. */
 double caps_kernel_M0_EE(int i, int j, void *args);
 double caps_kernel_M0_MM(int i, int j, void *args);
-/* ξ = 0 blocks EE and MM (ldim x ldim); MM_plasma must be NULL (the plasma
- * ξ = 0 integrals are not part of this library: abort with a message).
+/* ξ = 0 plasma-model MM kernel (args: caps_M_t with caps, m, lmin and
+ * integration_plasma).  Reference: src/libcaps.c:882-906. */
+double caps_kernel_M0_MM_plasma(int i, int j, void *args);
+/* ξ = 0 plasma integrals: ω = ωp (L+R)/c, I(l1+l2) = (1/ν!) ∫_0^∞ -r_TE(t)
+ * e^{-t} t^ν dt and the ratios I_{l-1/2}(α)/I_{l+1/2}(α), α = ω/(1 + L/R),
+ * cached per object (epsrel is accepted for compatibility: the integral is
+ * evaluated to full precision).  Reference: src/integration.c:897-1031
+ * (src/include/integration.h:24-26). */
+integration_plasma_t *caps_integrate_plasma_init(caps_t *caps, double omegap, double epsrel);
+double caps_integrate_plasma(integration_plasma_t *self, int l1, int l2, int m, double *ratio1, double *ratio2);
+void caps_integrate_plasma_free(integration_plasma_t *self);
+/* ξ = 0 blocks EE, MM and the plasma-model MM (omegap in rad/s), each
+ * ldim x ldim, filled and factored on the GPU; NULL outputs are skipped.
  * Reference: src/libcaps.c:984-1017. */
 void caps_logdetD0(caps_t *self, int m, double omegap, double *EE, double *MM, double *MM_plasma);
 /* ξ = 0 perfect-reflector free energy.  Reference: src/libcaps.c:1089-1116. */
 double caps_ht_perf(caps_t *caps, double eps);
+/* ξ = 0 plasma-model free energy (omegap in rad/s), in units of k_B T.
+ * Reference: src/libcaps.c:1127-1145. */
+double caps_ht_plasma(caps_t *caps, double omegap, double eps);
 
 /* ---- extensions ------------------------------------------------------- */
 

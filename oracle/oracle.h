@@ -79,6 +79,7 @@ int    orc_estimate_lmin(const orc_caps_t *c, int m);
 void   orc_fresnel(double epsm1, double xi_, double k_, double *rTE, double *rTM);
 void   orc_mie(const orc_caps_t *c, double xi_, int l, double *lna, double *lnb);
 double orc_K_estimate(int nu, int m, double alpha, double eps, double *a, double *b, double *approx);
+double orc_integrate_plasma(int nu, double omega, double epsrel);   /* ξ = 0 plasma integral */
 
 typedef struct orc_block orc_block_t;
 orc_block_t *orc_block_new(const orc_caps_t *c, double xi_, int m);
@@ -96,6 +97,11 @@ double orc_logdet_1mM(const double *M, int n, int detalg);  /* log det(1-M) with
 double orc_logdetD(const orc_caps_t *c, double xi_, int m);
 void   orc_logdetD_many(const orc_caps_t *c, const double *xi_, const int *m, int n, double *out, int threads);
 double orc_ht_drude(double LbyR);
+
+/* ---- xi0.c ------------------------------------------------------------- */
+void   orc_logdetD0(const orc_caps_t *c, int m, double omegap, double *EE, double *MM, double *MMp);
+double orc_ht_perf(const orc_caps_t *c, double eps);
+double orc_ht_plasma(const orc_caps_t *c, double omegap, double eps);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,10 @@ def lib():
             "orc_logdetD": (d, [ctypes.POINTER(OrcCaps), d, i]),
             "orc_logdetD_many": (None, [ctypes.POINTER(OrcCaps), dp, ctypes.POINTER(ctypes.c_int), i, dp, i]),
             "orc_ht_drude": (d, [d]),
+            "orc_integrate_plasma": (d, [i, d, d]),
+            "orc_logdetD0": (None, [ctypes.POINTER(OrcCaps), i, d, dp, dp, dp]),
+            "orc_ht_perf": (d, [ctypes.POINTER(OrcCaps), d]),
+            "orc_ht_plasma": (d, [ctypes.POINTER(OrcCaps), d, d]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)

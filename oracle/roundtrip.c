@@ -388,6 +388,47 @@ static double K_compute(orc_block_t *B, int nu, int p, orc_sign_t *sign)
     return log(fabs(sum)) + lognorm;
 }
 
+/* ξ = 0 plasma-model integral (1/ν!) ∫_0^∞ -r_TE(t) e^{-t} t^ν dt with
+ * -r_TE = β₋/(2 + β₋), β₋ = sqrt(1 + (2ω/t)²) - 1, ω = ωp (L+R)/c.
+ * Reference: src/integration.c:924-1019 (dqags on a window around the
+ * peak, dqage GK7-15 left of it, dqagi on the tail, all to epsrel).  The
+ * oracle: adaptive Gauss-Legendre on [0, ν], [ν, ν + 40√ν + 40] and the
+ * tail t = b + s/(1-s), s in [0, 1), to epsrel. */
+typedef struct { int nu; double omega, lp; } pint_t;
+typedef struct { const pint_t *p; double b; } ptail_t;
+
+static double plasma_integrand(double t, void *a_)
+{
+    const pint_t *a = (const pint_t *)a_;
+    if(t <= 0)
+        return 0;
+    const double q = 2*a->omega/t;
+    const double bm1 = orc_sqrtpm1(q*q);
+    return bm1/(2 + bm1)*exp(a->lp - t + a->nu*log(t));
+}
+
+static double plasma_tail(double s, void *a_)
+{
+    const ptail_t *a = (const ptail_t *)a_;
+    if(s >= 1)
+        return 0;
+    const double u = 1 - s;
+    return plasma_integrand(a->b + s/u, (void *)a->p)/(u*u);
+}
+
+double orc_integrate_plasma(int nu, double omega, double epsrel)
+{
+    gl_init();
+    pint_t p = { nu, omega, -orc_lfac((unsigned)nu) };
+    const double a = nu, b = nu + 40*sqrt((double)nu) + 40;
+    long nev = 0;
+    const double I2 = gl_adapt(plasma_integrand, &p, a, b, 0, epsrel, &nev);
+    const double I1 = gl_adapt(plasma_integrand, &p, 0, a, 0.01*epsrel*fabs(I2), epsrel, &nev);
+    ptail_t tl = { &p, b };
+    const double I3 = gl_adapt(plasma_tail, &tl, 0, 1, 0.01*epsrel*fabs(I2), epsrel, &nev);
+    return I1 + I2 + I3;
+}
+
 /* Cached K.  Reference: src/integration.c:452-489 (sign preset: TE -, TM +). */
 double orc_block_K(orc_block_t *B, int nu, int p, orc_sign_t *sign)
 {

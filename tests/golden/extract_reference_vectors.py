@@ -177,6 +177,38 @@ def logdetD_rows(fname):
     return rows
 
 
+def logdetD0_rows(fname):
+    """ξ = 0 assertions of test_logdetD0: caps_ht_plasma(omegap) values
+    (R/L = 100, ldim = 500) and the m = 0 EE, MM blocks (R/L = 1000,
+    ldim = 12000)."""
+    lines = read_lines(fname)
+    tols = tolerance_by_line(lines)
+    rows = []
+    state, env, m0 = {}, {"CAPS_c": CAPS_C}, None
+    for i, ln in enumerate(lines, 1):
+        m = re.search(r"const double R\s*=\s*(%s)\s*,\s*L\s*=\s*(%s)\s*;" % (NUM, NUM), ln)
+        if m:
+            env["R"], env["L"] = float(m.group(1)), float(m.group(2))
+        m = re.search(r"caps_init\(\s*(\w+|%s)\s*,\s*(\w+|%s)\s*\)" % (NUM, NUM), ln)
+        if m:
+            state = {"R": evaluate(m.group(1), env), "L": evaluate(m.group(2), env)}
+        m = re.search(r"caps_set_ldim\(\w+,\s*(\d+)\)", ln)
+        if m:
+            state["ldim"] = int(m.group(1))
+        m = re.search(r"caps_logdetD0\(\w+,\s*(\d+),", ln)
+        if m:
+            m0 = int(m.group(1))
+        m = re.search(r"AssertAlmostEqual\(&\w+,\s*caps_ht_plasma\(\w+,\s*([^,]+),\s*eps\),\s*(%s)\)" % NUM, ln)
+        if m and tols[i] and tols[i][0] == "caps_logdetD0":
+            rows.append(dict(state, fn="ht_plasma", omegap=evaluate(m.group(1), env), eps=1e-13,
+                             expect=float(m.group(2)), tol=tols[i][1], src="src/tests/%s:%d" % (fname, i)))
+        m = re.search(r"AssertAlmostEqual\(&\w+,\s*(EE|MM),\s*(%s)\)" % NUM, ln)
+        if m and tols[i] and tols[i][0] == "caps_logdetD0":
+            rows.append(dict(state, fn="logdetD0_" + m.group(1), m=m0, expect=float(m.group(2)), tol=tols[i][1],
+                             src="src/tests/%s:%d" % (fname, i)))
+    return rows
+
+
 def main():
     if not os.path.isdir(TESTS):
         sys.exit("reference tests not found at %s" % TESTS)
@@ -187,6 +219,7 @@ def main():
         "mie_drude.json": mie_rows("test_mie_drude.c"),
         "fresnel.json": fresnel_rows("test_fresnel.c"),
         "logdetD.json": logdetD_rows("test_logdetD.c"),
+        "logdetD0.json": logdetD0_rows("test_logdetD.c"),
     }
     # Bessel assertions take one or two arguments depending on the function.
     lines = read_lines("test_bessels.c")

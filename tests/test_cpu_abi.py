@@ -61,3 +61,22 @@ def test_host_only_entry_points():
     assert c.lminmax(100) == (400, 1600)
     assert c.lminmax(200) == (1400, 2600)
     assert abs(c.ht_drude() - po.lib().orc_ht_drude(1e-6 / 200e-6)) < 1e-14 * abs(c.ht_drude())
+
+
+def test_plasma_integrals_host():
+    """The library's ξ = 0 plasma integrals (trapezoid rule in log t, host
+    code) and Bessel ratios (Lentz continued fraction) against the oracle's
+    adaptive quadrature and the reference-style continued fraction."""
+    import caps_amd
+    import pyoracle as po
+    L = po.lib()
+    R, Lx = 100e-6, 1e-6
+    c = caps_amd.Caps(R, Lx, ldim=500)
+    for w in (0.01, 1.0, 100.0, 1000.0):
+        omegap = w * po.CAPS_C / (R + Lx)
+        alpha = w / (1 + Lx / R)
+        for l1, l2 in [(1, 1), (3, 7), (50, 60), (300, 500), (1000, 1200)]:
+            I, r1, r2 = c.plasma_integrals(omegap, l1, l2)
+            assert abs(I / L.orc_integrate_plasma(l1 + l2, w, 1e-13) - 1) < 1e-12, (w, l1, l2)
+            assert abs(r1 / L.orc_bessel_ratioI(l1 - 0.5, alpha) - 1) < 1e-13, (w, l1)
+            assert abs(r2 / L.orc_bessel_ratioI(l2 - 0.5, alpha) - 1) < 1e-13, (w, l2)

@@ -429,3 +429,58 @@ def test_c5_blocks_vs_reference(ca):
         t["total"] / 1e3, t["lu"] / 1e3, max(abs(v / r["logdetD"] - 1) for v, r in zip(out, rows))))
     bad = [(r["xi_"], r["m"], v, r["logdetD"]) for v, r in zip(out, rows) if not close_block(v, r["logdetD"])]
     assert not bad, bad
+
+
+XI0_ROWS = load_golden("logdetD0.json") if os.path.exists(os.path.join(GOLDEN, "logdetD0.json")) else []
+
+
+@pytest.mark.parametrize("row", [r for r in XI0_ROWS if r["fn"] == "ht_plasma"],
+                         ids=lambda r: "omegap%.3g" % r["omegap"])
+def test_ht_plasma_vs_reference(ca, row):
+    """ξ = 0 plasma-model free energy (reference test_logdetD0: R/L = 100,
+    ldim = 500, ωp (L+R)/c = 0.01 .. 1000): plasma integrals on the host,
+    blocks filled and factored on the GPU, m-sum to 1e-13."""
+    c = ca.Caps(row["R"], row["L"], ldim=row["ldim"])
+    v = c.ht_plasma(row["omegap"], row["eps"])
+    print("ht_plasma omegap=%.4g: %.16g reference %.16g (rel %.1e)" % (
+        row["omegap"], v, row["expect"], abs(v / row["expect"] - 1)))
+    assert relerr(v, row["expect"]) < row["tol"], (v, row["expect"])
+
+
+def test_logdetD0_m0_vs_reference(ca):
+    """ξ = 0, m = 0 EE and MM blocks at R/L = 1000, ldim = 12000 (12000 x
+    12000 matrices filled and factored on the GPU) against the reference's
+    test_logdetD0 values."""
+    rows = {r["fn"]: r for r in XI0_ROWS if r["fn"].startswith("logdetD0_")}
+    r = rows["logdetD0_EE"]
+    c = ca.Caps(r["R"], r["L"], ldim=r["ldim"])
+    ee, mm = c.logdetD0(0)
+    print("logdetD0 m=0 ldim=12000: EE %.16g (%.16g), MM %.16g (%.16g)" % (
+        ee, r["expect"], mm, rows["logdetD0_MM"]["expect"]))
+    assert relerr(ee, r["expect"]) < r["tol"], (ee, r["expect"])
+    assert relerr(mm, rows["logdetD0_MM"]["expect"]) < rows["logdetD0_MM"]["tol"]
+
+
+def test_logdetD0_blocks_vs_oracle(ca):
+    """ξ = 0 EE, MM and plasma MM blocks of the GPU path against the oracle
+    (dense CPU LU, adaptive quadrature) at R/L = 50, ldim = 150, for several
+    m and plasma frequencies; and caps_ht_perf against the oracle's."""
+    import ctypes
+    import pyoracle as po
+    L = po.lib()
+    R, Lx = 50e-6, 1e-6
+    c = ca.Caps(R, Lx, ldim=150)
+    oc = po.caps(R, Lx, ldim=150, epsrel=1e-12)
+    for m in (0, 1, 7, 30):
+        for w in (0.1, 10.0, 500.0):
+            omegap = w * po.CAPS_C / (R + Lx)
+            ee, mm, mp = c.logdetD0(m, omegap)
+            oe, om, op = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            L.orc_logdetD0(ctypes.byref(oc), m, omegap, ctypes.byref(oe), ctypes.byref(om), ctypes.byref(op))
+            for got, want in [(ee, oe.value), (mm, om.value), (mp, op.value)]:
+                # absolute floor: a log det of a near-identity block carries
+                # ~N ulp of rounding (these blocks have |log det| down to 1e-4)
+                assert abs(got - want) <= 1e-10 * abs(want) + 1e-13, (m, w, got, want)
+    oc2 = po.caps(10e-6, 1e-6, ldim=60, epsrel=1e-12)
+    c2 = ca.Caps(10e-6, 1e-6, ldim=60)
+    assert abs(c2.ht_perf(1e-13) / L.orc_ht_perf(ctypes.byref(oc2), 1e-13) - 1) < 1e-12

@@ -219,3 +219,57 @@ def test_logdetD_golden(row):
     c = po.caps(row["R"], row["L"], ldim=row["ldim"])
     v = po.logdetD(c, row["xi_"], row["m"])
     assert relerr(v, row["expect"]) < row["tol"], (v, row["expect"])
+
+
+# ξ = 0 blocks (reference test_logdetD0): the plasma-model free energies at
+# R/L = 100, ldim = 500; the m = 0 EE/MM values at ldim = 12000 are checked
+# by the GPU tests (a 12000 x 12000 LU is too long for the CPU oracle).
+XI0_ROWS = load_golden("logdetD0.json")
+
+
+@pytest.mark.parametrize("row", [r for r in XI0_ROWS if r["fn"] == "ht_plasma"],
+                         ids=lambda r: "omegap%.3g" % r["omegap"])
+def test_ht_plasma_golden(row):
+    import ctypes
+    c = po.caps(row["R"], row["L"], ldim=row["ldim"], epsrel=1e-12)
+    v = po.lib().orc_ht_plasma(ctypes.byref(c), row["omegap"], row["eps"])
+    assert relerr(v, row["expect"]) < row["tol"], (v, row["expect"])
+
+
+def test_xi0_blocks_closed_forms():
+    """ξ = 0 kernels against closed forms at R/L = 10 (ldim = 160, where the
+    window has converged): the m = 0 MM block equals twice Bimonte's series
+    Σ_l ½ log(1 - Z^{2l+3}) (the m = 0 term of caps_ht_perf, which is halved
+    in the m-sum), and Σ'_m of the EE blocks equals the closed-form Drude
+    term (Bimonte-Emig)."""
+    import ctypes
+    L = po.lib()
+    c = po.caps(10e-6, 1e-6, ldim=160, epsrel=1e-12)
+    Z = 1 / (1 + c.LbyR + math.sqrt(c.LbyR * (2 + c.LbyR)))
+    series = sum(0.5 * math.log1p(-Z ** (2 * l + 3)) for l in range(2000))
+    EE, MM = ctypes.c_double(), ctypes.c_double()
+    L.orc_logdetD0(ctypes.byref(c), 0, 0.0, ctypes.byref(EE), ctypes.byref(MM), None)
+    assert abs(MM.value / (2 * series) - 1) < 1e-12, (MM.value, 2 * series)
+    s = 0.0
+    for m in range(200):
+        L.orc_logdetD0(ctypes.byref(c), m, 0.0, ctypes.byref(EE), None, None)
+        s += EE.value / 2 if m == 0 else EE.value
+        if m > 0 and abs(EE.value / s) < 1e-15:
+            break
+    drude = L.orc_ht_drude(c.LbyR)
+    assert abs(s / drude - 1) < 1e-12, (s, drude)
+
+
+def test_plasma_integral_limits():
+    """ξ = 0 plasma integral (1/ν!) ∫ -r_TE e^{-t} t^ν dt: tends to
+    1 - (ν+1)/ω as ω -> ∞ (-r_TE ≈ 1 - t/ω) and to ω²/(ν(ν-1)) as ω -> 0
+    (-r_TE ≈ ω²/t²), ω = ωp (L+R)/c."""
+    L = po.lib()
+    for nu in (2, 7, 40, 300):
+        big = L.orc_integrate_plasma(nu, 1e9, 1e-13)
+        assert abs(big - (1 - (nu + 1) / 1e9)) < 1e-12, (nu, big)
+        # the next order is O(ω) relative at ν = 2 (t < ω, where -r_TE ≈ 1),
+        # O(ω²/ν²) above
+        w = 1e-8 if nu == 2 else 1e-4
+        small = L.orc_integrate_plasma(nu, w, 1e-13)
+        assert abs(small / (w * w / (nu * (nu - 1))) - 1) < 1e-6, (nu, small)
