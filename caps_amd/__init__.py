@@ -73,6 +73,9 @@ def lib():
             "capsx_lu_grid_panel_rows": (i, [i]),
             "capsx_surrogate_report": (None, [p, dp, i]),
             "caps_ht_plasma": (d, [p, d, d]),
+            "capsx_npy_save": (i, [ctypes.c_char_p, dp, i]),
+            "capsx_npy_load": (dp, [ctypes.c_char_p, ip]),
+            "capsx_npy_free": (None, [dp]),
             "caps_integrate_plasma_init": (p, [p, d, d]),
             "caps_integrate_plasma": (d, [p, i, i, i, dp, dp]),
             "caps_integrate_plasma_free": (None, [p]),
@@ -281,6 +284,30 @@ class _MObj:
         if getattr(self, "ptr", None):
             lib().caps_M_free(self.ptr)
             self.ptr = None
+
+
+def npy_save(path, M):
+    """Write a square matrix as .npy in the reference's byte layout
+    (element (i, j) at i*dim + j under a fortran_order header)."""
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    if M.ndim != 2 or M.shape[0] != M.shape[1]:
+        raise ValueError("square matrix required")
+    if lib().capsx_npy_save(str(path).encode(), _dptr(M), M.shape[0]) != 0:
+        raise OSError("cannot write %s" % path)
+
+
+def npy_load(path):
+    """Read a matrix file of that layout; returns the dim x dim array with
+    element (i, j) = file element i*dim + j (numpy.load gives its
+    transpose)."""
+    dim = ctypes.c_int()
+    p = lib().capsx_npy_load(str(path).encode(), ctypes.byref(dim))
+    if not p:
+        raise ValueError("%s is not a square '<f8' fortran-order .npy file" % path)
+    n = dim.value
+    out = np.ctypeslib.as_array(p, shape=(n * n,)).copy().reshape(n, n)
+    lib().capsx_npy_free(p)
+    return out
 
 
 def logdetD_multi(objs, xis, ms):

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -733,6 +734,36 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     }
 }
 
+// .npy file of a dim x dim matrix in the reference's byte layout
+// (src/matrix.c:211-233): magic, version 1.0, header
+// "{'descr': '<f8', 'fortran_order': True, 'shape': (dim, dim), }", then the
+// elements with (i, j) at i*dim + j.  at(i, j) supplies element (i, j).
+bool npy_write(const char* path, int dim, const std::function<double(int, int)>& at)
+{
+    FILE* f = fopen(path, "wb");
+    if (!f) return false;
+    char hdr[128];
+    snprintf(hdr, sizeof(hdr), "{'descr': '<f8', 'fortran_order': True, 'shape': (%d, %d), }", dim, dim);
+    const uint16_t len = (uint16_t)strlen(hdr);
+    bool ok = fwrite("\x93NUMPY\x01\x00", 1, 8, f) == 8 && fwrite(&len, sizeof(len), 1, f) == 1 &&
+              fwrite(hdr, 1, len, f) == len;
+    std::vector<double> row(dim);
+    for (int i = 0; i < dim && ok; i++) {
+        for (int j = 0; j < dim; j++) row[j] = at(i, j);
+        ok = fwrite(row.data(), sizeof(double), dim, f) == (size_t)dim;
+    }
+    return fclose(f) == 0 && ok;
+}
+
+// CAPS_DUMP: with a dense determinant algorithm the reference's
+// kernel_logdet writes every matrix M it factors (after the trace shortcut)
+// to the file the variable names (src/matrix.c:84-109).
+const char* dump_path(detalg_t detalg)
+{
+    const char* p = getenv("CAPS_DUMP");
+    return (p && *p && detalg != DETALG_HODLR) ? p : nullptr;
+}
+
 // log det(1-M) from the block scalars with the reference's shortcut rules.
 double finish_block(const BlockOut& o, int lu_mode)
 {
@@ -995,9 +1026,31 @@ int caps_logdetD_batch(caps_t* self, const double* xi_, const int* m, int n, dou
     return capsx_logdetD_multi(objs.data(), xi_, m, n, out);
 }
 
+// Host copy of a block's matrix (the reference's integration_t slot of
+// caps_M_t).
+struct caps_block_state {
+    double* M;   // N x N, column-major, lda = N
+    int N;
+};
+
 double caps_logdetD(caps_t* self, double xi_, int m)
 {
     TERMINATE(xi_ <= 0, "Matsubara frequency must be positive");
+    if (const char* path = dump_path(self->detalg)) {
+        // the matrix M of the block (built once more, without the LU)
+        caps_M_t* M = caps_M_init(self, m, xi_);
+        const caps_block_state* st = M->integration;
+        double tr = 0, comp = 0;
+        for (int i = 0; i < st->N; i++) {
+            const double y = st->M[i + (size_t)i * st->N] - comp, t = tr + y;
+            comp = (t - tr) - y;
+            tr = t;
+        }
+        if (fabs(tr) >= 1e-8)
+            TERMINATE(!npy_write(path, st->N, [&](int i, int j) { return st->M[i + (size_t)j * st->N]; }),
+                      "cannot write %s", path);
+        caps_M_free(M);
+    }
     double out;
     caps_logdetD_batch(self, &xi_, &m, 1, &out);
     return out;
@@ -1010,11 +1063,6 @@ void caps_last_timing(caps_t* self, double ms[6])
 }
 
 // ---- round-trip matrix object ----------------------------------------------
-
-struct caps_block_state {
-    double* M;   // N x N, column-major, lda = N
-    int N;
-};
 
 caps_M_t* caps_M_init(caps_t* caps, int m, double xi_)
 {
@@ -1096,6 +1144,9 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
             A[k + (size_t)(md + k) * lda] = kernel(k, md + k, args);
             A[(md + k) + (size_t)k * lda] = kernel(md + k, k, args);
         }
+    if (const char* path = dump_path(detalg))
+        TERMINATE(!npy_write(path, dim, [&](int i, int j) { return A[i + (size_t)j * lda]; }), "cannot write %s",
+                  path);
     const int lu_mode = is_lu_mode(detalg);
     if (lu_mode) {
         double norm = 0;
@@ -1244,6 +1295,17 @@ double sep_logdet(caps_t* self, const std::vector<double>& h, const std::vector<
     const int lu_mode = is_lu_mode(self->detalg);
     capsd_launch_fill_xi0(c->d_desc, c->arena, off_tab, n, s);
     capsd_launch_trace(c->d_desc, c->arena, c->d_out, 1, lu_mode, s);
+    if (const char* path = dump_path(self->detalg)) {
+        BlockOut o0;
+        HIPCHECK(hipMemcpyAsync(&o0, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, s));
+        std::vector<double> A((size_t)lda * n), dg(n);
+        HIPCHECK(hipMemcpyAsync(A.data(), c->arena + d.off_mat, A.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(dg.data(), c->arena + d.off_diag, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        if (fabs(o0.trace) >= 1e-8)   // M = 1 - A off the diagonal, the exact diagonal of M on it
+            TERMINATE(!npy_write(path, n, [&](int i, int j) { return i == j ? dg[i] : -A[i + (size_t)j * lda]; }),
+                      "cannot write %s", path);
+    }
     HIPCHECK(hipEventRecord(c->ev[6], s));
     launch_lu_groups(c, 1, n, lu_config(n, 1), c->ev[6], false);
     BlockOut o;
@@ -2112,6 +2174,52 @@ void capsx_set_surrogate(caps_t* self, struct capsx_mlp* mlp, float f_omegap, fl
     c->sur_feat[0] = f_omegap;
     c->sur_feat[1] = f_gamma;
 }
+
+// Extension: write a dim x dim matrix (element (i, j) at M[i*dim + j]) as
+// .npy in the reference's byte layout (matrix_save_to_file,
+// src/matrix.c:211-254).  Returns 0 on success.
+int capsx_npy_save(const char* path, const double* M, int dim)
+{
+    if (!path || !M || dim <= 0) return 1;
+    return npy_write(path, dim, [&](int i, int j) { return M[(size_t)i * dim + j]; }) ? 0 : 1;
+}
+
+// Extension: read a square '<f8' .npy file with fortran_order True, as
+// the reference's matrix_load_from_file does (src/matrix.c:270-335; the
+// elements in file order).  Returns a malloc'd array (capsx_npy_free) and
+// its dimension, or NULL for a file of another layout.
+double* capsx_npy_load(const char* path, int* dim)
+{
+    FILE* f = fopen(path, "rb");
+    if (!f) return nullptr;
+    char magic[8];
+    uint16_t len = 0;
+    char hdr[512] = {0};
+    double* out = nullptr;
+    if (fread(magic, 1, 8, f) == 8 && memcmp(magic, "\x93NUMPY\x01\x00", 8) == 0 && fread(&len, 2, 1, f) == 1 &&
+        len >= 2 && len < sizeof(hdr) && fread(hdr, 1, len, f) == len && strstr(hdr, "'descr': '<f8'") &&
+        strstr(hdr, "'fortran_order': True")) {
+        const char* p = strstr(hdr, "'shape': (");
+        if (p) {
+            p += 10;
+            const long d1 = strtol(p, nullptr, 10);
+            const char* q = strchr(p, ',');
+            const long d2 = q ? strtol(q + 1, nullptr, 10) : -1;
+            if (d1 > 0 && d1 == d2 && d1 <= 1000000) {
+                out = (double*)malloc((size_t)d1 * d1 * sizeof(double));
+                if (out && fread(out, sizeof(double), (size_t)d1 * d1, f) != (size_t)d1 * d1) {
+                    free(out);
+                    out = nullptr;
+                }
+                if (out) *dim = (int)d1;
+            }
+        }
+    }
+    fclose(f);
+    return out;
+}
+
+void capsx_npy_free(double* M) { free(M); }
 
 // Extension: accuracy of the planner surrogate against the blocks computed
 // for its plans since the last reset (resets when reset != 0):

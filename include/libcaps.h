@@ -265,6 +265,23 @@ void caps_last_timing(caps_t *self, double ms[6]);
 double caps_free_energy(caps_t *self, double T, double cutoff, double epsrel, int mode,
                         int drude_xi0, long *nblocks, long *nused);
 
+/* ---- matrix files ------------------------------------------------------
+ * CAPS_DUMP: with a dense determinant algorithm (detalg LU, QR, CHOLESKY),
+ * caps_logdetD, caps_logdetD0 and kernel_logdet write each matrix M they
+ * factor (not the trace-shortcut ones) to the file the environment variable
+ * names, as the reference's kernel_logdet does (src/matrix.c:84-109); the
+ * batched extensions do not.  The byte layout is the reference's. */
+/* Extension: write a dim x dim matrix (element (i, j) at M[i*dim + j]) as
+ * .npy: '<f8', 'fortran_order': True, the elements in that order (the
+ * reference's matrix_save_to_file, src/matrix.c:211-254).  Returns 0 on
+ * success. */
+int capsx_npy_save(const char *path, const double *M, int dim);
+/* Extension: read such a file (the reference's matrix_load_from_file,
+ * src/matrix.c:270-335): a malloc'd dim*dim array in file order, NULL for a
+ * file of another layout.  Release with capsx_npy_free. */
+double *capsx_npy_load(const char *path, int *dim);
+void capsx_npy_free(double *M);
+
 /* ---- planner surrogate (no counterpart in the reference) ---------------
  * An MLP 5 -> 64 -> 64 -> 1 (tanh, FP32) predicting log|logdetD| from
  * (log R/L, log ξ(L+R)/c, m/ldim, log ωp/eV, log γ/eV), trained and

@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from conftest import ROOT
 
 
@@ -80,3 +82,26 @@ def test_plasma_integrals_host():
             assert abs(I / L.orc_integrate_plasma(l1 + l2, w, 1e-13) - 1) < 1e-12, (w, l1, l2)
             assert abs(r1 / L.orc_bessel_ratioI(l1 - 0.5, alpha) - 1) < 1e-13, (w, l1)
             assert abs(r2 / L.orc_bessel_ratioI(l2 - 0.5, alpha) - 1) < 1e-13, (w, l2)
+
+
+def test_npy_files(tmp_path):
+    """Matrix files in the reference's layout (src/matrix.c:211-335): the
+    header bytes, a round trip, and numpy's reading of the same bytes (the
+    header says fortran order, so numpy returns the transpose); files of
+    another layout are refused."""
+    import numpy as np
+    import caps_amd
+    rng = np.random.default_rng(3)
+    M = rng.normal(size=(7, 7))
+    p = tmp_path / "m.npy"
+    caps_amd.npy_save(p, M)
+    raw = p.read_bytes()
+    hdr = b"{'descr': '<f8', 'fortran_order': True, 'shape': (7, 7), }"
+    assert raw[:8] == b"\x93NUMPY\x01\x00" and raw[8:10] == len(hdr).to_bytes(2, "little")
+    assert raw[10:10 + len(hdr)] == hdr and len(raw) == 10 + len(hdr) + 8 * 49
+    assert np.array_equal(caps_amd.npy_load(p), M)
+    assert np.array_equal(np.load(p, allow_pickle=False), M.T)
+    q = tmp_path / "c.npy"
+    np.save(q, np.ones((3, 3)))   # C order: not the reference's layout
+    with pytest.raises(ValueError):
+        caps_amd.npy_load(q)

@@ -484,3 +484,38 @@ def test_logdetD0_blocks_vs_oracle(ca):
     oc2 = po.caps(10e-6, 1e-6, ldim=60, epsrel=1e-12)
     c2 = ca.Caps(10e-6, 1e-6, ldim=60)
     assert abs(c2.ht_perf(1e-13) / L.orc_ht_perf(ctypes.byref(oc2), 1e-13) - 1) < 1e-12
+
+
+def test_caps_dump_matrix(ca, tmp_path, monkeypatch):
+    """CAPS_DUMP with a dense determinant algorithm: caps_logdetD writes the
+    block's M in the reference's file layout; it is the matrix caps_M_init
+    builds, the oracle's block matrix to 1e-10, and log det(1 - M) of it is
+    the returned value.  caps_logdetD0 writes its last block (MM), whose
+    elements follow the closed form."""
+    import pyoracle as po
+    path = tmp_path / "M.npy"
+    monkeypatch.setenv("CAPS_DUMP", str(path))
+    c = make(ca, 20e-6, 1e-6, 40, detalg="LU")
+    v = c.logdetD(1.0, 2)
+    M = ca.npy_load(path)
+    Mb, lmin, obj = c.matrix(1.0, 2)
+    assert np.array_equal(M, Mb)
+    N = M.shape[0]
+    assert abs(np.linalg.slogdet(np.eye(N) - M)[1] - v) < 1e-12 * max(1, abs(v))
+    oc = po.caps(20e-6, 1e-6, ldim=40, detalg=1)
+    B = po.Block(oc, 1.0, 2)
+    Mo = B.matrix()
+    assert np.max(np.abs(M - Mo)) < 1e-10 * np.max(np.abs(Mo))
+    # xi = 0: the MM block of m = 3 is the file's last matrix
+    ee, mm = c.logdetD0(3)
+    M0 = ca.npy_load(path)
+    lmin0 = c.lminmax(3)[0]
+    l = lmin0 + np.arange(c.ldim)
+    from math import lgamma
+    lf = np.vectorize(lambda n: lgamma(n + 1.0))
+    y = -math.log(2) - math.log1p(1 / 20)
+    L1, L2 = np.meshgrid(l, l, indexing="ij")
+    ref = np.exp((L1 + L2 + 1) * y + lf(L1 + L2) - 0.5 * (lf(L1 + 3) + lf(L1 - 3) + lf(L2 + 3) + lf(L2 - 3)))
+    ref *= np.sqrt(L1 * L2 / ((L1 + 1.0) * (L2 + 1.0)))
+    assert np.max(np.abs(M0 - ref) / ref) < 1e-11
+    assert abs(np.linalg.slogdet(np.eye(c.ldim) - M0)[1] - mm) < 1e-12 * max(1, abs(mm))
