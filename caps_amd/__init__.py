@@ -73,6 +73,9 @@ def lib():
             "capsx_lu_grid_panel_rows": (i, [i]),
             "capsx_surrogate_report": (None, [p, dp, i]),
             "caps_ht_plasma": (d, [p, d, d]),
+            "capsx_free_energy": (d, [p, d, d, d, i, i, d, i, ctypes.POINTER(ctypes.c_long),
+                                      ctypes.POINTER(ctypes.c_long)]),
+            "capsx_psd": (i, [i, dp, dp]),
             "capsx_npy_save": (i, [ctypes.c_char_p, dp, i]),
             "capsx_npy_load": (dp, [ctypes.c_char_p, ip]),
             "capsx_npy_free": (None, [dp]),
@@ -224,6 +227,18 @@ class Caps:
     def ht_drude(self):
         return lib().caps_ht_drude(self.ptr)
 
+    def free_energy_ex(self, T, cutoff=1e-9, epsrel=1e-6, mode="batched", xi0="drude", omegap=0.0, psd_order=0):
+        """Free energy with the reference driver's options (capsx_free_energy):
+        xi0 in ("drude", "pr", "plasma") selects the ξ = 0 term (plasma:
+        omegap in rad/s); psd_order 0 = Matsubara sum, > 0 = Padé spectrum
+        decomposition of that order, < 0 = automatic order.  Returns (F,
+        determinants computed, determinants entering the sums)."""
+        nb, nu = ctypes.c_long(), ctypes.c_long()
+        F = lib().capsx_free_energy(self.ptr, T, cutoff, epsrel, {"plain": 0, "batched": 1, "surrogate": 2}[mode],
+                                    {"drude": 1, "pr": 2, "plasma": 3}[xi0], omegap, psd_order,
+                                    ctypes.byref(nb), ctypes.byref(nu))
+        return F, nb.value, nu.value
+
     def free_energy(self, T, cutoff=1e-9, epsrel=1e-6, mode="batched", drude_xi0=True):
         """F (L+R)/(ħc) at temperature T with the reference driver's sums
         (src/caps.c:235-396, 722-850); T = 0 integrates over ξ with the
@@ -284,6 +299,15 @@ class _MObj:
         if getattr(self, "ptr", None):
             lib().caps_M_free(self.ptr)
             self.ptr = None
+
+
+def psd(N):
+    """Poles ξ_j and residues η_j of the Padé spectrum decomposition of
+    order N (capsx_psd)."""
+    xi, eta = np.empty(N), np.empty(N)
+    if lib().capsx_psd(N, _dptr(xi), _dptr(eta)) != 0:
+        raise ValueError("invalid order %d" % N)
+    return xi, eta
 
 
 def npy_save(path, M):

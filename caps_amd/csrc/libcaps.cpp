@@ -1252,6 +1252,8 @@ double caps_kernel_M0_MM(int i, int j, void* args_)
            * sqrt(((double)l1 * (double)l2) / ((l1 + 1.) * (l2 + 1.)));
 }
 
+}  // extern "C"
+
 namespace {
 
 // log det(1 - M) of the n x n block M_ij = exp(h_i + h_j + T_{i+j}) (the
@@ -1367,6 +1369,8 @@ double plasma_integral(int nu, double omega)
 
 }  // namespace
 
+extern "C" {
+
 // Plasma-model ξ = 0 integrals (the reference's integration_plasma_t):
 // ω = ωp (L+R)/c, α = ω/(1 + L/R), cached I(ν) and Bessel ratios.
 struct caps_block_state_plasma {
@@ -1408,6 +1412,8 @@ void caps_integrate_plasma_free(integration_plasma_t* self) { delete self; }
 
 }  // extern "C"
 
+}  // extern "C"
+
 namespace {
 
 // ln of the common ξ = 0 element factor, split as h(l1) + h(l2) + T(l1+l2)
@@ -1429,6 +1435,8 @@ void xi0_tables(caps_t* self, int m, int lmin, std::vector<double>& h, std::vect
 }
 
 }  // namespace
+
+extern "C" {
 
 // Plasma-model MM kernel.  Reference: src/libcaps.c:882-906.
 double caps_kernel_M0_MM_plasma(int i, int j, void* args_)
@@ -1521,6 +1529,8 @@ double caps_ht_perf(caps_t* caps, double eps)
 }
 
 // ---- free energy (reference driver semantics) -------------------------------
+
+}  // extern "C"
 
 namespace {
 
@@ -1767,16 +1777,234 @@ double free_energy_T0(caps_t* self, double cutoff, double epsrel, int mode, long
 
 }  // namespace
 
+namespace {
+
+double free_energy_msd(caps_t* self, double T_scaled, double cutoff, double epsrel, int mode, double v0,
+                       long* nblocks, long* nused, long computed0);
+
+// Eigenvalues below zero of the 2N x 2N symmetric tridiagonal matrix with
+// zero diagonal and off-diagonal 1/sqrt((2j+1)(2j+3)), j = 1 .. 2N-1, by
+// bisection on Sturm counts (each to full precision), ascending.
+std::vector<double> psd_eigen_negative(int N)
+{
+    const int n = 2 * N;
+    std::vector<double> e2(n);
+    for (int j = 1; j < n; j++) e2[j] = 1.0 / ((2.0 * j + 1) * (2.0 * j + 3));
+    auto count_below = [&](double x) {   // eigenvalues < x
+        int c = 0;
+        double q = -x;
+        if (q < 0) c++;
+        for (int i = 1; i < n; i++) {
+            if (q == 0) q = 1e-300;
+            q = -x - e2[i] / q;
+            if (q < 0) c++;
+        }
+        return c;
+    };
+    std::vector<double> w(N);
+    for (int k = 0; k < N; k++) {   // the (k+1)-th smallest eigenvalue
+        double lo = -2, hi = 0;       // |λ| <= max row sum < 2/sqrt(15) + 1/sqrt(35) < 1
+        for (int it = 0; it < 200 && hi - lo > 0; it++) {
+            const double mid = 0.5 * (lo + hi);
+            if (mid == lo || mid == hi) break;
+            if (count_below(mid) > k) hi = mid;
+            else lo = mid;
+        }
+        w[k] = 0.5 * (lo + hi);
+    }
+    return w;
+}
+
+// Residue η_j of the Padé pole ξ_j (z = -ξ_j²): the three-term recurrences
+// of the numerator and denominator polynomials of the [N-1/N] Padé
+// approximant, A_M = (2M+1) A_{M-1} + z A_{M-2}/4 (same for B, and for dB/dz
+// by differentiation), from A_1 = 1/4, A_2 = 5/4, B_1 = 3, B_2 = 15 + z/4,
+// rescaled every step; η = A/(2 dB/dz) at M = 2N (Hu, Xu, Yan, J. Chem.
+// Phys. 133, 101106 (2010), eqs. (12)-(13); the reference's psd.c:20-62).
+double psd_residue(int M2, double z)
+{
+    double a1 = 0.25, a2 = 1.25, b1 = 3, b2 = 15 + z / 4, d1 = 0, d2 = 0.25;
+    for (int M = 3; M <= M2; M++) {
+        const double a = (2 * M + 1) * a2 + z * a1 / 4;
+        const double b = (2 * M + 1) * b2 + z * b1 / 4;
+        const double d = (2 * M + 1) * d2 + b1 / 4 + z * d1 / 4;
+        a1 = a2 / a;
+        b1 = b2 / a;
+        d1 = d2 / a;
+        a2 = 1;
+        b2 = b / a;
+        d2 = d / a;
+    }
+    return a2 / (2 * d2);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Extension: poles ξ_j (imaginary frequencies, ascending) and residues η_j
+// of the Padé spectrum decomposition of order N of the Bose function,
+// 1/(1 - e^{-x}) = 1/x + 1/2 + Σ_j 2 η_j x/(x² + ξ_j²) (Hu, Xu, Yan 2010):
+// ξ_j = -2/w_j from the N negative eigenvalues w_j of the tridiagonal
+// matrix of eq. (14).  Returns 0 (N <= 0: 1).  Reference: src/psd.c:80-136.
+int capsx_psd(int N, double* xi, double* eta)
+{
+    if (N <= 0) return 1;
+    const std::vector<double> w = psd_eigen_negative(N);
+    for (int j = 0; j < N; j++) {
+        xi[j] = -2 / w[j];
+        eta[j] = psd_residue(2 * N, -xi[j] * xi[j]);
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// ξ = 0 term of the driver's sum (src/caps.c:270-300, 307-396): model 1
+// Drude (caps_ht_drude); 2 perfect reflector and 3 plasma (omegap, rad/s):
+// the Drude EE part plus Σ'_m of the MM blocks (m = 0 halved) up to the
+// first m with v = 0 or v/v_0 < cutoff.
+double xi0_term(caps_t* self, int model, double omegap, double cutoff, long* computed)
+{
+    const double drude = caps_ht_drude(self);
+    if (model == 1) return drude;
+    TERMINATE(model != 2 && model != 3, "unknown xi = 0 model %d", model);
+    std::vector<double> v;
+    for (int m = 0;; m++) {
+        TERMINATE(m >= 100000, "xi = 0 sum over m did not converge");
+        double x;
+        if (model == 2) caps_logdetD0(self, m, 0, NULL, &x, NULL);
+        else caps_logdetD0(self, m, omegap, NULL, NULL, &x);
+        (*computed)++;
+        v.push_back(x);
+        if (x == 0 || x / v[0] < cutoff) break;
+    }
+    v[0] /= 2;
+    return drude + kahan(v);
+}
+
+// T > 0 by the Padé spectrum decomposition (src/caps.c:811-832):
+// F = T_scaled/π [v_0/2 + Σ_j η_j F_ξ(ξ_j T_scaled/(2π))]; the frequencies'
+// m-sums planned as the Matsubara driver's (mode 0 plain, 1 batched with
+// m-ranges from the previous pole, 2 surrogate-planned).
+double free_energy_psd(caps_t* self, double T_scaled, double cutoff, int order, int mode, double v0,
+                       long* nblocks, long* nused, long computed)
+{
+    Ctx* cx = ctx_of(self);
+    std::vector<double> pole(order), eta(order);
+    capsx_psd(order, pole.data(), eta.data());
+    std::vector<Term> terms(order);
+    for (int j = 0; j < order; j++) terms[j].xi = pole[j] * T_scaled / (2 * M_PI);
+    long used = 0;
+    if (mode == 0) {
+        for (Term& t : terms)
+            for (int m = 0;; m++) {
+                TERMINATE(m >= 100000, "sum over m did not converge at xi_=%g", t.xi);
+                t.v.push_back(caps_logdetD(self, t.xi, m));
+                computed++;
+                if (msum(t.v, cutoff, &t.value, &t.mstop)) break;
+            }
+    } else {
+        std::vector<std::pair<int, int>> req;
+        const int MCAP = std::min(self->ldim - 1, 4000), per = MCAP + 1;
+        std::vector<float> P;
+        std::vector<int> mpred(order, 64);
+        if (mode == 2) {
+            TERMINATE(!cx->sur, "mode 2 needs a surrogate (capsx_set_surrogate)");
+            std::vector<double> xs;
+            for (const Term& t : terms) xs.push_back(t.xi);
+            sur_predict(self, xs, MCAP, P);
+            for (int j = 0; j < order; j++) mpred[j] = sur_mcut(&P[(size_t)j * per], MCAP, cutoff);
+        }
+        // mode 1: the first pole in a growing batch, then the others with
+        // the m-range of the pole before (m-cutoffs fall with ξ)
+        int j0 = 0;
+        if (mode == 1) {
+            int upto = 64;
+            while (true) {
+                run_requests(self, terms, {{0, upto}}, &computed);
+                cx->plan_batches++;
+                if (msum(terms[0].v, cutoff, &terms[0].value, &terms[0].mstop)) break;
+                upto += 32;
+            }
+            j0 = 1;
+        }
+        for (int j = j0; j < order; j++) {
+            const int ms = mode == 2 ? mpred[j] : terms[0].mstop;
+            req.push_back({j, std::min(self->ldim * 8, ms + std::max(4, ms / 25) + 1)});
+        }
+        run_requests(self, terms, req, &computed);
+        cx->plan_batches++;
+        for (int j = 0; j < order; j++) {
+            Term& t = terms[j];
+            int upto = (int)t.v.size();
+            while (!msum(t.v, cutoff, &t.value, &t.mstop)) {
+                TERMINATE(upto >= 100000, "sum over m did not converge at xi_=%g", t.xi);
+                upto += 16;
+                run_requests(self, terms, {{j, upto}}, &computed);
+                cx->plan_batches++;
+            }
+            if (mode == 2) sur_score(cx, &P[(size_t)j * per], MCAP, mpred[j], t);
+        }
+    }
+    std::vector<double> v{v0 / 2};
+    for (int j = 0; j < order; j++) {
+        v.push_back(eta[j] * terms[j].value);
+        used += terms[j].mstop + 1;
+    }
+    if (nblocks) *nblocks = computed;
+    if (nused) *nused = used;
+    return T_scaled / M_PI * kahan(v);
+}
+
+}  // namespace
+
+extern "C" {
+
+// Extension: the free energy with the reference driver's options.  xi0:
+// ξ = 0 term (1 Drude, 2 perfect reflector, 3 plasma with omegap in rad/s);
+// psd_order: 0 Matsubara sum, > 0 Padé spectrum decomposition of that
+// order, < 0 the driver's automatic order ceil((1 - 1.5 log10 epsrel)/
+// sqrt(T_eff)), T_eff = 4π k_B T L/(ħc) (src/caps.c:635-639).  T = 0
+// integrates over ξ (xi0 and psd_order unused).
+double capsx_free_energy(caps_t* self, double T, double cutoff, double epsrel, int mode, int xi0, double omegap,
+                         int psd_order, long* nblocks, long* nused)
+{
+    TERMINATE(!(T >= 0), "the free energy needs T >= 0 (got %g)", T);
+    if (T == 0) return free_energy_T0(self, cutoff, epsrel, mode, nblocks, nused);
+    long computed = 0;
+    const double v0 = xi0_term(self, xi0, omegap, cutoff, &computed);
+    const double T_scaled = 2 * M_PI * CAPS_kB * (self->R + self->L) * T / (CAPS_hbar * CAPS_c);
+    if (psd_order == 0) return free_energy_msd(self, T_scaled, cutoff, epsrel, mode, v0, nblocks, nused, computed);
+    if (psd_order < 0) {
+        const double Teff = 4 * M_PI * CAPS_kB / CAPS_hbar / CAPS_c * T * self->L;
+        psd_order = (int)ceil((1 - 1.5 * log10(epsrel)) / sqrt(Teff));
+    }
+    return free_energy_psd(self, T_scaled, cutoff, psd_order, mode, v0, nblocks, nused, computed);
+}
+
 double caps_free_energy(caps_t* self, double T, double cutoff, double epsrel, int mode, int drude_xi0,
                         long* nblocks, long* nused)
 {
     TERMINATE(!(T >= 0), "caps_free_energy needs T >= 0 (got %g)", T);
     if (T == 0) return free_energy_T0(self, cutoff, epsrel, mode, nblocks, nused);
-    TERMINATE(!drude_xi0, "the xi = 0 term is implemented for the Drude model only");
-    const double T_scaled = 2 * M_PI * CAPS_kB * (self->R + self->L) * T / (CAPS_hbar * CAPS_c);
-    long computed = 0, used = 0;
+    TERMINATE(!drude_xi0, "caps_free_energy: for other xi = 0 models use capsx_free_energy");
+    return capsx_free_energy(self, T, cutoff, epsrel, mode, 1, 0, 0, nblocks, nused);
+}
+
+}  // extern "C"
+
+namespace {
+
+// T > 0 by the Matsubara sum (src/caps.c:836-846) with the ξ = 0 term v0.
+double free_energy_msd(caps_t* self, double T_scaled, double cutoff, double epsrel, int mode, double v0,
+                       long* nblocks, long* nused, long computed0)
+{
+    long computed = computed0, used = 0;
     std::vector<double> v;      // Matsubara terms, v[0] = ξ = 0 term
-    v.push_back(caps_ht_drude(self));
+    v.push_back(v0);
     std::vector<Term> terms(1);  // terms[n] for n >= 1
     auto term = [&](int n) -> Term& {
         while ((int)terms.size() <= n) {
@@ -1904,6 +2132,10 @@ double caps_free_energy(caps_t* self, double T, double cutoff, double epsrel, in
     if (nused) *nused = used;
     return T_scaled / M_PI * s;
 }
+
+}  // namespace
+
+extern "C" {
 
 // ---- LU of user matrices and the MFMA probe (test hooks) --------------------
 

@@ -265,6 +265,24 @@ void caps_last_timing(caps_t *self, double ms[6]);
 double caps_free_energy(caps_t *self, double T, double cutoff, double epsrel, int mode,
                         int drude_xi0, long *nblocks, long *nused);
 
+/* Extension: the free energy with the reference driver's further options
+ * (src/caps.c:270-396, 635-639, 752-850).  xi0 = ξ = 0 term: 1 Drude
+ * (caps_ht_drude), 2 perfect reflector and 3 plasma model with omegap
+ * (rad/s): the Drude term plus Σ'_m of the MM blocks of caps_logdetD0 up to
+ * the m-cutoff.  psd_order = 0: Matsubara sum; > 0: Padé spectrum
+ * decomposition of that order, F = T_scaled/π [v_0/2 + Σ_j η_j
+ * F_ξ(ξ_j T_scaled/(2π))] (capsx_psd); < 0: the driver's order
+ * ceil((1 - 1.5 log10 epsrel)/sqrt(T_eff)), T_eff = 4π k_B T L/(ħc).  mode
+ * as for caps_free_energy.  T = 0: as caps_free_energy. */
+double capsx_free_energy(caps_t *self, double T, double cutoff, double epsrel, int mode, int xi0,
+                         double omegap, int psd_order, long *nblocks, long *nused);
+/* Extension: poles ξ_j (ascending) and residues η_j, j < N, of the Padé
+ * spectrum decomposition of the Bose function (Hu, Xu, Yan, J. Chem. Phys.
+ * 133, 101106 (2010)): 1/(1 - e^{-x}) ≈ 1/x + 1/2 + Σ_j 2η_j x/(x² + ξ_j²).
+ * Returns 0 on success.  Reference: src/psd.c:80-136 (LAPACK dstemr there;
+ * bisection on Sturm sequences here). */
+int capsx_psd(int N, double *xi, double *eta);
+
 /* ---- matrix files ------------------------------------------------------
  * CAPS_DUMP: with a dense determinant algorithm (detalg LU, QR, CHOLESKY),
  * caps_logdetD, caps_logdetD0 and kernel_logdet write each matrix M they

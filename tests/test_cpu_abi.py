@@ -105,3 +105,30 @@ def test_npy_files(tmp_path):
     np.save(q, np.ones((3, 3)))   # C order: not the reference's layout
     with pytest.raises(ValueError):
         caps_amd.npy_load(q)
+
+
+def test_psd_poles_and_residues():
+    """Padé spectrum decomposition (host code): the poles are -2/w of the
+    negative eigenvalues w of the tridiagonal matrix (numpy's eigvalsh as
+    the independent solver), the lowest ones sit at the Matsubara
+    frequencies 2πj with residue 1, and the expansion reproduces the Bose
+    function 1/(1 - e^{-x}) = 1/x + 1/2 + Σ 2η x/(x² + ξ²) to rounding for
+    x well inside the spectrum."""
+    import math
+    import numpy as np
+    import caps_amd
+    for N in (5, 20, 60):
+        xi, eta = caps_amd.psd(N)
+        n = 2 * N
+        T = np.zeros((n, n))
+        for j in range(1, n):
+            T[j - 1, j] = T[j, j - 1] = 1 / math.sqrt((2 * j + 1) * (2 * j + 3))
+        w = np.sort(np.linalg.eigvalsh(T))[:N]
+        assert np.max(np.abs(xi / (-2 / w) - 1)) < 1e-13
+        assert np.all(np.diff(xi) > 0)
+        if N >= 20:
+            assert np.max(np.abs(xi[:3] / (2 * math.pi * np.arange(1, 4)) - 1)) < 1e-13
+            assert np.max(np.abs(eta[:3] - 1)) < 1e-13
+            for x in (0.3, 2.0, 8.0, 20.0):
+                bose = 1 / (-math.expm1(-x))
+                assert abs((1 / x + 0.5 + np.sum(2 * eta * x / (x * x + xi * xi))) / bose - 1) < 1e-14

@@ -519,3 +519,50 @@ def test_caps_dump_matrix(ca, tmp_path, monkeypatch):
     ref *= np.sqrt(L1 * L2 / ((L1 + 1.0) * (L2 + 1.0)))
     assert np.max(np.abs(M0 - ref) / ref) < 1e-11
     assert abs(np.linalg.slogdet(np.eye(c.ldim) - M0)[1] - mm) < 1e-12 * max(1, abs(mm))
+
+
+def test_free_energy_psd_equals_matsubara(ca):
+    """T > 0 by the Padé spectrum decomposition (the driver's automatic
+    order) and by the Matsubara sum give the same free energy to the
+    requested accuracy (Drude, R/L = 50, ldim = 300, T = 300 K, epsrel
+    1e-8), with fewer frequencies; the plain and batched PSD forms sum the
+    same blocks."""
+    c = make(ca, 50e-6, 1e-6, 300, 9 / HBAR_EV, 0.035 / HBAR_EV)
+    Fm, cm, um = c.free_energy_ex(300.0, epsrel=1e-8)
+    Fp, cp, up = c.free_energy_ex(300.0, epsrel=1e-8, psd_order=-1)
+    print("MSD F=%.16g (%d blocks), PSD F=%.16g (%d blocks), rel %.2e" % (Fm, um, Fp, up, abs(Fp / Fm - 1)))
+    assert abs(Fp / Fm - 1) < 1e-7 and up < um
+    Fp0, cp0, up0 = c.free_energy_ex(300.0, epsrel=1e-8, mode="plain", psd_order=-1)
+    assert up0 == up and cp0 == up0 and abs(Fp0 - Fp) <= 1e-12 * abs(Fp)
+
+
+def test_free_energy_xi0_models(ca):
+    """The driver's ξ = 0 terms for perfect reflectors and the plasma model
+    (Drude EE term plus Σ'_m of the MM blocks up to the m-cutoff): the
+    difference to the Drude-term energy is T_scaled/(2π) Σ'_m MM, checked
+    against the oracle's blocks (R/L = 10, ldim = 60)."""
+    import ctypes
+    import pyoracle as po
+    L = po.lib()
+    R, Lx, T, cutoff = 10e-6, 1e-6, 300.0, 1e-9
+    c = ca.Caps(R, Lx, ldim=60)
+    oc = po.caps(R, Lx, ldim=60, epsrel=1e-12)
+    T_scaled = 2 * math.pi * 1.38064852e-23 * (R + Lx) * T / (1.0545718e-34 * 299792458.0)
+    Fd, _, _ = c.free_energy_ex(T, cutoff=cutoff, xi0="drude")
+    wp = 9 / HBAR_EV
+    for model, omegap in [("pr", 0.0), ("plasma", wp)]:
+        F, _, _ = c.free_energy_ex(T, cutoff=cutoff, xi0=model, omegap=omegap)
+        got = (F - Fd) * math.pi / T_scaled * 2
+        v = []
+        for m in range(1000):
+            x, y = ctypes.c_double(), ctypes.c_double()
+            if model == "pr":
+                L.orc_logdetD0(ctypes.byref(oc), m, 0.0, None, ctypes.byref(x), None)
+            else:
+                L.orc_logdetD0(ctypes.byref(oc), m, omegap, None, None, ctypes.byref(x))
+            v.append(x.value)
+            if x.value == 0 or x.value / v[0] < cutoff:
+                break
+        want = math.fsum([v[0] / 2] + v[1:])
+        print("xi=0 %s: %.16g oracle %.16g (%d m)" % (model, got, want, len(v)))
+        assert abs(got - want) <= 1e-9 * abs(want), (model, got, want)
