@@ -73,6 +73,10 @@ def lib():
             "capsx_lu_grid_panel_rows": (i, [i]),
             "capsx_surrogate_report": (None, [p, dp, i]),
             "caps_ht_plasma": (d, [p, d, d]),
+            "capsx_material_init": (p, [ctypes.c_char_p, d]),
+            "capsx_material_free": (None, [p]),
+            "capsx_material_epsilonm1": (d, [d, p]),
+            "capsx_material_get_extrapolation": (None, [p, dp, dp, dp, dp]),
             "capsx_free_energy": (d, [p, d, d, d, i, i, d, i, ctypes.POINTER(ctypes.c_long),
                                       ctypes.POINTER(ctypes.c_long)]),
             "capsx_psd": (i, [i, dp, dp]),
@@ -137,6 +141,19 @@ class Caps:
         setter = {"both": lib().caps_set_epsilonm1, "plate": lib().caps_set_epsilonm1_plate,
                   "sphere": lib().caps_set_epsilonm1_sphere}[body]
         setter(self.ptr, fn, ctypes.cast(ud, ctypes.c_void_p))
+
+    def set_material(self, path, body="both"):
+        """Tabulated ε(iξ) from a file in the reference's material format
+        (capsx_material_init); linear interpolation, Drude extrapolation."""
+        h = lib().capsx_material_init(str(path).encode(), self.R + self.L)
+        if not h:
+            raise ValueError("cannot read material file %s" % path)
+        self._materials = getattr(self, "_materials", []) + [_Material(h)]
+        fn = ctypes.cast(lib().capsx_material_epsilonm1, ctypes.c_void_p)
+        setter = {"both": lib().caps_set_epsilonm1, "plate": lib().caps_set_epsilonm1_plate,
+                  "sphere": lib().caps_set_epsilonm1_sphere}[body]
+        setter(self.ptr, fn, ctypes.c_void_p(h))
+        return self._materials[-1]
 
     def lminmax(self, m):
         a, b = ctypes.c_size_t(), ctypes.c_size_t()
@@ -286,6 +303,26 @@ class Caps:
         v = L.caps_integrate_plasma(h, l1, l2, m, ctypes.byref(r1), ctypes.byref(r2))
         L.caps_integrate_plasma_free(h)
         return v, r1.value, r2.value
+
+
+class _Material:
+    """A loaded material table (freed with the object that holds it)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def epsilonm1(self, xi):
+        return lib().capsx_material_epsilonm1(xi, self.ptr)
+
+    def extrapolation(self):
+        v = [ctypes.c_double() for _ in range(4)]
+        lib().capsx_material_get_extrapolation(self.ptr, *[ctypes.byref(x) for x in v])
+        return tuple(x.value for x in v)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().capsx_material_free(self.ptr)
+            self.ptr = None
 
 
 class _MObj:

@@ -908,6 +908,94 @@ double caps_epsilonm1_drude(double xi, void* userdata)
     return p[0] * p[0] / (xi * (xi + p[1]));
 }
 
+// ---- tabulated materials ------------------------------------------------------
+
+// ε(iξ) - 1 from a table (linear interpolation in ξ) with Drude
+// extrapolation below and above it.  File format of the reference's
+// material_init (src/material.c:63-180): lines "# omegap_low = <eV>",
+// "# gamma_low", "# omegap_high", "# gamma_high" (eV), other '#' lines
+// ignored, data lines "ξ[rad/s] ε(iξ)" separated by blanks or tabs with ξ
+// strictly ascending.
+struct capsx_material {
+    double calL = 0;
+    double omegap_low = 0, gamma_low = 0, omegap_high = 0, gamma_high = 0;   // rad/s
+    std::vector<double> xi, epsm1;
+};
+
+capsx_material* capsx_material_init(const char* filename, double calL)
+{
+    FILE* f = fopen(filename, "r");
+    if (!f) return nullptr;
+    capsx_material* mt = new capsx_material();
+    mt->calL = calL;
+    char line[2048];
+    bool ok = true;
+    auto key = [&](const char* ln, const char* k, double* v) {
+        const char* p = strstr(ln, k);
+        if (!p) return;
+        p = strchr(p + strlen(k), '=');
+        if (p) *v = strtod(p + 1, nullptr);
+    };
+    while (fgets(line, sizeof(line), f)) {
+        char* b = line;
+        while (*b == ' ' || *b == '\t') b++;
+        if (*b == '#') {
+            key(b, "omegap_low", &mt->omegap_low);
+            key(b, "gamma_low", &mt->gamma_low);
+            key(b, "omegap_high", &mt->omegap_high);
+            key(b, "gamma_high", &mt->gamma_high);
+            continue;
+        }
+        char* end = nullptr;
+        const double x = strtod(b, &end);
+        if (end == b || (*end != ' ' && *end != '\t')) continue;   // not a data line
+        const double eps = strtod(end, nullptr);
+        if (!mt->xi.empty() && !(x > mt->xi.back())) {
+            ok = false;   // not ascending
+            break;
+        }
+        mt->xi.push_back(x);
+        mt->epsm1.push_back(eps - 1);
+    }
+    fclose(f);
+    if (!ok || mt->xi.size() < 2) {
+        delete mt;
+        return nullptr;
+    }
+    const double hev = CAPS_hbar_eV;
+    mt->omegap_low /= hev;
+    mt->gamma_low /= hev;
+    mt->omegap_high /= hev;
+    mt->gamma_high /= hev;
+    return mt;
+}
+
+void capsx_material_free(capsx_material* mt) { delete mt; }
+
+// ε(iξ) - 1 at ξ (rad/s); userdata is the capsx_material (a callback for
+// caps_set_epsilonm1).  Reference: src/material.c:255-293.
+double capsx_material_epsilonm1(double xi, void* userdata)
+{
+    const capsx_material* mt = (const capsx_material*)userdata;
+    if (xi < mt->xi.front()) return mt->omegap_low * mt->omegap_low / (xi * (xi + mt->gamma_low));
+    if (xi > mt->xi.back()) return mt->omegap_high * mt->omegap_high / (xi * (xi + mt->gamma_high));
+    size_t r = std::upper_bound(mt->xi.begin(), mt->xi.end(), xi) - mt->xi.begin();
+    if (r >= mt->xi.size()) r = mt->xi.size() - 1;
+    const size_t l = r - 1;
+    return mt->epsm1[l] + (xi - mt->xi[l]) * (mt->epsm1[r] - mt->epsm1[l]) / (mt->xi[r] - mt->xi[l]);
+}
+
+// Drude parameters of the extrapolations (rad/s); NULL outputs skipped.
+// Reference: src/material.c:200-210.
+void capsx_material_get_extrapolation(const capsx_material* mt, double* omegap_low, double* gamma_low,
+                                      double* omegap_high, double* gamma_high)
+{
+    if (omegap_low) *omegap_low = mt->omegap_low;
+    if (gamma_low) *gamma_low = mt->gamma_low;
+    if (omegap_high) *omegap_high = mt->omegap_high;
+    if (gamma_high) *gamma_high = mt->gamma_high;
+}
+
 static double host_lfac(unsigned n)
 {
     if (n < 1024) return lgamma((double)n + 1.0);

@@ -121,6 +121,20 @@ double caps_epsilonm1_sphere(caps_t *self, double xi_);
 double caps_epsilonm1_perf(double xi_, void *userdata);
 double caps_epsilonm1_drude(double xi_, void *userdata);
 
+/* Extension: tabulated dielectric functions (the file format of the
+ * reference's material_init, src/material.c:63-180: "# omegap_low = <eV>",
+ * "# gamma_low", "# omegap_high", "# gamma_high" headers and data lines
+ * "xi[rad/s] epsilon(i xi)" with xi ascending).  ε(iξ) - 1 is interpolated
+ * linearly, with Drude extrapolation below and above the table
+ * (src/material.c:255-293); evaluated on the host once per ξ and body.
+ * Use as caps_set_epsilonm1(caps, capsx_material_epsilonm1, material). */
+typedef struct capsx_material capsx_material_t;
+capsx_material_t *capsx_material_init(const char *filename, double calL);   /* NULL on error */
+void capsx_material_free(capsx_material_t *material);
+double capsx_material_epsilonm1(double xi, void *material);
+void capsx_material_get_extrapolation(const capsx_material_t *material, double *omegap_low, double *gamma_low,
+                                      double *omegap_high, double *gamma_high);
+
 /* ---- scalar physics: src/include/libcaps.h:116,142-157 ---------------- */
 
 /* log Λ_{l1 l2}^{(m)}.  Reference: src/libcaps.c:45-48. */

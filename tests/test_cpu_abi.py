@@ -132,3 +132,44 @@ def test_psd_poles_and_residues():
             for x in (0.3, 2.0, 8.0, 20.0):
                 bose = 1 / (-math.expm1(-x))
                 assert abs((1 / x + 0.5 + np.sum(2 * eta * x / (x * x + xi * xi))) / bose - 1) < 1e-14
+
+
+def _write_drude_table(path, xis, wp, gm, wp_hi=None, gm_hi=None):
+    hev = 6.582119514e-16
+    with open(path, "w") as f:
+        f.write("# synthetic Drude table\n#\n# omegap_low = %.17geV\n# gamma_low = %.17geV\n" % (wp, gm))
+        f.write("# omegap_high = %.17geV\n# gamma_high = %.17geV\n#\n# xi in rad/s\tepsilon(i*xi)\n" % (
+            wp if wp_hi is None else wp_hi, gm if gm_hi is None else gm_hi))
+        for x in xis:
+            W, G = wp / hev, gm / hev
+            f.write("%.17g\t%.17g\n" % (x, 1 + W * W / (x * (x + G))))
+
+
+def test_material_table(tmp_path):
+    """Tabulated ε(iξ) (host code, the reference's material file format):
+    tabulated values at the nodes, linear interpolation between them, Drude
+    extrapolation with the header parameters outside; a table whose ξ do
+    not ascend is refused."""
+    import numpy as np
+    import caps_amd
+    hev = 6.582119514e-16
+    xis = np.logspace(12, 16, 200)
+    p = tmp_path / "drude.csv"
+    _write_drude_table(p, xis, 9.0, 0.035, 50.0, 200.0)
+    c = caps_amd.Caps(200e-6, 1e-6, ldim=100)
+    m = c.set_material(p)
+    W, G = 9.0 / hev, 0.035 / hev
+    drude = lambda x, w, g: w * w / (x * (x + g))
+    for k in (0, 17, 199):
+        assert abs(m.epsilonm1(xis[k]) / drude(xis[k], W, G) - 1) < 1e-14
+    x = 0.3 * xis[40] + 0.7 * xis[41]
+    lin = drude(xis[40], W, G) + (x - xis[40]) * (drude(xis[41], W, G) - drude(xis[40], W, G)) / (xis[41] - xis[40])
+    assert abs(m.epsilonm1(x) / lin - 1) < 1e-13
+    assert abs(m.epsilonm1(1e10) / drude(1e10, W, G) - 1) < 1e-14
+    assert abs(m.epsilonm1(1e18) / drude(1e18, 50.0 / hev, 200.0 / hev) - 1) < 1e-14
+    ext = m.extrapolation()
+    assert np.allclose(ext, (W, G, 50.0 / hev, 200.0 / hev), rtol=1e-14)
+    q = tmp_path / "bad.csv"
+    _write_drude_table(q, xis[::-1], 9.0, 0.035)
+    with pytest.raises(ValueError):
+        c.set_material(q)

@@ -566,3 +566,29 @@ def test_free_energy_xi0_models(ca):
         want = math.fsum([v[0] / 2] + v[1:])
         print("xi=0 %s: %.16g oracle %.16g (%d m)" % (model, got, want, len(v)))
         assert abs(got - want) <= 1e-9 * abs(want), (model, got, want)
+
+
+def test_tabulated_material_block(ca, tmp_path):
+    """A tabulated material (the reference's file format) that lists the
+    block's frequency reproduces the analytic Drude block to rounding; on a
+    coarse table the block follows the interpolated ε (bounded change)."""
+    import numpy as np
+    from test_cpu_abi import _write_drude_table
+    R, Lx = 100e-6, 1e-6
+    xi_ = 3.0
+    xi = xi_ * 299792458.0 / (R + Lx)          # rad/s of the block
+    grid = np.unique(np.concatenate([np.logspace(11, 17, 300), [xi]]))
+    p = tmp_path / "drude.csv"
+    _write_drude_table(p, grid, 9.0, 0.035)
+    a = make(ca, R, Lx, 600, 9 / HBAR_EV, 0.035 / HBAR_EV)
+    b = ca.Caps(R, Lx, ldim=600)
+    b.set_material(p)
+    for m in (0, 5, 40):
+        va, vb = a.logdetD(xi_, m), b.logdetD(xi_, m)
+        assert abs(va - vb) <= 1e-12 * abs(va), (m, va, vb)
+    q = tmp_path / "coarse.csv"
+    _write_drude_table(q, np.logspace(11, 17, 13), 9.0, 0.035)
+    cc = ca.Caps(R, Lx, ldim=600)
+    cc.set_material(q)
+    v, ref = cc.logdetD(xi_, 5), a.logdetD(xi_, 5)
+    assert v != ref and abs(v / ref - 1) < 1e-2
