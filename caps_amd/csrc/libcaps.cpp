@@ -36,9 +36,11 @@ extern "C" {
 void capsx_mlp_predict(capsx_mlp* h, const float* X, int n, float* Y);
 void capsd_launch_mie(const BlockDesc*, char*, const double*, int, int, hipStream_t);
 void capsd_launch_K(const BlockDesc*, char*, const double*, int, int, int, int*, hipStream_t);
-void capsd_launch_I(const BlockDesc*, char*, const void*, const int*, const double*, int64_t, int*, hipStream_t);
-void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int, hipStream_t);
+void capsd_launch_I(const BlockDesc*, char*, const void*, const int*, const double*, int64_t, int*, const BlockOut*,
+                    int, int, hipStream_t);
+void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int, const BlockOut*, int, hipStream_t);
 void capsd_launch_trace(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
+void capsd_launch_frobenius(const BlockDesc*, char*, BlockOut*, int, hipStream_t);
 void capsd_launch_fill_xi0(const BlockDesc*, char*, int64_t, int, hipStream_t);
 void capsd_launch_mie_points(int, const double*, const int*, double, const double*, double*, hipStream_t);
 void capsd_launch_fresnel_points(int, const double*, const double*, const double*, double*, hipStream_t);
@@ -369,7 +371,7 @@ void grid_for(BlockDesc& d)
 // while a second factored the parts in turn was slower, 0.31-0.44 s for
 // 2-8 parts).
 struct BatchPlan {
-    int n = 0, maxN = 0, max_ldim = 0, max_waves = 0, max_nK = 0;
+    int n = 0, maxN = 0, max_ldim = 0, max_waves = 0, max_nK = 0, max_nl = 0;
     int64_t max_ntri = 0;
     size_t bytes = 0;
     std::vector<IGroup> groups;   // per part, ordered by class (see build_igroups)
@@ -487,6 +489,7 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         p.max_waves = std::max(p.max_waves, d.n_waves);
         p.max_nK = std::max(p.max_nK, d.nK);
         p.max_ntri = std::max(p.max_ntri, (int64_t)d.nl * (d.nl + 1) / 2);
+        p.max_nl = std::max(p.max_nl, d.nl);
     }
     p.bytes = (size_t)off;
     // parts = LU stream groups of the tuned schedule (one without an LU,
@@ -573,12 +576,23 @@ int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, b
         const int b0 = p.part_b[g], nb = p.part_b[g + 1] - b0;
         const bool last = g == p.parts - 1;
         if (g > 0) HIPCHECK(hipStreamWaitEvent(sg, c->part_i_done[g - 1], 0));
-        capsd_launch_I(c->d_desc, c->arena, c->d_groups + p.part_goff[g], p.part_cls[g], g_lfac_dev, p.max_ntri,
-                       c->d_status, sg);
+        const IGroup* gr = c->d_groups + p.part_goff[g];
+        if (mode == 0) {
+            // trace pass: the I pairs of the diagonal band, the diagonal of M,
+            // the trace and its shortcut; the full I and matrix then only
+            // for the blocks that need an LU
+            capsd_launch_I(c->d_desc, c->arena, gr, p.part_cls[g], g_lfac_dev, p.max_nl, c->d_status, c->d_out, 1, 0,
+                           sg);
+            capsd_launch_assemble(c->d_desc + b0, c->arena, g_lfac_dev, nb, p.max_ldim, mode, c->d_out + b0, 1, sg);
+            capsd_launch_trace(c->d_desc + b0, c->arena, c->d_out + b0, nb, 0, sg);
+        }
+        capsd_launch_I(c->d_desc, c->arena, gr, p.part_cls[g], g_lfac_dev, p.max_ntri, c->d_status, c->d_out,
+                       0, mode == 0, sg);
         if (!last) HIPCHECK(hipEventRecord(c->part_i_done[g], sg));
         if (events && last) HIPCHECK(hipEventRecord(c->ev[3], sg));
-        capsd_launch_assemble(c->d_desc + b0, c->arena, g_lfac_dev, nb, p.max_ldim, mode, sg);
-        if (mode == 0) capsd_launch_trace(c->d_desc + b0, c->arena, c->d_out + b0, nb, lu_mode_shortcuts, sg);
+        capsd_launch_assemble(c->d_desc + b0, c->arena, g_lfac_dev, nb, p.max_ldim, mode, c->d_out + b0,
+                              mode == 0 ? 2 : 0, sg);
+        if (mode == 0 && lu_mode_shortcuts) capsd_launch_frobenius(c->d_desc + b0, c->arena, c->d_out + b0, nb, sg);
         if (events && last) HIPCHECK(hipEventRecord(c->ev[4], sg));
         if (mode != 0) continue;
         const bool prof = c->prof && p.parts == 1;

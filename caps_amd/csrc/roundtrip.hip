@@ -265,23 +265,44 @@ __device__ __forceinline__ AlphaND alpha_nd(double p, double s2, double d2)
 // One group: up to GM blocks that share (m, ℓ-window, polarizations) — in
 // a free energy the blocks of one m at different frequencies — so the a_q of
 // a pair are computed once and applied to every block's K table.
+// band != 0: only the pairs with i1 - i2 <= 2, which the diagonal of M
+// needs (the trace pass); use_skip != 0: blocks whose trace shortcut is
+// already decided (out[b].skip) take no part.
 template <int NPOL, int GM>
 __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                            const IGroup* __restrict__ groups, const double* __restrict__ lfac_tab,
-                                           int* __restrict__ status)
+                                           int* __restrict__ status, const BlockOut* __restrict__ out, int band,
+                                           int use_skip)
 {
     const IGroup& grp = groups[blockIdx.y];
     const int G = grp.count;
+    // the group's blocks that take part (uniform over the workgroup)
+    bool act[GM];
+    bool any = false;
+#pragma unroll
+    for (int g = 0; g < GM; g++) {
+        act[g] = g < G && !(use_skip && out[grp.blk[g]].skip);
+        any = any || act[g];
+    }
+    if (!any) return;
     const BlockDesc& d = desc[grp.blk[0]];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int nl = d.nl;
-    const int64_t ntri = (int64_t)nl * (nl + 1) / 2;
-    if (t >= ntri) return;
-    // triangular index -> (i1 >= i2)
-    int i1 = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-    while ((int64_t)i1 * (i1 + 1) / 2 > t) i1--;
-    while ((int64_t)(i1 + 1) * (i1 + 2) / 2 <= t) i1++;
-    const int i2 = (int)(t - (int64_t)i1 * (i1 + 1) / 2);
+    int i1, i2;
+    if (band) {
+        if (t >= 3 * (int64_t)nl) return;
+        i1 = (int)(t / 3);
+        i2 = i1 - (int)(t % 3);
+        if (i2 < 0) return;
+    } else {
+        const int64_t ntri = (int64_t)nl * (nl + 1) / 2;
+        if (t >= ntri) return;
+        // triangular index -> (i1 >= i2)
+        i1 = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((int64_t)i1 * (i1 + 1) / 2 > t) i1--;
+        while ((int64_t)(i1 + 1) * (i1 + 2) / 2 <= t) i1++;
+        i2 = (int)(t - (int64_t)i1 * (i1 + 1) / 2);
+    }
     const int l1 = d.lo + i1, l2 = d.lo + i2;
     const int64_t a = (int64_t)i1 * nl + i2, b = (int64_t)i2 * nl + i1;
     const int64_t nl2 = (int64_t)nl * nl;
@@ -289,6 +310,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     const int m_ = d.m > 0 ? d.m : 1;
     if (l2 < m_ || l2 < d.m) {
         for (int g = 0; g < G; g++) {
+            if (!act[g]) continue;
             const BlockDesc& dg = desc[grp.blk[g]];
             double* Im = at<double>(arena, dg.off_i);
             int* Ie = at<int>(arena, dg.off_i + 16 * nl2);
@@ -326,11 +348,11 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     int ndone = 0;
 #pragma unroll
     for (int g = 0; g < GM; g++) {
-        kb[g] = g < G ? at<KEnt>(arena, desc[grp.blk[g]].off_k) : nullptr;
+        kb[g] = act[g] ? at<KEnt>(arena, desc[grp.blk[g]].off_k) : nullptr;
 #pragma unroll
         for (int pp = 0; pp < NPOL; pp++) {
             const int pol = POL0 + pp;
-            if (g < G) {
+            if (act[g]) {
                 const KEnt k0 = kb[g][pol * nK + l12 - mu];
                 E0[g][pp] = k0.e;
                 sh[g][pp] = -k0.e;   // S - E0, S = 0
@@ -343,7 +365,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 sh[g][pp] = 0;
                 sum[g][pp] = 0;
                 tthr[g][pp] = 0;
-                cnt[g][pp] = 3;   // absent: counts as done
+                cnt[g][pp] = 3;   // absent or skipped: counts as done
                 cs[g][pp] = 0;
                 ndone++;
             }
@@ -364,24 +386,24 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
         for (int g = 0; g < GM; g++)
 #pragma unroll
             for (int pp = 0; pp < NPOL; pp++)
-                if (g < G) kv[g][pp] = kb[g][(POL0 + pp) * nK + j];   // g < G: uniform
+                if (act[g]) kv[g][pp] = kb[g][(POL0 + pp) * nK + j];   // uniform
     };
     auto accumulate = [&](const KEnt (&kv)[GM][NPOL], double aq, bool check) {
 #pragma unroll
         for (int g = 0; g < GM; g++) {
-            if (g >= G) continue;
+            if (!act[g]) continue;
 #pragma unroll
             for (int pp = 0; pp < NPOL; pp++) {
-                const bool act = cnt[g][pp] < 3;
+                const bool live = cnt[g][pp] < 3;
                 const double term = ldexp(aq * kv[g][pp].m, kv[g][pp].e + sh[g][pp]);
-                sum[g][pp] += act ? term : 0.0;
+                sum[g][pp] += live ? term : 0.0;
                 if (check) {
                     const double aterm = fabs(term);
                     const int c1 = aterm < tthr[g][pp] ? cnt[g][pp] + 1 : 0;
                     const int c2 = aterm * 0x1p56 < fabs(sum[g][pp]) ? cs[g][pp] + 1 : 0;
-                    const bool fin = act && (c1 >= 3 || c2 >= 3);
-                    cnt[g][pp] = act ? (fin ? 3 : c1) : cnt[g][pp];
-                    cs[g][pp] = act ? c2 : cs[g][pp];
+                    const bool fin = live && (c1 >= 3 || c2 >= 3);
+                    cnt[g][pp] = live ? (fin ? 3 : c1) : cnt[g][pp];
+                    cs[g][pp] = live ? c2 : cs[g][pp];
                     ndone += fin ? 1 : 0;
                 }
             }
@@ -468,7 +490,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     const int a0e = (int)fl;
 #pragma unroll
     for (int g = 0; g < GM; g++) {
-        if (g >= G) continue;
+        if (!act[g]) continue;
         const BlockDesc& dg = desc[grp.blk[g]];
         double* Im = at<double>(arena, dg.off_i);                 // [2][nl*nl]
         int* Ie = at<int>(arena, dg.off_i + 16 * nl2);            // [2][nl*nl]
@@ -525,11 +547,17 @@ __device__ __forceinline__ UV uv(int l, int m)
     return {(l + 1.) * (l + m) * r, l * (l - m + 1.) * r};
 }
 
+// flags bit 0: the trace pass: only the diagonal tiles (blockIdx.y unused)
+// and only the diagonal of M (from I pairs with |l1 - l2| <= 2); bit 1:
+// leave out the blocks with out[b].skip set.
 __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                                  const double* __restrict__ lfac_tab, int mode)
+                                                  const double* __restrict__ lfac_tab, int mode,
+                                                  const BlockOut* __restrict__ out, int flags)
 {
     const BlockDesc& d = desc[blockIdx.z];
-    const int L1 = d.lmin + blockIdx.x * 16, L2 = d.lmin + blockIdx.y * 16;
+    if ((flags & 2) && out[blockIdx.z].skip) return;
+    const bool diag_only = flags & 1;
+    const int L1 = d.lmin + blockIdx.x * 16, L2 = d.lmin + (diag_only ? blockIdx.x : blockIdx.y) * 16;
     if (L1 >= d.lmin + d.ldim || L2 >= d.lmin + d.ldim) return;   // uniform over the workgroup
     const int m = d.m;
     // the I values of the workgroup's 16 x 16 tiles and their ±1 halo, both
@@ -552,8 +580,9 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     }
     __syncthreads();
     const int c0 = threadIdx.x & 15, r0 = threadIdx.x >> 4;
-    const int i2 = blockIdx.x * 16 + c0, j2 = blockIdx.y * 16 + r0;
+    const int i2 = (L1 - d.lmin) + c0, j2 = (L2 - d.lmin) + r0;
     if (i2 >= d.ldim || j2 >= d.ldim) return;
+    if (diag_only && i2 != j2) return;
     const int l1 = d.lmin + i2, l2 = d.lmin + j2;
     // I(l1 + a, l2 + b) of polarization pol times the prefactor pm 2^pe
     auto X = [&](int pol, int a, int b, double pm, int pe) {
@@ -587,6 +616,12 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     const int eEE = xa1 + xa2, eMM = xb1 + xb2;
     const double EE = B(TM, pEE, eEE) - A(TE, pEE, eEE);
     const double MM = A(TM, pMM, eMM) - B(TE, pMM, eMM);
+    if (diag_only) {
+        double* dg = at<double>(arena, d.off_diag);
+        dg[2 * i2] = EE;
+        dg[2 * i2 + 1] = MM;
+        return;
+    }
     double EM = 0, ME = 0;
     if (m != 0) {
         // EM: D_TM - C_TE with (a_l1 + b_l2)/2;  ME: C_TM - D_TE with (b_l1 + a_l2)/2
@@ -661,6 +696,7 @@ __global__ void __launch_bounds__(256) k_frobenius(const BlockDesc* __restrict__
                                                    BlockOut* __restrict__ out)
 {
     const BlockDesc& d = desc[blockIdx.x];
+    if (out[blockIdx.x].skip == 1) return;   // trace shortcut already taken
     const double* A = at<double>(arena, d.off_mat);
     const double* dg = at<double>(arena, d.off_diag);
     double f = 0, t2 = 0;
@@ -720,27 +756,36 @@ void capsd_launch_K(const BlockDesc* desc, char* arena, const double* lfac_tab, 
 // I tables of the batch's groups, ordered by class: perfect reflectors
 // (TM only) with at most 1, 2, 4, 8 blocks per group, then both
 // polarizations likewise; counts[8] holds the groups per class.
+// band: only the pairs the diagonal of M needs (max_ntri is then the
+// largest nl); use_skip: leave out the blocks with out[b].skip set.
 void capsd_launch_I(const BlockDesc* desc, char* arena, const void* groups, const int* counts,
-                    const double* lfac_tab, int64_t max_ntri, int* status, hipStream_t s)
+                    const double* lfac_tab, int64_t max_ntri, int* status, const BlockOut* out, int band,
+                    int use_skip, hipStream_t s)
 {
     const IGroup* gr = (const IGroup*)groups;
-    const unsigned gx = (unsigned)((max_ntri + 255) / 256);
+    const unsigned gx = (unsigned)(((band ? 3 * max_ntri : max_ntri) + 255) / 256);
     int off = 0;
 #define KI(NP, GMV, CLS)                                                                                    \
     if (counts[CLS] > 0)                                                                                   \
         hipLaunchKernelGGL((k_I<NP, GMV>), dim3(gx, counts[CLS]), dim3(256), 0, s, desc, arena, gr + off,  \
-                           lfac_tab, status);                                                              \
+                           lfac_tab, status, out, band, use_skip);                                         \
     off += counts[CLS];
     KI(1, 1, 0) KI(1, 2, 1) KI(1, 4, 2) KI(1, 8, 3)
     KI(2, 1, 4) KI(2, 2, 5) KI(2, 4, 6) KI(2, 8, 7)
 #undef KI
 }
 
+// flags: see k_assemble (1 = diagonal only, 2 = leave out skipped blocks)
 void capsd_launch_assemble(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int max_ldim,
-                           int mode, hipStream_t s)
+                           int mode, const BlockOut* out, int flags, hipStream_t s)
 {
-    dim3 g((max_ldim + 15) / 16, (max_ldim + 15) / 16, nblocks);
-    hipLaunchKernelGGL(k_assemble, g, dim3(256), 0, s, desc, arena, lfac_tab, mode);
+    dim3 g((max_ldim + 15) / 16, (flags & 1) ? 1 : (max_ldim + 15) / 16, nblocks);
+    hipLaunchKernelGGL(k_assemble, g, dim3(256), 0, s, desc, arena, lfac_tab, mode, out, flags);
+}
+
+void capsd_launch_frobenius(const BlockDesc* desc, char* arena, BlockOut* out, int nblocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_frobenius, dim3(nblocks), dim3(256), 0, s, desc, arena, out);
 }
 
 void capsd_launch_fill_xi0(const BlockDesc* desc, char* arena, int64_t off_tab, int n, hipStream_t s)
