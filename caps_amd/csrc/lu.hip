@@ -76,9 +76,13 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx)
     }
 }
 
-// Panel body: R rows per lane, leaves of LF columns, NT threads; the
-// logical->physical row map lives in dynamic LDS (R*NT ints).
-template <int R, int LF, int NT>
+// Panel body: R rows per lane, leaves of LF columns, NT threads, the
+// left-looking update in chunks of CW columns of L (CW divides LF); the
+// logical->physical row map lives in dynamic LDS (R*NT ints).  Every leaf
+// re-reads the L columns left of it (64 columns, leaves of 4: 480 L columns
+// per panel; of 8: 224), while each column's pivot exchange and elimination
+// touch all R x LF leaf registers: leaves of 8 are the fastest measured.
+template <int R, int LF, int NT, int CW>
 __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                               const LuSide* __restrict__ side, const BlockOut* __restrict__ out, int k,
                                               int pb)
@@ -100,7 +104,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
     __shared__ double red_v2[2][NT / 64];
     __shared__ int red_i2[2][NT / 64];
     __shared__ double prow2[2][LF], xrow2[2][LF];
-    __shared__ double Uc[2][LF * LF];   // solved U rows of one chunk, [row][col]
+    __shared__ double Uc[2][CW * LF];   // solved U rows of one chunk, [row][col]
     __shared__ double pivv[NB];         // |pivot| per panel column
 
     for (int r = tid; r < np; r += NT) perm[r] = r;
@@ -117,52 +121,49 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
             for (int j = 0; j < LF; j++)
                 a[i][j] = (r < np && j < w) ? A[(k + perm[r]) + (k + c0 + j) * lda] : 0.0;
         }
-        // left-looking update with the panel's earlier columns, LF columns of
-        // L per chunk.  Rows [cc0, cc0+LF) of the leaf (lanes cc0.. of wave 0,
-        // register row 0) become final U rows by a forward substitution
-        // inside the wavefront; every row below then subtracts the chunk's
-        // rank-LF product.
-        for (int cc0 = 0; cc0 < c0; cc0 += LF) {
-            const int buf = (cc0 / LF) & 1;
-            double Lr[R][LF];
+        // left-looking update with the panel's earlier columns, CW columns
+        // of L per chunk.  Rows [cc0, cc0+CW) of the leaf (lanes cc0.. of
+        // wave 0, register row 0) become final U rows by a forward
+        // substitution inside the wavefront; every row below then subtracts
+        // the chunk's rank-CW product.
+        for (int cc0 = 0; cc0 < c0; cc0 += CW) {
+            const int buf = (cc0 / CW) & 1;
+            double Lr[R][CW];
 #pragma unroll
             for (int i = 0; i < R; i++) {
                 const int r = tid + i * NT;
                 const bool row_ok = r > cc0 && r < np;
                 const int64_t pr = k + (row_ok ? perm[r] : 0);
 #pragma unroll
-                for (int jj = 0; jj < LF; jj++)
+                for (int jj = 0; jj < CW; jj++)
                     Lr[i][jj] = (row_ok && cc0 + jj < r) ? A[pr + (k + cc0 + jj) * lda] : 0.0;
             }
             if (wid == 0) {
 #pragma unroll
-                for (int t = 0; t < LF; t++) {
+                for (int t = 0; t < CW; t++) {
                     const int q = cc0 + t;
                     double uq[LF];
 #pragma unroll
                     for (int j = 0; j < LF; j++) uq[j] = __shfl(a[0][j], q, 64);
-                    if (lane > q && lane < cc0 + LF) {
+                    if (lane > q && lane < cc0 + CW) {
 #pragma unroll
                         for (int j = 0; j < LF; j++) a[0][j] -= Lr[0][t] * uq[j];
                     }
                 }
-                if (lane >= cc0 && lane < cc0 + LF) {
+                if (lane >= cc0 && lane < cc0 + CW) {
 #pragma unroll
                     for (int j = 0; j < LF; j++) Uc[buf][(lane - cc0) * LF + j] = a[0][j];
                 }
             }
             __syncthreads();
 #pragma unroll
-            for (int t = 0; t < LF; t++) {
-                double u[LF];
+            for (int t = 0; t < CW; t++) {
 #pragma unroll
-                for (int j = 0; j < LF; j++) u[j] = Uc[buf][t * LF + j];
+                for (int j = 0; j < LF; j++) {
+                    const double u = Uc[buf][t * LF + j];
 #pragma unroll
-                for (int i = 0; i < R; i++) {
-                    if (i > 0 || tid >= cc0 + LF) {
-#pragma unroll
-                        for (int j = 0; j < LF; j++) a[i][j] -= Lr[i][t] * u[j];
-                    }
+                    for (int i = 0; i < R; i++)
+                        if (i > 0 || tid >= cc0 + CW) a[i][j] -= Lr[i][t] * u;
                 }
             }
         }
@@ -242,11 +243,16 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
                     for (int jj = 0; jj < LF; jj++) a[i][jj] = xrow[jj];
                 }
             }
+            // l = a / u as a product with 1/u, as LAPACK's dgetf2 scales the
+            // column (a division only below the smallest normal number); u is
+            // the same in every lane, so the choice is uniform
+            const int by_rcp = __builtin_amdgcn_readfirstlane((int)(fabs(u) >= 2.2250738585072014e-308));
+            const double ru = 1.0 / u;
 #pragma unroll
             for (int i = 0; i < R; i++) {
                 const int r = tid + i * NT;
                 if (r > c && r < np) {
-                    const double l = a[i][j] / u;
+                    const double l = by_rcp ? a[i][j] * ru : a[i][j] / u;
                     a[i][j] = l;
 #pragma unroll
                     for (int jj = 0; jj < LF; jj++)
@@ -422,15 +428,15 @@ __global__ void __launch_bounds__(GP_ROWS) k_gp_copy(const BlockDesc* __restrict
     for (int j = 0; j < nbk; j++) P[(r - k) + (int64_t)j * lda] = A[r + (int64_t)(k + j) * lda];
 }
 
-// R rows per lane, leaves of LF columns, NT lanes; OCC waves per SIMD (OCC
-// NT/256 workgroups per CU share the registers that hold the leaf and one
-// chunk of L).
-template <int R, int LF, int NT, int OCC>
+// R rows per lane, leaves of LF columns, NT lanes, chunks of CW L columns;
+// OCC waves per SIMD (OCC NT/256 workgroups per CU share the registers that
+// hold the leaf and one chunk of L).
+template <int R, int LF, int NT, int OCC, int CW = LF>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_lu_panel(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
            const BlockOut* __restrict__ out, int k, int pb)
 {
-    lu_panel_body<R, LF, NT>(desc, arena, side, out, k, pb);
+    lu_panel_body<R, LF, NT, CW>(desc, arena, side, out, k, pb);
 }
 
 // Row interchanges of one column (lane), in place: the panel's pivots as
@@ -758,9 +764,10 @@ int capsd_lu_nb(void) { return NB; }
 // (CAPS_AMD_GRID_PANEL_ROWS, default 6144: above it the one-workgroup panel
 // needs 24 rows per lane).
 static int grid_panel_rows = 6144;
-// one-workgroup panel variants: 1 = two workgroups per CU (leaves of 4
-// columns above 512 rows), 0 = one workgroup per CU with leaves of 8 up to
-// 2048 rows (CAPS_AMD_PANEL_SET, default 1)
+// one-workgroup panel variants (CAPS_AMD_PANEL_SET): 1 (default) = leaves
+// of 8 columns with 256-lane workgroups up to 2560 rows, 0 = leaves of 4
+// above 512 rows.  Both give bit-identical factors: every entry subtracts
+// the same products in the same column order.
 static int panel_set = 1;
 
 static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
@@ -769,6 +776,9 @@ static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuS
 #define PANEL(RR, LF, NT, OCC)                                                                              \
     hipLaunchKernelGGL((k_lu_panel<RR, LF, NT, OCC>), dim3(nb), dim3(NT), (RR) * (NT) * sizeof(int), s, desc, arena, \
                        side, out, k, pb)
+#define PANELC(RR, LF, NT, OCC, CW)                                                                       \
+    hipLaunchKernelGGL((k_lu_panel<RR, LF, NT, OCC, CW>), dim3(nb), dim3(NT), (RR) * (NT) * sizeof(int), s, desc, \
+                       arena, side, out, k, pb)
     if (rows > grid_panel_rows) {
         const int nch = (rows + GP_ROWS - 1) / GP_ROWS;
         hipLaunchKernelGGL(k_gp_max, dim3(nch, nb), dim3(GP_ROWS), 0, s, desc, arena, side, out, k, pb);
@@ -778,22 +788,26 @@ static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuS
         }
         hipLaunchKernelGGL(k_gp_copy, dim3(nch, nb), dim3(GP_ROWS), 0, s, desc, arena, side, out, k, pb);
     } else if (panel_set == 1) {
-        // two workgroups per CU: more matrices' panels in flight at once
+        // leaves of 8 left-looking in chunks of 4 (2 at 2560 rows), two
+        // 256-lane workgroups per CU up to 2560 rows
+        if (rows <= 2 * 256) PANEL(2, 8, 256, 2);
+        else if (rows <= 4 * 256) PANELC(4, 8, 256, 2, 4);
+        else if (rows <= 8 * 256) PANELC(8, 8, 256, 2, 4);
+        else if (rows <= 10 * 256) PANELC(10, 8, 256, 2, 2);
+        else if (rows <= 6 * 512) PANEL(6, 4, 512, 2);
+        else if (rows <= 12 * 512) PANEL(12, 4, 512, 2);
+        else PANEL(24, 2, 1024, 4);
+    } else {
+        // leaves of 4 above 512 rows
         if (rows <= 2 * 256) PANEL(2, 8, 256, 2);
         else if (rows <= 4 * 256) PANEL(4, 4, 256, 2);
         else if (rows <= 8 * 256) PANEL(8, 4, 256, 2);
         else if (rows <= 6 * 512) PANEL(6, 4, 512, 2);
         else if (rows <= 12 * 512) PANEL(12, 4, 512, 2);
         else PANEL(24, 2, 1024, 4);
-    } else {
-        if (rows <= 2 * 256) PANEL(2, 8, 256, 1);
-        else if (rows <= 4 * 256) PANEL(4, 8, 256, 1);
-        else if (rows <= 8 * 256) PANEL(8, 8, 256, 1);
-        else if (rows <= 6 * 512) PANEL(6, 4, 512, 2);
-        else if (rows <= 12 * 512) PANEL(12, 4, 512, 2);
-        else PANEL(24, 2, 1024, 4);
     }
 #undef PANEL
+#undef PANELC
 }
 
 int capsd_lu_init(void)
