@@ -75,7 +75,7 @@ void capsd_launch_mfma_probe(const double*, const double*, double*, hipStream_t)
 namespace {
 
 struct LuSide {
-    int64_t off_panel, off_logs, off_moved;
+    int64_t off_panel, off_logs, off_moved, off_ut;
 };
 
 double* g_lfac_dev = nullptr;
@@ -232,6 +232,24 @@ void ensure_arena(Ctx* c, size_t bytes)
 }
 
 inline int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
+
+// LU side buffers of one N x N block (leading dimension lda) from arena
+// offset off: the factored panels of a step, log|u_ii|, the pivots of each
+// panel and the step's U rows in row-major order (capsd_lu_npanels() * NB
+// rows of lda doubles, the trailing updates' B operand); returns the offset
+// after them.
+int64_t lu_side_layout(LuSide& s, int64_t off, int N, int lda)
+{
+    const int nb = capsd_lu_nb();
+    s.off_panel = off;
+    off = align256(off + capsd_lu_npanels() * (int64_t)lda * nb * 8);
+    s.off_logs = off;
+    off = align256(off + (int64_t)N * 8);
+    s.off_moved = off;
+    off = align256(off + capsd_lu_npanels() * (int64_t)nb * 4);
+    s.off_ut = off;
+    return align256(off + capsd_lu_npanels() * (int64_t)nb * lda * 8);
+}
 
 double env_double(const char* name, double dflt)
 {
@@ -476,14 +494,9 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         off = align256(off + (int64_t)N * 4);
         LuSide& s = c->h_side[b];
         if (with_lu) {
-            s.off_panel = off;
-            off = align256(off + capsd_lu_npanels() * (int64_t)lda * nb * 8);
-            s.off_logs = off;
-            off = align256(off + (int64_t)N * 8);
-            s.off_moved = off;
-            off = align256(off + capsd_lu_npanels() * (int64_t)nb * 4);
+            off = lu_side_layout(s, off, N, lda);
         } else
-            s.off_panel = s.off_logs = s.off_moved = 0;
+            s.off_panel = s.off_logs = s.off_moved = s.off_ut = 0;
         p.maxN = std::max(p.maxN, N);
         p.max_ldim = std::max(p.max_ldim, ldim);
         p.max_waves = std::max(p.max_waves, d.n_waves);
@@ -519,7 +532,7 @@ size_t block_bytes_estimate(caps_t* self, int m)
     size_t lmin, lmax;
     caps_estimate_lminmax(self, m, &lmin, &lmax);
     const size_t nK = 2 * (lmin + ldim) + 4;
-    return (size_t)2 * (ldim + 2) * (ldim + 2) * 12 + lda * N * 8 + lda * 128 * 8 + nK * (24 * 2 + 24 * 64) +
+    return (size_t)2 * (ldim + 2) * (ldim + 2) * 12 + lda * N * 8 + lda * 512 * 8 + nK * (24 * 2 + 24 * 64) +
            (size_t)N * 32 + 65536;
 }
 
@@ -1288,12 +1301,7 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     d.off_mat = off;
     off = align256(off + (int64_t)lda * dim * 8);
     LuSide& s = c->h_side[0];
-    s.off_panel = off;
-    off = align256(off + capsd_lu_npanels() * (int64_t)lda * capsd_lu_nb() * 8);
-    s.off_logs = off;
-    off = align256(off + (int64_t)dim * 8);
-    s.off_moved = off;
-    off = align256(off + capsd_lu_npanels() * (int64_t)capsd_lu_nb() * 4);
+    off = lu_side_layout(s, off, dim, lda);
     ensure_arena(c, off);
     BlockOut o;
     memset(&o, 0, sizeof(o));
@@ -1383,12 +1391,7 @@ double sep_logdet(caps_t* self, const std::vector<double>& h, const std::vector<
     const int64_t off_tab = off;
     off = align256(off + (int64_t)(3 * n - 1) * 8);
     LuSide& sd = c->h_side[0];
-    sd.off_panel = off;
-    off = align256(off + capsd_lu_npanels() * (int64_t)lda * capsd_lu_nb() * 8);
-    sd.off_logs = off;
-    off = align256(off + (int64_t)n * 8);
-    sd.off_moved = off;
-    off = align256(off + capsd_lu_npanels() * (int64_t)capsd_lu_nb() * 4);
+    off = lu_side_layout(sd, off, n, lda);
     ensure_arena(c, off);
     std::vector<double> tab(h);
     tab.insert(tab.end(), T.begin(), T.end());
@@ -2260,12 +2263,7 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
         d.off_mat = off;
         off = align256(off + (int64_t)lda * dim * 8);
         LuSide& s = c->h_side[b];
-        s.off_panel = off;
-        off = align256(off + capsd_lu_npanels() * (int64_t)lda * capsd_lu_nb() * 8);
-        s.off_logs = off;
-        off = align256(off + (int64_t)dim * 8);
-        s.off_moved = off;
-        off = align256(off + capsd_lu_npanels() * (int64_t)capsd_lu_nb() * 4);
+        off = lu_side_layout(s, off, dim, lda);
     }
     ensure_arena(c, off);
     std::vector<BlockOut> o(n);
@@ -2310,12 +2308,7 @@ int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, in
         d.off_mat = off;
         off = align256(off + (int64_t)lda * dim * 8);
         LuSide& sd = c->h_side[b];
-        sd.off_panel = off;
-        off = align256(off + capsd_lu_npanels() * (int64_t)lda * nb * 8);
-        sd.off_logs = off;
-        off = align256(off + (int64_t)dim * 8);
-        sd.off_moved = off;
-        off = align256(off + capsd_lu_npanels() * (int64_t)nb * 4);
+        off = lu_side_layout(sd, off, dim, lda);
     }
     ensure_arena(c, off);
     HIPCHECK(hipMemcpyAsync(c->d_desc, c->h_desc, n * sizeof(BlockDesc), hipMemcpyHostToDevice, c->stream));

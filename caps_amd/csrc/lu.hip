@@ -51,6 +51,8 @@ struct LuSide {
                          // panel, LAPACK ipiv convention relative to the
                          // panel's first row (at column c rows c and ipiv[c]
                          // were exchanged)
+    int64_t off_ut;      // NPANEL NB x lda doubles: the step's U rows, row
+                         // k + i at i lda (columns at their matrix index)
 };
 constexpr int NPANEL = 4;    // panels per step of the three-level schedule
 
@@ -488,8 +490,14 @@ __global__ void __launch_bounds__(64) k_lu_swap_trsm(const BlockDesc* __restrict
 #pragma unroll
         for (int j = i + 1; j < NB; j++) x[j] -= P[j + (int64_t)i * lda] * x[i];
     }
+    // the solved U rows also go to the step's row-major U buffer, the B
+    // operand of the trailing updates (coalesced: lanes are columns)
+    double* Ut = at<double>(arena, side[b].off_ut) + (int64_t)pb * NB * lda + col;
 #pragma unroll
-    for (int i = 0; i < NB; i++) Ac[r0 + i] = x[i];
+    for (int i = 0; i < NB; i++) {
+        Ac[r0 + i] = x[i];
+        Ut[i * lda] = x[i];
+    }
 }
 
 // Later panels' row interchanges applied to the L rows of earlier panels of
@@ -699,6 +707,128 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
 #endif
 }
 
+// A22 -= L21 U12 on 128x128 tiles (2x2 wavefronts of 64x64) with the
+// operand stages moved straight from HBM into LDS (global_load_lds_dwordx4:
+// no staging registers, no LDS store instructions) and double-buffered, so
+// one barrier per 16-deep stage suffices and the next stage's loads run
+// under the current stage's MFMAs.  L comes from the step's panel buffers,
+// U from the row-major U buffer, so both stages are rows of 128 contiguous
+// doubles, one 1 KB row per wavefront instruction (lane l moves elements 2l
+// and 2l+1).  The loads are issued from inline assembly: the compiler's
+// wait-count tracking would otherwise make every LDS read wait for all
+// outstanding LDS-bound loads, the next stage's included.  They are
+// waited for explicitly (vmcnt(0) before the barrier); the compiler's own
+// vmcnt waits stay correct, only stricter, because loads retire in order.
+// Rows (columns) past the region load from inside the padded leading
+// dimension (lda > N when N is odd) or not at all, and only feed rows
+// (columns) of the product that are not stored.
+typedef __attribute__((address_space(3))) void lds_void;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
+              const BlockOut* __restrict__ out, int k, int r_off, int r_lim, int c_off, int c_lim, int kk0, int KK,
+              int gx, int gy, int nblocks, int xcd_order)
+{
+    constexpr int GT = 128, KS = 16, PITCH = GT + 16;
+    const int wg = blockIdx.x, tiles = gx * gy;
+    int b, tile;
+    if (xcd_order) {
+        const int xcd = wg & 7, local = wg >> 3;
+        b = (local / tiles) * 8 + xcd;
+        tile = local % tiles;
+    } else {
+        b = wg / tiles;
+        tile = wg % tiles;
+    }
+    if (b >= nblocks) return;
+    const BlockDesc& d = desc[b];
+    if (out[b].skip) return;
+    const int N = d.N;
+    const int r0 = k + r_off, rend = min(N, r0 + r_lim);
+    const int c0 = k + c_off, cend = min(N, c0 + c_lim);
+    const int m0 = r0 + (tile % gx) * GT;
+    const int n0 = c0 + (tile / gx) * GT;
+    if (m0 >= rend || n0 >= cend) return;
+    const int64_t lda = d.lda;
+    double* A = at<double>(arena, d.off_mat);
+    const double* Pb = panel_buf(arena, side[b], lda, 0);
+    const double* Ut = at<double>(arena, side[b].off_ut);
+    __shared__ double sA[2][KS][PITCH];   // [stage][kk][row]
+    __shared__ double sB[2][KS][PITCH];   // [stage][kk][col]
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
+    double4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (double4_t){0, 0, 0, 0};
+    const int li = lane & 15, lk = lane >> 4;
+    const bool arow_ok = m0 + 2 * lane < rend, bcol_ok = n0 + 2 * lane < cend;
+    // stage of depth KS at kk0 + ks into buffer buf: wavefront wv moves kk
+    // rows 4 wv .. 4 wv + 3 of both operands
+    auto issue = [&](int ks, int buf) {
+        const int j = kk0 + ks, q = j / NB;
+        const double* Lp = Pb + (int64_t)q * lda * NB + (int64_t)(j % NB) * lda - (k + q * NB) + m0 + 2 * lane;
+        const double* Up = Ut + (int64_t)j * lda + n0 + 2 * lane;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int kk = 4 * wv + t;
+            const uint32_t la = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(lds_void*)&sA[buf][kk][0]);
+            const uint32_t lb = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(lds_void*)&sB[buf][kk][0]);
+            if (arow_ok)
+                asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                             :: "s"(la), "v"(Lp + (int64_t)kk * lda) : "memory", "m0");
+            if (bcol_ok)
+                asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                             :: "s"(lb), "v"(Up + (int64_t)kk * lda) : "memory", "m0");
+        }
+    };
+    issue(0, 0);
+    const int S = KK / KS;
+    for (int s = 0; s < S; s++) {
+        const int buf = s & 1;
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wavefront's stage-s loads are in LDS
+        __syncthreads();                       // everyone's are, and stage s-1's buffer is free
+        if (s + 1 < S) issue((s + 1) * KS, buf ^ 1);
+#pragma unroll
+        for (int kk = 0; kk < KS; kk += 4) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) av[i] = sA[buf][kk + lk][wr + 16 * i + li];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bv[j] = sB[buf][kk + lk][wc + 16 * j + li];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+        }
+    }
+    // swapped operands: lane l, register q of acc[i][j] holds C row
+    // wr + 16i + l%16, column wc + 16j + l/16 + 4q; C is updated straight
+    // from the accumulators one row band at a time
+    const int rr = lane & 15, cq = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = m0 + wr + 16 * i + rr;
+        const bool rok = row < rend;
+        double cv[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int col = n0 + wc + 16 * j + cq + 4 * q;
+                cv[j][q] = (rok && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
+            }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int col = n0 + wc + 16 * j + cq + 4 * q;
+                if (rok && col < cend) A[row + (int64_t)col * lda] = cv[j][q] - acc[i][j][q];
+            }
+    }
+}
+
 // Kahan sum of log|u_ii| (src/matrix.c:462-471).
 __global__ void k_lu_finalize(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                               const LuSide* __restrict__ side, BlockOut* __restrict__ out, int nblocks)
@@ -769,6 +899,10 @@ static int grid_panel_rows = 6144;
 // above 512 rows.  Both give bit-identical factors: every entry subtracts
 // the same products in the same column order.
 static int panel_set = 1;
+// trailing-update kernel of the 128 x 128 tile: 1 (default) = k_lu_gemm_dma
+// (operand stages moved straight into LDS, double-buffered), 0 = k_lu_gemm
+// (stages through registers) (CAPS_AMD_GEMM_KERNEL)
+static int gemm_kernel = 1;
 
 static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
                          int k, int pb, hipStream_t s)
@@ -814,6 +948,7 @@ int capsd_lu_init(void)
 {
     if (const char* e = getenv("CAPS_AMD_GRID_PANEL_ROWS")) grid_panel_rows = atoi(e);
     if (const char* e = getenv("CAPS_AMD_PANEL_SET")) panel_set = atoi(e);
+    if (const char* e = getenv("CAPS_AMD_GEMM_KERNEL")) gemm_kernel = atoi(e);
     return (int)hipFuncSetAttribute((const void*)k_lu_panel<24, 2, 1024, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     24 * 1024 * sizeof(int));
 }
@@ -875,10 +1010,14 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
             const int gx = (rows + 255) / 256, gy = (cols + 127) / 128;
             hipLaunchKernelGGL((k_lu_gemm<4, 2>), dim3(gx * gy * nb8), dim3(512), 0, s, desc, arena, sd, out, k,
                                r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
-        } else {
+        } else if (gemm_kernel == 0) {
             const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
             hipLaunchKernelGGL((k_lu_gemm<2, 2>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k,
                                r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+        } else {
+            const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
+            hipLaunchKernelGGL(k_lu_gemm_dma, dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k, r_off,
+                               r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
         }
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
         if (gemm_info) {
