@@ -153,10 +153,10 @@ def hbm_probe(nbytes=1 << 30, reps=3):
 
 
 def gemm_traffic():
-    """HBM bytes per k_lu_gemm launch measured with FETCH_SIZE and WRITE_SIZE
-    (profiles/k_lu_gemm_c2_traffic.txt, scaled by the copy calibration), or
-    None when that summary is absent."""
-    path = os.path.join(ROOT, "profiles", "k_lu_gemm_c2_traffic.txt")
+    """HBM bytes per k_lu_gemm_dma launch measured with FETCH_SIZE and
+    WRITE_SIZE (profiles/k_lu_gemm_dma_c2_traffic.txt, scaled by the copy
+    calibration), or None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_lu_gemm_dma_c2_traffic.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -257,10 +257,10 @@ def main():
                  "speedup_vs_plain": (nb / (total / args.steps)) / (nbp / dtp)}
         print("bench: plain step %.3f s, %d blocks" % (dtp, nbp), file=sys.stderr, flush=True)
 
-    # dominant kernel: the LU's trailing update (k_lu_gemm), FP64 MFMA
+    # dominant kernel: the LU's trailing update (k_lu_gemm_dma), FP64 MFMA
     gemm_s = prof["gemm_ms"] / 1e3
     achieved = prof["gemm_flops"] / gemm_s / 1e12 if gemm_s > 0 else None
-    roof = {"kernel": "k_lu_gemm", "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+    roof = {"kernel": "k_lu_gemm_dma", "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS if achieved else None, "traffic": gemm_traffic(),
             "launches": prof["gemm_launches"],
             "avg_launch_us": 1e3 * prof["gemm_ms"] / max(1, prof["gemm_launches"]),
