@@ -458,15 +458,19 @@ __device__ __forceinline__ void swap_rows_ipiv(double* __restrict__ Ac, const in
 
 // Row interchanges and U12 = L11^{-1} A12 for the 64-row tile of rows
 // [r0, r0+64), r0 = k + pb NB, columns [r0+cs_off, min(N, r0+cs_off+climit)),
-// one lane per column.  For each set bit q of swaps (in increasing q) the
-// rows are exchanged by the pivots of panel q (relative to row k + q NB);
-// then the column's 64 entries are solved in registers with the unit lower
-// L11 of panel buffer pb (uniform loads along the columns of L11).
-__global__ void __launch_bounds__(64) k_lu_swap_trsm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                                     const LuSide* __restrict__ side,
-                                                     const BlockOut* __restrict__ out, int k, int pb, int cs_off,
-                                                     int climit, int swaps)
+// one lane per column, NTH columns per workgroup.  For each set bit q of
+// swaps (in increasing q) the rows are exchanged by the pivots of panel q
+// (relative to row k + q NB); then the column's 64 entries are solved in
+// registers with the unit lower L11 of panel buffer pb, staged once per
+// workgroup in LDS (every lane reads the same L11 entry at a time: one
+// broadcast LDS read instead of a vector load per entry).
+template <int NTH>
+__global__ void __launch_bounds__(NTH) k_lu_swap_trsm(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                      const LuSide* __restrict__ side,
+                                                      const BlockOut* __restrict__ out, int k, int pb, int cs_off,
+                                                      int climit, int swaps)
 {
+    __shared__ double sL[NB * NB];   // L11, [column][row]
     const int b = blockIdx.y;
     const BlockDesc& d = desc[b];
     const int N = d.N;
@@ -474,10 +478,12 @@ __global__ void __launch_bounds__(64) k_lu_swap_trsm(const BlockDesc* __restrict
     if (out[b].skip || r0 + cs_off >= N) return;
     const int cend = min(N, r0 + cs_off + climit);
     const int64_t lda = d.lda;
-    const int col = r0 + cs_off + blockIdx.x * 64 + threadIdx.x;
+    const double* P = panel_buf(arena, side[b], lda, pb);
+    for (int idx = threadIdx.x; idx < NB * NB; idx += NTH) sL[idx] = P[(idx % NB) + (int64_t)(idx / NB) * lda];
+    __syncthreads();
+    const int col = r0 + cs_off + blockIdx.x * NTH + threadIdx.x;
     if (col >= cend) return;
     double* Ac = at<double>(arena, d.off_mat) + (int64_t)col * lda;
-    const double* P = panel_buf(arena, side[b], lda, pb);
     // the pivots of the panel at row k + q NB cover min(NB, N - k - q NB)
     // columns
     for (int q = 0; q < NPANEL; q++)
@@ -488,7 +494,7 @@ __global__ void __launch_bounds__(64) k_lu_swap_trsm(const BlockDesc* __restrict
 #pragma unroll
     for (int i = 0; i < NB; i++) {
 #pragma unroll
-        for (int j = i + 1; j < NB; j++) x[j] -= P[j + (int64_t)i * lda] * x[i];
+        for (int j = i + 1; j < NB; j++) x[j] -= sL[i * NB + j] * x[i];
     }
     // the solved U rows also go to the step's row-major U buffer, the B
     // operand of the trailing updates (coalesced: lanes are columns)
@@ -903,6 +909,9 @@ static int panel_set = 1;
 // (operand stages moved straight into LDS, double-buffered), 0 = k_lu_gemm
 // (stages through registers) (CAPS_AMD_GEMM_KERNEL)
 static int gemm_kernel = 1;
+// 128 x 64 and 64 x 128 tiles for regions one panel wide or high, with the
+// 128 x 128 tile (CAPS_AMD_NARROW_TILES, default 1)
+static int narrow_tiles = 1;
 
 static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
                          int k, int pb, hipStream_t s)
@@ -949,6 +958,7 @@ int capsd_lu_init(void)
     if (const char* e = getenv("CAPS_AMD_GRID_PANEL_ROWS")) grid_panel_rows = atoi(e);
     if (const char* e = getenv("CAPS_AMD_PANEL_SET")) panel_set = atoi(e);
     if (const char* e = getenv("CAPS_AMD_GEMM_KERNEL")) gemm_kernel = atoi(e);
+    if (const char* e = getenv("CAPS_AMD_NARROW_TILES")) narrow_tiles = atoi(e);
     return (int)hipFuncSetAttribute((const void*)k_lu_panel<24, 2, 1024, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     24 * 1024 * sizeof(int));
 }
@@ -1010,6 +1020,16 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
             const int gx = (rows + 255) / 256, gy = (cols + 127) / 128;
             hipLaunchKernelGGL((k_lu_gemm<4, 2>), dim3(gx * gy * nb8), dim3(512), 0, s, desc, arena, sd, out, k,
                                r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+        } else if (narrow_tiles && cols <= 64 && rows > 64) {
+            // regions one panel wide: 128 x 64 tiles (a 128 x 128 tile would
+            // compute half of its product for nothing)
+            const int gx = (rows + 127) / 128, gy = (cols + 63) / 64;
+            hipLaunchKernelGGL((k_lu_gemm<2, 1>), dim3(gx * gy * nb8), dim3(128), 0, s, desc, arena, sd, out, k,
+                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+        } else if (narrow_tiles && rows <= 64 && cols > 64) {
+            const int gx = (rows + 63) / 64, gy = (cols + 127) / 128;
+            hipLaunchKernelGGL((k_lu_gemm<1, 2>), dim3(gx * gy * nb8), dim3(128), 0, s, desc, arena, sd, out, k,
+                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
         } else if (gemm_kernel == 0) {
             const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
             hipLaunchKernelGGL((k_lu_gemm<2, 2>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k,
@@ -1032,8 +1052,12 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         const int c0 = k + pb * NB + cs_off;
         const int n = min(maxN - c0, cols);
         if (n <= 0) return;
-        hipLaunchKernelGGL(k_lu_swap_trsm, dim3((n + 63) / 64, nblocks), dim3(64), 0, s, desc, arena, sd, out, k,
-                           pb, cs_off, cols, swaps);
+        if (n > 64)
+            hipLaunchKernelGGL(k_lu_swap_trsm<256>, dim3((n + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, sd,
+                               out, k, pb, cs_off, cols, swaps);
+        else
+            hipLaunchKernelGGL(k_lu_swap_trsm<64>, dim3((n + 63) / 64, nblocks), dim3(64), 0, s, desc, arena, sd, out,
+                               k, pb, cs_off, cols, swaps);
     };
     auto panel = [&](int k, int pb) {
         if (k + pb * NB < maxN) launch_panel(maxN - k - pb * NB, desc, arena, sd, out, nblocks, k + pb * NB, pb, s);
