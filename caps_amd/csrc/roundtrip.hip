@@ -345,7 +345,10 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     const int nK = d.nK;
     double sum[GM][NPOL], tthr[GM][NPOL];
     int E0[GM][NPOL], sh[GM][NPOL], cnt[GM][NPOL], cs[GM][NPOL];
-    int ndone = 0;
+    // a finished sum gets the shift DONE: its later terms are exact zeros
+    // (ldexp far below the smallest subnormal), so it stays unchanged
+    // without a select per term
+    constexpr int DONE = -1000000, LIVE_MIN = DONE / 2;
 #pragma unroll
     for (int g = 0; g < GM; g++) {
         kb[g] = act[g] ? at<KEnt>(arena, desc[grp.blk[g]].off_k) : nullptr;
@@ -362,12 +365,11 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 cs[g][pp] = 0;
             } else {
                 E0[g][pp] = 0;
-                sh[g][pp] = 0;
+                sh[g][pp] = DONE;   // absent or skipped
                 sum[g][pp] = 0;
                 tthr[g][pp] = 0;
-                cnt[g][pp] = 3;   // absent or skipped: counts as done
+                cnt[g][pp] = 0;
                 cs[g][pp] = 0;
-                ndone++;
             }
         }
     }
@@ -378,8 +380,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     // (below half an ulp of the sum, they leave it unchanged)
     // All of a step's K loads are issued together, before the step's a_q is
     // computed (one memory latency per step, hidden by the recurrence, not
-    // one per block and polarization); the updates are selects, so finished
-    // sums stay unchanged without divergent branches.
+    // one per block and polarization).  Returns whether a sum is still live.
     auto load_k = [&](int q, KEnt (&kv)[GM][NPOL]) {
         const int j = l12 - 2 * q - mu;
 #pragma unroll
@@ -389,25 +390,24 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 if (act[g]) kv[g][pp] = kb[g][(POL0 + pp) * nK + j];   // uniform
     };
     auto accumulate = [&](const KEnt (&kv)[GM][NPOL], double aq, bool check) {
+        bool live = false;
 #pragma unroll
         for (int g = 0; g < GM; g++) {
             if (!act[g]) continue;
 #pragma unroll
             for (int pp = 0; pp < NPOL; pp++) {
-                const bool live = cnt[g][pp] < 3;
                 const double term = ldexp(aq * kv[g][pp].m, kv[g][pp].e + sh[g][pp]);
-                sum[g][pp] += live ? term : 0.0;
+                sum[g][pp] += term;
                 if (check) {
                     const double aterm = fabs(term);
-                    const int c1 = aterm < tthr[g][pp] ? cnt[g][pp] + 1 : 0;
-                    const int c2 = aterm * 0x1p56 < fabs(sum[g][pp]) ? cs[g][pp] + 1 : 0;
-                    const bool fin = live && (c1 >= 3 || c2 >= 3);
-                    cnt[g][pp] = live ? (fin ? 3 : c1) : cnt[g][pp];
-                    cs[g][pp] = live ? c2 : cs[g][pp];
-                    ndone += fin ? 1 : 0;
+                    cnt[g][pp] = aterm < tthr[g][pp] ? cnt[g][pp] + 1 : 0;
+                    cs[g][pp] = aterm * 0x1p56 < fabs(sum[g][pp]) ? cs[g][pp] + 1 : 0;
+                    if (cnt[g][pp] >= 3 || cs[g][pp] >= 3) sh[g][pp] = DONE;
+                    live = live || sh[g][pp] > LIVE_MIN;
                 }
             }
         }
+        return live;
     };
 
     double aqm2 = 1, aqm1 = 1, aq = 1;  // a_{q-2}, a_{q-1}, a_q
@@ -440,8 +440,9 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     // step before (p = l1 + l2 - 2q).
     const double s2 = (n + nu + 1) * (n + nu + 1), d2 = (n - nu) * (n - nu);
     AlphaND al3 = alpha_nd(n + nu - 3, s2, d2), al4 = alpha_nd(n + nu - 2, s2, d2);
+    bool live = true;
 #pragma unroll 2
-    for (int q = 3; q <= qmax && ndone < GM * NPOL; q++) {
+    for (int q = 3; q <= qmax && live; q++) {
         // this step's K values are fetched first; the recurrence below
         // covers their latency
         KEnt kv[GM][NPOL];
@@ -480,7 +481,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
 #pragma unroll
                 for (int pp = 0; pp < NPOL; pp++) sh[g][pp] += e;
         }
-        accumulate(kv, aq, true);
+        live = accumulate(kv, aq, true);
     }
 
     // I = e^{log a0} · sum · 2^{E0}, stored as a signed mantissa times 2^exp
