@@ -511,7 +511,14 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
     if (with_lu && !c->prof)
         p.parts = std::max(1, std::min(lu_config(p.maxN, n).streams, std::min(MAX_PARTS, n / 4)));
     p.groups.clear();
-    for (int g = 0; g <= p.parts; g++) p.part_b[g] = (int)((int64_t)n * g / p.parts);
+    // part 0 takes the fraction f0 of the blocks (its I runs alone, before
+    // any LU can start), the others share the rest equally
+    // (CAPS_AMD_PART0_FRAC; default: equal parts)
+    const double f0 = p.parts > 1 ? env_double("CAPS_AMD_PART0_FRAC", 1.0 / p.parts) : 1.0;
+    p.part_b[0] = 0;
+    p.part_b[1] = std::max(1, std::min(n - (p.parts - 1), (int)std::lround(n * f0)));
+    for (int g = 2; g <= p.parts; g++)
+        p.part_b[g] = p.part_b[1] + (int)((int64_t)(n - p.part_b[1]) * (g - 1) / (p.parts - 1));
     for (int g = 0; g < p.parts; g++) {
         p.part_goff[g] = (int)p.groups.size();
         build_igroups(objs, c->h_desc, p.part_b[g], p.part_b[g + 1], p, p.part_cls[g]);

@@ -716,8 +716,8 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
 // A22 -= L21 U12 on 128x128 tiles (2x2 wavefronts of 64x64) with the
 // operand stages moved straight from HBM into LDS (global_load_lds_dwordx4:
 // no staging registers, no LDS store instructions) and double-buffered, so
-// one barrier per 16-deep stage suffices and the next stage's loads run
-// under the current stage's MFMAs.  L comes from the step's panel buffers,
+// one barrier per stage suffices and the next stage's loads run under the
+// current stage's MFMAs.  L comes from the step's panel buffers,
 // U from the row-major U buffer, so both stages are rows of 128 contiguous
 // doubles, one 1 KB row per wavefront instruction (lane l moves elements 2l
 // and 2l+1).  The loads are issued from inline assembly: the compiler's
@@ -729,12 +729,16 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
 // dimension (lda > N when N is odd) or not at all, and only feed rows
 // (columns) of the product that are not stored.
 typedef __attribute__((address_space(3))) void lds_void;
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+// KS = 16: 73.7 KB of LDS, two workgroups per CU; KS = 8: 36.9 KB, OCC
+// workgroups per CU as the registers allow
+template <int KS, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
               const BlockOut* __restrict__ out, int k, int r_off, int r_lim, int c_off, int c_lim, int kk0, int KK,
               int gx, int gy, int nblocks, int xcd_order)
 {
-    constexpr int GT = 128, KS = 16, PITCH = GT + 16;
+    constexpr int GT = 128, PITCH = GT + 16;
+    constexpr int KW = KS / 4;   // kk rows of a stage moved by each wavefront
     const int wg = blockIdx.x, tiles = gx * gy;
     int b, tile;
     if (xcd_order) {
@@ -770,14 +774,14 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
     const int li = lane & 15, lk = lane >> 4;
     const bool arow_ok = m0 + 2 * lane < rend, bcol_ok = n0 + 2 * lane < cend;
     // stage of depth KS at kk0 + ks into buffer buf: wavefront wv moves kk
-    // rows 4 wv .. 4 wv + 3 of both operands
+    // rows KW wv .. KW wv + KW - 1 of both operands
     auto issue = [&](int ks, int buf) {
         const int j = kk0 + ks, q = j / NB;
         const double* Lp = Pb + (int64_t)q * lda * NB + (int64_t)(j % NB) * lda - (k + q * NB) + m0 + 2 * lane;
         const double* Up = Ut + (int64_t)j * lda + n0 + 2 * lane;
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int kk = 4 * wv + t;
+        for (int t = 0; t < KW; t++) {
+            const int kk = KW * wv + t;
             const uint32_t la = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(lds_void*)&sA[buf][kk][0]);
             const uint32_t lb = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(lds_void*)&sB[buf][kk][0]);
             if (arow_ok)
@@ -905,10 +909,11 @@ static int grid_panel_rows = 6144;
 // above 512 rows.  Both give bit-identical factors: every entry subtracts
 // the same products in the same column order.
 static int panel_set = 1;
-// trailing-update kernel of the 128 x 128 tile: 1 (default) = k_lu_gemm_dma
-// (operand stages moved straight into LDS, double-buffered), 0 = k_lu_gemm
-// (stages through registers) (CAPS_AMD_GEMM_KERNEL)
-static int gemm_kernel = 1;
+// trailing-update kernel of the 128 x 128 tile (CAPS_AMD_GEMM_KERNEL):
+// k_lu_gemm_dma (operand stages moved straight into LDS, double-buffered)
+// with 8-deep stages and three workgroups per CU (2, the default) or 16-deep
+// stages and two (1), or k_lu_gemm (stages through registers, 0)
+static int gemm_kernel = 2;
 // 128 x 64 and 64 x 128 tiles for regions one panel wide or high, with the
 // 128 x 128 tile (CAPS_AMD_NARROW_TILES, default 1)
 static int narrow_tiles = 1;
@@ -1036,8 +1041,12 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
                                r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
         } else {
             const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
-            hipLaunchKernelGGL(k_lu_gemm_dma, dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k, r_off,
-                               r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+            if (gemm_kernel == 2)
+                hipLaunchKernelGGL((k_lu_gemm_dma<8, 3>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out,
+                                   k, r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+            else
+                hipLaunchKernelGGL((k_lu_gemm_dma<16, 2>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out,
+                                   k, r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
         }
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
         if (gemm_info) {
