@@ -505,11 +505,17 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         p.max_nl = std::max(p.max_nl, d.nl);
     }
     p.bytes = (size_t)off;
-    // parts = LU stream groups of the tuned schedule (one without an LU,
-    // and in profiled runs, which time each update launch alone)
+    // parts = LU stream groups of the tuned schedule, at most two (one
+    // without an LU, and in profiled runs, which time each update launch
+    // alone): the tuning times the LU alone, where three groups can win,
+    // but in the pipeline every further part adds I work that runs before
+    // its LU can start (C2: three parts 253 ms, two 235 ms per step);
+    // CAPS_AMD_PARTS overrides the cap
     p.parts = 1;
-    if (with_lu && !c->prof)
-        p.parts = std::max(1, std::min(lu_config(p.maxN, n).streams, std::min(MAX_PARTS, n / 4)));
+    if (with_lu && !c->prof) {
+        const int cap = std::max(1, (int)env_double("CAPS_AMD_PARTS", 2));
+        p.parts = std::max(1, std::min(std::min(lu_config(p.maxN, n).streams, cap), std::min(MAX_PARTS, n / 4)));
+    }
     p.groups.clear();
     // part 0 takes the fraction f0 of the blocks (its I runs alone, before
     // any LU can start), the others share the rest equally
