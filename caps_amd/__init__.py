@@ -71,6 +71,7 @@ def lib():
             "capsx_profile": (None, [p, i]),
             "capsx_set_graph": (None, [p, i]),
             "capsx_lu_grid_panel_rows": (i, [i]),
+            "capsx_lu_kernels": (i, [i, i, i, ip]),
             "capsx_surrogate_report": (None, [p, dp, i]),
             "caps_ht_plasma": (d, [p, d, d]),
             "capsx_material_init": (p, [ctypes.c_char_p, d]),

@@ -53,6 +53,7 @@ int capsd_lu_init(void);
 int capsd_lu_max_dim(void);
 int capsd_lu_nb(void);
 int capsd_lu_npanels(void);
+int capsd_lu_kernels(int gk, int ps, int nt, int prev[3]);
 void capsd_launch_mfma_probe(const double*, const double*, double*, hipStream_t);
 }
 
@@ -2400,6 +2401,14 @@ int capsx_lu_grid_panel_rows(int rows)
 {
     device_init();
     return capsd_lu_grid_panel_rows(rows);
+}
+
+// Extension: the LU's kernel variants (trailing update, panel set, narrow
+// tiles); values < 0 keep the current ones, prev receives the old ones.
+int capsx_lu_kernels(int gemm_kernel, int panel_set, int narrow_tiles, int prev[3])
+{
+    device_init();
+    return capsd_lu_kernels(gemm_kernel, panel_set, narrow_tiles, prev);
 }
 
 // Extension: the schedule used for n blocks of largest dimension maxN:

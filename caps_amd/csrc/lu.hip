@@ -968,6 +968,24 @@ int capsd_lu_init(void)
                                     24 * 1024 * sizeof(int));
 }
 
+// Kernel variants (values < 0 keep the current one; prev receives the
+// values before the call): the 128 x 128 trailing-update kernel (0, 1, 2),
+// the panel variant set (0, 1), the narrow tiles (0, 1).  Returns 1 for
+// invalid values.
+int capsd_lu_kernels(int gk, int ps, int nt, int prev[3])
+{
+    if (prev) {
+        prev[0] = gemm_kernel;
+        prev[1] = panel_set;
+        prev[2] = narrow_tiles;
+    }
+    if (gk > 2 || ps > 1 || nt > 1) return 1;
+    if (gk >= 0) gemm_kernel = gk;
+    if (ps >= 0) panel_set = ps;
+    if (nt >= 0) narrow_tiles = nt;
+    return 0;
+}
+
 // Panel height above which the grid-wide panel kernels run; returns the
 // previous value (rows <= 0: query only).
 int capsd_lu_grid_panel_rows(int rows)

@@ -245,6 +245,18 @@ int capsx_lu_tuning_load(const char *path);
  * kernels (default 6144, above which one workgroup would hold more than 12
  * rows per lane); rows <= 0 only queries.  Returns the previous value. */
 int capsx_lu_grid_panel_rows(int rows);
+/* Kernel variants of the LU for all later batches (values < 0 keep the
+ * current one; prev, if not NULL, receives the values before the call):
+ * gemm_kernel = the 128 x 128 trailing update (2, the default: operand
+ * stages moved straight into LDS, 8 deep, three workgroups per CU; 1: 16
+ * deep, two per CU; 0: stages through registers), panel_set (1, the
+ * default: leaves of 8 columns; 0: leaves of 4), narrow_tiles (1, the
+ * default: 128 x 64 and 64 x 128 tiles for regions one panel wide or high;
+ * 0: 128 x 128 for all).  All give bit-identical factors.  Same as the
+ * CAPS_AMD_GEMM_KERNEL, CAPS_AMD_PANEL_SET and CAPS_AMD_NARROW_TILES
+ * variables read at start-up.  Returns 1 for invalid values.  No
+ * counterpart in the reference. */
+int capsx_lu_kernels(int gemm_kernel, int panel_set, int narrow_tiles, int prev[3]);
 /* Milliseconds of one batched LU of n synthetic dim x dim matrices with the
  * given schedule (values < 0 keep the configured ones), averaged over reps.
  * Returns 0 on success. */

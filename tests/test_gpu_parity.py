@@ -349,6 +349,37 @@ def test_lu_schedules_agree(ca):
             at.set_config(0)
 
 
+def test_lu_kernel_variants_bitwise(ca):
+    """Every kernel variant of the LU (trailing update with operand stages
+    straight into LDS, 8 or 16 deep, or through registers; panel leaves of
+    8 or 4 columns; narrow tiles on or off) gives the same determinants bit
+    for bit under every schedule level: each matrix entry receives the same
+    products in the same order."""
+    import ctypes
+    import itertools
+    from caps_amd import autotune as at
+    L = ca.lib()
+    rng = np.random.default_rng(11)
+    prev = (ctypes.c_int * 3)()
+    assert L.capsx_lu_kernels(-1, -1, -1, prev) == 0
+    default = tuple(prev)
+    assert L.capsx_lu_kernels(3, -1, -1, None) == 1
+    try:
+        for N in (517, 300, 129):
+            mats = np.array([np.eye(N) - 0.3 * rng.normal(size=(N, N)) / math.sqrt(N) for _ in range(6)])
+            for levels in (1, 2, 3):
+                at.set_config(levels, 1, 0, 128)
+                L.capsx_lu_kernels(*default, None)
+                ref = ca.lu_logdet_batch(mats)
+                for gk, ps, nt in itertools.product((0, 1, 2), (0, 1), (0, 1)):
+                    assert L.capsx_lu_kernels(gk, ps, nt, None) == 0
+                    out = ca.lu_logdet_batch(mats)
+                    assert np.array_equal(out, ref), (N, levels, gk, ps, nt, np.abs(out - ref).max())
+    finally:
+        L.capsx_lu_kernels(*default, None)
+        at.set_config(0)
+
+
 def test_autotune_table_and_timer(ca, tmp_path):
     from caps_amd import autotune as at
     p = tmp_path / "lu_tuning.txt"
