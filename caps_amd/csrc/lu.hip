@@ -65,17 +65,56 @@ __device__ __forceinline__ int* moved_buf(char* arena, const LuSide& sd, int pb)
     return at<int>(arena, sd.off_moved) + pb * NB;
 }
 
+// Cross-lane moves without the LDS pipeline: a DPP lane permutation inside
+// rows of 16 lanes, and a broadcast of lane l (uniform l) through a scalar
+// register.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x)
+{
+    const int2 v = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(double, make_int2(dpp_i<CTRL>(v.x), dpp_i<CTRL>(v.y)));
+}
+__device__ __forceinline__ double lane_d(double x, int l)
+{
+    const int2 v = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(double,
+                              make_int2(__builtin_amdgcn_readlane(v.x, l), __builtin_amdgcn_readlane(v.y, l)));
+}
+
+// Largest v over the wavefront, lowest idx among equal values (a total
+// order, so the result does not depend on the reduction tree), in every
+// lane: within rows of 16 by DPP (quad exchanges 1 and 2, half-row and row
+// mirrors), then across the four rows through scalar registers.
 __device__ __forceinline__ void wave_argmax(double& v, int& idx)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(v, o, 64);
-        const int oi = __shfl_xor(idx, o, 64);
+    auto take = [&](double ov, int oi) {
         if (ov > v || (ov == v && oi < idx)) {
             v = ov;
             idx = oi;
         }
+    };
+    take(dpp_d<0xB1>(v), dpp_i<0xB1>(idx));     // quad_perm [1,0,3,2]
+    take(dpp_d<0x4E>(v), dpp_i<0x4E>(idx));     // quad_perm [2,3,0,1]
+    take(dpp_d<0x141>(v), dpp_i<0x141>(idx));   // row_half_mirror
+    take(dpp_d<0x140>(v), dpp_i<0x140>(idx));   // row_mirror
+    double bv = lane_d(v, 0);
+    int bi = __builtin_amdgcn_readlane(idx, 0);
+#pragma unroll
+    for (int r = 16; r < 64; r += 16) {
+        const double ov = lane_d(v, r);
+        const int oi = __builtin_amdgcn_readlane(idx, r);
+        if (ov > bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+        }
     }
+    v = bv;
+    idx = bi;
 }
 
 // Panel body: R rows per lane, leaves of LF columns, NT threads, the
@@ -146,7 +185,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
                     const int q = cc0 + t;
                     double uq[LF];
 #pragma unroll
-                    for (int j = 0; j < LF; j++) uq[j] = __shfl(a[0][j], q, 64);
+                    for (int j = 0; j < LF; j++) uq[j] = lane_d(a[0][j], q);
                     if (lane > q && lane < cc0 + CW) {
 #pragma unroll
                         for (int j = 0; j < LF; j++) a[0][j] -= Lr[0][t] * uq[j];
