@@ -420,9 +420,13 @@ void build_igroups(caps_t* const* objs, const BlockDesc* desc, int b0, int b1, B
         IGroup g;
         memset(&g, 0, sizeof(g));
         int j = i;
-        // groups of at most 4 by default: the 8-block instantiation needs 231
-        // VGPRs (2 waves per SIMD) and ran the C2 I stage 25% slower
-        static const int gmax = std::max(1, std::min(IG_MAX, (int)env_double("CAPS_AMD_IGROUP_MAX", 4)));
+        // at most four running sums per lane by default: groups of 2 blocks
+        // with both polarizations, of 4 for perfect reflectors (TM only);
+        // those instantiations take 110 VGPRs (4 waves per SIMD).  C2 (two
+        // polarizations) I stage: groups of 1, 2, 3, 4, 8: 61, 56, 66, 65,
+        // 86 ms.  CAPS_AMD_IGROUP_MAX sets one maximum for both.
+        static const int genv = (int)env_double("CAPS_AMD_IGROUP_MAX", 0);
+        const int gmax = std::max(1, std::min(IG_MAX, genv > 0 ? genv : (desc[idx[i]].is_pr ? 4 : 2)));
         while (j < n && j - i < gmax && key(idx[j]) == key(idx[i])) {
             g.blk[j - i] = idx[j];
             j++;
