@@ -25,6 +25,7 @@
 #include <stdlib.h>
 
 #include "blocks.h"
+#include "wave.h"
 
 namespace capsd {
 
@@ -65,27 +66,6 @@ __device__ __forceinline__ int* moved_buf(char* arena, const LuSide& sd, int pb)
     return at<int>(arena, sd.off_moved) + pb * NB;
 }
 
-// Cross-lane moves without the LDS pipeline: a DPP lane permutation inside
-// rows of 16 lanes, and a broadcast of lane l (uniform l) through a scalar
-// register.
-template <int CTRL>
-__device__ __forceinline__ int dpp_i(int x)
-{
-    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_d(double x)
-{
-    const int2 v = __builtin_bit_cast(int2, x);
-    return __builtin_bit_cast(double, make_int2(dpp_i<CTRL>(v.x), dpp_i<CTRL>(v.y)));
-}
-__device__ __forceinline__ double lane_d(double x, int l)
-{
-    const int2 v = __builtin_bit_cast(int2, x);
-    return __builtin_bit_cast(double,
-                              make_int2(__builtin_amdgcn_readlane(v.x, l), __builtin_amdgcn_readlane(v.y, l)));
-}
-
 // Largest v over the wavefront, lowest idx among equal values (a total
 // order, so the result does not depend on the reduction tree), in every
 // lane: within rows of 16 by DPP (quad exchanges 1 and 2, half-row and row
@@ -98,10 +78,10 @@ __device__ __forceinline__ void wave_argmax(double& v, int& idx)
             idx = oi;
         }
     };
-    take(dpp_d<0xB1>(v), dpp_i<0xB1>(idx));     // quad_perm [1,0,3,2]
-    take(dpp_d<0x4E>(v), dpp_i<0x4E>(idx));     // quad_perm [2,3,0,1]
-    take(dpp_d<0x141>(v), dpp_i<0x141>(idx));   // row_half_mirror
-    take(dpp_d<0x140>(v), dpp_i<0x140>(idx));   // row_mirror
+    take(dpp_d<DPP_QUAD_X1>(v), dpp_i<DPP_QUAD_X1>(idx));
+    take(dpp_d<DPP_QUAD_X2>(v), dpp_i<DPP_QUAD_X2>(idx));
+    take(dpp_d<DPP_HALF_MIRROR>(v), dpp_i<DPP_HALF_MIRROR>(idx));
+    take(dpp_d<DPP_MIRROR>(v), dpp_i<DPP_MIRROR>(idx));
     double bv = lane_d(v, 0);
     int bi = __builtin_amdgcn_readlane(idx, 0);
 #pragma unroll

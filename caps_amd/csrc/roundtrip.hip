@@ -10,6 +10,7 @@
 
 #include "blocks.h"
 #include "device_math.h"
+#include "wave.h"
 
 namespace capsd {
 
@@ -106,20 +107,6 @@ __global__ void k_fresnel_points(int n, const double* __restrict__ xi_, const do
 // all ν), kept as mantissa × 2^E.  For every ν the wave sums its 64 node
 // terms after aligning them to the wave's largest exponent (ldexp, exact);
 // one KPart per (ν, wave) goes to HBM and k_K_combine adds the waves.
-
-__device__ __forceinline__ int wave_max_int(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 __global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                   const double* __restrict__ lfac_tab)

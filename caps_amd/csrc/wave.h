@@ -1,0 +1,57 @@
+// Cross-lane moves within a 64-lane wavefront without the LDS pipeline
+// (ds_bpermute): DPP lane permutations inside rows of 16 lanes and
+// broadcasts of one lane through a scalar register.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace capsd {
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int x)
+{
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x)
+{
+    const int2 v = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(double, make_int2(dpp_i<CTRL>(v.x), dpp_i<CTRL>(v.y)));
+}
+// value of lane l (uniform l)
+__device__ __forceinline__ double lane_d(double x, int l)
+{
+    const int2 v = __builtin_bit_cast(int2, x);
+    return __builtin_bit_cast(double,
+                              make_int2(__builtin_amdgcn_readlane(v.x, l), __builtin_amdgcn_readlane(v.y, l)));
+}
+
+// DPP controls: exchanges within quads (lane i <-> i^1, i <-> i^2) and the
+// mirrors of 8- and 16-lane groups (i <-> 7-i, i <-> 15-i)
+constexpr int DPP_QUAD_X1 = 0xB1, DPP_QUAD_X2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+
+// Sum over the wavefront, the same value in every lane, with the additions
+// of the xor butterfly (pairs 1, 2, 4, 8, 16, 32): after the quad stages all
+// lanes of a quad hold the same partial sum, so the mirrors pair the same
+// operands as xor 4 and xor 8, and the last two stages are
+// (row0 + row1) + (row2 + row3) of the 16-lane rows.  Bit-identical to the
+// butterfly (each addition has the same two operands).
+__device__ __forceinline__ double wave_sum(double v)
+{
+    v += dpp_d<DPP_QUAD_X1>(v);
+    v += dpp_d<DPP_QUAD_X2>(v);
+    v += dpp_d<DPP_HALF_MIRROR>(v);
+    v += dpp_d<DPP_MIRROR>(v);
+    return (lane_d(v, 0) + lane_d(v, 16)) + (lane_d(v, 32) + lane_d(v, 48));
+}
+
+__device__ __forceinline__ int wave_max_int(int v)
+{
+    v = max(v, dpp_i<DPP_QUAD_X1>(v));
+    v = max(v, dpp_i<DPP_QUAD_X2>(v));
+    v = max(v, dpp_i<DPP_HALF_MIRROR>(v));
+    v = max(v, dpp_i<DPP_MIRROR>(v));
+    return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+}  // namespace capsd
