@@ -859,22 +859,14 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
 }
 
 // Kahan sum of log|u_ii| (src/matrix.c:462-471).
-__global__ void k_lu_finalize(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                              const LuSide* __restrict__ side, BlockOut* __restrict__ out, int nblocks)
+__global__ void __launch_bounds__(64) k_lu_finalize(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                   const LuSide* __restrict__ side, BlockOut* __restrict__ out,
+                                                   int nblocks)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblocks) return;
-    if (out[b].skip) return;
-    const BlockDesc& d = desc[b];
-    const double* logs = at<double>(arena, side[b].off_logs);
-    double s = 0, c = 0;
-    for (int i = 0; i < d.N; i++) {
-        const double y = logs[i] - c;
-        const double t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    out[b].logdet = s;
+    const int b = blockIdx.x;   // one wavefront per block
+    if (b >= nblocks || out[b].skip) return;
+    const double s = wave_kahan_sum(at<double>(arena, side[b].off_logs), desc[b].N);
+    if (threadIdx.x == 0) out[b].logdet = s;
 }
 
 // Synthetic LU input (autotuning): A = 1 - 0.3 X / sqrt(N) with X_ij uniform
@@ -1152,7 +1144,7 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
             update(k, 4 * NB, ALL, 4 * NB, ALL, 0, 4 * NB);
         }
     }
-    hipLaunchKernelGGL(k_lu_finalize, dim3((nblocks + 63) / 64), dim3(64), 0, s, desc, arena, sd, out, nblocks);
+    hipLaunchKernelGGL(k_lu_finalize, dim3(nblocks), dim3(64), 0, s, desc, arena, sd, out, nblocks);
     return p;
 }
 

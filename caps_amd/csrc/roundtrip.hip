@@ -654,24 +654,19 @@ __global__ void __launch_bounds__(256) k_fill_xi0(const BlockDesc* __restrict__ 
 
 // Trace of M (Kahan, in index order as src/matrix.c:63-68) and the trace
 // shortcut flag |tr M| < 1e-8 (src/matrix.c:78-82).  One lane per block.
-__global__ void k_trace(const BlockDesc* __restrict__ desc, char* __restrict__ arena, BlockOut* __restrict__ out,
-                        int nblocks)
+__global__ void __launch_bounds__(64) k_trace(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                             BlockOut* __restrict__ out, int nblocks)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x;   // one wavefront per block
     if (b >= nblocks) return;
     const BlockDesc& d = desc[b];
-    const double* dg = at<double>(arena, d.off_diag);
-    double s = 0, c = 0;
-    for (int i = 0; i < d.N; i++) {
-        const double y = dg[i] - c;
-        const double t = s + y;
-        c = (t - s) - y;
-        s = t;
+    const double s = wave_kahan_sum(at<double>(arena, d.off_diag), d.N);
+    if (threadIdx.x == 0) {
+        out[b].trace = s;
+        out[b].skip = fabs(s) < 1e-8 ? 1 : 0;
+        out[b].logdet = NAN;
+        out[b].status = 0;
     }
-    out[b].trace = s;
-    out[b].skip = fabs(s) < 1e-8 ? 1 : 0;
-    out[b].logdet = NAN;
-    out[b].status = 0;
 }
 
 // ||M||_F² and tr M² = Σ_ij M_ij M_ji of M (from the stored 1 - M and the
@@ -783,7 +778,7 @@ void capsd_launch_fill_xi0(const BlockDesc* desc, char* arena, int64_t off_tab, 
 
 void capsd_launch_trace(const BlockDesc* desc, char* arena, BlockOut* out, int nblocks, int frobenius, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_trace, dim3((nblocks + 63) / 64), dim3(64), 0, s, desc, arena, out, nblocks);
+    hipLaunchKernelGGL(k_trace, dim3(nblocks), dim3(64), 0, s, desc, arena, out, nblocks);
     if (frobenius) hipLaunchKernelGGL(k_frobenius, dim3(nblocks), dim3(256), 0, s, desc, arena, out);
 }
 

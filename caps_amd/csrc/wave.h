@@ -54,4 +54,28 @@ __device__ __forceinline__ int wave_max_int(int v)
                max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
 }
 
+// Kahan sum of x[0..n) in index order (the recurrence of a single thread
+// running through the array), by a whole wavefront: 64 consecutive entries
+// per load (the next chunk loaded while this one is summed), each lane
+// running the same recurrence on the lane broadcasts.  Same value in every
+// lane, bit-identical to the sequential loop.
+__device__ __forceinline__ double wave_kahan_sum(const double* __restrict__ x, int n)
+{
+    const int lane = threadIdx.x & 63;
+    double s = 0, c = 0;
+    double v = lane < n ? x[lane] : 0.0;
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        const double nv = i0 + 64 + lane < n ? x[i0 + 64 + lane] : 0.0;
+        const int m = n - i0 < 64 ? n - i0 : 64;
+        for (int t = 0; t < m; t++) {
+            const double y = lane_d(v, t) - c;
+            const double u = s + y;
+            c = (u - s) - y;
+            s = u;
+        }
+        v = nv;
+    }
+    return s;
+}
+
 }  // namespace capsd
