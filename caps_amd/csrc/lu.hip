@@ -515,14 +515,14 @@ __global__ void __launch_bounds__(NTH) k_lu_swap_trsm(const BlockDesc* __restric
 #pragma unroll
         for (int j = i + 1; j < NB; j++) x[j] -= sL[i * NB + j] * x[i];
     }
-    // the solved U rows also go to the step's row-major U buffer, the B
-    // operand of the trailing updates (coalesced: lanes are columns)
+    // the solved U rows go to the step's row-major U buffer (coalesced:
+    // lanes are columns), the B operand of every trailing update; nothing
+    // reads them from the matrix again (the determinant needs only the
+    // diagonal), so they are not written back there, which saved 64 stores
+    // per lane that each touched 64 columns
     double* Ut = at<double>(arena, side[b].off_ut) + (int64_t)pb * NB * lda + col;
 #pragma unroll
-    for (int i = 0; i < NB; i++) {
-        Ac[r0 + i] = x[i];
-        Ut[i * lda] = x[i];
-    }
+    for (int i = 0; i < NB; i++) Ut[i * lda] = x[i];
 }
 
 // Later panels' row interchanges applied to the L rows of earlier panels of
@@ -558,8 +558,8 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     constexpr int NT = 64 * WM * WN, GTM = 64 * WM, GTN = 64 * WN;
     constexpr int KS = NT >= 256 ? LU_KS : 8;   // K per LDS stage (prefetch registers)
     // padded LDS rows: GTPM keeps the A rows 128-byte aligned (its stores and
-    // reads run along rows); GTPN is odd, so the B stores, which run down a
-    // column (16 consecutive k), fall on distinct banks
+    // reads run along rows); the B stores run along rows too (U from the
+    // row-major U buffer), GTPN odd as when they ran down columns
     constexpr int GTPM = GTM + 16, GTPN = GTN + 17;
     // XCD-aware order (xcd_order = 1): workgroup i runs on XCD i % 8, so all
     // tiles of block b go to XCD b % 8 and share its L2 for the L and U
@@ -590,6 +590,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
     const double* Pb = panel_buf(arena, side[b], lda, 0);
+    const double* Ut = at<double>(arena, side[b].off_ut);
     // one LDS buffer: the A and B stages, reused for the C strips at the end
     constexpr int STG = KS * (GTPM + GTPN), STRIPS = WM * WN * (64 * 17 + 4);
     constexpr int SMEM = (LU_EPI_DIRECT || STG > STRIPS) ? STG : STRIPS;   // stages (then the epilogue strips)
@@ -621,12 +622,14 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
             const int row = m0 + r;
             pa[t] = row < rend ? Lp[row + (int64_t)kk * lda] : 0.0;
         }
+        // U rows from the step's row-major U buffer (consecutive lanes read
+        // consecutive columns)
 #pragma unroll
         for (int t = 0; t < PERB; t++) {
             const int idx = tid + NT * t;
-            const int kk2 = idx % KS, c = idx / KS;
+            const int kk2 = idx / GTN, c = idx % GTN;
             const int colg = n0 + c;
-            pb[t] = colg < cend ? A[(k + kk0 + ks + kk2) + (int64_t)colg * lda] : 0.0;
+            pb[t] = colg < cend ? Ut[(int64_t)(kk0 + ks + kk2) * lda + colg] : 0.0;
         }
     };
     fetch(0);
@@ -640,7 +643,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
 #pragma unroll
         for (int t = 0; t < PERB; t++) {
             const int idx = tid + NT * t;
-            Bs[idx % KS][idx / KS] = pb[t];
+            Bs[idx / GTN][idx % GTN] = pb[t];
         }
         __syncthreads();
         if (ks + KS < KK) fetch(ks + KS);
