@@ -15,8 +15,14 @@
 //                   factored panel in pivoted order to a side buffer, the
 //                   pivots (LAPACK ipiv convention) and log|u_ii|;
 //   k_lu_swap_trsm  applies the row interchanges to the trailing columns
-//                   and solves U12 = L11^{-1} A12 (one lane per column);
-//   k_lu_gemm       A22 -= L21 U12 on FP64 MFMA (v_mfma_f64_16x16x4f64).
+//                   and solves U12 = L11^{-1} A12 (one lane per column),
+//                   writing U12 to the step's row-major U buffer;
+//   k_lu_gemm_dma   A22 -= L21 U12 on FP64 MFMA (v_mfma_f64_16x16x4f64),
+//                   128 x 128 tiles, operand stages moved straight from
+//                   HBM into LDS (L from the panel buffers, U from the U
+//                   buffer), double-buffered;
+//   k_lu_gemm       the same product with register-staged operands, for
+//                   128 x 64, 64 x 128, 64 x 64 and 256 x 128 tiles.
 // k_lu_finalize sums log|u_ii| with Kahan compensation.
 // Matrices are column-major with leading dimension lda.
 #include <hip/hip_runtime.h>
