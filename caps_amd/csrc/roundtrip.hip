@@ -622,10 +622,10 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     const int64_t r = 2 * i2, c = 2 * j2, lda = d.lda;
     const double sg = mode == 0 ? -1.0 : 1.0;
     const double one = mode == 0 ? 1.0 : 0.0;
-    Am[r + c * lda] = sg * EE + (i2 == j2 ? one : 0.0);
-    Am[r + (c + 1) * lda] = sg * EM;
-    Am[r + 1 + c * lda] = sg * ME;
-    Am[r + 1 + (c + 1) * lda] = sg * MM + (i2 == j2 ? one : 0.0);
+    // rows r, r+1 of a column are adjacent: one 16-byte store per column
+    // (r and lda even, the matrix 256-byte aligned)
+    *reinterpret_cast<double2*>(Am + r + c * lda) = make_double2(sg * EE + (i2 == j2 ? one : 0.0), sg * ME);
+    *reinterpret_cast<double2*>(Am + r + (c + 1) * lda) = make_double2(sg * EM, sg * MM + (i2 == j2 ? one : 0.0));
     if (i2 == j2) {
         double* dg = at<double>(arena, d.off_diag);
         dg[r] = EE;
@@ -653,7 +653,7 @@ __global__ void __launch_bounds__(256) k_fill_xi0(const BlockDesc* __restrict__ 
 }
 
 // Trace of M (Kahan, in index order as src/matrix.c:63-68) and the trace
-// shortcut flag |tr M| < 1e-8 (src/matrix.c:78-82).  One lane per block.
+// shortcut flag |tr M| < 1e-8 (src/matrix.c:78-82).  One wavefront per block.
 __global__ void __launch_bounds__(64) k_trace(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                              BlockOut* __restrict__ out, int nblocks)
 {
