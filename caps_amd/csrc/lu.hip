@@ -962,8 +962,8 @@ static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuS
         else if (rows <= 4 * 256) PANELC(4, 8, 256, 2, 4);
         else if (rows <= 8 * 256) PANELC(8, 8, 256, 2, 4);
         else if (rows <= 10 * 256) PANELC(10, 8, 256, 2, 2);
-        else if (rows <= 6 * 512) PANEL(6, 4, 512, 2);
-        else if (rows <= 12 * 512) PANEL(12, 4, 512, 2);
+        else if (rows <= 6 * 512) PANELC(6, 8, 512, 2, 4);
+        else if (rows <= 12 * 512) PANELC(12, 8, 512, 2, 1);
         else PANEL(24, 2, 1024, 4);
     } else {
         // leaves of 4 above 512 rows
