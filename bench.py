@@ -209,7 +209,7 @@ def main():
         hbm_probe()
         return
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (every (xi, m) block fits one MI355X); "
+        print("bench.py: the library runs on one GPU only (every (xi, m) block fits one MI355X); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
