@@ -156,11 +156,23 @@ __global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ 
         }
         __syncthreads();
         const int nend = min(base + 64, d.nu_max + 1);
-        for (int nu = base; nu < nend; nu++) {
-            const int e_lane = active ? E + ilogb(p) : -1000000;
+        // ν's terms are p·mant·2^E; the recurrence gives p at ν+1, kept
+        // below 2^600 by moving powers of two into E
+        auto advance = [&](double& pv, double& pp, int& Ev, int j) {
+            const double pn = c1s[j] * x * pv - c2s[j] * pp;
+            pp = pv;
+            pv = pn;
+            if (pv > 0x1p600) {
+                pv *= 0x1p-600;
+                pp *= 0x1p-600;
+                Ev += 600;
+            }
+        };
+        auto reduce = [&](double pv, int Ev, int nu) {
+            const int e_lane = active ? Ev + ilogb(pv) : -1000000;
             const int F = wave_max_int(e_lane);
-            const double s_te = wave_sum(active ? ldexp(p * mant_te, E - F) : 0.0);
-            const double s_tm = wave_sum(active ? ldexp(p * mant_tm, E - F) : 0.0);
+            const double s_te = wave_sum(active ? ldexp(pv * mant_te, Ev - F) : 0.0);
+            const double s_tm = wave_sum(active ? ldexp(pv * mant_tm, Ev - F) : 0.0);
             if (lane == 0) {
                 KPart kp;
                 kp.s_te = s_te;
@@ -169,14 +181,24 @@ __global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ 
                 kp.pad_ = 0;
                 part[(int64_t)(nu - mu) * d.n_waves + wave] = kp;
             }
-            const double pn = c1s[nu - base] * x * p - c2s[nu - base] * p_prev;
-            p_prev = p;
-            p = pn;
-            if (p > 0x1p600) {
-                p *= 0x1p-600;
-                p_prev *= 0x1p-600;
-                E += 600;
-            }
+        };
+        // two ν per iteration: their reductions are independent, so the
+        // second's cross-lane chains overlap the first's
+        int nu = base;
+        for (; nu + 1 < nend; nu += 2) {
+            double p1 = p, pp1 = p_prev;
+            int E1 = E;
+            advance(p1, pp1, E1, nu - base);   // ν+1
+            reduce(p, E, nu);
+            reduce(p1, E1, nu + 1);
+            p = p1;
+            p_prev = pp1;
+            E = E1;
+            advance(p, p_prev, E, nu + 1 - base);
+        }
+        if (nu < nend) {
+            reduce(p, E, nu);
+            advance(p, p_prev, E, nu - base);
         }
         __syncthreads();
     }
