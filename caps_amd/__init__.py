@@ -55,6 +55,10 @@ def lib():
             "caps_logdetD": (d, [p, d, i]),
             "caps_logdetD_batch": (i, [p, dp, ip, i, dp]),
             "capsx_logdetD_multi": (i, [ctypes.POINTER(p), dp, ip, i, dp]),
+            "capsx_logdetD_multi_info": (i, [ctypes.POINTER(p), dp, ip, i, dp, dp, ip]),
+            "capsx_mie_points": (i, [p, i, dp, ip, dp, dp]),
+            "capsx_fresnel_points": (i, [p, i, dp, dp, dp, dp]),
+            "capsx_bessel_points": (i, [i, ip, dp, dp, dp, dp, dp]),
             "caps_last_timing": (None, [p, dp]),
             "caps_M_init": (p, [p, i, d]), "caps_M_elem": (d, [p, i, i, ctypes.c_char, ctypes.c_char]),
             "caps_M_free": (None, [p]), "caps_kernel_M": (d, [i, i, p]),
@@ -63,7 +67,9 @@ def lib():
             "caps_logdetD0": (None, [p, i, d, dp, dp, dp]),
             "caps_ht_perf": (d, [p, d]),
             "capsx_lu_logdet_batch": (i, [dp, i, i, dp]),
+            "capsx_logdet_1mM_batch": (i, [dp, i, i, dp, ip]),
             "capsx_mfma_probe": (i, [dp, dp, dp]),
+            "kernel_logdet": (d, [i, p, p, i, i]),
             "capsx_block_tables": (i, [p, d, i, ip, ip, dp, ip, ip, dp, ctypes.POINTER(ctypes.c_byte), i, i]),
             "capsx_block_nodes": (i, [p, d, i]),
             "capsx_M_copy": (i, [p, dp]),
@@ -164,11 +170,17 @@ class Caps:
     def logdetD(self, xi_, m):
         return lib().caps_logdetD(self.ptr, float(xi_), int(m))
 
-    def logdetD_batch(self, xis, ms):
+    def logdetD_batch(self, xis, ms, info=False):
+        """logdetD of the blocks (xis[i], ms[i]) in one batch.  info=True
+        also returns each block's tr M and flags (capsx_logdetD_multi_info:
+        bits 0-1 shortcut 0 LU / 1 trace / 2 Mercator / 3 non-finite, bit 2
+        delta-form LU, bit 3 HODLR-mode rule gave -tr M)."""
         xis = np.ascontiguousarray(xis, dtype=np.float64)
         ms = np.ascontiguousarray(ms, dtype=np.int32)
         if xis.shape != ms.shape:
             raise ValueError("xi_ and m must have the same length")
+        if info:
+            return logdetD_multi([self] * len(xis), xis, ms, info=True)
         out = np.empty(len(xis))
         rc = lib().caps_logdetD_batch(self.ptr, _dptr(xis), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                       len(xis), _dptr(out))
@@ -213,6 +225,25 @@ class Caps:
         a, b = ctypes.c_double(), ctypes.c_double()
         lib().caps_fresnel(self.ptr, xi_, k_, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def mie_points(self, xis, ls):
+        """log|a_l|, log|b_l| at the points (xis[i], ls[i]) in one launch."""
+        xis = np.ascontiguousarray(xis, dtype=np.float64)
+        ls = np.ascontiguousarray(ls, dtype=np.int32)
+        a, b = np.empty(len(xis)), np.empty(len(xis))
+        if lib().capsx_mie_points(self.ptr, len(xis), _dptr(xis), ls.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                  _dptr(a), _dptr(b)) != 0:
+            raise ValueError("capsx_mie_points rejected its input")
+        return a, b
+
+    def fresnel_points(self, xis, ks):
+        """r_TE, r_TM at the points (xis[i], ks[i]) in one launch."""
+        xis = np.ascontiguousarray(xis, dtype=np.float64)
+        ks = np.ascontiguousarray(ks, dtype=np.float64)
+        te, tm = np.empty(len(xis)), np.empty(len(xis))
+        if lib().capsx_fresnel_points(self.ptr, len(xis), _dptr(xis), _dptr(ks), _dptr(te), _dptr(tm)) != 0:
+            raise ValueError("capsx_fresnel_points rejected its input")
+        return te, tm
 
     def matrix(self, xi_, m):
         """The round-trip matrix M (N x N, N = 2 ldim) and lmin, via caps_M_init."""
@@ -372,7 +403,22 @@ def npy_load(path):
     return out
 
 
-def logdetD_multi(objs, xis, ms):
+def bessel_points(orders, nus, xs):
+    """log I_{n+1/2}(x), log K_{n+1/2}(x) (orders[i] = n, < 0 skipped) and
+    I_ν(x)/I_{ν+1}(x) (nus[i] = ν, < 0 skipped) at the points xs, in one
+    launch of the device functions."""
+    orders = np.ascontiguousarray(orders, dtype=np.int32)
+    nus = np.ascontiguousarray(nus, dtype=np.float64)
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    n = len(xs)
+    lI, lK, r = np.empty(n), np.empty(n), np.empty(n)
+    if lib().capsx_bessel_points(n, orders.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dptr(nus), _dptr(xs),
+                                 _dptr(lI), _dptr(lK), _dptr(r)) != 0:
+        raise ValueError("capsx_bessel_points rejected its input")
+    return lI, lK, r
+
+
+def logdetD_multi(objs, xis, ms, info=False):
     """logdetD of blocks i = (objs[i], xis[i], ms[i]) in one batched launch
     sequence; the objects may differ in R/L, ldim and material (uneven
     dimensions 2 ldim).  Runs on objs[0]'s device context, whose stage
@@ -383,11 +429,18 @@ def logdetD_multi(objs, xis, ms):
         raise ValueError("objs, xi_ and m must have the same length")
     ptrs = (ctypes.c_void_p * len(objs))(*[o.ptr for o in objs])
     out = np.empty(len(xis))
-    rc = lib().capsx_logdetD_multi(ptrs, _dptr(xis), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                   len(xis), _dptr(out))
+    if info:
+        tr = np.empty(len(xis))
+        flags = np.zeros(len(xis), dtype=np.int32)
+        rc = lib().capsx_logdetD_multi_info(ptrs, _dptr(xis), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            len(xis), _dptr(out), _dptr(tr),
+                                            flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    else:
+        rc = lib().capsx_logdetD_multi(ptrs, _dptr(xis), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       len(xis), _dptr(out))
     if rc != 0:
         raise ValueError("capsx_logdetD_multi rejected its input")
-    return out
+    return (out, tr, flags) if info else out
 
 
 def lu_logdet_batch(mats):
@@ -399,6 +452,32 @@ def lu_logdet_batch(mats):
     if lib().capsx_lu_logdet_batch(_dptr(buf), N, n, _dptr(out)) != 0:
         raise ValueError("bad LU batch")
     return out
+
+
+def logdet_1mM_batch(Ms):
+    """log|det(1 - M)| of square matrices M through the device's column
+    dominance test and LU (test hook): returns (values, delta flags), flag 1
+    where 1 - M is column diagonally dominant and the LU ran in delta form
+    (no pivot search, log1p of the pivots' deviations from 1)."""
+    Ms = np.asarray(Ms, dtype=np.float64)
+    n, N, _ = Ms.shape
+    buf = np.ascontiguousarray(np.transpose(Ms, (0, 2, 1)))   # column-major per matrix
+    out = np.empty(n)
+    flags = np.zeros(n, dtype=np.int32)
+    if lib().capsx_logdet_1mM_batch(_dptr(buf), N, n, _dptr(out), flags.ctypes.data_as(ctypes.POINTER(ctypes.c_int))) != 0:
+        raise ValueError("bad batch")
+    return out, flags
+
+
+KERNEL_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
+
+
+def kernel_logdet(dim, kernel, detalg="HODLR", sym_spd=1):
+    """log det(1 - M) of the dim x dim matrix M_ij = kernel(i, j) through the
+    library's kernel_logdet (host callback, as src/capc.cpp:110-114 calls it)."""
+    cb = KERNEL_FN(lambda i, j, _a: kernel(i, j))
+    return lib().kernel_logdet(dim, ctypes.cast(cb, ctypes.c_void_p), None, sym_spd,
+                               DETALG[detalg] if isinstance(detalg, str) else int(detalg))
 
 
 def mfma_probe(A, B):

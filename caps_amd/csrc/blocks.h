@@ -43,6 +43,10 @@ struct BlockDesc {
     int64_t off_mat;    // lda * N doubles (column-major)
     int64_t off_diag;   // N doubles: diagonal of M
     int64_t off_piv;    // N ints: LU pivots
+    int64_t off_cpart;  // n_rtiles x N doubles: per row tile, the column sums
+                        // of |M_ij| over i != j (written with the matrix)
+    int n_rtiles;       // row tiles of off_cpart (0: none)
+    int pad2_;
 };
 
 // Blocks whose I tables share the coefficients a_q: same m, ℓ-window and
@@ -76,6 +80,13 @@ struct BlockOut {
     double tr2;         // tr M^2 (LU mode only)
     int skip;           // LU skipped (trace shortcut)
     int status;         // 0 ok; nonzero: numerical failure flag
+    int delta;          // 1: the matrix holds -M (1 - M without its unit
+                        // diagonal; 1 - M is column diagonally dominant, so
+                        // partial pivoting exchanges no rows) and the LU
+                        // runs without pivot search on u = 1 + d, summing
+                        // log1p(d); 0: the matrix holds 1 - M (or a user
+                        // matrix), LU with partial pivoting
+    int pad_;
 };
 
 }  // namespace capsd

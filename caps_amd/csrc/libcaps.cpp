@@ -40,10 +40,12 @@ void capsd_launch_I(const BlockDesc*, char*, const void*, const int*, const doub
                     int, int, hipStream_t);
 void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int, const BlockOut*, int, hipStream_t);
 void capsd_launch_trace(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
-void capsd_launch_frobenius(const BlockDesc*, char*, BlockOut*, int, hipStream_t);
+void capsd_launch_dominance(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
+void capsd_launch_frobenius(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
 void capsd_launch_fill_xi0(const BlockDesc*, char*, int64_t, int, hipStream_t);
 void capsd_launch_mie_points(int, const double*, const int*, double, const double*, double*, hipStream_t);
 void capsd_launch_fresnel_points(int, const double*, const double*, const double*, double*, hipStream_t);
+void capsd_launch_bessel_points(int, const int*, const double*, const double*, double*, hipStream_t);
 int capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t, int, int, int,
                     hipEvent_t*, int*);
 void capsd_launch_lu_fill(const BlockDesc*, char*, int, unsigned long long, hipStream_t);
@@ -125,6 +127,9 @@ struct Ctx {
     LuSide* h_side = nullptr;
     BlockOut* h_out = nullptr;
     int* h_status = nullptr;
+    // staging buffer of the point APIs (capsx_*_points), grown on demand
+    char* pts = nullptr;
+    size_t pts_bytes = 0;
     double timing[6] = {0, 0, 0, 0, 0, 0};
     // accumulated since capsx_stats_reset: stage ms [0..5], blocks, blocks
     // factored (not skipped), algorithmic LU flops, batches, I pairs, and
@@ -183,6 +188,7 @@ void ctx_free(Ctx* c)
     if (c->h_status) hipHostFree(c->h_status);
     if (c->d_groups) hipFree(c->d_groups);
     if (c->h_groups) hipHostFree(c->h_groups);
+    if (c->pts) hipFree(c->pts);
     for (auto& e : c->ev) hipEventDestroy(e);
     for (auto& e : c->gemm_ev) hipEventDestroy(e);
     for (auto& g : c->graphs) hipGraphExecDestroy(g.second);
@@ -233,6 +239,20 @@ void ensure_arena(Ctx* c, size_t bytes)
 }
 
 inline int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
+
+// The point APIs' device staging buffer, at least `bytes` long.
+char* ensure_pts(Ctx* c, size_t bytes)
+{
+    if (bytes > c->pts_bytes) {
+        HIPCHECK(hipStreamSynchronize(c->stream));
+        if (c->pts) HIPCHECK(hipFree(c->pts));
+        c->pts = nullptr;
+        const size_t sz = std::max(bytes, (size_t)(c->pts_bytes * 1.5));
+        HIPCHECK(hipMalloc(&c->pts, sz));
+        c->pts_bytes = sz;
+    }
+    return c->pts;
+}
 
 // LU side buffers of one N x N block (leading dimension lda) from arena
 // offset off: the factored panels of a step, log|u_ii|, the pivots of each
@@ -497,6 +517,11 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         off = align256(off + (int64_t)N * 8);
         d.off_piv = off;
         off = align256(off + (int64_t)N * 4);
+        // column sums of |M| per 32-row tile of the assembly (k_assemble),
+        // for the column dominance test of the LU's delta form
+        d.n_rtiles = with_lu ? (ldim + 15) / 16 : 0;
+        d.off_cpart = off;
+        off = align256(off + (int64_t)d.n_rtiles * N * 8);
         LuSide& s = c->h_side[b];
         if (with_lu) {
             off = lu_side_layout(s, off, N, lda);
@@ -551,7 +576,7 @@ size_t block_bytes_estimate(caps_t* self, int m)
     caps_estimate_lminmax(self, m, &lmin, &lmax);
     const size_t nK = 2 * (lmin + ldim) + 4;
     return (size_t)2 * (ldim + 2) * (ldim + 2) * 12 + lda * N * 8 + lda * 512 * 8 + nK * (24 * 2 + 24 * 64) +
-           (size_t)N * 32 + 65536;
+           (size_t)N * 32 + (size_t)((ldim + 15) / 16) * N * 8 + 65536;
 }
 
 // The LU of blocks [0, n) (descriptors already on the device) after event
@@ -623,7 +648,9 @@ int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, b
         if (events && last) HIPCHECK(hipEventRecord(c->ev[3], sg));
         capsd_launch_assemble(c->d_desc + b0, c->arena, g_lfac_dev, nb, p.max_ldim, mode, c->d_out + b0,
                               mode == 0 ? 2 : 0, sg);
-        if (mode == 0 && lu_mode_shortcuts) capsd_launch_frobenius(c->d_desc + b0, c->arena, c->d_out + b0, nb, sg);
+        if (mode == 0) capsd_launch_dominance(c->d_desc + b0, c->arena, c->d_out + b0, nb, p.maxN, sg);
+        if (mode == 0 && lu_mode_shortcuts)
+            capsd_launch_frobenius(c->d_desc + b0, c->arena, c->d_out + b0, nb, lu_mode_shortcuts == 2, sg);
         if (events && last) HIPCHECK(hipEventRecord(c->ev[4], sg));
         if (mode != 0) continue;
         const bool prof = c->prof && p.parts == 1;
@@ -651,6 +678,7 @@ std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_sh
         int64_t max_ntri;
         int parts, part_b[MAX_PARTS + 1], part_goff[MAX_PARTS], cls[MAX_PARTS][8];
         int levels, streams, xcd, tile;
+        int lu_kernels[3], grid_panel_rows;   // LU kernel variants (capsx_lu_kernels, capsx_lu_grid_panel_rows)
         const void *arena, *desc, *side, *out, *groups;
     } k;
     memset(&k, 0, sizeof(k));
@@ -670,6 +698,8 @@ std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_sh
     k.streams = cfg.streams;
     k.xcd = cfg.xcd;
     k.tile = cfg.tile;
+    capsd_lu_kernels(-1, -1, -1, k.lu_kernels);
+    k.grid_panel_rows = capsd_lu_grid_panel_rows(0);
     k.arena = c->arena;
     k.desc = c->d_desc;
     k.side = c->d_side;
@@ -813,6 +843,7 @@ const char* dump_path(detalg_t detalg)
 double finish_block(const BlockOut& o, int lu_mode)
 {
     const double tr = o.trace;
+    if (o.skip == 3) return NAN;   // non-finite trace: the LU did not run
     if (fabs(tr) < 1e-8) return -tr;
     if (lu_mode && o.skip == 2) return -tr - o.tr2 / 2;
     const double ld = o.logdet;
@@ -820,7 +851,13 @@ double finish_block(const BlockOut& o, int lu_mode)
     return ld;
 }
 
-bool is_lu_mode(detalg_t a) { return a == DETALG_LU || a == DETALG_QR; }
+// Dense determinant mode (every detalg but HODLR: the reference's
+// kernel_logdet takes its dense branch with the Mercator shortcut for LU, QR
+// and CHOLESKY, src/matrix.c:84-113, 489-505).  Returns 0 for HODLR, 1 for LU
+// and QR, 2 for CHOLESKY (whose dense matrix the reference fills below the
+// diagonal only, so its Mercator test sees the lower triangle).
+int dense_mode(detalg_t a) { return a == DETALG_HODLR ? 0 : a == DETALG_CHOLESKY ? 2 : 1; }
+bool is_lu_mode(detalg_t a) { return dense_mode(a) != 0; }
 
 // Largest number of blocks per launch sequence that fits the HBM budget.
 int chunk_blocks_multi(caps_t* const* objs, const int* m, int n)
@@ -1070,24 +1107,81 @@ int caps_estimate_lminmax(caps_t* self, int m, size_t* lmin_p, size_t* lmax_p)
     return l;
 }
 
-void caps_mie(caps_t* self, double xi_, int l, double* lna, double* lnb)
+// Extension: log|a_l|, log|b_l| of the sphere at n points (xi_[i], l[i]) in
+// one launch (ε(iξ) - 1 of the sphere evaluated on the host per point).  As
+// the reference's caps_mie, no argument checks (xi_ <= 0 or l < 1 give NaN
+// or meaningless values).  Returns 0 on success (1: n < 0).
+int capsx_mie_points(caps_t* self, int n, const double* xi_, const int* l, double* lna, double* lnb)
 {
+    if (n < 0) return 1;
+    if (n == 0) return 0;
     Ctx* c = ctx_of(self);
     std::lock_guard<std::mutex> lock(c->mu);
-    const double eps = caps_epsilonm1_sphere(self, xi_);
-    double* d;
-    HIPCHECK(hipMalloc(&d, 6 * sizeof(double)));
-    double h[3] = {xi_, eps, 0};
-    int li = l;
-    HIPCHECK(hipMemcpyAsync(d, h, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipMemcpyAsync(d + 2, &li, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    capsd_launch_mie_points(1, d, (const int*)(d + 2), self->LbyR, d + 1, d + 4, c->stream);
-    double r[2];
-    HIPCHECK(hipMemcpyAsync(r, d + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> h(2 * (size_t)n);   // xi_, eps
+    double last_xi = NAN, eps = 0;
+    for (int i = 0; i < n; i++) {
+        if (!(xi_[i] == last_xi)) {
+            eps = caps_epsilonm1_sphere(self, xi_[i]);
+            last_xi = xi_[i];
+        }
+        h[i] = xi_[i];
+        h[n + i] = eps;
+    }
+    const size_t o_l = align256(16 * (size_t)n), o_out = o_l + align256(4 * (size_t)n);
+    char* d = ensure_pts(c, o_out + 16 * (size_t)n);
+    std::vector<double> r(2 * (size_t)n);
+    HIPCHECK(hipMemcpyAsync(d, h.data(), 16 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(d + o_l, l, 4 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    capsd_launch_mie_points(n, (const double*)d, (const int*)(d + o_l), self->LbyR, (const double*)d + n,
+                            (double*)(d + o_out), c->stream);
+    HIPCHECK(hipMemcpyAsync(r.data(), d + o_out, 16 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
-    HIPCHECK(hipFree(d));
-    *lna = r[0];
-    *lnb = r[1];
+    HIPCHECK(hipGetLastError());
+    for (int i = 0; i < n; i++) {
+        lna[i] = r[2 * i];
+        lnb[i] = r[2 * i + 1];
+    }
+    return 0;
+}
+
+// Extension: Fresnel coefficients r_TE, r_TM of the plate at n points
+// (xi_[i], k_[i]) in one launch.  Returns 0 on success (1: n < 0).
+int capsx_fresnel_points(caps_t* self, int n, const double* xi_, const double* k_, double* r_TE, double* r_TM)
+{
+    if (n < 0) return 1;
+    if (n == 0) return 0;
+    Ctx* c = ctx_of(self);
+    std::lock_guard<std::mutex> lock(c->mu);
+    std::vector<double> h(3 * (size_t)n);   // xi_, k_, eps
+    double last_xi = NAN, eps = 0;
+    for (int i = 0; i < n; i++) {
+        if (!(xi_[i] == last_xi)) {
+            eps = caps_epsilonm1_plate(self, xi_[i]);
+            last_xi = xi_[i];
+        }
+        h[i] = xi_[i];
+        h[n + i] = k_[i];
+        h[2 * (size_t)n + i] = eps;
+    }
+    const size_t o_out = align256(24 * (size_t)n);
+    char* d = ensure_pts(c, o_out + 16 * (size_t)n);
+    std::vector<double> r(2 * (size_t)n);
+    const double* dd = (const double*)d;
+    HIPCHECK(hipMemcpyAsync(d, h.data(), 24 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    capsd_launch_fresnel_points(n, dd, dd + n, dd + 2 * (size_t)n, (double*)(d + o_out), c->stream);
+    HIPCHECK(hipMemcpyAsync(r.data(), d + o_out, 16 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipGetLastError());
+    for (int i = 0; i < n; i++) {
+        r_TE[i] = r[2 * i];
+        r_TM[i] = r[2 * i + 1];
+    }
+    return 0;
+}
+
+void caps_mie(caps_t* self, double xi_, int l, double* lna, double* lnb)
+{
+    capsx_mie_points(self, 1, &xi_, &l, lna, lnb);
 }
 
 void caps_mie_perf(caps_t* self, double xi_, int l, double* lna, double* lnb)
@@ -1102,26 +1196,54 @@ void caps_mie_perf(caps_t* self, double xi_, int l, double* lna, double* lnb)
 
 void caps_fresnel(caps_t* self, double xi_, double k_, double* r_TE, double* r_TM)
 {
-    Ctx* c = ctx_of(self);
-    std::lock_guard<std::mutex> lock(c->mu);
-    const double eps = caps_epsilonm1_plate(self, xi_);
-    double* d;
-    HIPCHECK(hipMalloc(&d, 6 * sizeof(double)));
-    double h[3] = {xi_, k_, eps};
-    HIPCHECK(hipMemcpyAsync(d, h, 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    capsd_launch_fresnel_points(1, d, d + 1, d + 2, d + 4, c->stream);
-    double r[2];
-    HIPCHECK(hipMemcpyAsync(r, d + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    capsx_fresnel_points(self, 1, &xi_, &k_, r_TE, r_TM);
+}
+
+// Extension: log I_{n+1/2}(x), log K_{n+1/2}(x) (order[i] = n >= 0; NaN for
+// order < 0) and I_ν(x)/I_{ν+1}(x) (nu[i] >= 0; NaN for nu < 0) at n points,
+// the device functions of the Mie coefficients (src/bessel.c:635-660,
+// 781-833), in one launch.  Returns 0 on success.
+int capsx_bessel_points(int n, const int* order, const double* nu, const double* x, double* logI, double* logK,
+                        double* ratio)
+{
+    if (n < 0) return 1;
+    for (int i = 0; i < n; i++)
+        if (!(x[i] > 0)) return 1;
+    if (n == 0) return 0;
+    caps_t* tmp = caps_init(1, 1);
+    Ctx* c = ctx_of(tmp);
+    const size_t o_nu = align256(4 * (size_t)n), o_x = o_nu + align256(8 * (size_t)n),
+                 o_out = o_x + align256(8 * (size_t)n);
+    char* d = ensure_pts(c, o_out + 24 * (size_t)n);
+    std::vector<double> r(3 * (size_t)n);
+    HIPCHECK(hipMemcpyAsync(d, order, 4 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(d + o_nu, nu, 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(d + o_x, x, 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    capsd_launch_bessel_points(n, (const int*)d, (const double*)(d + o_nu), (const double*)(d + o_x),
+                               (double*)(d + o_out), c->stream);
+    HIPCHECK(hipMemcpyAsync(r.data(), d + o_out, 24 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
-    HIPCHECK(hipFree(d));
-    *r_TE = r[0];
-    *r_TM = r[1];
+    HIPCHECK(hipGetLastError());
+    caps_free(tmp);
+    for (int i = 0; i < n; i++) {
+        if (logI) logI[i] = r[3 * i];
+        if (logK) logK[i] = r[3 * i + 1];
+        if (ratio) ratio[i] = r[3 * i + 2];
+    }
+    return 0;
 }
 
 // Extension: n blocks, block i of caps object objs[i] (objects may differ in
 // R/L, ldim, material), in one batched launch sequence on objs[0]'s device
-// context.  All objects must use the same detalg.  Returns 0 on success.
-int capsx_logdetD_multi(caps_t* const* objs, const double* xi_, const int* m, int n, double* out)
+// context.  All objects must use the same detalg.  With trace / flags
+// non-NULL, also each block's tr M and how its value was obtained: flags
+// bits 0-1 the shortcut (0 LU, 1 trace |tr M| < 1e-8, 2 LU-mode Mercator,
+// 3 non-finite trace), bit 2 set when the LU ran in delta form (1 - M
+// column diagonally dominant: no pivot search, log1p of u_ii - 1), bit 3
+// set when the HODLR-mode rule |tr M| > |log det| replaced the LU value by
+// -tr M.  Returns 0 on success.
+int capsx_logdetD_multi_info(caps_t* const* objs, const double* xi_, const int* m, int n, double* out, double* trace,
+                             int* flags)
 {
     if (n <= 0) return 0;
     for (int i = 0; i < n; i++) {
@@ -1138,19 +1260,33 @@ int capsx_logdetD_multi(caps_t* const* objs, const double* xi_, const int* m, in
     }
     Ctx* c = ctx_of(objs[0]);
     std::lock_guard<std::mutex> lock(c->mu);
-    const int lu_mode = is_lu_mode(objs[0]->detalg) ? 1 : 0;
+    const int lu_mode = dense_mode(objs[0]->detalg);
     double tot[6] = {0, 0, 0, 0, 0, 0};
     for (int b0 = 0; b0 < n;) {
         const int k = chunk_blocks_multi(objs + b0, m + b0, n - b0);
         ensure_blocks(c, k);
         BatchPlan p = plan_batch_multi(objs + b0, c, xi_ + b0, m + b0, k, 1);
         run_batch(objs[0], c, p, 0, lu_mode);
-        for (int i = 0; i < k; i++) out[b0 + i] = finish_block(c->h_out[i], lu_mode);
+        for (int i = 0; i < k; i++) {
+            const BlockOut& o = c->h_out[i];
+            out[b0 + i] = finish_block(o, lu_mode);
+            if (trace) trace[b0 + i] = o.trace;
+            if (flags) {
+                const bool lu_ran = o.skip == 0;
+                flags[b0 + i] = (o.skip & 3) | (lu_ran && o.delta ? 4 : 0) |
+                                (lu_ran && !lu_mode && fabs(o.trace) > fabs(o.logdet) ? 8 : 0);
+            }
+        }
         for (int i = 0; i < 6; i++) tot[i] += c->timing[i];
         b0 += k;
     }
     memcpy(c->timing, tot, sizeof(tot));
     return 0;
+}
+
+int capsx_logdetD_multi(caps_t* const* objs, const double* xi_, const int* m, int n, double* out)
+{
+    return capsx_logdetD_multi_info(objs, xi_, m, n, out, nullptr, nullptr);
 }
 
 int caps_logdetD_batch(caps_t* self, const double* xi_, const int* m, int n, double* out)
@@ -1270,11 +1406,14 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     }
     if (fabs(tr) < 1e-8) return -tr;
     const int lda = (dim + 15) & ~15;
+    const int dmode = dense_mode(detalg);
     std::vector<double> A((size_t)lda * dim, 0.0);
     for (int i = 0; i < dim; i++) A[i + (size_t)i * lda] = diag[i];
+    // by minor diagonals as src/matrix.c:93-102; CHOLESKY fills the lower
+    // triangle only (the reference's Cholesky reads only that half)
     for (int md = 1; md < dim; md++)
         for (int k = 0; k < dim - md; k++) {
-            A[k + (size_t)(md + k) * lda] = kernel(k, md + k, args);
+            if (dmode != 2) A[k + (size_t)(md + k) * lda] = kernel(k, md + k, args);
             A[(md + k) + (size_t)k * lda] = kernel(md + k, k, args);
         }
     if (const char* path = dump_path(detalg))
@@ -1304,8 +1443,24 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
             if (fabs(err / mercator) < 1e-8) return mercator;
         }
     }
+    // the symmetric matrix the reference's Cholesky factors (from the lower
+    // triangle) for the LU
+    if (dmode == 2)
+        for (int j = 1; j < dim; j++)
+            for (int i = 0; i < j; i++) A[i + (size_t)j * lda] = A[j + (size_t)i * lda];
+    // -M, and 1 - M (the unit diagonal back) unless 1 - M is column
+    // diagonally dominant (then the LU's delta form: no pivot search,
+    // log1p of the pivots' deviations from 1)
     for (size_t i = 0; i < A.size(); i++) A[i] = -A[i];
-    for (int i = 0; i < dim; i++) A[i + (size_t)i * lda] += 1;
+    bool dominant = true;
+    for (int j = 0; j < dim && dominant; j++) {
+        double sj = 0;
+        for (int i = 0; i < dim; i++)
+            if (i != j) sj += fabs(A[i + (size_t)j * lda]);
+        dominant = sj < fabs(1.0 + A[j + (size_t)j * lda]);
+    }
+    if (!dominant)
+        for (int i = 0; i < dim; i++) A[i + (size_t)i * lda] += 1;
     // one-block batch with a user matrix
     caps_t* tmp = caps_init(1, 1);
     Ctx* c = ctx_of(tmp);
@@ -1323,6 +1478,7 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     ensure_arena(c, off);
     BlockOut o;
     memset(&o, 0, sizeof(o));
+    o.delta = dominant ? 1 : 0;
     HIPCHECK(hipMemcpyAsync(c->d_desc, &d, sizeof(BlockDesc), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->d_side, &s, sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->d_out, &o, sizeof(BlockOut), hipMemcpyHostToDevice, c->stream));
@@ -1406,6 +1562,9 @@ double sep_logdet(caps_t* self, const std::vector<double>& h, const std::vector<
     off = align256(off + (int64_t)lda * n * 8);
     d.off_diag = off;
     off = align256(off + (int64_t)n * 8);
+    d.n_rtiles = (n + 63) / 64;   // k_fill_xi0's 64-row tiles
+    d.off_cpart = off;
+    off = align256(off + (int64_t)d.n_rtiles * n * 8);
     const int64_t off_tab = off;
     off = align256(off + (int64_t)(3 * n - 1) * 8);
     LuSide& sd = c->h_side[0];
@@ -1417,9 +1576,10 @@ double sep_logdet(caps_t* self, const std::vector<double>& h, const std::vector<
     HIPCHECK(hipMemcpyAsync(c->d_side, &sd, sizeof(LuSide), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemsetAsync(c->d_out, 0, sizeof(BlockOut), s));
     HIPCHECK(hipMemcpyAsync(c->arena + off_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, s));
-    const int lu_mode = is_lu_mode(self->detalg);
+    const int lu_mode = dense_mode(self->detalg);
     capsd_launch_fill_xi0(c->d_desc, c->arena, off_tab, n, s);
     capsd_launch_trace(c->d_desc, c->arena, c->d_out, 1, lu_mode, s);
+    capsd_launch_dominance(c->d_desc, c->arena, c->d_out, 1, n, s);
     if (const char* path = dump_path(self->detalg)) {
         BlockOut o0;
         HIPCHECK(hipMemcpyAsync(&o0, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, s));
@@ -1427,7 +1587,7 @@ double sep_logdet(caps_t* self, const std::vector<double>& h, const std::vector<
         HIPCHECK(hipMemcpyAsync(A.data(), c->arena + d.off_mat, A.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipMemcpyAsync(dg.data(), c->arena + d.off_diag, n * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
-        if (fabs(o0.trace) >= 1e-8)   // M = 1 - A off the diagonal, the exact diagonal of M on it
+        if (fabs(o0.trace) >= 1e-8)   // M = -A off the diagonal, the exact diagonal of M on it
             TERMINATE(!npy_write(path, n, [&](int i, int j) { return i == j ? dg[i] : -A[i + (size_t)j * lda]; }),
                       "cannot write %s", path);
     }
@@ -2300,6 +2460,71 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
     float ms;
     HIPCHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     for (int b = 0; b < n; b++) out[b] = o[b].logdet;
+    caps_free(tmp);
+    return 0;
+}
+
+// Extension (test hook): log|det(1 - M)| of n column-major dim x dim
+// matrices M (lda = dim) through the device's column dominance test and LU:
+// -M is uploaded with the off-diagonal column sums of |M|, k_dominance
+// decides the form (delta[b] = 1: no pivot search, log1p of u_ii - 1; 0: the
+// unit diagonal written back, partial pivoting).  No trace shortcut.
+// Returns 0 on success.
+int capsx_logdet_1mM_batch(const double* M, int dim, int n, double* out, int* delta)
+{
+    if (dim <= 0 || dim > capsd_lu_max_dim() || n <= 0) return 1;
+    caps_t* tmp = caps_init(1, 1);
+    Ctx* c = ctx_of(tmp);
+    ensure_blocks(c, n);
+    const int lda = (dim + 15) & ~15;
+    int64_t off = 0;
+    for (int b = 0; b < n; b++) {
+        BlockDesc& d = c->h_desc[b];
+        memset(&d, 0, sizeof(d));
+        d.N = dim;
+        d.ldim = (dim + 1) / 2;
+        d.lda = lda;
+        d.off_mat = off;
+        off = align256(off + (int64_t)lda * dim * 8);
+        d.n_rtiles = 1;
+        d.off_cpart = off;
+        off = align256(off + (int64_t)dim * 8);
+        LuSide& s = c->h_side[b];
+        off = lu_side_layout(s, off, dim, lda);
+    }
+    ensure_arena(c, off);
+    std::vector<BlockOut> o(n);
+    memset(o.data(), 0, n * sizeof(BlockOut));
+    for (int b = 0; b < n; b++) o[b].delta = 1;
+    HIPCHECK(hipMemcpyAsync(c->d_desc, c->h_desc, n * sizeof(BlockDesc), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->d_side, c->h_side, n * sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->d_out, o.data(), n * sizeof(BlockOut), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> A((size_t)lda * dim, 0.0), cs(dim);
+    for (int b = 0; b < n; b++) {
+        const double* Mb = M + (size_t)b * dim * dim;
+        for (int j = 0; j < dim; j++) {
+            double sj = 0;
+            for (int i = 0; i < dim; i++) {
+                const double v = Mb[i + (size_t)j * dim];
+                A[i + (size_t)j * lda] = -v;
+                if (i != j) sj += fabs(v);
+            }
+            cs[j] = sj;
+        }
+        const BlockDesc& d = c->h_desc[b];
+        HIPCHECK(hipMemcpy(c->arena + d.off_mat, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(c->arena + d.off_cpart, cs.data(), dim * sizeof(double), hipMemcpyHostToDevice));
+    }
+    capsd_launch_dominance(c->d_desc, c->arena, c->d_out, n, dim, c->stream);
+    HIPCHECK(hipEventRecord(c->ev[0], c->stream));
+    launch_lu_groups(c, n, dim, lu_config(dim, n), c->ev[0], false);
+    HIPCHECK(hipMemcpyAsync(o.data(), c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipGetLastError());
+    for (int b = 0; b < n; b++) {
+        out[b] = o[b].logdet;
+        if (delta) delta[b] = o[b].delta;
+    }
     caps_free(tmp);
     return 0;
 }

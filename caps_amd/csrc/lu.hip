@@ -72,6 +72,10 @@ __device__ __forceinline__ int* moved_buf(char* arena, const LuSide& sd, int pb)
     return at<int>(arena, sd.off_moved) + pb * NB;
 }
 
+// log|1 + d| of a delta-form pivot u = 1 + d: log1p keeps the relative
+// accuracy of d (a pivot near 1 stored as 1 + d loses d below one ulp of 1).
+__device__ __forceinline__ double log_abs_1p(double d) { return d > -0.5 ? log1p(d) : log(fabs(1.0 + d)); }
+
 // Largest v over the wavefront, lowest idx among equal values (a total
 // order, so the result does not depend on the reduction tree), in every
 // lane: within rows of 16 by DPP (quad exchanges 1 and 2, half-row and row
@@ -126,13 +130,17 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
     double* __restrict__ logs = at<double>(arena, side[b].off_logs);
     int* __restrict__ ipiv = moved_buf(arena, side[b], pb);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    // delta form (uniform per block): the matrix holds 1 - M without its
+    // unit diagonal and is column diagonally dominant, so the pivot of
+    // column c is row c (no search, no exchanges), u = 1 + d
+    const bool delta = out[b].delta != 0;
 
     extern __shared__ int perm[];   // R * NT
     __shared__ double red_v2[2][NT / 64];
     __shared__ int red_i2[2][NT / 64];
     __shared__ double prow2[2][LF], xrow2[2][LF];
     __shared__ double Uc[2][CW * LF];   // solved U rows of one chunk, [row][col]
-    __shared__ double pivv[NB];         // |pivot| per panel column
+    __shared__ double pivv[NB];         // |pivot| per panel column (delta form: d of u = 1 + d)
 
     for (int r = tid; r < np; r += NT) perm[r] = r;
     __syncthreads();
@@ -206,30 +214,41 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
             int* red_i = red_i2[c & 1];
             double* prow = prow2[c & 1];
             double* xrow = xrow2[c & 1];
-            double bv = -1.0;
-            int bi = 0x7fffffff;
+            double gv = 0.0;
+            int p = c;
+            if (!delta) {
+                double bv = -1.0;
+                int bi = 0x7fffffff;
 #pragma unroll
-            for (int i = 0; i < R; i++) {
-                const int r = tid + i * NT;
-                const double v = fabs(a[i][j]);
-                if (r >= c && r < np && (v > bv || (v == bv && r < bi))) {
-                    bv = v;
-                    bi = r;
+                for (int i = 0; i < R; i++) {
+                    const int r = tid + i * NT;
+                    const double v = fabs(a[i][j]);
+                    if (r >= c && r < np && (v > bv || (v == bv && r < bi))) {
+                        bv = v;
+                        bi = r;
+                    }
+                }
+                wave_argmax(bv, bi);
+                if (lane == 0) {
+                    red_v[wid] = bv;
+                    red_i[wid] = bi;
+                }
+                __syncthreads();
+                gv = red_v[0];
+                p = red_i[0];
+                for (int q = 1; q < NT / 64; q++)
+                    if (red_v[q] > gv || (red_v[q] == gv && red_i[q] < p)) {
+                        gv = red_v[q];
+                        p = red_i[q];
+                    }
+                // no finite candidate (a NaN column): keep row c, so the row
+                // map and ipiv stay in range; the NaN reaches log|u| and the
+                // determinant
+                if (p >= np) {
+                    p = c;
+                    gv = NAN;
                 }
             }
-            wave_argmax(bv, bi);
-            if (lane == 0) {
-                red_v[wid] = bv;
-                red_i[wid] = bi;
-            }
-            __syncthreads();
-            double gv = red_v[0];
-            int p = red_i[0];
-            for (int q = 1; q < NT / 64; q++)
-                if (red_v[q] > gv || (red_v[q] == gv && red_i[q] < p)) {
-                    gv = red_v[q];
-                    p = red_i[q];
-                }
             // owners of rows p and c publish them
             const int own_p = p % NT, own_c = c % NT;
             if (tid == own_p) {
@@ -254,20 +273,23 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
                     perm[c] = perm[p];
                     perm[p] = t;
                 }
-                pivv[c] = gv;
+                if (!delta) pivv[c] = gv;
                 ipiv[c] = p;
             }
             __syncthreads();
-            const double u = prow[j];
+            const double u = delta ? 1.0 + prow[j] : prow[j];
+            if (delta && tid == 0) pivv[c] = prow[j];
+            if (p != c) {
 #pragma unroll
-            for (int i = 0; i < R; i++) {
-                const int r = tid + i * NT;
-                if (r == c) {
+                for (int i = 0; i < R; i++) {
+                    const int r = tid + i * NT;
+                    if (r == c) {
 #pragma unroll
-                    for (int jj = 0; jj < LF; jj++) a[i][jj] = prow[jj];
-                } else if (r == p) {
+                        for (int jj = 0; jj < LF; jj++) a[i][jj] = prow[jj];
+                    } else if (r == p) {
 #pragma unroll
-                    for (int jj = 0; jj < LF; jj++) a[i][jj] = xrow[jj];
+                        for (int jj = 0; jj < LF; jj++) a[i][jj] = xrow[jj];
+                    }
                 }
             }
             // l = a / u as a product with 1/u, as LAPACK's dgetf2 scales the
@@ -301,7 +323,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
         __syncthreads();
     }
     // log|u_ii| and the panel in pivoted (logical) order
-    if (tid < nbk) logs[k + tid] = log(pivv[tid]);
+    if (tid < nbk) logs[k + tid] = delta ? log_abs_1p(pivv[tid]) : log(pivv[tid]);
     for (int r = tid; r < np; r += NT) {
         const int64_t pr = k + perm[r];
 #pragma unroll
@@ -356,7 +378,7 @@ __global__ void __launch_bounds__(GP_ROWS) k_gp_max(const BlockDesc* __restrict_
     const int b = blockIdx.y;
     const BlockDesc& d = desc[b];
     const int N = d.N;
-    if (out[b].skip || k >= N || k + (int)blockIdx.x * GP_ROWS >= N) return;
+    if (out[b].skip || out[b].delta || k >= N || k + (int)blockIdx.x * GP_ROWS >= N) return;
     const int64_t lda = d.lda;
     const double* A = at<double>(arena, d.off_mat);
     const int r = k + blockIdx.x * GP_ROWS + threadIdx.x;
@@ -365,7 +387,8 @@ __global__ void __launch_bounds__(GP_ROWS) k_gp_max(const BlockDesc* __restrict_
 }
 
 // pivot of column c: reduce the candidates, exchange rows k+c and p within
-// the panel, record ipiv and log|u_cc|
+// the panel, record ipiv and log|u_cc|; delta form: the pivot is row k+c
+// (u = 1 + d), no exchange
 __global__ void __launch_bounds__(64) k_gp_pivot(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                  const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
                                                  int k, int pb, int c)
@@ -378,6 +401,13 @@ __global__ void __launch_bounds__(64) k_gp_pivot(const BlockDesc* __restrict__ d
     if (c >= nbk) return;
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
+    if (out[b].delta) {
+        if (threadIdx.x == 0) {
+            moved_buf(arena, side[b], pb)[c] = c;
+            at<double>(arena, side[b].off_logs)[k + c] = log_abs_1p(A[(k + c) + (int64_t)(k + c) * lda]);
+        }
+        return;
+    }
     const double* sc = panel_buf(arena, side[b], lda, pb);
     const int nch = (N - k + GP_ROWS - 1) / GP_ROWS;
     double v = -1.0;
@@ -391,6 +421,12 @@ __global__ void __launch_bounds__(64) k_gp_pivot(const BlockDesc* __restrict__ d
         }
     }
     wave_argmax(v, p);
+    // no finite candidate (a NaN column): keep row k+c; the NaN reaches
+    // log|u| and the determinant
+    if (p >= N) {
+        p = k + c;
+        v = NAN;
+    }
     const int j = threadIdx.x;
     if (j < nbk && p != k + c) {
         double* col = A + (int64_t)(k + j) * lda;
@@ -420,9 +456,10 @@ __global__ void __launch_bounds__(GP_ROWS) k_gp_update(const BlockDesc* __restri
     double* A = at<double>(arena, d.off_mat);
     const int r = k + blockIdx.x * GP_ROWS + threadIdx.x;
     const int pr = k + c;
+    const bool delta = out[b].delta != 0;   // uniform per block
     double vnext = -1.0;
     if (r > pr && r < N) {
-        const double u = A[pr + (int64_t)pr * lda];
+        const double u = A[pr + (int64_t)pr * lda] + (delta ? 1.0 : 0.0);
         const double l = A[r + (int64_t)pr * lda] / u;
         A[r + (int64_t)pr * lda] = l;
         for (int j = c + 1; j < nbk; j++) {
@@ -432,7 +469,7 @@ __global__ void __launch_bounds__(GP_ROWS) k_gp_update(const BlockDesc* __restri
             if (j == c + 1) vnext = fabs(x);
         }
     }
-    if (c + 1 < nbk)
+    if (c + 1 < nbk && !delta)
         block_argmax_store(vnext, (r > pr && r < N) ? r : 0x7fffffff, panel_buf(arena, side[b], lda, pb),
                            blockIdx.x);
 }
@@ -511,8 +548,9 @@ __global__ void __launch_bounds__(NTH) k_lu_swap_trsm(const BlockDesc* __restric
     double* Ac = at<double>(arena, d.off_mat) + (int64_t)col * lda;
     // the pivots of the panel at row k + q NB cover min(NB, N - k - q NB)
     // columns
-    for (int q = 0; q < NPANEL; q++)
-        if ((swaps >> q) & 1) swap_rows_ipiv(Ac, moved_buf(arena, side[b], q), min(NB, N - k - q * NB), k + q * NB);
+    if (!out[b].delta)
+        for (int q = 0; q < NPANEL; q++)
+            if ((swaps >> q) & 1) swap_rows_ipiv(Ac, moved_buf(arena, side[b], q), min(NB, N - k - q * NB), k + q * NB);
     double x[NB];
 #pragma unroll
     for (int i = 0; i < NB; i++) x[i] = Ac[r0 + i];
@@ -543,7 +581,7 @@ __global__ void __launch_bounds__(64 * (NPANEL - 1))
     const int b = blockIdx.x;
     const BlockDesc& d = desc[b];
     const int q = qbegin + (threadIdx.x >> 6);
-    if (out[b].skip) return;
+    if (out[b].skip || out[b].delta) return;   // delta form: no interchanges
     double* Pq = panel_buf(arena, side[b], d.lda, q) + (int64_t)(threadIdx.x & 63) * d.lda;
     for (int p = max(pfirst, q + 1); p <= plast; p++)
         if (k + p * NB < d.N) swap_rows_ipiv(Pq, moved_buf(arena, side[b], p), min(NB, d.N - k - p * NB), (p - q) * NB);

@@ -92,6 +92,22 @@ __global__ void k_fresnel_points(int n, const double* __restrict__ xi_, const do
     out[2 * i + 1] = tm;
 }
 
+// log I_{n+1/2}(x), log K_{n+1/2}(x) and I_ν(x)/I_{ν+1}(x) at arbitrary
+// points (capsx_bessel_points): out[3i..3i+2].  order[i] < 0 skips the
+// half-integer pair (NaN), nu[i] < 0 the ratio.
+__global__ void k_bessel_points(int n, const int* __restrict__ order, const double* __restrict__ nu,
+                                const double* __restrict__ x, double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double lI = NAN, lK = NAN, r = NAN;
+    if (order[i] >= 0) logInKn_half(order[i], x[i], lI, lK);
+    if (nu[i] >= 0) r = ratioI(nu[i], x[i]);
+    out[3 * i] = lI;
+    out[3 * i + 1] = lK;
+    out[3 * i + 2] = r;
+}
+
 // ---------------------------------------------------------------------------
 // K_ν for all ν of a block.
 //
@@ -560,6 +576,11 @@ __device__ __forceinline__ UV uv(int l, int m)
 // flags bit 0: the trace pass: only the diagonal tiles (blockIdx.y unused)
 // and only the diagonal of M (from I pairs with |l1 - l2| <= 2); bit 1:
 // leave out the blocks with out[b].skip set.
+// mode 0 (LU path) writes -M, i.e. 1 - M without its unit diagonal (the LU's
+// delta form; k_dominance puts the unit diagonal back where the LU needs
+// partial pivoting), and for every column of the workgroup's 32 rows the sum
+// of |M_ij| over its rows i != j into off_cpart (row tile blockIdx.x);
+// mode 1 writes M.
 __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                   const double* __restrict__ lfac_tab, int mode,
                                                   const BlockOut* __restrict__ out, int flags)
@@ -575,6 +596,7 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     // symmetric, the window holds I(l1, l2) at [pol][l2 - L2 + 1][l1 - L1 + 1]
     __shared__ double wm[2][18][18];
     __shared__ int we[2][18][18];
+    __shared__ double csum[16][2][16];   // [column pair][parity][row pair]: |M| of the lane's 2 rows
     {
         const double* Im = at<double>(arena, d.off_i);
         const int* Ie = at<int>(arena, d.off_i + 16 * (int64_t)d.nl * d.nl);
@@ -591,91 +613,160 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     __syncthreads();
     const int c0 = threadIdx.x & 15, r0 = threadIdx.x >> 4;
     const int i2 = (L1 - d.lmin) + c0, j2 = (L2 - d.lmin) + r0;
-    if (i2 >= d.ldim || j2 >= d.ldim) return;
-    if (diag_only && i2 != j2) return;
-    const int l1 = d.lmin + i2, l2 = d.lmin + j2;
-    // I(l1 + a, l2 + b) of polarization pol times the prefactor pm 2^pe
-    auto X = [&](int pol, int a, int b, double pm, int pe) {
-        return ldexp(wm[pol][r0 + 1 + b][c0 + 1 + a] * pm, we[pol][r0 + 1 + b][c0 + 1 + a] + pe);
-    };
-    const double* ma = at<double>(arena, d.off_mie) + 2 * d.ldim;   // mantissas of e^{ea}, e^{eb}
-    const double* mb = ma + d.ldim;
-    const int* xa = reinterpret_cast<const int*>(ma + 2 * d.ldim);  // their exponents
-    const int* xb = xa + d.ldim;
-    const double ma1 = ma[i2], mb1 = mb[i2], ma2 = ma[j2], mb2 = mb[j2];
-    const int xa1 = xa[i2], xb1 = xb[i2], xa2 = xa[j2], xb2 = xb[j2];
-    const int TE = 0, TM = 1;
-    const UV c1 = uv(l1, m), c2 = uv(l2, m);
-    // A = m² I;  B = u1 u2 I(-1,-1) - v1 u2 I(+1,-1) - u1 v2 I(-1,+1) + v1 v2 I(+1,+1)
-    // (m = 0: B = I);  C(l1,l2) = m [-u2 I(0,-1) + v2 I(0,+1)];
-    // D(l1,l2) = C(l2,l1) = m [-u1 I(-1,0) + v1 I(+1,0)]   (I symmetric)
-    auto A = [&](int pol, double pm, int pe) { return m == 0 ? 0.0 : (double)m * m * X(pol, 0, 0, pm, pe); };
-    auto B = [&](int pol, double pm, int pe) {
-        if (m == 0) return X(pol, 0, 0, pm, pe);
-        return c1.u * c2.u * X(pol, -1, -1, pm, pe) - c1.v * c2.u * X(pol, 1, -1, pm, pe)
-             - c1.u * c2.v * X(pol, -1, 1, pm, pe) + c1.v * c2.v * X(pol, 1, 1, pm, pe);
-    };
-    auto C = [&](int pol, double pm, int pe) {
-        return m * (-c2.u * X(pol, 0, -1, pm, pe) + c2.v * X(pol, 0, 1, pm, pe));
-    };
-    auto D = [&](int pol, double pm, int pe) {
-        return m * (-c1.u * X(pol, -1, 0, pm, pe) + c1.v * X(pol, 1, 0, pm, pe));
-    };
-    // EE: √(a_l1 a_l2) Λ (B_TM - A_TE);  MM: √(b_l1 b_l2) Λ (A_TM - B_TE)
-    const double pEE = ma1 * ma2, pMM = mb1 * mb2;
-    const int eEE = xa1 + xa2, eMM = xb1 + xb2;
-    const double EE = B(TM, pEE, eEE) - A(TE, pEE, eEE);
-    const double MM = A(TM, pMM, eMM) - B(TE, pMM, eMM);
-    if (diag_only) {
-        double* dg = at<double>(arena, d.off_diag);
-        dg[2 * i2] = EE;
-        dg[2 * i2 + 1] = MM;
-        return;
+    const bool partials = mode == 0 && !diag_only && d.n_rtiles > 0;
+    const bool in = i2 < d.ldim && j2 < d.ldim && (!diag_only || i2 == j2);
+    if (in) {
+        const int l1 = d.lmin + i2, l2 = d.lmin + j2;
+        // I(l1 + a, l2 + b) of polarization pol times the prefactor pm 2^pe
+        auto X = [&](int pol, int a, int b, double pm, int pe) {
+            return ldexp(wm[pol][r0 + 1 + b][c0 + 1 + a] * pm, we[pol][r0 + 1 + b][c0 + 1 + a] + pe);
+        };
+        const double* ma = at<double>(arena, d.off_mie) + 2 * d.ldim;   // mantissas of e^{ea}, e^{eb}
+        const double* mb = ma + d.ldim;
+        const int* xa = reinterpret_cast<const int*>(ma + 2 * d.ldim);  // their exponents
+        const int* xb = xa + d.ldim;
+        const double ma1 = ma[i2], mb1 = mb[i2], ma2 = ma[j2], mb2 = mb[j2];
+        const int xa1 = xa[i2], xb1 = xb[i2], xa2 = xa[j2], xb2 = xb[j2];
+        const int TE = 0, TM = 1;
+        const UV c1 = uv(l1, m), c2 = uv(l2, m);
+        // A = m² I;  B = u1 u2 I(-1,-1) - v1 u2 I(+1,-1) - u1 v2 I(-1,+1) + v1 v2 I(+1,+1)
+        // (m = 0: B = I);  C(l1,l2) = m [-u2 I(0,-1) + v2 I(0,+1)];
+        // D(l1,l2) = C(l2,l1) = m [-u1 I(-1,0) + v1 I(+1,0)]   (I symmetric)
+        auto A = [&](int pol, double pm, int pe) { return m == 0 ? 0.0 : (double)m * m * X(pol, 0, 0, pm, pe); };
+        auto B = [&](int pol, double pm, int pe) {
+            if (m == 0) return X(pol, 0, 0, pm, pe);
+            return c1.u * c2.u * X(pol, -1, -1, pm, pe) - c1.v * c2.u * X(pol, 1, -1, pm, pe)
+                 - c1.u * c2.v * X(pol, -1, 1, pm, pe) + c1.v * c2.v * X(pol, 1, 1, pm, pe);
+        };
+        auto C = [&](int pol, double pm, int pe) {
+            return m * (-c2.u * X(pol, 0, -1, pm, pe) + c2.v * X(pol, 0, 1, pm, pe));
+        };
+        auto D = [&](int pol, double pm, int pe) {
+            return m * (-c1.u * X(pol, -1, 0, pm, pe) + c1.v * X(pol, 1, 0, pm, pe));
+        };
+        // EE: √(a_l1 a_l2) Λ (B_TM - A_TE);  MM: √(b_l1 b_l2) Λ (A_TM - B_TE)
+        const double pEE = ma1 * ma2, pMM = mb1 * mb2;
+        const int eEE = xa1 + xa2, eMM = xb1 + xb2;
+        const double EE = B(TM, pEE, eEE) - A(TE, pEE, eEE);
+        const double MM = A(TM, pMM, eMM) - B(TE, pMM, eMM);
+        if (diag_only) {
+            double* dg = at<double>(arena, d.off_diag);
+            dg[2 * i2] = EE;
+            dg[2 * i2 + 1] = MM;
+            return;   // no barrier follows in the trace pass
+        }
+        double EM = 0, ME = 0;
+        if (m != 0) {
+            // EM: D_TM - C_TE with (a_l1 + b_l2)/2;  ME: C_TM - D_TE with (b_l1 + a_l2)/2
+            const double pEM = ma1 * mb2, pME = mb1 * ma2;
+            const int eEM = xa1 + xb2, eME = xb1 + xa2;
+            EM = D(TM, pEM, eEM) - C(TE, pEM, eEM);
+            ME = C(TM, pME, eME) - D(TE, pME, eME);
+        }
+        double* Am = at<double>(arena, d.off_mat);
+        const int64_t r = 2 * i2, c = 2 * j2, lda = d.lda;
+        const double sg = mode == 0 ? -1.0 : 1.0;
+        // rows r, r+1 of a column are adjacent: one 16-byte store per column
+        // (r and lda even, the matrix 256-byte aligned)
+        *reinterpret_cast<double2*>(Am + r + c * lda) = make_double2(sg * EE, sg * ME);
+        *reinterpret_cast<double2*>(Am + r + (c + 1) * lda) = make_double2(sg * EM, sg * MM);
+        if (i2 == j2) {
+            double* dg = at<double>(arena, d.off_diag);
+            dg[r] = EE;
+            dg[r + 1] = MM;
+        }
+        if (partials) {
+            // column c: EE (row r, the diagonal when i2 = j2) and ME (row r+1);
+            // column c+1: EM (row r) and MM (row r+1, the diagonal when i2 = j2)
+            csum[r0][0][c0] = (i2 == j2 ? 0.0 : fabs(EE)) + fabs(ME);
+            csum[r0][1][c0] = fabs(EM) + (i2 == j2 ? 0.0 : fabs(MM));
+        }
+    } else if (partials) {
+        csum[r0][0][c0] = 0.0;
+        csum[r0][1][c0] = 0.0;
     }
-    double EM = 0, ME = 0;
-    if (m != 0) {
-        // EM: D_TM - C_TE with (a_l1 + b_l2)/2;  ME: C_TM - D_TE with (b_l1 + a_l2)/2
-        const double pEM = ma1 * mb2, pME = mb1 * ma2;
-        const int eEM = xa1 + xb2, eME = xb1 + xa2;
-        EM = D(TM, pEM, eEM) - C(TE, pEM, eEM);
-        ME = C(TM, pME, eME) - D(TE, pME, eME);
+    if (!partials) return;
+    __syncthreads();
+    // 32 columns of the workgroup: the sum over its 32 rows in row order
+    if (threadIdx.x < 32) {
+        const int jp = threadIdx.x >> 1, par = threadIdx.x & 1;
+        const int col = 2 * ((L2 - d.lmin) + jp) + par;
+        if (col < d.N) {
+            double s = 0;
+#pragma unroll
+            for (int q = 0; q < 16; q++) s += csum[jp][par][q];
+            at<double>(arena, d.off_cpart)[(int64_t)blockIdx.x * d.N + col] = s;
+        }
     }
-    double* Am = at<double>(arena, d.off_mat);
-    const int64_t r = 2 * i2, c = 2 * j2, lda = d.lda;
-    const double sg = mode == 0 ? -1.0 : 1.0;
-    const double one = mode == 0 ? 1.0 : 0.0;
-    // rows r, r+1 of a column are adjacent: one 16-byte store per column
-    // (r and lda even, the matrix 256-byte aligned)
-    *reinterpret_cast<double2*>(Am + r + c * lda) = make_double2(sg * EE + (i2 == j2 ? one : 0.0), sg * ME);
-    *reinterpret_cast<double2*>(Am + r + (c + 1) * lda) = make_double2(sg * EM, sg * MM + (i2 == j2 ? one : 0.0));
-    if (i2 == j2) {
-        double* dg = at<double>(arena, d.off_diag);
-        dg[r] = EE;
-        dg[r + 1] = MM;
+}
+
+// Column diagonal dominance of 1 - M (the matrix holds -M, the column sums
+// of |M_ij|, i != j, per row tile are in off_cpart): column j is dominant if
+// Σ_{i≠j} |M_ij| < |1 - M_jj|.  A matrix whose columns are all strictly
+// dominant keeps that property in every Schur complement, so LU with partial
+// pivoting exchanges no rows and the elimination without pivot search is the
+// same factorization (growth factor <= 2).  out[b].delta starts at 1 (k_trace)
+// and any non-dominant column clears it (an AND, independent of the order).
+__global__ void __launch_bounds__(256) k_dominance(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                   BlockOut* __restrict__ out)
+{
+    const int b = blockIdx.y;
+    const BlockDesc& d = desc[b];
+    if (out[b].skip || d.n_rtiles <= 0) return;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    bool ok = true;
+    if (j < d.N) {
+        const double* cp = at<double>(arena, d.off_cpart) + j;
+        double s = 0;
+        for (int t = 0; t < d.n_rtiles; t++) s += cp[(int64_t)t * d.N];
+        const double djj = at<double>(arena, d.off_mat)[j + (int64_t)j * d.lda];   // -M_jj
+        ok = s < fabs(1.0 + djj);
     }
+    if (__syncthreads_or(!ok) && threadIdx.x == 0) atomicAnd(&out[b].delta, 0);
+}
+
+// Blocks left without column dominance get their unit diagonal back: the
+// matrix becomes 1 - M (the same values as writing 1 - M directly) for the LU
+// with partial pivoting.
+__global__ void __launch_bounds__(256) k_dominance_fix(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                       const BlockOut* __restrict__ out)
+{
+    const int b = blockIdx.y;
+    const BlockDesc& d = desc[b];
+    if (out[b].skip || d.n_rtiles <= 0 || out[b].delta) return;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < d.N) at<double>(arena, d.off_mat)[j + (int64_t)j * d.lda] += 1.0;
 }
 
 // ξ = 0 blocks (EE, MM, plasma MM; src/libcaps.c:860-941): elements
 // M_ij = exp(h_i + h_j + T_{i+j}) with per-row terms h (n doubles) and
 // per-(l1+l2) terms T (2n - 1 doubles), both at arena offset off_tab.
-// Writes 1 - M column-major (leading dimension lda) and the diagonal of M.
-// Lanes run down the columns (coalesced stores), 64 x 4 per workgroup.
+// Writes -M column-major (leading dimension lda; the LU's delta form, see
+// k_assemble), the diagonal of M, and per 64-row tile (blockIdx.x) the
+// column sums of |M_ij|, i != j.  Lanes run down the columns (coalesced
+// stores), 64 x 4 per workgroup.
 __global__ void __launch_bounds__(256) k_fill_xi0(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                   int64_t off_tab)
 {
     const BlockDesc& d = desc[0];
     const int n = d.N;
     const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (i >= n || j >= n) return;
-    const double* h = at<double>(arena, off_tab);
-    const double* T = h + n;
-    const double v = exp(h[i] + h[j] + T[i + j]);
-    at<double>(arena, d.off_mat)[i + (int64_t)j * d.lda] = (i == j ? 1.0 : 0.0) - v;
-    if (i == j) at<double>(arena, d.off_diag)[i] = v;
+    double v = 0;
+    if (i < n && j < n) {
+        const double* h = at<double>(arena, off_tab);
+        const double* T = h + n;
+        v = exp(h[i] + h[j] + T[i + j]);
+        at<double>(arena, d.off_mat)[i + (int64_t)j * d.lda] = -v;
+        if (i == j) at<double>(arena, d.off_diag)[i] = v;
+    }
+    // all 64 lanes of the wavefront take part in the reduction
+    const double s = wave_sum(i == j ? 0.0 : fabs(v));
+    if ((threadIdx.x & 63) == 0 && j < n) at<double>(arena, d.off_cpart)[(int64_t)blockIdx.x * n + j] = s;
 }
 
 // Trace of M (Kahan, in index order as src/matrix.c:63-68) and the trace
 // shortcut flag |tr M| < 1e-8 (src/matrix.c:78-82).  One wavefront per block.
+// Also starts the block's delta flag at 1 (cleared by k_dominance).
 __global__ void __launch_bounds__(64) k_trace(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                              BlockOut* __restrict__ out, int nblocks)
 {
@@ -685,20 +776,25 @@ __global__ void __launch_bounds__(64) k_trace(const BlockDesc* __restrict__ desc
     const double s = wave_kahan_sum(at<double>(arena, d.off_diag), d.N);
     if (threadIdx.x == 0) {
         out[b].trace = s;
-        out[b].skip = fabs(s) < 1e-8 ? 1 : 0;
+        // skip 1: trace shortcut; 3: non-finite trace (no LU, NaN result)
+        out[b].skip = fabs(s) < 1e-8 ? 1 : (isfinite(s) ? 0 : 3);
         out[b].logdet = NAN;
         out[b].status = 0;
+        out[b].delta = d.n_rtiles > 0 ? 1 : 0;
     }
 }
 
-// ||M||_F² and tr M² = Σ_ij M_ij M_ji of M (from the stored 1 - M and the
-// exact diagonal of M) and the LU-mode Mercator shortcut of
-// src/matrix.c:489-505: if ||M||_F < 1 and the bound on the neglected terms,
-// |n²/2 + n + log(1-n)|, is below 1e-8 of -tr M - tr M²/2, the block's LU is
-// skipped (skip = 2) and the host returns the two-term series.  One
-// workgroup per block.
+// ||M||_F² and tr M² = Σ_ij M_ij M_ji of M (from the stored matrix, -M off
+// the diagonal in both forms, and the exact diagonal of M) and the LU-mode
+// Mercator shortcut of src/matrix.c:489-505: if ||M||_F < 1 and the bound on
+// the neglected terms, |n²/2 + n + log(1-n)|, is below 1e-8 of
+// -tr M - tr M²/2, the block's LU is skipped (skip = 2) and the host returns
+// the two-term series.  lower != 0 (CHOLESKY): both sums over the matrix the
+// reference builds for that mode, M with its upper triangle left at zero
+// (src/matrix.c:93-102), i.e. ||M||_F² over i >= j and tr M² over the
+// diagonal.  One workgroup per block.
 __global__ void __launch_bounds__(256) k_frobenius(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                                   BlockOut* __restrict__ out)
+                                                   BlockOut* __restrict__ out, int lower)
 {
     const BlockDesc& d = desc[blockIdx.x];
     if (out[blockIdx.x].skip == 1) return;   // trace shortcut already taken
@@ -707,10 +803,15 @@ __global__ void __launch_bounds__(256) k_frobenius(const BlockDesc* __restrict__
     double f = 0, t2 = 0;
     for (int64_t idx = threadIdx.x; idx < (int64_t)d.N * d.N; idx += blockDim.x) {
         const int i = (int)(idx % d.N), j = (int)(idx / d.N);
+        if (lower && i < j) continue;
         const double mij = i == j ? dg[i] : -A[i + (int64_t)j * d.lda];
-        const double mji = i == j ? dg[i] : -A[j + (int64_t)i * d.lda];
         f += mij * mij;
-        t2 += mij * mji;
+        if (!lower) {
+            const double mji = i == j ? dg[i] : -A[j + (int64_t)i * d.lda];
+            t2 += mij * mji;
+        } else if (i == j) {
+            t2 += mij * mij;
+        }
     }
     __shared__ double sf[256], st[256];
     sf[threadIdx.x] = f;
@@ -788,9 +889,9 @@ void capsd_launch_assemble(const BlockDesc* desc, char* arena, const double* lfa
     hipLaunchKernelGGL(k_assemble, g, dim3(256), 0, s, desc, arena, lfac_tab, mode, out, flags);
 }
 
-void capsd_launch_frobenius(const BlockDesc* desc, char* arena, BlockOut* out, int nblocks, hipStream_t s)
+void capsd_launch_frobenius(const BlockDesc* desc, char* arena, BlockOut* out, int nblocks, int lower, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_frobenius, dim3(nblocks), dim3(256), 0, s, desc, arena, out);
+    hipLaunchKernelGGL(k_frobenius, dim3(nblocks), dim3(256), 0, s, desc, arena, out, lower);
 }
 
 void capsd_launch_fill_xi0(const BlockDesc* desc, char* arena, int64_t off_tab, int n, hipStream_t s)
@@ -798,16 +899,31 @@ void capsd_launch_fill_xi0(const BlockDesc* desc, char* arena, int64_t off_tab, 
     hipLaunchKernelGGL(k_fill_xi0, dim3((n + 63) / 64, (n + 3) / 4), dim3(256), 0, s, desc, arena, off_tab);
 }
 
+// Column dominance of the blocks' 1 - M (after their assembly): the delta
+// flags, and the unit diagonal written back where a block is not dominant.
+void capsd_launch_dominance(const BlockDesc* desc, char* arena, BlockOut* out, int nblocks, int maxN, hipStream_t s)
+{
+    const dim3 g((maxN + 255) / 256, nblocks);
+    hipLaunchKernelGGL(k_dominance, g, dim3(256), 0, s, desc, arena, out);
+    hipLaunchKernelGGL(k_dominance_fix, g, dim3(256), 0, s, desc, arena, out);
+}
+
 void capsd_launch_trace(const BlockDesc* desc, char* arena, BlockOut* out, int nblocks, int frobenius, hipStream_t s)
 {
     hipLaunchKernelGGL(k_trace, dim3(nblocks), dim3(64), 0, s, desc, arena, out, nblocks);
-    if (frobenius) hipLaunchKernelGGL(k_frobenius, dim3(nblocks), dim3(256), 0, s, desc, arena, out);
+    if (frobenius) hipLaunchKernelGGL(k_frobenius, dim3(nblocks), dim3(256), 0, s, desc, arena, out, frobenius == 2);
 }
 
 void capsd_launch_mie_points(int n, const double* xi_, const int* l, double LbyR, const double* epsm1, double* out,
                              hipStream_t s)
 {
     hipLaunchKernelGGL(k_mie_points, dim3((n + 63) / 64), dim3(64), 0, s, n, xi_, l, LbyR, epsm1, out);
+}
+
+void capsd_launch_bessel_points(int n, const int* order, const double* nu, const double* x, double* out,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bessel_points, dim3((n + 63) / 64), dim3(64), 0, s, n, order, nu, x, out);
 }
 
 void capsd_launch_fresnel_points(int n, const double* xi_, const double* k_, const double* epsm1, double* out,

@@ -45,11 +45,63 @@ double orc_logdet_lu(double *A, int n)
     return r;
 }
 
+/* log|1 + d| keeping the relative accuracy of d (a pivot u = 1 + d near 1
+ * stored as u itself loses d below one ulp of 1). */
+static double log_abs_1p(double d) { return d > -0.5 ? log1p(d) : log(fabs(1 + d)); }
+
+/* 1 if 1 - M (row-major M) is strictly column diagonally dominant:
+ * sum_{i != j} |M_ij| < |1 - M_jj| for every column j.  Then every Schur
+ * complement is too, and LU with partial pivoting exchanges no rows. */
+int orc_dominant_1mM(const double *M, int n)
+{
+    for(int j = 0; j < n; j++) {
+        double s = 0;
+        for(int i = 0; i < n; i++)
+            if(i != j) s += fabs(M[(size_t)i*n + j]);
+        if(!(s < fabs(1 - M[(size_t)j*n + j])))
+            return 0;
+    }
+    return 1;
+}
+
+/* log|det(1 - M)| for column diagonally dominant 1 - M, in delta form: the
+ * elimination runs on D = -M (1 - M without its unit diagonal) with pivot
+ * u_k = 1 + d_kk and no row exchanges (partial pivoting would make none),
+ * every update being the same on D as on 1 - M; log|u_k| = log1p(d_kk),
+ * Kahan-summed.  The reference factors 1 - M with LAPACK (src/matrix.c:
+ * 555-577); the delta form gives the same determinant without the rounding
+ * of 1 - M_kk to the nearest double near 1. */
+double orc_logdet_1mM_delta(const double *M, int n)
+{
+    double *D = malloc((size_t)n*n*sizeof(double));
+    double *logs = malloc((size_t)n*sizeof(double));
+    for(size_t i = 0; i < (size_t)n*n; i++) D[i] = -M[i];
+    for(int k = 0; k < n; k++) {
+        const double d = D[(size_t)k*n + k];
+        logs[k] = log_abs_1p(d);
+        const double inv = 1/(1 + d);
+        for(int i = k + 1; i < n; i++) {
+            double *ri = D + (size_t)i*n;
+            const double l = ri[k]*inv;
+            ri[k] = l;
+            if(l == 0) continue;
+            const double *rk = D + (size_t)k*n;
+            for(int j = k + 1; j < n; j++)
+                ri[j] -= l*rk[j];
+        }
+    }
+    const double r = orc_kahan_sum(logs, n);
+    free(logs);
+    free(D);
+    return r;
+}
+
 /* log det(1-M) with the reference's shortcuts.  Reference: src/matrix.c:57-149
  * (|tr M| < 1e-8 -> -tr M; HODLR mode: -tr M if |tr M| > |logdet|) and
  * 489-544 (LU mode: Mercator -tr M - tr M²/2 if ||M||_F < 1 and its error bound
  * is below 1e-8 relative).  detalg 0 = HODLR mode (the reference default),
- * 1 = LU mode.  The determinant itself is always a dense LU here. */
+ * 1 = LU mode.  The determinant itself is always a dense LU here: the delta
+ * form for column diagonally dominant 1 - M, else partial pivoting. */
 double orc_logdet_1mM(const double *M, int n, int detalg)
 {
     double *d = malloc((size_t)n*sizeof(double));
@@ -80,11 +132,16 @@ double orc_logdet_1mM(const double *M, int n, int detalg)
         }
     }
 
-    double *A = malloc((size_t)n*n*sizeof(double));
-    for(size_t i = 0; i < (size_t)n*n; i++) A[i] = -M[i];
-    for(int i = 0; i < n; i++) A[(size_t)i*n + i] += 1;
-    const double ld = orc_logdet_lu(A, n);
-    free(A);
+    double ld;
+    if(orc_dominant_1mM(M, n))
+        ld = orc_logdet_1mM_delta(M, n);
+    else {
+        double *A = malloc((size_t)n*n*sizeof(double));
+        for(size_t i = 0; i < (size_t)n*n; i++) A[i] = -M[i];
+        for(int i = 0; i < n; i++) A[(size_t)i*n + i] += 1;
+        ld = orc_logdet_lu(A, n);
+        free(A);
+    }
     if(detalg == 0 && fabs(tr) > fabs(ld))
         return -tr;
     return ld;

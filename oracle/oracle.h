@@ -17,7 +17,7 @@
  *   - the determinant is a dense LU with partial pivoting for every detalg;
  *     the HODLR mode keeps the reference's trace shortcuts around it.
  *
- * Parity status of every function is listed in oracle/README.md.
+ * Parity status of every function is listed in oracle/README.md (what pins it).
  */
 #ifndef CAPS_ORACLE_H
 #define CAPS_ORACLE_H
@@ -87,6 +87,7 @@ void   orc_block_free(orc_block_t *b);
 int    orc_block_lmin(const orc_block_t *b);
 double orc_block_K(orc_block_t *b, int nu, int p, orc_sign_t *sign);
 double orc_block_I(orc_block_t *b, int l1, int l2, int p, orc_sign_t *sign);
+double orc_block_I_abs(orc_block_t *B, int l1, int l2, int p);   /* log Σ|terms| of I */
 double orc_block_elem(orc_block_t *b, int l1, int l2, char p1, char p2);
 void   orc_block_matrix(orc_block_t *b, double *M);   /* N x N row-major, N = 2 ldim */
 long   orc_block_counters(const orc_block_t *b, long *n_K, long *n_Kevals, long *n_I, long *n_qterms);
@@ -94,6 +95,8 @@ long   orc_block_counters(const orc_block_t *b, long *n_K, long *n_Kevals, long 
 /* ---- logdet.c ---------------------------------------------------------- */
 double orc_logdet_lu(double *A, int n);              /* log|det A|, A overwritten */
 double orc_logdet_1mM(const double *M, int n, int detalg);  /* log det(1-M) with shortcuts */
+int    orc_dominant_1mM(const double *M, int n);              /* 1 - M column diagonally dominant */
+double orc_logdet_1mM_delta(const double *M, int n);          /* log|det(1-M)|, delta form, no pivoting */
 double orc_logdetD(const orc_caps_t *c, double xi_, int m);
 void   orc_logdetD_many(const orc_caps_t *c, const double *xi_, const int *m, int n, double *out, int threads);
 double orc_ht_drude(double LbyR);

@@ -61,10 +61,12 @@ def lib():
             "orc_block_new": (p, [ctypes.POINTER(OrcCaps), d, i]),
             "orc_block_free": (None, [p]), "orc_block_lmin": (i, [p]),
             "orc_block_K": (d, [p, i, i, sp]), "orc_block_I": (d, [p, i, i, i, sp]),
+            "orc_block_I_abs": (d, [p, i, i, i]),
             "orc_block_elem": (d, [p, i, i, ctypes.c_char, ctypes.c_char]),
             "orc_block_matrix": (None, [p, dp]),
             "orc_block_counters": (ctypes.c_long, [p] + [ctypes.POINTER(ctypes.c_long)] * 4),
             "orc_logdet_lu": (d, [dp, i]), "orc_logdet_1mM": (d, [dp, i, i]),
+            "orc_dominant_1mM": (i, [dp, i]), "orc_logdet_1mM_delta": (d, [dp, i]),
             "orc_logdetD": (d, [ctypes.POINTER(OrcCaps), d, i]),
             "orc_logdetD_many": (None, [ctypes.POINTER(OrcCaps), dp, ctypes.POINTER(ctypes.c_int), i, dp, i]),
             "orc_ht_drude": (d, [d]),
@@ -135,6 +137,10 @@ class Block:
         v = lib().orc_block_I(self.ptr, l1, l2, p, ctypes.byref(s))
         return v, s.value
 
+    def I_abs(self, l1, l2, p):
+        """log of the sum of the q-sum's term magnitudes of I_{l1 l2}^p."""
+        return lib().orc_block_I_abs(self.ptr, l1, l2, p)
+
     def elem(self, l1, l2, p1, p2):
         return lib().orc_block_elem(self.ptr, l1, l2, p1.encode(), p2.encode())
 
@@ -160,6 +166,18 @@ def logdet_1mM(M, detalg=0):
     import numpy as np
     M = np.ascontiguousarray(M, dtype=np.float64)
     return lib().orc_logdet_1mM(M.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), M.shape[0], detalg)
+
+
+def dominant_1mM(M):
+    import numpy as np
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    return bool(lib().orc_dominant_1mM(M.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), M.shape[0]))
+
+
+def logdet_1mM_delta(M):
+    import numpy as np
+    M = np.ascontiguousarray(M, dtype=np.float64)
+    return lib().orc_logdet_1mM_delta(M.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), M.shape[0])
 
 
 def mie(c, xi_, l):

@@ -456,7 +456,17 @@ static double alpha_pnnu(double p, double n, double nu)
  * Reference: src/integration.c:497-609 (eqs. (20),(24),(26)-(30),(35); the
  * 1e±100 rescaling of a_q; the early exit after three consecutive terms
  * below e^{-60} times the first). */
+static double I_compute_abs(orc_block_t *B, int l1, int l2, int p, orc_sign_t *sign, double *log_abs);
+
 static double I_compute(orc_block_t *B, int l1, int l2, int p, orc_sign_t *sign)
+{
+    return I_compute_abs(B, l1, l2, p, sign, NULL);
+}
+
+/* The same sum, and (log_abs != NULL) log of Σ_q |a_q K_{l1+l2-2q}| e^{log a0}:
+ * the sum of the terms' magnitudes, the scale of its rounding and of the
+ * K-integrals' errors (the condition of the sum is that over |I|). */
+static double I_compute_abs(orc_block_t *B, int l1, int l2, int p, orc_sign_t *sign, double *log_abs)
 {
     const int m_ = B->m > 0 ? B->m : 1;
     const double n = l1, nu = l2, m = m_;
@@ -537,9 +547,28 @@ done:
         const int nt = (q < qmax ? q : qmax) + 1;
         B->n_qterms += nt;
         const double r = log_a0 + orc_logadd_ms(tv, ts, nt, sign);
+        if(log_abs) {
+            double a = -INFINITY;
+            for(int k = 0; k < nt; k++) a = orc_logadd(a, tv[k]);
+            *log_abs = log_a0 + a;
+        }
         free(aq); free(tv); free(ts);
         return r;
     }
+}
+
+/* log Σ_q |a_q K_{l1+l2-2q}| e^{log a0} of I_{l1 l2}^p (TM for perfect
+ * reflectors, whose TE is -|TM|); l1 >= l2 >= m.  Test aid: the scale of
+ * the sum's absolute error. */
+double orc_block_I_abs(orc_block_t *B, int l1, int l2, int p)
+{
+    if(l1 < l2) { const int t = l1; l1 = l2; l2 = t; }
+    if(l2 < B->m) return -INFINITY;
+    if(B->is_pr) p = 1;
+    orc_sign_t s;
+    double a;
+    I_compute_abs(B, l1, l2, p, &s, &a);
+    return a;
 }
 
 /* Cached, symmetrised I.  Reference: src/integration.c:627-671 (zero for

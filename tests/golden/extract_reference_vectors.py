@@ -209,6 +209,50 @@ def logdetD0_rows(fname):
     return rows
 
 
+def manual_runs():
+    """The free-energy runs the reference's manual prints
+    (docs/manual/index.rst): per run its command-line options, the parameters
+    of its header (ldim, cutoff, epsrel, cores), every printed per-frequency
+    value "xi*(L+R)/c=..., logdetD=..." (the driver's F_xi, src/caps.c:
+    307-396), the "# plasma = " value where printed, the number of
+    determinants and the final line's values; each with its line number."""
+    path = os.path.join(REF, "docs", "manual", "index.rst")
+    lines = open(path).read().splitlines()
+    runs, run = [], None
+    for i, ln in enumerate(lines, 1):
+        t = ln.strip()
+        m = re.match(r"\$ mpirun -n (\d+) \./caps (.*)$", t)
+        if m:
+            if run:
+                runs.append(run)
+            opts, toks = {}, m.group(2).split()
+            for a, b in zip(toks[::2], toks[1::2]):
+                opts[a.lstrip("-")] = b
+            run = {"src": "docs/manual/index.rst:%d" % i, "cores": int(m.group(1)), "options": opts,
+                   "xi": [], "header": {}}
+            continue
+        if run is None:
+            continue
+        m = re.match(r"# (ldim|cutoff|epsrel|iepsrel|T|cores) = (%s)$" % NUM, t)
+        if m:
+            run["header"][m.group(1)] = float(m.group(2))
+        m = re.match(r"# xi\*\(L\+R\)/c=(%s), logdetD=(%s), t=" % (NUM, NUM), t)
+        if m:
+            run["xi"].append({"xi_": float(m.group(1)), "value": float(m.group(2)), "line": i})
+        m = re.match(r"# plasma = (%s) " % NUM, t)
+        if m:
+            run["plasma_xi0"] = float(m.group(1))
+        m = re.match(r"# (\d+) determinants computed", t)
+        if m:
+            run["determinants"] = int(m.group(1))
+        if re.match(r"(%s), (%s), (%s)" % (NUM, NUM, NUM), t) and not t.startswith("#"):
+            run["final"] = [float(x) for x in t.split(",")[:3]] + [float(x) for x in t.split(",")[3:]]
+            run["final_line"] = i
+            runs.append(run)
+            run = None
+    return [r for r in runs if "final" in r]
+
+
 def main():
     if not os.path.isdir(TESTS):
         sys.exit("reference tests not found at %s" % TESTS)
@@ -238,6 +282,15 @@ def main():
             json.dump({"source": "CaPS v0.5 src/tests (transcribed by extract_reference_vectors.py)",
                        "rows": rows}, f, indent=0)
         print("%-16s %5d rows" % (name, len(rows)))
+    runs = manual_runs()
+    with open(os.path.join(OUT, "manual_runs.json"), "w") as f:
+        json.dump({"source": "CaPS v0.5 docs/manual/index.rst (transcribed by extract_reference_vectors.py)",
+                   "rows": runs}, f, indent=1)
+    print("%-16s %5d runs" % ("manual_runs.json", len(runs)))
+    # the optical data of gold the manual's material run reads (a data
+    # table: xi in rad/s, epsilon(i xi), with its Drude extrapolation header)
+    with open(os.path.join(REF, "materials", "gold.csv")) as f, open(os.path.join(OUT, "gold.csv"), "w") as g:
+        g.write(f.read())
 
 
 if __name__ == "__main__":

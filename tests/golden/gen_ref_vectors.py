@@ -7,6 +7,7 @@ machines without the reference.
 
     python tests/golden/gen_ref_vectors.py blocks      # block logdetD values
     python tests/golden/gen_ref_vectors.py tables      # K, I and M elements
+    python tests/golden/gen_ref_vectors.py tables_c2   # the same at R/L = 200, ldim = 1200
     python tests/golden/gen_ref_vectors.py free_energy # C2 free energy (~10 min, 8 cores)
     python tests/golden/gen_ref_vectors.py mixed       # 64 mixed-aspect PR blocks
     python tests/golden/gen_ref_vectors.py t0          # T = 0 free energies (dqagi)
@@ -115,18 +116,29 @@ def gen_blocks(pool):
     return rows
 
 
-def gen_tables():
-    """K_ν, I_{l1 l2} and matrix elements of a few small blocks."""
+TABLE_CASES = [
+    ("pr", 20e-6, 1e-6, 40, None, 1.0, 0),
+    ("pr", 20e-6, 1e-6, 40, None, 1.0, 3),
+    ("pr", 100e-6, 1e-6, 60, None, 0.1, 1),
+    ("pr", 100e-6, 1e-6, 60, None, 10.0, 20),
+    ("drude", 200e-6, 1e-6, 50, drude_c2(), 165.4564, 0),
+    ("drude", 200e-6, 1e-6, 50, drude_c2(), 165.4564, 7),
+    ("drude", 200e-6, 1e-6, 50, drude_c2(), 1489.1076, 30),
+]
+# The R/L = 200 Drude configuration at its own ldim = 1200: ξ_1 at m = 0, at
+# the m-cutoff (m = 125, window l in [650, 1850)) and beyond it (m = 190,
+# ν up to ~5000), ξ_9 at m = 10.
+TABLE_CASES_C2 = [
+    ("drude", 200e-6, 1e-6, 1200, drude_c2(), 165.4563560875888, 0),
+    ("drude", 200e-6, 1e-6, 1200, drude_c2(), 165.4563560875888, 125),
+    ("drude", 200e-6, 1e-6, 1200, drude_c2(), 165.4563560875888, 190),
+    ("drude", 200e-6, 1e-6, 1200, drude_c2(), 1489.1072047882992, 10),
+]
+
+
+def gen_tables(cases=TABLE_CASES):
+    """K_ν, I_{l1 l2} and matrix elements of a few blocks."""
     out = []
-    cases = [
-        ("pr", 20e-6, 1e-6, 40, None, 1.0, 0),
-        ("pr", 20e-6, 1e-6, 40, None, 1.0, 3),
-        ("pr", 100e-6, 1e-6, 60, None, 0.1, 1),
-        ("pr", 100e-6, 1e-6, 60, None, 10.0, 20),
-        ("drude", 200e-6, 1e-6, 50, drude_c2(), 165.4564, 0),
-        ("drude", 200e-6, 1e-6, 50, drude_c2(), 165.4564, 7),
-        ("drude", 200e-6, 1e-6, 50, drude_c2(), 1489.1076, 30),
-    ]
     L_ = lib()
     for tag, R, L, ldim, drude, xi, m in cases:
         c = make_caps(R, L, ldim, drude)
@@ -137,7 +149,10 @@ def gen_tables():
         s = ctypes.c_byte()
         mu = 2 * m if m > 0 else 2
         K = []
-        for nu in sorted(set([mu, mu + 1, mu + 2, mu + 5, mu + 17, mu + 40, 2 * (lmin + ldim) - 3])):
+        nus = [mu, mu + 1, mu + 2, mu + 5, mu + 17, mu + 40, 2 * (lmin + ldim) - 3]
+        if ldim >= 1000:
+            nus += [mu + 501, lmin + ldim, 2 * (lmin + ldim) - 400]
+        for nu in sorted(set(nus)):
             for p in (0, 1):
                 v = L_.caps_integrate_K(integ, nu, p, ctypes.byref(s))
                 K.append([nu, p, v, s.value])
@@ -298,6 +313,9 @@ def main():
     if what == "tables":
         data = gen_tables()
         name = "ref_tables.json"
+    elif what == "tables_c2":
+        data = gen_tables(TABLE_CASES_C2)
+        name = "ref_tables_c2.json"
     else:
         with mp.Pool(8) as pool:
             if what == "blocks":

@@ -1,0 +1,119 @@
+"""The determinant's two forms on the GPU, through the C ABI.
+
+Delta form: for a column diagonally dominant 1 - M the matrix holds -M (no
+unit diagonal), the LU runs without pivot search on u = 1 + d and sums
+log1p(d); partial pivoting would exchange no rows there, so it is the same
+factorization, but no pivot near 1 is rounded to a double near 1.  Checked
+against the Mercator series of M (conftest.mercator_logdet, which never
+forms 1 - M) at 1e-13 relative.  Explicit form (1 - M not dominant): partial
+pivoting, checked against numpy's slogdet.  kernel_logdet with a caller's
+callback (src/capc.cpp:110-114) under every detalg, against the oracle and
+the reference's rules (src/matrix.c:57-149, 489-544).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import mercator_logdet
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ca():
+    import caps_amd
+    caps_amd.lib()
+    return caps_amd
+
+
+def sym_matrix(rng, N, scale):
+    X = rng.normal(size=(N, N))
+    return scale * (X + X.T) / (2 * N)
+
+
+@pytest.mark.parametrize("N", [7, 64, 130, 300, 700])
+def test_delta_form_small_norm(ca, N):
+    """Near-identity 1 - M (||M|| from 1e-2 down to 1e-9): every block runs in
+    delta form and matches the Mercator series of M to 1e-13 relative; a
+    plain LU of 1 - M would lose up to sqrt(N) eps / |log det| (5e-8
+    relative at |log det| = 1e-9)."""
+    rng = np.random.default_rng(N)
+    mats = [sym_matrix(rng, N, s) + np.diag(np.full(N, s / N)) for s in (1e-2, 1e-4, 1e-6, 1e-9)]
+    out, flags = ca.logdet_1mM_batch(np.array(mats))
+    assert list(flags) == [1, 1, 1, 1], flags
+    for M, v in zip(mats, out):
+        ref = mercator_logdet(M)
+        assert abs(v / ref - 1) < 1e-13, (N, v, ref)
+
+
+def test_delta_form_physical_block(ca):
+    """A round-trip block (R/L = 20, ξ = 1, m = 24, |log det| = 8e-7) from the
+    oracle: delta form, against its Mercator series."""
+    import pyoracle as po
+    oc = po.caps(20e-6, 1e-6, ldim=120, epsrel=1e-12)
+    M = po.Block(oc, 1.0, 24).matrix()
+    out, flags = ca.logdet_1mM_batch(M[None])
+    ref = mercator_logdet(M)
+    assert flags[0] == 1 and abs(out[0] / ref - 1) < 1e-13, (out[0], ref)
+
+
+@pytest.mark.parametrize("N", [50, 300, 517])
+def test_explicit_form_when_not_dominant(ca, N):
+    """Column sums of |M| above |1 - M_jj|: the unit diagonal is written back
+    and the LU pivots (slogdet at 1e-10 N); a dominant matrix of the same
+    batch stays in delta form."""
+    rng = np.random.default_rng(100 + N)
+    big = rng.normal(size=(N, N)) * (3.0 / math.sqrt(N))       # column sums ~ 2.4 sqrt(N)
+    small = sym_matrix(rng, N, 0.3)
+    out, flags = ca.logdet_1mM_batch(np.array([big, small]))
+    assert list(flags) == [0, 1], flags
+    for M, v in zip([big, small], out):
+        ref = np.linalg.slogdet(np.eye(N) - M)[1]
+        assert abs(v - ref) < 1e-10 * N * max(1, abs(ref)), (N, v, ref)
+
+
+def test_lu_nan_entry_gives_nan(ca):
+    """A NaN entry in a matrix of the pivoted LU: no candidate row wins a
+    column whose entries are all NaN; the pivot falls back to the diagonal
+    row (indices stay in range) and the determinant is NaN."""
+    rng = np.random.default_rng(3)
+    N = 200
+    A = np.eye(N) + 0.1 * rng.normal(size=(N, N)) / math.sqrt(N)
+    B = A.copy()
+    B[:, 5] = np.nan
+    out = ca.lu_logdet_batch(np.array([A, B]))
+    assert abs(out[0] - np.linalg.slogdet(A)[1]) < 1e-10
+    assert np.isnan(out[1]), out[1]
+
+
+@pytest.mark.parametrize("detalg", ["HODLR", "LU", "QR", "CHOLESKY"])
+def test_kernel_logdet_callback(ca, detalg):
+    """kernel_logdet with a caller's callback, as the reference's cylinder
+    program calls it (src/capc.cpp:110-114): a block-sized symmetric matrix
+    (LU), a small one (LU-mode Mercator: -tr M - tr M²/2, taken only in the
+    dense modes; CHOLESKY fills and tests the lower triangle only, as
+    src/matrix.c:93-102 does) and a tiny one (trace shortcut)."""
+    import pyoracle as po
+    rng = np.random.default_rng(8)
+    N = 160
+    big = sym_matrix(rng, N, 4.0)                    # ||M||_F ~ 3, eigenvalues < 1/2: LU
+    small = sym_matrix(rng, N, 1e-4) + np.diag(np.full(N, 1e-5))   # ||M||_F ~ 1.5e-4: Mercator
+    tiny = sym_matrix(rng, N, 1e-9)
+    for M in (big, small, tiny):
+        v = ca.kernel_logdet(N, lambda i, j: float(M[i, j]), detalg)
+        tr = math.fsum(np.diag(M))
+        if abs(tr) < 1e-8:
+            expect = -tr
+        elif detalg == "HODLR":
+            ld = po.logdet_1mM(M, 0)            # dense LU with the HODLR-mode rule
+            expect = ld
+        else:
+            H = np.tril(M) if detalg == "CHOLESKY" else M
+            nrm = np.linalg.norm(H)
+            merc = -tr - float(np.sum(H * H.T)) / 2
+            if nrm < 1 and abs((nrm ** 2 / 2 + nrm + math.log1p(-nrm)) / merc) < 1e-8:
+                expect = merc
+            else:
+                expect = po.logdet_1mM_delta(M) if po.dominant_1mM(M) else np.linalg.slogdet(np.eye(N) - M)[1]
+        assert abs(v - expect) <= 1e-12 * abs(expect), (detalg, v, expect)

@@ -273,3 +273,74 @@ def test_plasma_integral_limits():
         w = 1e-8 if nu == 2 else 1e-4
         small = L.orc_integrate_plasma(nu, w, 1e-13)
         assert abs(small / (w * w / (nu * (nu - 1))) - 1) < 1e-6, (nu, small)
+
+
+def test_logdet_delta_form_vs_mercator():
+    """The oracle's delta-form determinant (no pivot search on u = 1 + d,
+    log1p(d)) of near-identity 1 - M against the Mercator series
+    -Σ tr(M^k)/k of M itself (closed form, never forming 1 - M), down to
+    |log det| ~ 1e-12; a plain LU of 1 - M is shown to lose the small ones
+    (the reason for the delta form).  A physical block (R/L = 20, m = 24)
+    too."""
+    from conftest import mercator_logdet
+    rng = np.random.default_rng(4)
+    for N in (5, 40, 200):
+        for scale in (1e-1, 1e-3, 1e-6, 1e-10):
+            X = rng.normal(size=(N, N))
+            M = scale * (X + X.T) / (2 * N) + np.diag(np.full(N, scale / N))
+            assert po.dominant_1mM(M)
+            ref = mercator_logdet(M)
+            assert abs(po.logdet_1mM_delta(M) / ref - 1) < 1e-14, (N, scale)
+    c = po.caps(20e-6, 1e-6, ldim=120, epsrel=1e-12)
+    M = po.Block(c, 1.0, 24).matrix()
+    ref = mercator_logdet(M)
+    assert abs(po.logdet_1mM_delta(M) / ref - 1) < 1e-14
+    plain = np.linalg.slogdet(np.eye(M.shape[0]) - M)[1]
+    assert abs(plain / ref - 1) > 1e-10        # the plain LU's rounding of 1 - M_ii
+
+
+def test_dominance_test_cases():
+    """orc_dominant_1mM: strict column dominance of 1 - M (row-major M),
+    Σ_{i≠j} |M_ij| < |1 - M_jj|, checked on hand-made cases."""
+    M = np.array([[0.2, 0.3], [0.4, 0.1]])
+    assert po.dominant_1mM(M)                      # columns: 0.4 < 0.8, 0.3 < 0.9
+    M = np.array([[0.2, 0.95], [0.4, 0.1]])
+    assert not po.dominant_1mM(M)                  # column 1: 0.95 >= 0.9
+    M = np.array([[1.5, 0.1], [0.1, 0.0]])
+    assert po.dominant_1mM(M)                      # |1 - 1.5| = 0.5 > 0.1
+    M = np.array([[0.0, 0.5], [0.5, 0.5]])
+    assert not po.dominant_1mM(M)                  # column 1: 0.5 = |1 - 0.5|, not strict
+
+
+def test_I_abs_bounds_I():
+    """log Σ_q |a_q K| >= log |I| for every pair, with equality where the
+    terms have one sign; the condition number of the far-off-diagonal pairs
+    of a Drude block at R/L = 200 is large (up to ~1e6)."""
+    c = po.caps(200e-6, 1e-6, ldim=200, epsrel=1e-10, drude=(9 / po.HBAR_EV, 0.035 / po.HBAR_EV))
+    B = po.Block(c, 1489.1072047882992, 10)
+    conds = []
+    for l1, l2 in [(10, 10), (60, 10), (150, 10), (209, 10), (120, 100), (209, 209)]:
+        for p in (0, 1):
+            v, _ = B.I(l1, l2, p)
+            a = B.I_abs(l1, l2, p)
+            assert a >= v - 1e-12, (l1, l2, p, a, v)
+            conds.append(math.exp(a - v))
+    assert min(conds) < 1 + 1e-12 and max(conds) > 10, conds
+
+
+def test_manual_runs_fixture():
+    """The transcribed manual runs: one T = 0 run, four T = 300 K runs (18,
+    17, 18, 18 printed frequencies starting at ξ = 0) and the high-temperature
+    limit; frequencies on the Matsubara grid 2π k_B T (L+R)/(ħc)."""
+    import json
+    import os
+    from conftest import GOLDEN
+    runs = json.load(open(os.path.join(GOLDEN, "manual_runs.json")))["rows"]
+    assert len(runs) == 6
+    T300 = [r for r in runs if r["header"].get("T") == 300.0]
+    assert [len(r["xi"]) for r in T300] == [18, 17, 18, 18]
+    for r in T300:
+        Ts = 2 * math.pi * 1.38064852e-23 * 50.5e-6 * 300 / (1.0545718e-34 * 299792458.0)
+        assert r["xi"][0]["xi_"] == 0
+        for k, row in enumerate(r["xi"][1:], 1):
+            assert abs(row["xi_"] / (k * Ts) - 1) < 1e-13
