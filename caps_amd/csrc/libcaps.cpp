@@ -57,6 +57,10 @@ int capsd_lu_nb(void);
 int capsd_lu_npanels(void);
 int capsd_lu_kernels(int gk, int ps, int nt, int prev[3]);
 void capsd_launch_mfma_probe(const double*, const double*, double*, hipStream_t);
+int capsd_noalloc_scratch(char** ptr, size_t* bytes);
+void capsd_launch_noalloc_setup(char*, int64_t, int, int, int, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t,
+                                int64_t, hipStream_t);
+void capsd_launch_noalloc_result(const void*, double*, int, hipStream_t);
 }
 
 #define TERMINATE(cond, ...)                                                                      \
@@ -2527,6 +2531,57 @@ int capsx_logdet_1mM_batch(const double* M, int dim, int n, double* out, int* de
     }
     caps_free(tmp);
     return 0;
+}
+
+// Extension without a counterpart in the reference, for callers that cannot
+// allocate device memory (code under HIP graph capture, serving loops):
+// capsx_lu_logdet_batch on device data, allocating nothing.  dA holds n
+// column-major dim x dim matrices with leading dimension lda (matrix b at
+// dA + b lda dim), d_logdet receives n values log|det A_b|, both device
+// pointers; the work is enqueued on `stream` (a hipStream_t, NULL for the
+// null stream) and the call returns without waiting.  The matrices are
+// overwritten.  Descriptors and the LU's side buffers (panels, pivots, U
+// rows: about 8 lda 64 doubles per matrix) live in a fixed 64 MiB device
+// array per GPU, set up by a kernel (no host copies); calls that overlap on
+// one GPU share it and race.  Returns 0; 1 for invalid arguments, 2 if the
+// array's address cannot be had, 3 if the batch needs more than the array
+// holds (nothing is launched then), 4 for a launch error.
+int capsx_lu_logdet_batch_noalloc(double* dA, int dim, int lda, int n, double* d_logdet, void* stream)
+{
+    if (!dA || !d_logdet || dim <= 0 || dim > capsd_lu_max_dim() || n <= 0 || lda < dim) {
+        fprintf(stderr, "capsx_lu_logdet_batch_noalloc: invalid arguments (dim=%d, lda=%d, n=%d)\n", dim, lda, n);
+        return 1;
+    }
+    char* base = nullptr;
+    size_t cap = 0;
+    if (capsd_noalloc_scratch(&base, &cap) != 0) {
+        fprintf(stderr, "capsx_lu_logdet_batch_noalloc: no scratch address on this device\n");
+        return 2;
+    }
+    LuSide rel;
+    const int64_t per = lu_side_layout(rel, 0, dim, lda);   // side buffers of one matrix, offsets from 0
+    const int64_t off_side = align256((int64_t)n * sizeof(BlockDesc));
+    const int64_t off_out = off_side + align256((int64_t)n * sizeof(LuSide));
+    const int64_t off_buf = off_out + align256((int64_t)n * sizeof(BlockOut));
+    // + 256: the trailing update's 16-byte operand loads may read one
+    // element past the last U buffer (odd lda), which must stay in the array
+    const int64_t need = off_buf + (int64_t)n * per + 256;
+    if (need > (int64_t)cap) {
+        fprintf(stderr,
+                "capsx_lu_logdet_batch_noalloc: %d matrices of dimension %d need %lld bytes of scratch, the "
+                "fixed array holds %zu\n",
+                n, dim, (long long)need, cap);
+        return 3;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t a_off = (int64_t)((char*)dA - base);
+    capsd_launch_noalloc_setup(base, a_off, dim, lda, n, off_side, off_out, off_buf, per, rel.off_logs,
+                               rel.off_moved, rel.off_ut, s);
+    const LuCfg cfg = lu_config(dim, n);
+    capsd_launch_lu((const BlockDesc*)base, base, base + off_side, (BlockOut*)(base + off_out), n, dim, s, cfg.levels,
+                    cfg.xcd, cfg.tile, nullptr, nullptr);
+    capsd_launch_noalloc_result(base + off_out, d_logdet, n, s);
+    return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
 // Extension (autotuning): milliseconds of one batched LU of n synthetic

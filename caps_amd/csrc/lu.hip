@@ -30,6 +30,8 @@
 #include <stdint.h>
 #include <stdlib.h>
 
+#include <mutex>
+
 #include "blocks.h"
 #include "wave.h"
 
@@ -936,6 +938,49 @@ __global__ void k_lu_fill(const BlockDesc* __restrict__ desc, char* __restrict__
     }
 }
 
+// ---- allocation-free LU (capsx_lu_logdet_batch_noalloc) ---------------------
+// Its descriptors and the LU side buffers live in this fixed device array
+// (each GPU holds its own copy until the process ends); host code never
+// uses the array's name as an address, only the pointer that
+// hipGetSymbolAddress returns for the device (noalloc_scratch_ptr).
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+// Descriptors of n column-major dim x dim matrices at the caller's pointer
+// (arena offset a_off, matrix b at a_off + b lda dim doubles): BlockDesc at
+// scratch offset 0, LuSide at off_side, BlockOut at off_out (zeroed: no
+// shortcut, explicit form, so the LU runs with partial pivoting as for any
+// matrix), and per block the side buffers from off_buf on, `per` bytes per
+// block in the layout of the library's lu_side_layout.
+__global__ void k_lu_noalloc_setup(char* __restrict__ base, int64_t a_off, int dim, int lda, int n, int64_t off_side,
+                                   int64_t off_out, int64_t off_buf, int64_t per, int64_t o_logs, int64_t o_moved,
+                                   int64_t o_ut)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    BlockDesc d = {};
+    d.N = dim;
+    d.ldim = (dim + 1) / 2;
+    d.lda = lda;
+    d.off_mat = a_off + (int64_t)b * lda * dim * 8;
+    reinterpret_cast<BlockDesc*>(base)[b] = d;
+    LuSide s;
+    s.off_panel = off_buf + b * per;
+    s.off_logs = s.off_panel + o_logs;
+    s.off_moved = s.off_panel + o_moved;
+    s.off_ut = s.off_panel + o_ut;
+    reinterpret_cast<LuSide*>(base + off_side)[b] = s;
+    BlockOut o = {};
+    o.logdet = NAN;
+    reinterpret_cast<BlockOut*>(base + off_out)[b] = o;
+}
+
+__global__ void k_lu_noalloc_result(const BlockOut* __restrict__ out, double* __restrict__ logdet, int n)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < n) logdet[b] = out[b].logdet;
+}
+
 // MFMA operand-layout probe: D = A(16x4) B(4x16) for one wave.  A is
 // row-major 16x4, B row-major 4x16, D row-major 16x16.
 __global__ void k_mfma_probe(const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ D)
@@ -1016,14 +1061,20 @@ static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuS
 #undef PANELC
 }
 
+int capsd_noalloc_scratch(char** ptr, size_t* bytes);
+
+// Library set-up on the current device: variant switches from the
+// environment, the 1024-lane panel's dynamic LDS limit, and the
+// allocation-free LU's scratch address for this device.
 int capsd_lu_init(void)
 {
     if (const char* e = getenv("CAPS_AMD_GRID_PANEL_ROWS")) grid_panel_rows = atoi(e);
     if (const char* e = getenv("CAPS_AMD_PANEL_SET")) panel_set = atoi(e);
     if (const char* e = getenv("CAPS_AMD_GEMM_KERNEL")) gemm_kernel = atoi(e);
     if (const char* e = getenv("CAPS_AMD_NARROW_TILES")) narrow_tiles = atoi(e);
-    return (int)hipFuncSetAttribute((const void*)k_lu_panel<24, 2, 1024, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    24 * 1024 * sizeof(int));
+    char* p;
+    size_t bytes;
+    return capsd_noalloc_scratch(&p, &bytes);
 }
 
 // Kernel variants (values < 0 keep the current one; prev receives the
@@ -1193,6 +1244,49 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
     }
     hipLaunchKernelGGL(k_lu_finalize, dim3(nblocks), dim3(64), 0, s, desc, arena, sd, out, nblocks);
     return p;
+}
+
+// Device address of the allocation-free LU's scratch array on the current
+// device, looked up once per device (hipGetSymbolAddress with that device
+// current) and kept; its size in *bytes.  Returns 0, or the HIP error of
+// the lookup.
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static char* noalloc_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex noalloc_mu;
+
+int capsd_noalloc_scratch(char** ptr, size_t* bytes)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        // the 1024-lane panel's dynamic LDS, as capsd_lu_init sets it
+        e = hipFuncSetAttribute((const void*)k_lu_panel<24, 2, 1024, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                24 * 1024 * sizeof(int));
+        if (e != hipSuccess) return (int)e;
+        noalloc_ptr[dev] = (char*)p;
+    }
+    *ptr = noalloc_ptr[dev];
+    *bytes = NOALLOC_SCRATCH_BYTES;
+    return 0;
+}
+
+void capsd_launch_noalloc_setup(char* base, int64_t a_off, int dim, int lda, int n, int64_t off_side, int64_t off_out,
+                                int64_t off_buf, int64_t per, int64_t o_logs, int64_t o_moved, int64_t o_ut,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lu_noalloc_setup, dim3((n + 63) / 64), dim3(64), 0, s, base, a_off, dim, lda, n, off_side,
+                       off_out, off_buf, per, o_logs, o_moved, o_ut);
+}
+
+void capsd_launch_noalloc_result(const void* out, double* logdet, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lu_noalloc_result, dim3((n + 63) / 64), dim3(64), 0, s, (const BlockOut*)out, logdet, n);
 }
 
 void capsd_launch_lu_fill(const BlockDesc* desc, char* arena, int nblocks, unsigned long long seed, hipStream_t s)

@@ -261,6 +261,18 @@ int capsx_lu_kernels(int gemm_kernel, int panel_set, int narrow_tiles, int prev[
  * given schedule (values < 0 keep the configured ones), averaged over reps.
  * Returns 0 on success. */
 int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, int tile, double *ms);
+/* Extension without a counterpart in the reference, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops): the batched LU log|det A_b| of n column-major dim x dim matrices
+ * already in device memory (dA, leading dimension lda, matrix b at
+ * dA + b*lda*dim; overwritten), written to the device array d_logdet,
+ * enqueued on `stream` (a hipStream_t; NULL: the null stream) without
+ * waiting.  It allocates nothing: descriptors and the LU's side buffers
+ * (about 8*lda*64 doubles per matrix) live in a fixed 64 MiB device array
+ * per GPU; batches needing more are refused (3) before any launch.  Calls
+ * that overlap on one GPU share the array and race.  Returns 0; 1 invalid
+ * arguments, 2 no scratch address, 3 too large, 4 launch error. */
+int capsx_lu_logdet_batch_noalloc(double *dA, int dim, int lda, int n, double *d_logdet, void *stream);
 
 /* Extension: per-stage timing of the last batch on this caps object, in
  * milliseconds: [0] Mie, [1] K, [2] I, [3] assembly, [4] LU, [5] total
