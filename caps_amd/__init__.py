@@ -70,6 +70,8 @@ def lib():
             "capsx_logdet_1mM_batch": (i, [dp, i, i, dp, ip]),
             "capsx_mfma_probe": (i, [dp, dp, dp]),
             "kernel_logdet": (d, [i, p, p, i, i]),
+            "capsx_lu_delta_mode": (i, [i]),
+            "capsx_lu_time_form": (i, [i, i, i, i, i, i, i, i, dp]),
             "capsx_block_tables": (i, [p, d, i, ip, ip, dp, ip, ip, dp, ctypes.POINTER(ctypes.c_byte), i, i]),
             "capsx_block_nodes": (i, [p, d, i]),
             "capsx_M_copy": (i, [p, dp]),

@@ -40,15 +40,15 @@ void capsd_launch_I(const BlockDesc*, char*, const void*, const int*, const doub
                     int, int, hipStream_t);
 void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int, const BlockOut*, int, hipStream_t);
 void capsd_launch_trace(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
-void capsd_launch_dominance(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
+void capsd_launch_dominance(const BlockDesc*, char*, BlockOut*, int, int, int, hipStream_t);
 void capsd_launch_frobenius(const BlockDesc*, char*, BlockOut*, int, int, hipStream_t);
 void capsd_launch_fill_xi0(const BlockDesc*, char*, int64_t, int, hipStream_t);
 void capsd_launch_mie_points(int, const double*, const int*, double, const double*, double*, hipStream_t);
 void capsd_launch_fresnel_points(int, const double*, const double*, const double*, double*, hipStream_t);
 void capsd_launch_bessel_points(int, const int*, const double*, const double*, double*, hipStream_t);
 int capsd_launch_lu(const BlockDesc*, char*, const void*, BlockOut*, int, int, hipStream_t, int, int, int,
-                    hipEvent_t*, int*);
-void capsd_launch_lu_fill(const BlockDesc*, char*, int, unsigned long long, hipStream_t);
+                    hipEvent_t*, int*, int);
+void capsd_launch_lu_fill(const BlockDesc*, char*, int, unsigned long long, int, hipStream_t);
 int capsd_lu_max_updates(int);
 int capsd_lu_grid_panel_rows(int);
 int capsd_lu_init(void);
@@ -59,7 +59,7 @@ int capsd_lu_kernels(int gk, int ps, int nt, int prev[3]);
 void capsd_launch_mfma_probe(const double*, const double*, double*, hipStream_t);
 int capsd_noalloc_scratch(char** ptr, size_t* bytes);
 void capsd_launch_noalloc_setup(char*, int64_t, int, int, int, int64_t, int64_t, int64_t, int64_t, int64_t, int64_t,
-                                int64_t, hipStream_t);
+                                int64_t, int64_t, hipStream_t);
 void capsd_launch_noalloc_result(const void*, double*, int, hipStream_t);
 }
 
@@ -82,7 +82,7 @@ void capsd_launch_noalloc_result(const void*, double*, int, hipStream_t);
 namespace {
 
 struct LuSide {
-    int64_t off_panel, off_logs, off_moved, off_ut;
+    int64_t off_panel, off_logs, off_moved, off_ut, off_inv;
 };
 
 double* g_lfac_dev = nullptr;
@@ -273,7 +273,9 @@ int64_t lu_side_layout(LuSide& s, int64_t off, int N, int lda)
     s.off_moved = off;
     off = align256(off + capsd_lu_npanels() * (int64_t)nb * 4);
     s.off_ut = off;
-    return align256(off + capsd_lu_npanels() * (int64_t)nb * lda * 8);
+    off = align256(off + capsd_lu_npanels() * (int64_t)nb * lda * 8);
+    s.off_inv = off;   // delta form: L11^{-1} and U11^{-1} of each panel
+    return align256(off + capsd_lu_npanels() * 2 * (int64_t)nb * nb * 8);
 }
 
 double env_double(const char* name, double dflt)
@@ -299,6 +301,9 @@ std::mutex g_cfg_mu;
 LuCfg g_override;                 // used when g_have_override
 bool g_have_override = false;
 std::vector<TuneEntry> g_tune;    // autotuned table (lu_tuning.txt)
+// 1: the delta form for every column dominant block (default); 0: the
+// pivoted LU for all blocks (capsx_lu_delta_mode, a test hook)
+int g_delta_mode = 1;
 std::string g_tune_path;
 
 // Table lines: "maxN n levels streams xcd tile" ('#' starts a comment).
@@ -422,6 +427,10 @@ struct BatchPlan {
     int part_b[MAX_PARTS + 1] = {};
     int part_goff[MAX_PARTS] = {};      // first I group of each part
     int part_cls[MAX_PARTS][8] = {};    // I groups per class of each part
+    // determinant forms the LU launches kernels for (capsd_launch_lu kinds):
+    // 2 = delta form only (round-trip blocks; one that turns out not column
+    // dominant is left unfactored and run again with 1 = pivoted)
+    int lu_kinds = 2;
 };
 
 // Groups of blocks [b0, b1) that share the I coefficients a_q: same caps
@@ -530,7 +539,7 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         if (with_lu) {
             off = lu_side_layout(s, off, N, lda);
         } else
-            s.off_panel = s.off_logs = s.off_moved = s.off_ut = 0;
+            s.off_panel = s.off_logs = s.off_moved = s.off_ut = s.off_inv = 0;
         p.maxN = std::max(p.maxN, N);
         p.max_ldim = std::max(p.max_ldim, ldim);
         p.max_waves = std::max(p.max_waves, d.n_waves);
@@ -586,14 +595,16 @@ size_t block_bytes_estimate(caps_t* self, int m)
 // The LU of blocks [0, n) (descriptors already on the device) after event
 // `ready` on the library's stream, in cfg.streams groups of blocks on their
 // own streams; the library's stream waits for all groups.  With prof, HIP
-// events around every trailing-update launch (one group).  Returns the
+// events around every trailing-update launch (one group).  kinds: the
+// determinant forms the blocks may take (capsd_launch_lu).  Returns the
 // number of update launches of the first group.
-int launch_lu_groups(Ctx* c, int n, int maxN, const LuCfg& cfg, hipEvent_t ready, bool prof)
+int launch_lu_groups(Ctx* c, int n, int maxN, const LuCfg& cfg, hipEvent_t ready, bool prof, int kinds)
 {
     const int groups = std::max(1, std::min(cfg.streams, std::min(MAX_LU_STREAMS, n / 4)));
     if (groups == 1)
         return capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, maxN, c->stream, cfg.levels, cfg.xcd,
-                               cfg.tile, prof ? c->gemm_ev.data() : nullptr, prof ? c->gemm_info.data() : nullptr);
+                               cfg.tile, prof ? c->gemm_ev.data() : nullptr, prof ? c->gemm_info.data() : nullptr,
+                               kinds);
     // group 0 runs on the library's stream itself, groups 1.. on their own
     // streams, forked at `ready` and joined back: HIP put two fresh streams
     // on one hardware queue (the trace showed both groups serialised), while
@@ -604,7 +615,7 @@ int launch_lu_groups(Ctx* c, int n, int maxN, const LuCfg& cfg, hipEvent_t ready
         hipStream_t sg = g == 0 ? c->stream : c->lu_stream[g - 1];
         if (g > 0) HIPCHECK(hipStreamWaitEvent(sg, ready, 0));
         const int k = capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, b1 - b0, maxN, sg,
-                                      cfg.levels, cfg.xcd, cfg.tile, nullptr, nullptr);
+                                      cfg.levels, cfg.xcd, cfg.tile, nullptr, nullptr, kinds);
         if (g == 0) launched = k;
     }
     for (int g = 1; g < groups; g++) {
@@ -652,7 +663,7 @@ int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, b
         if (events && last) HIPCHECK(hipEventRecord(c->ev[3], sg));
         capsd_launch_assemble(c->d_desc + b0, c->arena, g_lfac_dev, nb, p.max_ldim, mode, c->d_out + b0,
                               mode == 0 ? 2 : 0, sg);
-        if (mode == 0) capsd_launch_dominance(c->d_desc + b0, c->arena, c->d_out + b0, nb, p.maxN, sg);
+        if (mode == 0) capsd_launch_dominance(c->d_desc + b0, c->arena, c->d_out + b0, nb, p.maxN, g_delta_mode, sg);
         if (mode == 0 && lu_mode_shortcuts)
             capsd_launch_frobenius(c->d_desc + b0, c->arena, c->d_out + b0, nb, lu_mode_shortcuts == 2, sg);
         if (events && last) HIPCHECK(hipEventRecord(c->ev[4], sg));
@@ -661,7 +672,7 @@ int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, b
         const LuCfg cfg = lu_config(p.maxN, n);
         const int k = capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, nb, p.maxN, sg,
                                       cfg.levels, cfg.xcd, cfg.tile, prof ? c->gemm_ev.data() : nullptr,
-                                      prof ? c->gemm_info.data() : nullptr);
+                                      prof ? c->gemm_info.data() : nullptr, p.lu_kinds);
         if (g == 0) nlaunched = k;
     }
     for (int g = 1; g < p.parts; g++) {
@@ -678,7 +689,7 @@ std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_sh
 {
     const LuCfg cfg = lu_config(p.maxN, p.n);
     struct {
-        int n, maxN, max_ldim, max_waves, max_nK, mode, shortcuts;
+        int n, maxN, max_ldim, max_waves, max_nK, mode, shortcuts, lu_kinds, delta_mode;
         int64_t max_ntri;
         int parts, part_b[MAX_PARTS + 1], part_goff[MAX_PARTS], cls[MAX_PARTS][8];
         int levels, streams, xcd, tile;
@@ -693,6 +704,8 @@ std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_sh
     k.max_nK = p.max_nK;
     k.mode = mode;
     k.shortcuts = lu_mode_shortcuts;
+    k.lu_kinds = p.lu_kinds;
+    k.delta_mode = g_delta_mode;
     k.max_ntri = p.max_ntri;
     k.parts = p.parts;
     memcpy(k.part_b, p.part_b, sizeof(k.part_b));
@@ -1271,17 +1284,37 @@ int capsx_logdetD_multi_info(caps_t* const* objs, const double* xi_, const int* 
         ensure_blocks(c, k);
         BatchPlan p = plan_batch_multi(objs + b0, c, xi_ + b0, m + b0, k, 1);
         run_batch(objs[0], c, p, 0, lu_mode);
-        for (int i = 0; i < k; i++) {
-            const BlockOut& o = c->h_out[i];
-            out[b0 + i] = finish_block(o, lu_mode);
-            if (trace) trace[b0 + i] = o.trace;
+        // blocks whose 1 - M is not column dominant were left unfactored by
+        // the delta-form LU: run them again with the pivoted LU
+        std::vector<int> again;
+        for (int i = 0; i < k; i++)
+            if (c->h_out[i].skip == 0 && !c->h_out[i].delta) again.push_back(b0 + i);
+        auto take = [&](const BlockOut& o, int i) {
+            out[i] = finish_block(o, lu_mode);
+            if (trace) trace[i] = o.trace;
             if (flags) {
                 const bool lu_ran = o.skip == 0;
-                flags[b0 + i] = (o.skip & 3) | (lu_ran && o.delta ? 4 : 0) |
-                                (lu_ran && !lu_mode && fabs(o.trace) > fabs(o.logdet) ? 8 : 0);
+                flags[i] = (o.skip & 3) | (lu_ran && o.delta ? 4 : 0) |
+                           (lu_ran && !lu_mode && fabs(o.trace) > fabs(o.logdet) ? 8 : 0);
             }
-        }
+        };
+        for (int i = 0; i < k; i++) take(c->h_out[i], b0 + i);
         for (int i = 0; i < 6; i++) tot[i] += c->timing[i];
+        if (!again.empty()) {
+            std::vector<caps_t*> ro;
+            std::vector<double> rx;
+            std::vector<int> rm;
+            for (int i : again) {
+                ro.push_back(objs[i]);
+                rx.push_back(xi_[i]);
+                rm.push_back(m[i]);
+            }
+            BatchPlan q = plan_batch_multi(ro.data(), c, rx.data(), rm.data(), (int)again.size(), 1);
+            q.lu_kinds = 1;
+            run_batch(objs[0], c, q, 0, lu_mode);
+            for (size_t j = 0; j < again.size(); j++) take(c->h_out[j], again[j]);
+            for (int i = 0; i < 6; i++) tot[i] += c->timing[i];
+        }
         b0 += k;
     }
     memcpy(c->timing, tot, sizeof(tot));
@@ -1456,7 +1489,7 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     // diagonally dominant (then the LU's delta form: no pivot search,
     // log1p of the pivots' deviations from 1)
     for (size_t i = 0; i < A.size(); i++) A[i] = -A[i];
-    bool dominant = true;
+    bool dominant = g_delta_mode != 0;
     for (int j = 0; j < dim && dominant; j++) {
         double sj = 0;
         for (int i = 0; i < dim; i++)
@@ -1489,7 +1522,7 @@ double kernel_logdet(int dim, double (*kernel)(int, int, void*), void* args, int
     HIPCHECK(hipMemcpyAsync(c->arena, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     {
         HIPCHECK(hipEventRecord(c->ev[6], c->stream));
-        launch_lu_groups(c, 1, dim, lu_config(dim, 1), c->ev[6], false);
+        launch_lu_groups(c, 1, dim, lu_config(dim, 1), c->ev[6], false, dominant ? 2 : 1);
     }
     HIPCHECK(hipMemcpyAsync(&o, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1583,7 +1616,7 @@ double sep_logdet(caps_t* self, const std::vector<double>& h, const std::vector<
     const int lu_mode = dense_mode(self->detalg);
     capsd_launch_fill_xi0(c->d_desc, c->arena, off_tab, n, s);
     capsd_launch_trace(c->d_desc, c->arena, c->d_out, 1, lu_mode, s);
-    capsd_launch_dominance(c->d_desc, c->arena, c->d_out, 1, n, s);
+    capsd_launch_dominance(c->d_desc, c->arena, c->d_out, 1, n, g_delta_mode, s);
     if (const char* path = dump_path(self->detalg)) {
         BlockOut o0;
         HIPCHECK(hipMemcpyAsync(&o0, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, s));
@@ -1595,8 +1628,11 @@ double sep_logdet(caps_t* self, const std::vector<double>& h, const std::vector<
             TERMINATE(!npy_write(path, n, [&](int i, int j) { return i == j ? dg[i] : -A[i + (size_t)j * lda]; }),
                       "cannot write %s", path);
     }
+    BlockOut o0;   // the block's form (delta or pivoted) decides the LU's kernels
+    HIPCHECK(hipMemcpyAsync(&o0, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
     HIPCHECK(hipEventRecord(c->ev[6], s));
-    launch_lu_groups(c, 1, n, lu_config(n, 1), c->ev[6], false);
+    launch_lu_groups(c, 1, n, lu_config(n, 1), c->ev[6], false, o0.delta ? 2 : 1);
     BlockOut o;
     HIPCHECK(hipMemcpyAsync(&o, c->d_out, sizeof(BlockOut), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
@@ -2457,7 +2493,7 @@ int capsx_lu_logdet_batch(const double* A, int dim, int n, double* out)
         HIPCHECK(hipMemcpyAsync(c->arena + c->h_desc[b].off_mat, A + (size_t)b * dim * dim,
                                 (size_t)dim * dim * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-    launch_lu_groups(c, n, dim, lu_config(dim, n), c->ev[0], false);
+    launch_lu_groups(c, n, dim, lu_config(dim, n), c->ev[0], false, 1);
     HIPCHECK(hipEventRecord(c->ev[1], c->stream));
     HIPCHECK(hipMemcpyAsync(o.data(), c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -2519,9 +2555,9 @@ int capsx_logdet_1mM_batch(const double* M, int dim, int n, double* out, int* de
         HIPCHECK(hipMemcpy(c->arena + d.off_mat, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice));
         HIPCHECK(hipMemcpy(c->arena + d.off_cpart, cs.data(), dim * sizeof(double), hipMemcpyHostToDevice));
     }
-    capsd_launch_dominance(c->d_desc, c->arena, c->d_out, n, dim, c->stream);
+    capsd_launch_dominance(c->d_desc, c->arena, c->d_out, n, dim, g_delta_mode, c->stream);
     HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-    launch_lu_groups(c, n, dim, lu_config(dim, n), c->ev[0], false);
+    launch_lu_groups(c, n, dim, lu_config(dim, n), c->ev[0], false, 3);
     HIPCHECK(hipMemcpyAsync(o.data(), c->d_out, n * sizeof(BlockOut), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     HIPCHECK(hipGetLastError());
@@ -2576,26 +2612,28 @@ int capsx_lu_logdet_batch_noalloc(double* dA, int dim, int lda, int n, double* d
     hipStream_t s = (hipStream_t)stream;
     const int64_t a_off = (int64_t)((char*)dA - base);
     capsd_launch_noalloc_setup(base, a_off, dim, lda, n, off_side, off_out, off_buf, per, rel.off_logs,
-                               rel.off_moved, rel.off_ut, s);
+                               rel.off_moved, rel.off_ut, rel.off_inv, s);
     const LuCfg cfg = lu_config(dim, n);
     capsd_launch_lu((const BlockDesc*)base, base, base + off_side, (BlockOut*)(base + off_out), n, dim, s, cfg.levels,
-                    cfg.xcd, cfg.tile, nullptr, nullptr);
+                    cfg.xcd, cfg.tile, nullptr, nullptr, 1);
     capsd_launch_noalloc_result(base + off_out, d_logdet, n, s);
     return hipGetLastError() == hipSuccess ? 0 : 4;
 }
 
 // Extension (autotuning): milliseconds of one batched LU of n synthetic
-// dim x dim matrices (1 - 0.3 X/sqrt(dim), X uniform in [-1, 1)) with the
-// given schedule (a value < 0 keeps the configured one), averaged over reps;
-// the matrices are regenerated before each repetition, outside the timed
-// region.  Returns 0 on success.
-int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, int tile, double* ms_out)
+// dim x dim matrices with the given schedule (a value < 0 keeps the
+// configured one), averaged over reps; the matrices are regenerated before
+// each repetition, outside the timed region.  form 0: A = 1 - 0.3 X/sqrt(dim)
+// (X uniform in [-1, 1)), the pivoted LU; form 1: 1 - M with M = 0.3 X/dim,
+// column diagonally dominant, the delta form (the path of the round-trip
+// blocks).  Returns 0 on success.
+int capsx_lu_time_form(int dim, int n, int reps, int levels, int streams, int xcd, int tile, int form, double* ms_out)
 {
-    if (dim <= 0 || dim > capsd_lu_max_dim() || n <= 0 || reps <= 0) return 1;
+    if (dim <= 0 || dim > capsd_lu_max_dim() || n <= 0 || reps <= 0 || form < 0 || form > 1) return 1;
     caps_t* tmp = caps_init(1, 1);
     Ctx* c = ctx_of(tmp);
     ensure_blocks(c, n);
-    const int lda = (dim + 15) & ~15, nb = capsd_lu_nb();
+    const int lda = (dim + 15) & ~15;
     int64_t off = 0;
     for (int b = 0; b < n; b++) {
         BlockDesc& d = c->h_desc[b];
@@ -2611,6 +2649,10 @@ int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, in
     ensure_arena(c, off);
     HIPCHECK(hipMemcpyAsync(c->d_desc, c->h_desc, n * sizeof(BlockDesc), hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->d_side, c->h_side, n * sizeof(LuSide), hipMemcpyHostToDevice, c->stream));
+    for (int b = 0; b < n; b++) {
+        memset(&c->h_out[b], 0, sizeof(BlockOut));
+        c->h_out[b].delta = form;
+    }
     LuCfg cfg = lu_config(dim, n);
     if (levels > 0) cfg.levels = std::min(levels, 3);
     if (streams > 0) cfg.streams = streams;
@@ -2618,10 +2660,10 @@ int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, in
     if (tile > 0) cfg.tile = tile <= 64 ? 64 : tile >= 256 ? 256 : 128;
     double tot = 0;
     for (int r = 0; r < reps; r++) {
-        HIPCHECK(hipMemsetAsync(c->d_out, 0, n * sizeof(BlockOut), c->stream));
-        capsd_launch_lu_fill(c->d_desc, c->arena, n, 1234ull + r, c->stream);
+        HIPCHECK(hipMemcpyAsync(c->d_out, c->h_out, n * sizeof(BlockOut), hipMemcpyHostToDevice, c->stream));
+        capsd_launch_lu_fill(c->d_desc, c->arena, n, 1234ull + r, form, c->stream);
         HIPCHECK(hipEventRecord(c->ev[0], c->stream));
-        launch_lu_groups(c, n, dim, cfg, c->ev[0], false);
+        launch_lu_groups(c, n, dim, cfg, c->ev[0], false, form ? 2 : 1);
         HIPCHECK(hipEventRecord(c->ev[1], c->stream));
         HIPCHECK(hipStreamSynchronize(c->stream));
         HIPCHECK(hipGetLastError());
@@ -2632,6 +2674,11 @@ int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, in
     *ms_out = tot / reps;
     caps_free(tmp);
     return 0;
+}
+
+int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, int tile, double* ms_out)
+{
+    return capsx_lu_time_form(dim, n, reps, levels, streams, xcd, tile, 0, ms_out);
 }
 
 // Extension: capture each batch shape of this object into a HIP graph and
@@ -2693,6 +2740,17 @@ int capsx_lu_kernels(int gemm_kernel, int panel_set, int narrow_tiles, int prev[
 {
     device_init();
     return capsd_lu_kernels(gemm_kernel, panel_set, narrow_tiles, prev);
+}
+
+// Extension (test hook): 1 = the determinant's delta form for column
+// dominant blocks (the default), 0 = the pivoted LU for every block.
+// Returns the previous setting.
+int capsx_lu_delta_mode(int mode)
+{
+    std::lock_guard<std::mutex> lock(g_cfg_mu);
+    const int old = g_delta_mode;
+    if (mode == 0 || mode == 1) g_delta_mode = mode;
+    return old;
 }
 
 // Extension: the schedule used for n blocks of largest dimension maxN:

@@ -62,6 +62,8 @@ struct LuSide {
                          // were exchanged)
     int64_t off_ut;      // NPANEL NB x lda doubles: the step's U rows, row
                          // k + i at i lda (columns at their matrix index)
+    int64_t off_inv;     // NPANEL x 2 NB x NB doubles (delta form): per panel
+                         // L11^{-1} column-major, then U11^{-1} row-major
 };
 constexpr int NPANEL = 4;    // panels per step of the three-level schedule
 
@@ -73,6 +75,12 @@ __device__ __forceinline__ int* moved_buf(char* arena, const LuSide& sd, int pb)
 {
     return at<int>(arena, sd.off_moved) + pb * NB;
 }
+__device__ __forceinline__ double* inv_buf(char* arena, const LuSide& sd, int pb)
+{
+    return at<double>(arena, sd.off_inv) + (int64_t)pb * 2 * NB * NB;
+}
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // log|1 + d| of a delta-form pivot u = 1 + d: log1p keeps the relative
 // accuracy of d (a pivot near 1 stored as 1 + d loses d below one ulp of 1).
@@ -122,7 +130,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
 {
     const int b = blockIdx.x;
     const BlockDesc& d = desc[b];
-    if (out[b].skip || k >= d.N) return;
+    if (out[b].skip || out[b].delta || k >= d.N) return;   // delta form: k_lu_diag / k_lu_l21
     const int N = d.N;
     const int64_t lda = d.lda;
     const int np = N - k;                 // panel rows
@@ -132,17 +140,13 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
     double* __restrict__ logs = at<double>(arena, side[b].off_logs);
     int* __restrict__ ipiv = moved_buf(arena, side[b], pb);
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    // delta form (uniform per block): the matrix holds 1 - M without its
-    // unit diagonal and is column diagonally dominant, so the pivot of
-    // column c is row c (no search, no exchanges), u = 1 + d
-    const bool delta = out[b].delta != 0;
 
     extern __shared__ int perm[];   // R * NT
     __shared__ double red_v2[2][NT / 64];
     __shared__ int red_i2[2][NT / 64];
     __shared__ double prow2[2][LF], xrow2[2][LF];
     __shared__ double Uc[2][CW * LF];   // solved U rows of one chunk, [row][col]
-    __shared__ double pivv[NB];         // |pivot| per panel column (delta form: d of u = 1 + d)
+    __shared__ double pivv[NB];         // |pivot| per panel column
 
     for (int r = tid; r < np; r += NT) perm[r] = r;
     __syncthreads();
@@ -216,40 +220,36 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
             int* red_i = red_i2[c & 1];
             double* prow = prow2[c & 1];
             double* xrow = xrow2[c & 1];
-            double gv = 0.0;
-            int p = c;
-            if (!delta) {
-                double bv = -1.0;
-                int bi = 0x7fffffff;
+            double bv = -1.0;
+            int bi = 0x7fffffff;
 #pragma unroll
-                for (int i = 0; i < R; i++) {
-                    const int r = tid + i * NT;
-                    const double v = fabs(a[i][j]);
-                    if (r >= c && r < np && (v > bv || (v == bv && r < bi))) {
-                        bv = v;
-                        bi = r;
-                    }
+            for (int i = 0; i < R; i++) {
+                const int r = tid + i * NT;
+                const double v = fabs(a[i][j]);
+                if (r >= c && r < np && (v > bv || (v == bv && r < bi))) {
+                    bv = v;
+                    bi = r;
                 }
-                wave_argmax(bv, bi);
-                if (lane == 0) {
-                    red_v[wid] = bv;
-                    red_i[wid] = bi;
+            }
+            wave_argmax(bv, bi);
+            if (lane == 0) {
+                red_v[wid] = bv;
+                red_i[wid] = bi;
+            }
+            __syncthreads();
+            double gv = red_v[0];
+            int p = red_i[0];
+            for (int q = 1; q < NT / 64; q++)
+                if (red_v[q] > gv || (red_v[q] == gv && red_i[q] < p)) {
+                    gv = red_v[q];
+                    p = red_i[q];
                 }
-                __syncthreads();
-                gv = red_v[0];
-                p = red_i[0];
-                for (int q = 1; q < NT / 64; q++)
-                    if (red_v[q] > gv || (red_v[q] == gv && red_i[q] < p)) {
-                        gv = red_v[q];
-                        p = red_i[q];
-                    }
-                // no finite candidate (a NaN column): keep row c, so the row
-                // map and ipiv stay in range; the NaN reaches log|u| and the
-                // determinant
-                if (p >= np) {
-                    p = c;
-                    gv = NAN;
-                }
+            // no finite candidate (a column of NaN): keep row c, so the row
+            // map and ipiv stay in range; the NaN reaches log|u| and the
+            // determinant
+            if (p >= np) {
+                p = c;
+                gv = NAN;
             }
             // owners of rows p and c publish them
             const int own_p = p % NT, own_c = c % NT;
@@ -275,23 +275,20 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
                     perm[c] = perm[p];
                     perm[p] = t;
                 }
-                if (!delta) pivv[c] = gv;
+                pivv[c] = gv;
                 ipiv[c] = p;
             }
             __syncthreads();
-            const double u = delta ? 1.0 + prow[j] : prow[j];
-            if (delta && tid == 0) pivv[c] = prow[j];
-            if (p != c) {
+            const double u = prow[j];
 #pragma unroll
-                for (int i = 0; i < R; i++) {
-                    const int r = tid + i * NT;
-                    if (r == c) {
+            for (int i = 0; i < R; i++) {
+                const int r = tid + i * NT;
+                if (r == c) {
 #pragma unroll
-                        for (int jj = 0; jj < LF; jj++) a[i][jj] = prow[jj];
-                    } else if (r == p) {
+                    for (int jj = 0; jj < LF; jj++) a[i][jj] = prow[jj];
+                } else if (r == p) {
 #pragma unroll
-                        for (int jj = 0; jj < LF; jj++) a[i][jj] = xrow[jj];
-                    }
+                    for (int jj = 0; jj < LF; jj++) a[i][jj] = xrow[jj];
                 }
             }
             // l = a / u as a product with 1/u, as LAPACK's dgetf2 scales the
@@ -325,7 +322,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
         __syncthreads();
     }
     // log|u_ii| and the panel in pivoted (logical) order
-    if (tid < nbk) logs[k + tid] = delta ? log_abs_1p(pivv[tid]) : log(pivv[tid]);
+    if (tid < nbk) logs[k + tid] = log(pivv[tid]);
     for (int r = tid; r < np; r += NT) {
         const int64_t pr = k + perm[r];
 #pragma unroll
@@ -340,6 +337,211 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
     }
 }
 
+
+// ---- delta form: panels without pivot search --------------------------------
+// For a column diagonally dominant 1 - M (the matrix holds -M, out[b].delta)
+// partial pivoting exchanges no rows, so a panel is the unpivoted block
+// factorization [A11; A21] = [L11; L21] U11: the 64 x 64 diagonal block is
+// factored by one wavefront (k_lu_diag, which also forms L11^{-1} and
+// U11^{-1}), then L21 = A21 U11^{-1} (k_lu_l21) and, for the step's U rows,
+// U12 = L11^{-1} A12 (k_lu_trsm_mfma) are products on the FP64 MFMA units,
+// spread over as many workgroups as the panel has row (column) tiles.  The
+// diagonal blocks of a dominant matrix are well conditioned (|u_ii| close to
+// 1), so products with the triangular inverses are as accurate as the
+// substitutions they replace.
+
+// The 64 x 64 diagonal block of panel pb at step k: unpivoted LU with the
+// pivots u = 1 + d (d the stored diagonal), one lane per row, rows held in
+// registers and the pivot row broadcast through LDS; log|u_ii| = log1p(d);
+// the factored block goes to the panel buffer's top rows; then lane j forms
+// column j of L11^{-1} (forward substitution) and of U11^{-1} (backward), the
+// L11 and U11 entries read as LDS broadcasts.  Rows and columns past the
+// matrix (a last panel narrower than 64) act as identity.
+__global__ void __launch_bounds__(64) k_lu_diag(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
+                                                int k, int pb)
+{
+    const int b = blockIdx.x;
+    const BlockDesc& d = desc[b];
+    if (out[b].skip || !out[b].delta) return;
+    const int N = d.N, k0 = k + pb * NB;
+    if (k0 >= N) return;
+    const int nbk = min(NB, N - k0);
+    const int64_t lda = d.lda;
+    const double* __restrict__ A = at<double>(arena, d.off_mat);
+    double* __restrict__ P = panel_buf(arena, side[b], lda, pb);
+    double* __restrict__ inv = inv_buf(arena, side[b], pb);
+    __shared__ double S[NB][NB + 1];   // [row][column], odd pitch
+    const int r = threadIdx.x;
+    double a[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) a[j] = (r < nbk && j < nbk) ? A[(k0 + r) + (int64_t)(k0 + j) * lda] : 0.0;
+#pragma unroll
+    for (int c = 0; c < NB; c++) {
+        if (r == c) {
+#pragma unroll
+            for (int j = c; j < NB; j++) S[c][j] = a[j];
+        }
+        __syncthreads();
+        const double ru = 1.0 / (1.0 + S[c][c]);
+        if (r > c) {
+            const double l = a[c] * ru;
+            a[c] = l;
+#pragma unroll
+            for (int j = c + 1; j < NB; j++) a[j] -= l * S[c][j];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NB; j++) S[r][j] = a[j];
+    __syncthreads();
+    if (r < nbk) {
+        at<double>(arena, side[b].off_logs)[k0 + r] = log_abs_1p(S[r][r]);
+        moved_buf(arena, side[b], pb)[r] = r;
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            if (j < nbk) P[r + (int64_t)j * lda] = a[j];
+    }
+    // column r of L11^{-1} (unit lower): x_i = δ_ir - Σ_{q<i} L_iq x_q
+    {
+        double x[NB];
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            double t = (i == r) ? 1.0 : 0.0;
+#pragma unroll
+            for (int q = 0; q < i; q++) t -= S[i][q] * x[q];
+            x[i] = t;
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) inv[(int64_t)r * NB + i] = x[i];          // column-major
+    }
+    // column r of U11^{-1}: x_i = (δ_ir - Σ_{q>i} U_iq x_q) / (1 + d_i)
+    {
+        double x[NB];
+#pragma unroll
+        for (int i = NB - 1; i >= 0; i--) {
+            double t = (i == r) ? 1.0 : 0.0;
+#pragma unroll
+            for (int q = i + 1; q < NB; q++) t -= S[i][q] * x[q];
+            x[i] = t / (1.0 + S[i][i]);
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) inv[NB * NB + (int64_t)i * NB + r] = x[i];   // row-major
+    }
+}
+
+// L21 = A21 U11^{-1} for the rows below the diagonal block of panel pb:
+// 4 wavefronts of 64 rows per workgroup, each 4 x 4 MFMA tiles of 16 x 16
+// over K = 64; A21 read from the matrix (16-row column segments), U11^{-1}
+// (row-major, L2-resident) as the B operand; the product, with swapped
+// operands so each register holds whole 16-row column segments, goes to the
+// panel buffer (rows relative to the panel's first row), the trailing
+// updates' L operand.
+__global__ void __launch_bounds__(256) k_lu_l21(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                               const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
+                                               int k, int pb)
+{
+    const int b = blockIdx.y;
+    const BlockDesc& d = desc[b];
+    if (out[b].skip || !out[b].delta) return;
+    const int N = d.N, k0 = k + pb * NB;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m0 = k0 + NB + blockIdx.x * 256 + wv * 64;   // this wave's first row
+    if (m0 >= N) return;
+    const int64_t lda = d.lda;
+    const double* __restrict__ A = at<double>(arena, d.off_mat);
+    double* __restrict__ P = panel_buf(arena, side[b], lda, pb);
+    const double* __restrict__ Ui = inv_buf(arena, side[b], pb) + NB * NB;
+    const int li = lane & 15, lk = lane >> 4;
+    double4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (double4_t){0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < NB; kk += 4) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = m0 + 16 * i + li;
+            av[i] = row < N ? A[row + (int64_t)(k0 + kk + lk) * lda] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) bv[j] = Ui[(kk + lk) * NB + 16 * j + li];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+    }
+    // lane l, register q of acc[i][j]: row m0 + 16i + l%16, column 16j + l/16 + 4q
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = m0 + 16 * i + li;
+        if (row < N) {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) P[(row - k0) + (int64_t)(16 * j + lk + 4 * q) * lda] = acc[i][j][q];
+        }
+    }
+}
+
+// U12 = L11^{-1} A12 for the 64 rows of panel pb over the columns
+// [r0 + cs_off, min(N, r0 + cs_off + climit)), r0 = k + pb NB, written to the
+// step's row-major U buffer (the trailing updates' B operand).  A workgroup
+// takes 128 columns: the 64 x 128 tile of A12 is staged in LDS (one column's
+// 64 rows per wavefront load, 512 contiguous bytes), each wavefront computes
+// 16 rows x 128 columns (8 MFMA tiles) over K = 64 with L11^{-1}
+// (column-major, L2-resident) as the A operand; the result's 16-column row
+// segments go straight to the U buffer.
+__global__ void __launch_bounds__(256) k_lu_trsm_mfma(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                     const LuSide* __restrict__ side,
+                                                     const BlockOut* __restrict__ out, int k, int pb, int cs_off,
+                                                     int climit)
+{
+    constexpr int TC = 128, PITCH = TC + 1;
+    __shared__ double Bs[NB * PITCH];   // [k][column]
+    const int b = blockIdx.y;
+    const BlockDesc& d = desc[b];
+    if (out[b].skip || !out[b].delta) return;
+    const int N = d.N;
+    const int r0 = k + pb * NB;
+    if (r0 + cs_off >= N) return;
+    const int cend = min(N, r0 + cs_off + climit);
+    const int c0 = r0 + cs_off + blockIdx.x * TC;
+    if (c0 >= cend) return;
+    const int64_t lda = d.lda;
+    const double* __restrict__ A = at<double>(arena, d.off_mat);
+    const double* __restrict__ Li = inv_buf(arena, side[b], pb);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // stage A12[0..64) x [c0, c0+128): wavefront wv loads columns wv, wv+4, ...
+    for (int c = wv; c < TC; c += 4) {
+        const int col = c0 + c;
+        Bs[lane * PITCH + c] = col < cend ? A[(r0 + lane) + (int64_t)col * lda] : 0.0;
+    }
+    __syncthreads();
+    const int li = lane & 15, lk = lane >> 4;
+    double4_t acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[j] = (double4_t){0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < NB; kk += 4) {
+        const double av = Li[(int64_t)(kk + lk) * NB + 16 * wv + li];   // L11^{-1}[16 wv + li][kk + lk]
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Bs[(kk + lk) * PITCH + 16 * j + li], acc[j], 0, 0, 0);
+    }
+    // lane l, register q of acc[j]: row 16 wv + l/16 + 4q, column c0 + 16j + l%16
+    double* Ut = at<double>(arena, side[b].off_ut) + (int64_t)pb * NB * lda;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int col = c0 + 16 * j + li;
+        if (col < cend) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) Ut[(int64_t)(16 * wv + lk + 4 * q) * lda + col] = acc[j][q];
+        }
+    }
+}
 
 // Tall panels (more rows than one workgroup holds in registers): the panel
 // is factored one column at a time by grid-wide kernels, 256 rows per
@@ -389,8 +591,7 @@ __global__ void __launch_bounds__(GP_ROWS) k_gp_max(const BlockDesc* __restrict_
 }
 
 // pivot of column c: reduce the candidates, exchange rows k+c and p within
-// the panel, record ipiv and log|u_cc|; delta form: the pivot is row k+c
-// (u = 1 + d), no exchange
+// the panel, record ipiv and log|u_cc|
 __global__ void __launch_bounds__(64) k_gp_pivot(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                  const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
                                                  int k, int pb, int c)
@@ -398,18 +599,11 @@ __global__ void __launch_bounds__(64) k_gp_pivot(const BlockDesc* __restrict__ d
     const int b = blockIdx.x;
     const BlockDesc& d = desc[b];
     const int N = d.N;
-    if (out[b].skip || k >= N) return;
+    if (out[b].skip || out[b].delta || k >= N) return;
     const int nbk = min(NB, N - k);
     if (c >= nbk) return;
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
-    if (out[b].delta) {
-        if (threadIdx.x == 0) {
-            moved_buf(arena, side[b], pb)[c] = c;
-            at<double>(arena, side[b].off_logs)[k + c] = log_abs_1p(A[(k + c) + (int64_t)(k + c) * lda]);
-        }
-        return;
-    }
     const double* sc = panel_buf(arena, side[b], lda, pb);
     const int nch = (N - k + GP_ROWS - 1) / GP_ROWS;
     double v = -1.0;
@@ -451,17 +645,16 @@ __global__ void __launch_bounds__(GP_ROWS) k_gp_update(const BlockDesc* __restri
     const int b = blockIdx.y;
     const BlockDesc& d = desc[b];
     const int N = d.N;
-    if (out[b].skip || k >= N || k + (int)blockIdx.x * GP_ROWS >= N) return;
+    if (out[b].skip || out[b].delta || k >= N || k + (int)blockIdx.x * GP_ROWS >= N) return;
     const int nbk = min(NB, N - k);
     if (c >= nbk) return;
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
     const int r = k + blockIdx.x * GP_ROWS + threadIdx.x;
     const int pr = k + c;
-    const bool delta = out[b].delta != 0;   // uniform per block
     double vnext = -1.0;
     if (r > pr && r < N) {
-        const double u = A[pr + (int64_t)pr * lda] + (delta ? 1.0 : 0.0);
+        const double u = A[pr + (int64_t)pr * lda];
         const double l = A[r + (int64_t)pr * lda] / u;
         A[r + (int64_t)pr * lda] = l;
         for (int j = c + 1; j < nbk; j++) {
@@ -471,7 +664,7 @@ __global__ void __launch_bounds__(GP_ROWS) k_gp_update(const BlockDesc* __restri
             if (j == c + 1) vnext = fabs(x);
         }
     }
-    if (c + 1 < nbk && !delta)
+    if (c + 1 < nbk)
         block_argmax_store(vnext, (r > pr && r < N) ? r : 0x7fffffff, panel_buf(arena, side[b], lda, pb),
                            blockIdx.x);
 }
@@ -484,7 +677,7 @@ __global__ void __launch_bounds__(GP_ROWS) k_gp_copy(const BlockDesc* __restrict
     const int b = blockIdx.y;
     const BlockDesc& d = desc[b];
     const int N = d.N;
-    if (out[b].skip || k >= N) return;
+    if (out[b].skip || out[b].delta || k >= N) return;
     const int nbk = min(NB, N - k);
     const int64_t lda = d.lda;
     const double* A = at<double>(arena, d.off_mat);
@@ -539,7 +732,7 @@ __global__ void __launch_bounds__(NTH) k_lu_swap_trsm(const BlockDesc* __restric
     const BlockDesc& d = desc[b];
     const int N = d.N;
     const int r0 = k + pb * NB;
-    if (out[b].skip || r0 + cs_off >= N) return;
+    if (out[b].skip || out[b].delta || r0 + cs_off >= N) return;   // delta form: k_lu_trsm_mfma
     const int cend = min(N, r0 + cs_off + climit);
     const int64_t lda = d.lda;
     const double* P = panel_buf(arena, side[b], lda, pb);
@@ -550,9 +743,8 @@ __global__ void __launch_bounds__(NTH) k_lu_swap_trsm(const BlockDesc* __restric
     double* Ac = at<double>(arena, d.off_mat) + (int64_t)col * lda;
     // the pivots of the panel at row k + q NB cover min(NB, N - k - q NB)
     // columns
-    if (!out[b].delta)
-        for (int q = 0; q < NPANEL; q++)
-            if ((swaps >> q) & 1) swap_rows_ipiv(Ac, moved_buf(arena, side[b], q), min(NB, N - k - q * NB), k + q * NB);
+    for (int q = 0; q < NPANEL; q++)
+        if ((swaps >> q) & 1) swap_rows_ipiv(Ac, moved_buf(arena, side[b], q), min(NB, N - k - q * NB), k + q * NB);
     double x[NB];
 #pragma unroll
     for (int i = 0; i < NB; i++) x[i] = Ac[r0 + i];
@@ -592,7 +784,6 @@ __global__ void __launch_bounds__(64 * (NPANEL - 1))
 // A22 -= L21 U12 with FP64 MFMA.  Workgroup tile 128x128 (or 64x64) of A22,
 // waves of 64x64 (4x4 tiles of 16x16 per wave, 16 accumulators), K = 64 or
 // 128 staged through LDS 16 (one-wave tiles: 8) at a time.
-typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // WM x WN wavefronts of 64 x 64 each: workgroup tile (64 WM) x (64 WN).
 template <int WM, int WN>
@@ -918,14 +1109,17 @@ __global__ void __launch_bounds__(64) k_lu_finalize(const BlockDesc* __restrict_
     if (threadIdx.x == 0) out[b].logdet = s;
 }
 
-// Synthetic LU input (autotuning): A = 1 - 0.3 X / sqrt(N) with X_ij uniform
-// in [-1, 1) from a counter-based hash of (seed, block, i, j), column-major.
-__global__ void k_lu_fill(const BlockDesc* __restrict__ desc, char* __restrict__ arena, unsigned long long seed)
+// Synthetic LU input (autotuning), X_ij uniform in [-1, 1) from a
+// counter-based hash of (seed, block, i, j), column-major: A = 1 - 0.3 X /
+// sqrt(N) (pivoted form), or -M with M = 0.3 X / N (delta form: 1 - M
+// column diagonally dominant, column sums of |M| about 0.15).
+__global__ void k_lu_fill(const BlockDesc* __restrict__ desc, char* __restrict__ arena, unsigned long long seed,
+                          int delta)
 {
     const BlockDesc& d = desc[blockIdx.y];
     const int N = d.N;
     double* A = at<double>(arena, d.off_mat);
-    const double sc = 0.3 / sqrt((double)N);
+    const double sc = delta ? 0.3 / N : 0.3 / sqrt((double)N);
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)N * N;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(e % N), j = (int)(e / N);
@@ -934,7 +1128,7 @@ __global__ void k_lu_fill(const BlockDesc* __restrict__ desc, char* __restrict__
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
         z ^= z >> 31;
         const double x = (double)(z >> 11) * 0x1p-53 * 2 - 1;
-        A[i + (int64_t)j * d.lda] = (i == j ? 1.0 : 0.0) - sc * x;
+        A[i + (int64_t)j * d.lda] = (i == j && !delta ? 1.0 : 0.0) - sc * x;
     }
 }
 
@@ -954,7 +1148,7 @@ static __device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
 // block in the layout of the library's lu_side_layout.
 __global__ void k_lu_noalloc_setup(char* __restrict__ base, int64_t a_off, int dim, int lda, int n, int64_t off_side,
                                    int64_t off_out, int64_t off_buf, int64_t per, int64_t o_logs, int64_t o_moved,
-                                   int64_t o_ut)
+                                   int64_t o_ut, int64_t o_inv)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
@@ -969,6 +1163,7 @@ __global__ void k_lu_noalloc_setup(char* __restrict__ base, int64_t a_off, int d
     s.off_logs = s.off_panel + o_logs;
     s.off_moved = s.off_panel + o_moved;
     s.off_ut = s.off_panel + o_ut;
+    s.off_inv = s.off_panel + o_inv;
     reinterpret_cast<LuSide*>(base + off_side)[b] = s;
     BlockOut o = {};
     o.logdet = NAN;
@@ -1130,10 +1325,14 @@ int capsd_lu_npanels(void) { return NPANEL; }
 // update (k_lu_gemm).
 // gemm_ev: NULL, or 2 events per update launch recorded around it;
 // gemm_info receives (k, r_off, r_lim, c_off, c_lim, KK) of each update
-// launch.  Returns the number of update launches.
+// launch.  kinds: bit 0 set if the batch may hold blocks for the pivoted LU
+// (out[b].delta = 0), bit 1 if it may hold delta-form blocks; kernels for a
+// kind the caller rules out are not launched.  Returns the number of update
+// launches.
 int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockOut* out, int nblocks, int maxN,
-                    hipStream_t s, int levels, int xcd_order, int tile, hipEvent_t* gemm_ev, int* gemm_info)
+                    hipStream_t s, int levels, int xcd_order, int tile, hipEvent_t* gemm_ev, int* gemm_info, int kinds)
 {
+    const bool piv = kinds & 1, dlt = kinds & 2;
     const LuSide* sd = (const LuSide*)side;
     const int nb8 = xcd_order ? (nblocks + 7) & ~7 : nblocks;
     constexpr int ALL = 1 << 30;   // region limit: up to N
@@ -1184,23 +1383,38 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
     };
     // swaps (bit q: panel q's pivots) and TRSM of rows k + pb NB .. +NB in
     // the columns [k + pb NB + cs_off, +cols)
+    // pivoted blocks: k_lu_swap_trsm; delta-form blocks: k_lu_trsm_mfma
+    // (each kernel leaves the other kind of block alone)
     auto trsm = [&](int k, int pb, int cs_off, int cols, int swaps) {
         const int c0 = k + pb * NB + cs_off;
         const int n = min(maxN - c0, cols);
         if (n <= 0) return;
-        if (n > 64)
+        if (piv && n > 64)
             hipLaunchKernelGGL(k_lu_swap_trsm<256>, dim3((n + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, sd,
                                out, k, pb, cs_off, cols, swaps);
-        else
+        else if (piv)
             hipLaunchKernelGGL(k_lu_swap_trsm<64>, dim3((n + 63) / 64, nblocks), dim3(64), 0, s, desc, arena, sd, out,
                                k, pb, cs_off, cols, swaps);
+        if (dlt)
+            hipLaunchKernelGGL(k_lu_trsm_mfma, dim3((n + 127) / 128, nblocks), dim3(256), 0, s, desc, arena, sd, out,
+                               k, pb, cs_off, cols);
     };
+    // pivoted blocks: the one-workgroup (or grid-wide) panel; delta-form
+    // blocks: the diagonal block, then L21 = A21 U11^{-1}
     auto panel = [&](int k, int pb) {
-        if (k + pb * NB < maxN) launch_panel(maxN - k - pb * NB, desc, arena, sd, out, nblocks, k + pb * NB, pb, s);
+        const int k0 = k + pb * NB;
+        if (k0 >= maxN) return;
+        if (piv) launch_panel(maxN - k0, desc, arena, sd, out, nblocks, k0, pb, s);
+        if (!dlt) return;
+        hipLaunchKernelGGL(k_lu_diag, dim3(nblocks), dim3(64), 0, s, desc, arena, sd, out, k, pb);
+        const int below = maxN - k0 - NB;
+        if (below > 0)
+            hipLaunchKernelGGL(k_lu_l21, dim3((below + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, sd, out, k,
+                               pb);
     };
     // buffers [qbegin, qend) take the pivots of the later panels pfirst..plast
     auto swap_panels = [&](int k, int qbegin, int qend, int pfirst, int plast) {
-        if (k + pfirst * NB < maxN)
+        if (piv && k + pfirst * NB < maxN)
             hipLaunchKernelGGL(k_lu_swap_panel, dim3(nblocks), dim3(64 * (qend - qbegin)), 0, s, desc, arena, sd, out,
                                k, qbegin, pfirst, plast);
     };
@@ -1278,10 +1492,10 @@ int capsd_noalloc_scratch(char** ptr, size_t* bytes)
 
 void capsd_launch_noalloc_setup(char* base, int64_t a_off, int dim, int lda, int n, int64_t off_side, int64_t off_out,
                                 int64_t off_buf, int64_t per, int64_t o_logs, int64_t o_moved, int64_t o_ut,
-                                hipStream_t s)
+                                int64_t o_inv, hipStream_t s)
 {
     hipLaunchKernelGGL(k_lu_noalloc_setup, dim3((n + 63) / 64), dim3(64), 0, s, base, a_off, dim, lda, n, off_side,
-                       off_out, off_buf, per, o_logs, o_moved, o_ut);
+                       off_out, off_buf, per, o_logs, o_moved, o_ut, o_inv);
 }
 
 void capsd_launch_noalloc_result(const void* out, double* logdet, int n, hipStream_t s)
@@ -1289,9 +1503,10 @@ void capsd_launch_noalloc_result(const void* out, double* logdet, int n, hipStre
     hipLaunchKernelGGL(k_lu_noalloc_result, dim3((n + 63) / 64), dim3(64), 0, s, (const BlockOut*)out, logdet, n);
 }
 
-void capsd_launch_lu_fill(const BlockDesc* desc, char* arena, int nblocks, unsigned long long seed, hipStream_t s)
+void capsd_launch_lu_fill(const BlockDesc* desc, char* arena, int nblocks, unsigned long long seed, int delta,
+                          hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lu_fill, dim3(256, nblocks), dim3(256), 0, s, desc, arena, seed);
+    hipLaunchKernelGGL(k_lu_fill, dim3(256, nblocks), dim3(256), 0, s, desc, arena, seed, delta);
 }
 
 void capsd_launch_mfma_probe(const double* A, const double* B, double* D, hipStream_t s)

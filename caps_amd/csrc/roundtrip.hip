@@ -706,9 +706,10 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
 // dominant keeps that property in every Schur complement, so LU with partial
 // pivoting exchanges no rows and the elimination without pivot search is the
 // same factorization (growth factor <= 2).  out[b].delta starts at 1 (k_trace)
-// and any non-dominant column clears it (an AND, independent of the order).
+// and any non-dominant column clears it (an AND, independent of the order);
+// allow = 0 clears it for every block (the pivoted LU for all, a test hook).
 __global__ void __launch_bounds__(256) k_dominance(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                                   BlockOut* __restrict__ out)
+                                                   BlockOut* __restrict__ out, int allow)
 {
     const int b = blockIdx.y;
     const BlockDesc& d = desc[b];
@@ -720,7 +721,7 @@ __global__ void __launch_bounds__(256) k_dominance(const BlockDesc* __restrict__
         double s = 0;
         for (int t = 0; t < d.n_rtiles; t++) s += cp[(int64_t)t * d.N];
         const double djj = at<double>(arena, d.off_mat)[j + (int64_t)j * d.lda];   // -M_jj
-        ok = s < fabs(1.0 + djj);
+        ok = allow && s < fabs(1.0 + djj);
     }
     if (__syncthreads_or(!ok) && threadIdx.x == 0) atomicAnd(&out[b].delta, 0);
 }
@@ -901,10 +902,11 @@ void capsd_launch_fill_xi0(const BlockDesc* desc, char* arena, int64_t off_tab, 
 
 // Column dominance of the blocks' 1 - M (after their assembly): the delta
 // flags, and the unit diagonal written back where a block is not dominant.
-void capsd_launch_dominance(const BlockDesc* desc, char* arena, BlockOut* out, int nblocks, int maxN, hipStream_t s)
+void capsd_launch_dominance(const BlockDesc* desc, char* arena, BlockOut* out, int nblocks, int maxN, int allow,
+                            hipStream_t s)
 {
     const dim3 g((maxN + 255) / 256, nblocks);
-    hipLaunchKernelGGL(k_dominance, g, dim3(256), 0, s, desc, arena, out);
+    hipLaunchKernelGGL(k_dominance, g, dim3(256), 0, s, desc, arena, out, allow);
     hipLaunchKernelGGL(k_dominance_fix, g, dim3(256), 0, s, desc, arena, out);
 }
 

@@ -117,3 +117,38 @@ def test_kernel_logdet_callback(ca, detalg):
             else:
                 expect = po.logdet_1mM_delta(M) if po.dominant_1mM(M) else np.linalg.slogdet(np.eye(N) - M)[1]
         assert abs(v - expect) <= 1e-12 * abs(expect), (detalg, v, expect)
+
+
+def test_pivoted_fallback_on_round_trip_blocks(ca):
+    """Round-trip blocks whose 1 - M is not column dominant are left out by
+    the delta-form LU and factored again with the pivoted LU.  The test hook
+    capsx_lu_delta_mode(0) reports every block as not dominant: all blocks
+    of the R/L = 20 configuration and of the mixed-aspect batch then take
+    that path, and agree with the delta-form values to the pivoted LU's
+    rounding: each of the N pivots u_ii = fl(1 - M_ii - ...) near 1 and its
+    log carry up to an ulp of 1, so 1e-12 relative plus N eps absolute."""
+    from caps_amd.workloads import mixed_aspect_blocks, make_objects
+    L = ca.lib()
+    c = ca.Caps(20e-6, 1e-6, ldim=120)
+    ms = np.arange(121)
+    vd, _, fd = c.logdetD_batch(np.ones(121), ms, info=True)
+    blocks = mixed_aspect_blocks()
+    objs = make_objects(blocks)
+    xs, mm = [b["xi_"] for b in blocks], [b["m"] for b in blocks]
+    wd, _, gd = ca.logdetD_multi(objs, xs, mm, info=True)
+    old = L.capsx_lu_delta_mode(0)
+    try:
+        vp, _, fp = c.logdetD_batch(np.ones(121), ms, info=True)
+        wp, _, gp = ca.logdetD_multi(objs, xs, mm, info=True)
+    finally:
+        L.capsx_lu_delta_mode(old)
+    for vals_d, vals_p, fl_d, fl_p, dims in ((vd, vp, fd, fp, [240] * 121),
+                                              (wd, wp, gd, gp, [2 * b["ldim"] for b in blocks])):
+        factored = [(f & 3) == 0 for f in fl_d]
+        assert all(f & 4 for f, x in zip(fl_d, factored) if x)          # delta form by default
+        assert not any(f & 4 for f, x in zip(fl_p, factored) if x)      # pivoted when forced
+        for a, b, x, n in zip(vals_d, vals_p, factored, dims):
+            if x:
+                assert abs(a - b) <= 1e-12 * abs(a) + n * 2.2e-16, (a, b, n)
+            else:
+                assert a == b
