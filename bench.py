@@ -231,6 +231,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch the timed steps eagerly (default: HIP graph replay)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--hbm-probe", action="store_true",
@@ -255,19 +256,32 @@ def main():
         net.attach(c, OMEGAP_EV, GAMMA_EV)
         form = "surrogate"
 
+    # warmup eagerly, then (default) with graph capture on: the step's batch
+    # shape is captured once and the timed steps replay it
     for _ in range(args.warmup):
         gpu_step(c, form)
+    if not args.no_graph:
+        c.set_graph(True)
+        gpu_step(c, form)                  # captures the batch's launch sequence
     c.reset_stats()
     times, F, nb, ncomp = [], None, 0, 0
     for _ in range(args.steps):
         F, nb, ncomp, dt = gpu_step(c, form)
         times.append(dt)
-        print("bench: %s step %.3f s, %d blocks in the energy (%d computed), F=%.16g" % (form, dt, nb, ncomp, F),
-              file=sys.stderr, flush=True)
-    st = c.stats(reset=True)
+        print("bench: %s step %.3f s, %d blocks in the energy (%d computed), F=%.16g%s" % (
+            form, dt, nb, ncomp, F, "" if args.no_graph else " (graph replay)"), file=sys.stderr, flush=True)
     total = sum(times)
     value = nb * args.steps / total
     sur_acc = c.surrogate_report(reset=True) if form == "surrogate" else None
+    # one eager step: stage times and LU rate (a replayed graph is timed as
+    # a whole), and the same F bit for bit
+    c.set_graph(False)
+    c.reset_stats()
+    Fe, _, _, dte = gpu_step(c, form)
+    st = c.stats(reset=True)
+    eager = {"seconds": dte, "F_identical_to_timed": Fe == F}
+    print("bench: eager step %.3f s" % dte, file=sys.stderr, flush=True)
+    st_steps = 1
 
     # one more step with HIP events around every launch of the dominant
     # kernel (the LU's trailing update); its launches then run one at a time
@@ -318,9 +332,11 @@ def main():
         "plain": plain,
         "batched_without_surrogate": heuristic,
         "surrogate_vs_computed": sur_acc,
-        "stage_ms_per_step": {k: v / args.steps for k, v in st["stage_ms"].items()},
+        "timed_steps": "HIP graph replay" if not args.no_graph else "eager",
+        "eager_step": eager,
+        "stage_ms_per_step": {k: v / st_steps for k, v in st["stage_ms"].items()},
         "lu_tflops": st["lu_flops"] / lu_s / 1e12 if lu_s > 0 else None,
-        "blocks_factored_per_step": st["blocks_factored"] / args.steps,
+        "blocks_factored_per_step": st["blocks_factored"] / st_steps,
         "roofline": roof,
         "errors_vs_reference": errors,
         "mixed_aspect_batch": mixed,
