@@ -2815,6 +2815,16 @@ int capsx_block_tables(caps_t* self, double xi_, int m, int* mu, int* nK, double
             sgnI[i] = (signed char)((mt > 0) - (mt < 0));
             logI[i] = mt == 0 ? -INFINITY : log(fabs(mt)) + ex[i] * M_LN2;
         }
+        // the device keeps the lower triangle (row >= column); the upper one
+        // is its mirror image
+        const size_t nl_ = (size_t)d.nl;
+        for (size_t pol = 0; pol < 2; pol++)
+            for (size_t r = 0; r < nl_; r++)
+                for (size_t col = r + 1; col < nl_; col++) {
+                    const size_t i = pol * nl_ * nl_ + r * nl_ + col, src = pol * nl_ * nl_ + col * nl_ + r;
+                    logI[i] = logI[src];
+                    sgnI[i] = sgnI[src];
+                }
     }
     return 0;
 }

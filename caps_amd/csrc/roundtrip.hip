@@ -329,7 +329,10 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
         i2 = (int)(t - (int64_t)i1 * (i1 + 1) / 2);
     }
     const int l1 = d.lo + i1, l2 = d.lo + i2;
-    const int64_t a = (int64_t)i1 * nl + i2, b = (int64_t)i2 * nl + i1;
+    // the table keeps the lower triangle only (row i1 >= column i2, row
+    // major): a wave's stores are contiguous, and readers take I(l1, l2) at
+    // (max, min)
+    const int64_t a = (int64_t)i1 * nl + i2;
     const int64_t nl2 = (int64_t)nl * nl;
 
     const int m_ = d.m > 0 ? d.m : 1;
@@ -341,9 +344,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
             int* Ie = at<int>(arena, dg.off_i + 16 * nl2);
             for (int pol = 0; pol < 2; pol++) {
                 Im[pol * nl2 + a] = 0;
-                Im[pol * nl2 + b] = 0;
                 Ie[pol * nl2 + a] = 0;
-                Ie[pol * nl2 + b] = 0;
             }
         }
         return;
@@ -361,51 +362,42 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     const double* g2 = g1 + nl;
     const double log_a0 = g1[i1] + g1[i2] + g2[i1 + i2];
 
-    // per block and polarization: K table, running sum (exponent-aligned to
-    // the q = 0 term), |q = 0 term|, its exponent, early-exit count
+    // per block and polarization: running sum (exponent-aligned to the q = 0
+    // term), its shift S - E0 and the exit threshold e^{-60} |q = 0 term|
     const double thr = 8.7565107626965203e-27;  // e^{-60}
     // the group's K tables (same nK: same m and ℓ-window); polarization pol
     // of block g at kb[g] + pol nK
     const KEnt* kb[GM];
     const int nK = d.nK;
     double sum[GM][NPOL], tthr[GM][NPOL];
-    int E0[GM][NPOL], sh[GM][NPOL], cnt[GM][NPOL], cs[GM][NPOL];
-    // a finished sum gets the shift DONE: its later terms are exact zeros
-    // (ldexp far below the smallest subnormal), so it stays unchanged
-    // without a select per term
-    constexpr int DONE = -1000000, LIVE_MIN = DONE / 2;
+    int sh[GM][NPOL];
 #pragma unroll
     for (int g = 0; g < GM; g++) {
         kb[g] = act[g] ? at<KEnt>(arena, desc[grp.blk[g]].off_k) : nullptr;
 #pragma unroll
         for (int pp = 0; pp < NPOL; pp++) {
-            const int pol = POL0 + pp;
+            sh[g][pp] = 0;
+            sum[g][pp] = 0;
+            tthr[g][pp] = 0;
             if (act[g]) {
-                const KEnt k0 = kb[g][pol * nK + l12 - mu];
-                E0[g][pp] = k0.e;
+                const KEnt k0 = kb[g][(POL0 + pp) * nK + l12 - mu];
                 sh[g][pp] = -k0.e;   // S - E0, S = 0
                 sum[g][pp] = k0.m;
                 tthr[g][pp] = thr * fabs(k0.m);
-                cnt[g][pp] = 0;
-                cs[g][pp] = 0;
-            } else {
-                E0[g][pp] = 0;
-                sh[g][pp] = DONE;   // absent or skipped
-                sum[g][pp] = 0;
-                tthr[g][pp] = 0;
-                cnt[g][pp] = 0;
-                cs[g][pp] = 0;
             }
         }
     }
 
-    // add a_q K_{l1+l2-2q} (a_q = aq 2^S, sh = S - E0) to every block's sum;
-    // from q = 3 on with the exit rules: the reference's (three terms below
-    // e^-60 of the first), or three terms below 2^-56 of the running sum
-    // (below half an ulp of the sum, they leave it unchanged)
+    // add a_q K_{l1+l2-2q} (a_q = aq 2^S, sh = S - E0) to every block's sum.
+    // From q = 3 on the lane stops after three consecutive steps in which
+    // every sum's term is small: below e^{-60} times its q = 0 term (the
+    // reference's rule, per sum) or below 2^-56 of the running sum (less
+    // than half an ulp of it: the sum no longer changes).  A sum that would
+    // stop earlier on its own takes further terms below those bounds only.
     // All of a step's K loads are issued together, before the step's a_q is
     // computed (one memory latency per step, hidden by the recurrence, not
-    // one per block and polarization).  Returns whether a sum is still live.
+    // one per block and polarization).
+    int small_run = 0;
     auto load_k = [&](int q, KEnt (&kv)[GM][NPOL]) {
         const int j = l12 - 2 * q - mu;
 #pragma unroll
@@ -415,7 +407,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 if (act[g]) kv[g][pp] = kb[g][(POL0 + pp) * nK + j];   // uniform
     };
     auto accumulate = [&](const KEnt (&kv)[GM][NPOL], double aq, bool check) {
-        bool live = false;
+        bool small = true;
 #pragma unroll
         for (int g = 0; g < GM; g++) {
             if (!act[g]) continue;
@@ -423,16 +415,11 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
             for (int pp = 0; pp < NPOL; pp++) {
                 const double term = ldexp(aq * kv[g][pp].m, kv[g][pp].e + sh[g][pp]);
                 sum[g][pp] += term;
-                if (check) {
-                    const double aterm = fabs(term);
-                    cnt[g][pp] = aterm < tthr[g][pp] ? cnt[g][pp] + 1 : 0;
-                    cs[g][pp] = aterm * 0x1p56 < fabs(sum[g][pp]) ? cs[g][pp] + 1 : 0;
-                    if (cnt[g][pp] >= 3 || cs[g][pp] >= 3) sh[g][pp] = DONE;
-                    live = live || sh[g][pp] > LIVE_MIN;
-                }
+                if (check) small = small && fabs(term) < fmax(tthr[g][pp], fabs(sum[g][pp]) * 0x1p-56);
             }
         }
-        return live;
+        if (check) small_run = small ? small_run + 1 : 0;
+        return small_run < 3;
     };
 
     double aqm2 = 1, aqm1 = 1, aq = 1;  // a_{q-2}, a_{q-1}, a_q
@@ -466,6 +453,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     const double s2 = (n + nu + 1) * (n + nu + 1), d2 = (n - nu) * (n - nu);
     AlphaND al3 = alpha_nd(n + nu - 3, s2, d2), al4 = alpha_nd(n + nu - 2, s2, d2);
     bool live = true;
+    int S = 0;   // a_q = aq 2^S
 #pragma unroll 2
     for (int q = 3; q <= qmax && live; q++) {
         // this step's K values are fetched first; the recurrence below
@@ -501,6 +489,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
             const int e = ilogb(aq);
             aq = ldexp(aq, -e);
             aqm1 = ldexp(aqm1, -e);
+            S += e;
 #pragma unroll
             for (int g = 0; g < GM; g++)
 #pragma unroll
@@ -509,7 +498,8 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
         live = accumulate(kv, aq, true);
     }
 
-    // I = e^{log a0} · sum · 2^{E0}, stored as a signed mantissa times 2^exp
+    // I = e^{log a0} · sum · 2^{E0} (E0 = S - sh), stored as a signed
+    // mantissa times 2^exp at (i1, i2), i1 >= i2
     const double la2 = log_a0 * (1 / LOG2);
     const double fl = floor(la2);
     const double a0m = exp2(la2 - fl);
@@ -527,16 +517,12 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
             if (!(sv != 0) || !isfinite(sv)) atomicOr(status, 2);
             int e2;
             const double mt = frexp(sv * a0m, &e2);
-            const int ex = e2 + a0e + E0[g][pp];
+            const int ex = e2 + a0e + S - sh[g][pp];
             Im[pol * nl2 + a] = mt;
-            Im[pol * nl2 + b] = mt;
             Ie[pol * nl2 + a] = ex;
-            Ie[pol * nl2 + b] = ex;
             if (NPOL == 1) {
                 Im[a] = -fabs(mt);
-                Im[b] = -fabs(mt);
                 Ie[a] = ex;
-                Ie[b] = ex;
             }
         }
     }
@@ -592,20 +578,27 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     if (L1 >= d.lmin + d.ldim || L2 >= d.lmin + d.ldim) return;   // uniform over the workgroup
     const int m = d.m;
     // the I values of the workgroup's 16 x 16 tiles and their ±1 halo, both
-    // polarizations, loaded once into LDS with coalesced reads; I is
-    // symmetric, the window holds I(l1, l2) at [pol][l2 - L2 + 1][l1 - L1 + 1]
-    __shared__ double wm[2][18][18];
-    __shared__ int we[2][18][18];
+    // polarizations, loaded once into LDS; the window holds I(l1, l2) at
+    // [pol][l2 - L2 + 1][l1 - L1 + 1] (rows padded to 19: conflict-free
+    // column writes).  The table keeps I(l1, l2) = I(l2, l1) once, at row
+    // max(l1, l2), column min(l1, l2) (row major): below the diagonal
+    // (L1 > L2) consecutive lanes take consecutive l2, above it consecutive
+    // l1, so that the reads are contiguous either way
+    __shared__ double wm[2][18][19];
+    __shared__ int we[2][18][19];
     __shared__ double csum[16][2][16];   // [column pair][parity][row pair]: |M| of the lane's 2 rows
     {
         const double* Im = at<double>(arena, d.off_i);
         const int* Ie = at<int>(arena, d.off_i + 16 * (int64_t)d.nl * d.nl);
         const int64_t nl2 = (int64_t)d.nl * d.nl;
+        const bool below = L1 > L2;
         for (int t = threadIdx.x; t < 2 * 18 * 18; t += 256) {
-            const int pol = t / 324, r = (t / 18) % 18, c = t % 18;
+            const int pol = t / 324, u = (t / 18) % 18, v = t % 18;
+            const int r = below ? v : u, c = below ? u : v;
             const int l1 = L1 - 1 + c, l2 = L2 - 1 + r;
             const bool ok = l1 >= d.lo && l2 >= d.lo && l1 < d.lo + d.nl && l2 < d.lo + d.nl && l1 >= m && l2 >= m;
-            const int64_t idx = pol * nl2 + (int64_t)(l2 - d.lo) * d.nl + (l1 - d.lo);
+            const int hi = max(l1, l2), lw = min(l1, l2);
+            const int64_t idx = pol * nl2 + (int64_t)(hi - d.lo) * d.nl + (lw - d.lo);
             wm[pol][r][c] = ok ? Im[idx] : 0.0;
             we[pol][r][c] = ok ? Ie[idx] : 0;
         }
