@@ -449,25 +449,28 @@ void build_igroups(caps_t* const* objs, const BlockDesc* desc, int b0, int b1, B
     // classes: perfect reflector (TM only) or both polarizations, times at
     // most 1, 2, 4 or 8 blocks; the kernel is instantiated per class
     std::vector<IGroup> cls[8];
+    // at most 8 blocks per group (k_I keeps a chunk of a_q and every sum in
+    // registers: 125 VGPRs for 8 blocks with both polarizations, 4 waves
+    // per SIMD); a run of r blocks with one key is cut into ceil(r/8) groups
+    // of near-equal size, consecutive in the input order (in a free energy
+    // neighbouring frequencies, whose sums take similar numbers of terms).
+    // CAPS_AMD_IGROUP_MAX lowers the maximum.
+    static const int genv = (int)env_double("CAPS_AMD_IGROUP_MAX", 0);
+    const int gmax = std::max(1, std::min(IG_MAX, genv > 0 ? genv : IG_MAX));
     for (int i = 0; i < n;) {
-        IGroup g;
-        memset(&g, 0, sizeof(g));
-        int j = i;
-        // at most four running sums per lane by default: groups of 2 blocks
-        // with both polarizations, of 4 for perfect reflectors (TM only);
-        // those instantiations take 110 VGPRs (4 waves per SIMD).  C2 (two
-        // polarizations) I stage: groups of 1, 2, 3, 4, 8: 61, 56, 66, 65,
-        // 86 ms.  CAPS_AMD_IGROUP_MAX sets one maximum for both.
-        static const int genv = (int)env_double("CAPS_AMD_IGROUP_MAX", 0);
-        const int gmax = std::max(1, std::min(IG_MAX, genv > 0 ? genv : (desc[idx[i]].is_pr ? 4 : 2)));
-        while (j < n && j - i < gmax && key(idx[j]) == key(idx[i])) {
-            g.blk[j - i] = idx[j];
-            j++;
+        int r = 1;
+        while (i + r < n && key(idx[i + r]) == key(idx[i])) r++;
+        const int ng = (r + gmax - 1) / gmax;
+        for (int k = 0, at = i; k < ng; k++) {
+            IGroup g;
+            memset(&g, 0, sizeof(g));
+            g.count = r / ng + (k < r % ng ? 1 : 0);
+            for (int t = 0; t < g.count; t++) g.blk[t] = idx[at + t];
+            at += g.count;
+            const int sc = g.count <= 1 ? 0 : g.count <= 2 ? 1 : g.count <= 4 ? 2 : 3;
+            cls[(desc[idx[i]].is_pr ? 0 : 4) + sc].push_back(g);
         }
-        g.count = j - i;
-        const int sc = g.count <= 1 ? 0 : g.count <= 2 ? 1 : g.count <= 4 ? 2 : 3;
-        cls[(desc[idx[i]].is_pr ? 0 : 4) + sc].push_back(g);
-        i = j;
+        i += r;
     }
     for (int k = 0; k < 8; k++) {
         cls_count[k] = (int)cls[k].size();
@@ -1259,10 +1262,46 @@ int capsx_bessel_points(int n, const int* order, const double* nu, const double*
 // column diagonally dominant: no pivot search, log1p of u_ii - 1), bit 3
 // set when the HODLR-mode rule |tr M| > |log det| replaced the LU value by
 // -tr M.  Returns 0 on success.
+static int logdetD_multi_ordered(caps_t* const* objs, const double* xi_, const int* m, int n, double* out,
+                                 double* trace, int* flags);
+
 int capsx_logdetD_multi_info(caps_t* const* objs, const double* xi_, const int* m, int n, double* out, double* trace,
                              int* flags)
 {
     if (n <= 0) return 0;
+    // the blocks of one caps object and m (whose I tables share the a_q,
+    // k_I) are run next to each other, so that they fall into one part of
+    // the batch and into full groups
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; i++) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        return std::make_tuple((uintptr_t)objs[a], m[a]) < std::make_tuple((uintptr_t)objs[b], m[b]);
+    });
+    bool same = true;
+    for (int i = 0; i < n && same; i++) same = ord[i] == i;
+    if (same) return logdetD_multi_ordered(objs, xi_, m, n, out, trace, flags);
+    std::vector<caps_t*> po(n);
+    std::vector<double> px(n), pout(n), ptr(trace ? n : 0);
+    std::vector<int> pm(n), pf(flags ? n : 0);
+    for (int i = 0; i < n; i++) {
+        po[i] = objs[ord[i]];
+        px[i] = xi_[ord[i]];
+        pm[i] = m[ord[i]];
+    }
+    const int rc = logdetD_multi_ordered(po.data(), px.data(), pm.data(), n, pout.data(), trace ? ptr.data() : nullptr,
+                                         flags ? pf.data() : nullptr);
+    if (rc) return rc;
+    for (int i = 0; i < n; i++) {
+        out[ord[i]] = pout[i];
+        if (trace) trace[ord[i]] = ptr[i];
+        if (flags) flags[ord[i]] = pf[i];
+    }
+    return 0;
+}
+
+static int logdetD_multi_ordered(caps_t* const* objs, const double* xi_, const int* m, int n, double* out,
+                                 double* trace, int* flags)
+{
     for (int i = 0; i < n; i++) {
         if (!objs[i] || !(xi_[i] > 0) || m[i] < 0) {
             fprintf(stderr, "caps_logdetD_batch: block %d has xi_=%g, m=%d (need xi_ > 0, m >= 0)\n", i, xi_[i], m[i]);

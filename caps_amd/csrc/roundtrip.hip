@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "blocks.h"
 #include "device_math.h"
@@ -293,12 +294,14 @@ __device__ __forceinline__ AlphaND alpha_nd(double p, double s2, double d2)
 // band != 0: only the pairs with i1 - i2 <= 2, which the diagonal of M
 // needs (the trace pass); use_skip != 0: blocks whose trace shortcut is
 // already decided (out[b].skip) take no part.
-template <int NPOL, int GM>
+// C: steps of the a_q recurrence per chunk (see below)
+template <int NPOL, int GM, int C>
 __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                            const IGroup* __restrict__ groups, const double* __restrict__ lfac_tab,
                                            int* __restrict__ status, const BlockOut* __restrict__ out, int band,
                                            int use_skip)
 {
+    static_assert(C >= 3, "the convergence test looks at a chunk's last three terms");
     const IGroup& grp = groups[blockIdx.y];
     const int G = grp.count;
     // the group's blocks that take part (uniform over the workgroup)
@@ -363,14 +366,16 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
     const double log_a0 = g1[i1] + g1[i2] + g2[i1 + i2];
 
     // per block and polarization: running sum (exponent-aligned to the q = 0
-    // term), its shift S - E0 and the exit threshold e^{-60} |q = 0 term|
-    const double thr = 8.7565107626965203e-27;  // e^{-60}
+    // term) and its shift S - E0 (a_q = aq 2^S, E0 the exponent of the q = 0
+    // term)
     // the group's K tables (same nK: same m and ℓ-window); polarization pol
     // of block g at kb[g] + pol nK
     const KEnt* kb[GM];
     const int nK = d.nK;
-    double sum[GM][NPOL], tthr[GM][NPOL];
+    double sum[GM][NPOL];
     int sh[GM][NPOL];
+    // bit g NPOL + pp: that sum has converged (absent blocks count as such)
+    unsigned done = 0;
 #pragma unroll
     for (int g = 0; g < GM; g++) {
         kb[g] = act[g] ? at<KEnt>(arena, desc[grp.blk[g]].off_k) : nullptr;
@@ -378,63 +383,40 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
         for (int pp = 0; pp < NPOL; pp++) {
             sh[g][pp] = 0;
             sum[g][pp] = 0;
-            tthr[g][pp] = 0;
             if (act[g]) {
                 const KEnt k0 = kb[g][(POL0 + pp) * nK + l12 - mu];
                 sh[g][pp] = -k0.e;   // S - E0, S = 0
                 sum[g][pp] = k0.m;
-                tthr[g][pp] = thr * fabs(k0.m);
+            } else {
+                done |= 1u << (g * NPOL + pp);
             }
         }
     }
+    constexpr unsigned ALL = NPOL * GM >= 32 ? ~0u : (1u << (NPOL * GM)) - 1;
 
-    // add a_q K_{l1+l2-2q} (a_q = aq 2^S, sh = S - E0) to every block's sum.
-    // From q = 3 on the lane stops after three consecutive steps in which
-    // every sum's term is small: below e^{-60} times its q = 0 term (the
-    // reference's rule, per sum) or below 2^-56 of the running sum (less
-    // than half an ulp of it: the sum no longer changes).  A sum that would
-    // stop earlier on its own takes further terms below those bounds only.
-    // All of a step's K loads are issued together, before the step's a_q is
-    // computed (one memory latency per step, hidden by the recurrence, not
-    // one per block and polarization).
-    int small_run = 0;
-    auto load_k = [&](int q, KEnt (&kv)[GM][NPOL]) {
+    // q = 1, 2: eqs. (26)-(30), added to every sum
+    auto add_term = [&](int q, double aq) {
         const int j = l12 - 2 * q - mu;
 #pragma unroll
         for (int g = 0; g < GM; g++)
 #pragma unroll
             for (int pp = 0; pp < NPOL; pp++)
-                if (act[g]) kv[g][pp] = kb[g][(POL0 + pp) * nK + j];   // uniform
+                if (act[g]) {
+                    const KEnt kv = kb[g][(POL0 + pp) * nK + j];
+                    sum[g][pp] += ldexp(aq * kv.m, kv.e + sh[g][pp]);
+                }
     };
-    auto accumulate = [&](const KEnt (&kv)[GM][NPOL], double aq, bool check) {
-        bool small = true;
-#pragma unroll
-        for (int g = 0; g < GM; g++) {
-            if (!act[g]) continue;
-#pragma unroll
-            for (int pp = 0; pp < NPOL; pp++) {
-                const double term = ldexp(aq * kv[g][pp].m, kv[g][pp].e + sh[g][pp]);
-                sum[g][pp] += term;
-                if (check) small = small && fabs(term) < fmax(tthr[g][pp], fabs(sum[g][pp]) * 0x1p-56);
-            }
-        }
-        if (check) small_run = small ? small_run + 1 : 0;
-        return small_run < 3;
-    };
-
     double aqm2 = 1, aqm1 = 1, aq = 1;  // a_{q-2}, a_{q-1}, a_q
     double tn = 0, tv = 0, r3 = 0, rn1 = 0, rv1 = 0;
     if (qmax >= 1) {
-        // eqs. (26)-(30) with the reciprocals shared between q = 1 and 2
+        // the reciprocals shared between q = 1 and 2
         rn1 = 1 / (2 * n - 1);
         rv1 = 1 / (2 * nu - 1);
         r3 = 1 / (n4 * (n4 - 1));
         tn = (m - n) * (m - n + 1) * rn1;
         tv = (m - nu) * (m - nu + 1) * rv1;
         aq = (n + nu - 1.5) * (1 - (2 * n + 2 * nu - 1) * r3 * (tn + tv));
-        KEnt kv[GM][NPOL];
-        load_k(1, kv);
-        accumulate(kv, aq, false);
+        add_term(1, aq);
     }
     if (qmax >= 2) {
         aqm2 = 1;
@@ -444,58 +426,93 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
              * ((2 * n + 2 * nu - 5) * r6
              * (tn * (m - n + 2) * (m - n + 3) * rn3 + 2 * tn * tv + tv * (m - nu + 2) * (m - nu + 3) * rv3)
              - tn - tv) + 0.5);
-        KEnt kv[GM][NPOL];
-        load_k(2, kv);
-        accumulate(kv, aq, false);
+        add_term(2, aq);
     }
-    // q >= 3: eq. (35).  α(p+3), α(p+4) of a step are α(p+1), α(p+2) of the
-    // step before (p = l1 + l2 - 2q).
+
+    // q >= 3: eq. (35), in chunks of C steps.  A chunk first runs the
+    // recurrence for its C steps (a_q of steps past qmax are zero), then
+    // adds the C terms to every sum that has not converged, with the C K
+    // loads of a sum issued together.  A sum has converged when the last
+    // three terms of a chunk are each below 2^-56 of it (less than half an
+    // ulp: they leave it unchanged); the lane stops when all have.  Later
+    // terms of a converged sum are still added where other lanes of the
+    // wave need the chunk for that sum (they are below the same bound).
+    // The reference stops a sum after three consecutive terms below e^{-60}
+    // times its q = 0 term; where that bound is the larger (a sum that
+    // cancels to below 2^56 e^{-60} of its first term) this takes more
+    // terms than the reference, never fewer significant ones.
+    // α(p+3), α(p+4) of a step are α(p+1), α(p+2) of the step before
+    // (p = l1 + l2 - 2q).  aq is renormalized to a power of two at chunk
+    // starts, so that a chunk's steps share the scale 2^S.
     const double s2 = (n + nu + 1) * (n + nu + 1), d2 = (n - nu) * (n - nu);
     AlphaND al3 = alpha_nd(n + nu - 3, s2, d2), al4 = alpha_nd(n + nu - 2, s2, d2);
-    bool live = true;
     int S = 0;   // a_q = aq 2^S
-#pragma unroll 2
-    for (int q = 3; q <= qmax && live; q++) {
-        // this step's K values are fetched first; the recurrence below
-        // covers their latency
-        KEnt kv[GM][NPOL];
-        load_k(q, kv);
-        aqm2 = aqm1;
-        aqm1 = aq;
-        const double pq = n + nu - 2 * q, p1 = pq - 2 * m, p2 = pq + 2 * m;
-        const AlphaND al1 = alpha_nd(pq + 1, s2, d2), al2 = alpha_nd(pq + 2, s2, d2);
-        if (Ap != 0) {
-            // a_q = (c1 a_{q-1} + c2 a_{q-2}) / c0 of eq. (35) with
-            //   c0 = (p+2)(p+3)(p1+1)(p1+2) Ap α1,
-            //   c1 = Ap [Ap² + (p+1)(p+3)(p1+2)(p2+2) α2 + (p+2)(p+4)(p1+3)(p2+3) α3],
-            //   c2 = -(p+2)(p+3)(p2+3)(p2+4) Ap α4;
-            // Ap cancels, and numerator and denominator are multiplied by
-            // Dα1 Dα2 Dα3 Dα4
-            const double k2 = (pq + 1) * (pq + 3) * (p1 + 2) * (p2 + 2);
-            const double k3 = (pq + 2) * (pq + 4) * (p1 + 3) * (p2 + 3);
-            const double k4 = (pq + 2) * (pq + 3) * (p2 + 3) * (p2 + 4);
-            const double d23 = al2.D * al3.D;
-            const double num1 = (Ap * Ap * d23 + k2 * al2.N * al3.D + k3 * al3.N * al2.D) * al4.D;
-            const double num2 = k4 * al4.N * d23;
-            const double den = (pq + 2) * (pq + 3) * (p1 + 1) * (p1 + 2) * al1.N * d23 * al4.D;
-            aq = (num1 * aqm1 - num2 * aqm2) * al1.D / den;
-        } else {
-            aq = (pq + 1) * (p2 + 2) * al2.N * al1.D * aqm1 / ((pq + 2) * (p1 + 1) * al1.N * al2.D);
-        }
-        al3 = al1;
-        al4 = al2;
-        const double fa = fabs(aq);
-        if (fa > 0x1p332 || (fa < 0x1p-332 && fa > 0)) {
-            const int e = ilogb(aq);
-            aq = ldexp(aq, -e);
-            aqm1 = ldexp(aqm1, -e);
-            S += e;
+    for (int q0 = 3; q0 <= qmax && done != ALL; q0 += C) {
+        {
+            const double fa = fabs(aq);
+            if (fa > 0x1p200 || (fa < 0x1p-200 && fa > 0)) {
+                const int e = ilogb(aq);
+                aq = ldexp(aq, -e);
+                aqm1 = ldexp(aqm1, -e);
+                S += e;
 #pragma unroll
-            for (int g = 0; g < GM; g++)
+                for (int g = 0; g < GM; g++)
 #pragma unroll
-                for (int pp = 0; pp < NPOL; pp++) sh[g][pp] += e;
+                    for (int pp = 0; pp < NPOL; pp++) sh[g][pp] += e;
+            }
         }
-        live = accumulate(kv, aq, true);
+        double ac[C];
+#pragma unroll
+        for (int t = 0; t < C; t++) {
+            const int q = q0 + t;
+            aqm2 = aqm1;
+            aqm1 = aq;
+            const double pq = n + nu - 2 * q, p1 = pq - 2 * m, p2 = pq + 2 * m;
+            const AlphaND al1 = alpha_nd(pq + 1, s2, d2), al2 = alpha_nd(pq + 2, s2, d2);
+            if (Ap != 0) {
+                // a_q = (c1 a_{q-1} + c2 a_{q-2}) / c0 of eq. (35) with
+                //   c0 = (p+2)(p+3)(p1+1)(p1+2) Ap α1,
+                //   c1 = Ap [Ap² + (p+1)(p+3)(p1+2)(p2+2) α2 + (p+2)(p+4)(p1+3)(p2+3) α3],
+                //   c2 = -(p+2)(p+3)(p2+3)(p2+4) Ap α4;
+                // Ap cancels, and numerator and denominator are multiplied by
+                // Dα1 Dα2 Dα3 Dα4
+                const double k2 = (pq + 1) * (pq + 3) * (p1 + 2) * (p2 + 2);
+                const double k3 = (pq + 2) * (pq + 4) * (p1 + 3) * (p2 + 3);
+                const double k4 = (pq + 2) * (pq + 3) * (p2 + 3) * (p2 + 4);
+                const double d23 = al2.D * al3.D;
+                const double num1 = (Ap * Ap * d23 + k2 * al2.N * al3.D + k3 * al3.N * al2.D) * al4.D;
+                const double num2 = k4 * al4.N * d23;
+                const double den = (pq + 2) * (pq + 3) * (p1 + 1) * (p1 + 2) * al1.N * d23 * al4.D;
+                aq = (num1 * aqm1 - num2 * aqm2) * al1.D / den;
+            } else {
+                aq = (pq + 1) * (p2 + 2) * al2.N * al1.D * aqm1 / ((pq + 2) * (p1 + 1) * al1.N * al2.D);
+            }
+            al3 = al1;
+            al4 = al2;
+            ac[t] = q <= qmax ? aq : 0.0;
+        }
+        const int jb = l12 - 2 * q0 - mu;   // K index of the chunk's first step
+#pragma unroll
+        for (int g = 0; g < GM; g++) {
+            if (!act[g]) continue;
+#pragma unroll
+            for (int pp = 0; pp < NPOL; pp++) {
+                const unsigned bit = 1u << (g * NPOL + pp);
+                if (!__any(!(done & bit))) continue;   // uniform: converged in every lane
+                const KEnt* kp = kb[g] + (POL0 + pp) * nK;
+                KEnt kv[C];
+#pragma unroll
+                for (int t = 0; t < C; t++) kv[t] = kp[max(jb - 2 * t, 0)];   // steps past qmax: any entry
+                double tm[C];
+#pragma unroll
+                for (int t = 0; t < C; t++) {
+                    tm[t] = ldexp(ac[t] * kv[t].m, kv[t].e + sh[g][pp]);
+                    sum[g][pp] += tm[t];
+                }
+                const double lim = fabs(sum[g][pp]) * 0x1p-56;
+                if (fabs(tm[C - 1]) < lim && fabs(tm[C - 2]) < lim && fabs(tm[C - 3]) < lim) done |= bit;
+            }
+        }
     }
 
     // I = e^{log a0} · sum · 2^{E0} (E0 = S - sh), stored as a signed
@@ -864,11 +881,21 @@ void capsd_launch_I(const BlockDesc* desc, char* arena, const void* groups, cons
 {
     const IGroup* gr = (const IGroup*)groups;
     const unsigned gx = (unsigned)(((band ? 3 * max_ntri : max_ntri) + 255) / 256);
+    // recurrence steps per chunk: 4 (default) or 8 (CAPS_AMD_I_CHUNK=8)
+    static const int chunk = [] {
+        const char* e = getenv("CAPS_AMD_I_CHUNK");
+        return e && atoi(e) == 8 ? 8 : 4;
+    }();
     int off = 0;
-#define KI(NP, GMV, CLS)                                                                                    \
-    if (counts[CLS] > 0)                                                                                   \
-        hipLaunchKernelGGL((k_I<NP, GMV>), dim3(gx, counts[CLS]), dim3(256), 0, s, desc, arena, gr + off,  \
-                           lfac_tab, status, out, band, use_skip);                                         \
+#define KI(NP, GMV, CLS)                                                                                       \
+    if (counts[CLS] > 0) {                                                                                    \
+        if (chunk == 8)                                                                                       \
+            hipLaunchKernelGGL((k_I<NP, GMV, 8>), dim3(gx, counts[CLS]), dim3(256), 0, s, desc, arena,       \
+                               gr + off, lfac_tab, status, out, band, use_skip);                              \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_I<NP, GMV, 4>), dim3(gx, counts[CLS]), dim3(256), 0, s, desc, arena,       \
+                               gr + off, lfac_tab, status, out, band, use_skip);                              \
+    }                                                                                                         \
     off += counts[CLS];
     KI(1, 1, 0) KI(1, 2, 1) KI(1, 4, 2) KI(1, 8, 3)
     KI(2, 1, 4) KI(2, 2, 5) KI(2, 4, 6) KI(2, 8, 7)
