@@ -38,9 +38,6 @@
 namespace capsd {
 
 constexpr int NB = 64;       // panel width
-#ifndef LU_EPI_DIRECT
-#define LU_EPI_DIRECT 1
-#endif
 #ifndef LU_KS
 #define LU_KS 16
 #endif
@@ -489,17 +486,20 @@ __global__ void __launch_bounds__(256) k_lu_l21(const BlockDesc* __restrict__ de
 // U12 = L11^{-1} A12 for the 64 rows of panel pb over the columns
 // [r0 + cs_off, min(N, r0 + cs_off + climit)), r0 = k + pb NB, written to the
 // step's row-major U buffer (the trailing updates' B operand).  A workgroup
-// takes 128 columns: the 64 x 128 tile of A12 is staged in LDS (one column's
-// 64 rows per wavefront load, 512 contiguous bytes), each wavefront computes
-// 16 rows x 128 columns (8 MFMA tiles) over K = 64 with L11^{-1}
-// (column-major, L2-resident) as the A operand; the result's 16-column row
-// segments go straight to the U buffer.
+// takes TC columns: the 64 x TC tile of A12 is staged in LDS (one column's
+// 64 rows per wavefront load, 512 contiguous bytes, all of a wavefront's
+// loads issued before its LDS stores), each wavefront computes 16 rows x TC
+// columns (TC/16 MFMA tiles) over K = 64 with L11^{-1} (column-major,
+// L2-resident) as the A operand; the result's 16-column row segments go
+// straight to the U buffer.  TC = 64 keeps four workgroups per CU (33 KB of
+// LDS each), so one workgroup's loads run under another's MFMAs.
+template <int TC>
 __global__ void __launch_bounds__(256) k_lu_trsm_mfma(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                      const LuSide* __restrict__ side,
                                                      const BlockOut* __restrict__ out, int k, int pb, int cs_off,
                                                      int climit)
 {
-    constexpr int TC = 128, PITCH = TC + 1;
+    constexpr int PITCH = TC + 1, CPW = TC / 4, NT = TC / 16;
     __shared__ double Bs[NB * PITCH];   // [k][column]
     const int b = blockIdx.y;
     const BlockDesc& d = desc[b];
@@ -514,27 +514,33 @@ __global__ void __launch_bounds__(256) k_lu_trsm_mfma(const BlockDesc* __restric
     const double* __restrict__ A = at<double>(arena, d.off_mat);
     const double* __restrict__ Li = inv_buf(arena, side[b], pb);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // stage A12[0..64) x [c0, c0+128): wavefront wv loads columns wv, wv+4, ...
-    for (int c = wv; c < TC; c += 4) {
-        const int col = c0 + c;
-        Bs[lane * PITCH + c] = col < cend ? A[(r0 + lane) + (int64_t)col * lda] : 0.0;
+    // stage A12[0..64) x [c0, c0+TC): wavefront wv loads columns wv, wv+4, ...
+    {
+        double v[CPW];
+#pragma unroll
+        for (int t = 0; t < CPW; t++) {
+            const int col = c0 + wv + 4 * t;
+            v[t] = col < cend ? A[(r0 + lane) + (int64_t)col * lda] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < CPW; t++) Bs[lane * PITCH + wv + 4 * t] = v[t];
     }
     __syncthreads();
     const int li = lane & 15, lk = lane >> 4;
-    double4_t acc[8];
+    double4_t acc[NT];
 #pragma unroll
-    for (int j = 0; j < 8; j++) acc[j] = (double4_t){0, 0, 0, 0};
+    for (int j = 0; j < NT; j++) acc[j] = (double4_t){0, 0, 0, 0};
 #pragma unroll
     for (int kk = 0; kk < NB; kk += 4) {
         const double av = Li[(int64_t)(kk + lk) * NB + 16 * wv + li];   // L11^{-1}[16 wv + li][kk + lk]
 #pragma unroll
-        for (int j = 0; j < 8; j++)
+        for (int j = 0; j < NT; j++)
             acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Bs[(kk + lk) * PITCH + 16 * j + li], acc[j], 0, 0, 0);
     }
     // lane l, register q of acc[j]: row 16 wv + l/16 + 4q, column c0 + 16j + l%16
     double* Ut = at<double>(arena, side[b].off_ut) + (int64_t)pb * NB * lda;
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
+    for (int j = 0; j < NT; j++) {
         const int col = c0 + 16 * j + li;
         if (col < cend) {
 #pragma unroll
@@ -786,6 +792,51 @@ __global__ void __launch_bounds__(64 * (NPANEL - 1))
 // 128 staged through LDS 16 (one-wave tiles: 8) at a time.
 
 // WM x WN wavefronts of 64 x 64 each: workgroup tile (64 WM) x (64 WN).
+// End of a trailing-update tile (64 x 64 per wavefront, swapped-operand
+// MFMA layout: lane l, register q of acc[i][j] holds row r0 + 16i + rr,
+// column c0 + 16j + cq + 4q, rr = l%16, cq = l/16): acc holds C - L U
+// (pre) or -L U, to be added to C (loads of half a 16-row band issued
+// before its stores).
+__device__ __forceinline__ void gemm_store_tile(double* __restrict__ A, int64_t lda, const double4_t (&acc)[4][4],
+                                                int r0, int c0, int rr, int cq, int rend, int cend, bool pre)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = r0 + 16 * i + rr;
+        if (row >= rend) continue;
+        if (pre) {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int col = c0 + 16 * j + cq + 4 * q;
+                    if (col < cend) A[row + (int64_t)col * lda] = acc[i][j][q];
+                }
+        } else {
+            // half a band at a time (8 loads in flight: no spills beside
+            // the accumulators at three workgroups per CU)
+#pragma unroll
+            for (int h = 0; h < 4; h += 2) {
+                double cv[2][4];
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int col = c0 + 16 * (h + j) + cq + 4 * q;
+                        cv[j][q] = col < cend ? A[row + (int64_t)col * lda] : 0.0;
+                    }
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int col = c0 + 16 * (h + j) + cq + 4 * q;
+                        if (col < cend) A[row + (int64_t)col * lda] = cv[j][q] + acc[i][h + j][q];
+                    }
+            }
+        }
+    }
+}
+
 template <int WM, int WN>
 __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
@@ -828,21 +879,37 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     double* A = at<double>(arena, d.off_mat);
     const double* Pb = panel_buf(arena, side[b], lda, 0);
     const double* Ut = at<double>(arena, side[b].off_ut);
-    // one LDS buffer: the A and B stages, reused for the C strips at the end
-    constexpr int STG = KS * (GTPM + GTPN), STRIPS = WM * WN * (64 * 17 + 4);
-    constexpr int SMEM = (LU_EPI_DIRECT || STG > STRIPS) ? STG : STRIPS;   // stages (then the epilogue strips)
+    // one LDS buffer: the A and B stages
+    constexpr int SMEM = KS * (GTPM + GTPN);
     __shared__ double smem[SMEM];
     double(*As)[GTPM] = reinterpret_cast<double(*)[GTPM]>(smem);             // [kk][row]
     double(*Bs)[GTPN] = reinterpret_cast<double(*)[GTPN]>(smem + KS * GTPM);  // [kk][col]
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
     const int wr = (wv % WM) * 64, wc = (wv / WM) * 64;
+    const int li = lane & 15, lk = lane >> 4;
+    // With the operands swapped the MFMA yields the transposed tile: lane l,
+    // register q of acc[i][j] holds C row wr + 16i + l%16, column
+    // wc + 16j + l/16 + 4q, so each register's 64 lanes cover four whole
+    // 128-byte column segments.  The accumulators take -L U.  Delta-form
+    // blocks (entries of -M, small): they start as the C tile (loaded beside
+    // the first stage) and the tile ends with stores only.  Pivoted blocks
+    // (unit diagonal): they start at zero and C + acc is formed at the end,
+    // one rounding at the scale of C per update instead of one per MFMA.
+    const int rr = lane & 15, cq = lane >> 4;
+    const bool pre = out[b].delta;
     double4_t acc[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 4; i++) {
+        const int row = m0 + wr + 16 * i + rr;
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = (double4_t){0, 0, 0, 0};
-    const int li = lane & 15, lk = lane >> 4;
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int col = n0 + wc + 16 * j + cq + 4 * q;
+                acc[i][j][q] = (pre && row < rend && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
+            }
+    }
     // the next stage's operands are fetched into registers while the
     // current stage runs on the MFMA units
     constexpr int PERA = (KS * GTM) / NT, PERB = (KS * GTN) / NT;
@@ -888,88 +955,17 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
         for (int kk = 0; kk < KS; kk += 4) {
             double av[4], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) av[i] = As[kk + lk][wr + 16 * i + li];
+            for (int i = 0; i < 4; i++) av[i] = -As[kk + lk][wr + 16 * i + li];
 #pragma unroll
             for (int j = 0; j < 4; j++) bv[j] = Bs[kk + lk][wc + 16 * j + li];
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-#if LU_EPI_DIRECT
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
-#else
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
-#endif
         }
     }
-#if LU_EPI_DIRECT
-    // With the operands swapped the MFMA yields the transposed tile: lane l,
-    // register q of acc[i][j] holds C row wr + 16i + l%16, column
-    // wc + 16j + l/16 + 4q, so each register's 64 lanes cover four whole
-    // 128-byte column segments and C is updated straight from the
-    // accumulators, one row band i at a time (loads of the band before its
-    // stores).
-    {
-        const int rr = lane & 15, cq = lane >> 4;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = m0 + wr + 16 * i + rr;
-            const bool rok = row < rend;
-            double cv[4][4];
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int col = n0 + wc + 16 * j + cq + 4 * q;
-                    cv[j][q] = (rok && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
-                }
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int col = n0 + wc + 16 * j + cq + 4 * q;
-                    if (rok && col < cend) A[row + (int64_t)col * lda] = cv[j][q] - acc[i][j][q];
-                }
-        }
-    }
-#else
-    // D layout (measured with k_mfma_probe): lane l, register q holds row
-    // l/16 + 4q, column l%16 of the 16x16 tile.  Each wave passes its
-    // 64x64 result through LDS one 16-row strip at a time so that the
-    // read-modify-write of C touches whole 128-byte column segments: lane
-    // (r = l%16, c = l/16) updates rows r of columns c, c+4, ..., c+60, all
-    // loads of a strip issued before its stores.
-    // per-wave strip [col][row], 64 columns of 16 rows at a pitch of 17 so
-    // that the 16 lanes writing one row fall on distinct banks
-    double* Ts = smem + wv * (64 * 17 + 4);
-    __syncthreads();                                // As/Bs are reused
-    const int rr = lane & 15, cc = lane >> 4;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) Ts[(16 * j + li) * 17 + (lk + 4 * q)] = acc[i][j][q];
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int row = m0 + wr + 16 * i + rr;
-        double cv[16];
-#pragma unroll
-        for (int t = 0; t < 16; t++) {
-            const int col = n0 + wc + cc + 4 * t;
-            cv[t] = (row < rend && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
-        }
-#pragma unroll
-        for (int t = 0; t < 16; t++) {
-            const int col = n0 + wc + cc + 4 * t;
-            if (row < rend && col < cend) A[row + (int64_t)col * lda] = cv[t] - Ts[(cc + 4 * t) * 17 + rr];
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
-#endif
+    gemm_store_tile(A, lda, acc, m0 + wr, n0 + wc, rr, cq, rend, cend, pre);
 }
 
 // A22 -= L21 U12 on 128x128 tiles (2x2 wavefronts of 64x64) with the
@@ -1025,11 +1021,6 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
     __shared__ double sB[2][KS][PITCH];   // [stage][kk][col]
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
-    double4_t acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = (double4_t){0, 0, 0, 0};
     const int li = lane & 15, lk = lane >> 4;
     const bool arow_ok = m0 + 2 * lane < rend, bcol_ok = n0 + 2 * lane < cend;
     // stage of depth KS at kk0 + ks into buffer buf: wavefront wv moves kk
@@ -1051,6 +1042,27 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
                              :: "s"(lb), "v"(Up + (int64_t)kk * lda) : "memory", "m0");
         }
     };
+    // swapped operands: lane l, register q of acc[i][j] holds C row
+    // wr + 16i + l%16, column wc + 16j + l/16 + 4q.  The accumulators take
+    // -L U.  Delta-form blocks (entries of -M, small): they start as the C
+    // tile (loaded beside the first stage, one latency for both) and the
+    // tile ends with stores only.  Pivoted blocks (unit diagonal): they
+    // start at zero and C + acc is formed at the end, one rounding at the
+    // scale of C per update instead of one per MFMA.
+    const int rr = lane & 15, cq = lane >> 4;
+    const bool pre = out[b].delta;
+    double4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int row = m0 + wr + 16 * i + rr;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int col = n0 + wc + 16 * j + cq + 4 * q;
+                acc[i][j][q] = (pre && row < rend && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
+            }
+    }
     issue(0, 0);
     const int S = KK / KS;
     for (int s = 0; s < S; s++) {
@@ -1062,7 +1074,7 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
         for (int kk = 0; kk < KS; kk += 4) {
             double av[4], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) av[i] = sA[buf][kk + lk][wr + 16 * i + li];
+            for (int i = 0; i < 4; i++) av[i] = -sA[buf][kk + lk][wr + 16 * i + li];
 #pragma unroll
             for (int j = 0; j < 4; j++) bv[j] = sB[buf][kk + lk][wc + 16 * j + li];
 #pragma unroll
@@ -1072,30 +1084,7 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
         }
     }
-    // swapped operands: lane l, register q of acc[i][j] holds C row
-    // wr + 16i + l%16, column wc + 16j + l/16 + 4q; C is updated straight
-    // from the accumulators one row band at a time
-    const int rr = lane & 15, cq = lane >> 4;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int row = m0 + wr + 16 * i + rr;
-        const bool rok = row < rend;
-        double cv[4][4];
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int col = n0 + wc + 16 * j + cq + 4 * q;
-                cv[j][q] = (rok && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
-            }
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int col = n0 + wc + 16 * j + cq + 4 * q;
-                if (rok && col < cend) A[row + (int64_t)col * lda] = cv[j][q] - acc[i][j][q];
-            }
-    }
+    gemm_store_tile(A, lda, acc, m0 + wr, n0 + wc, rr, cq, rend, cend, pre);
 }
 
 // Kahan sum of log|u_ii| (src/matrix.c:462-471).
@@ -1215,6 +1204,9 @@ static int gemm_kernel = 2;
 // 128 x 64 and 64 x 128 tiles for regions one panel wide or high, with the
 // 128 x 128 tile (CAPS_AMD_NARROW_TILES, default 1)
 static int narrow_tiles = 1;
+// columns per workgroup of the delta form's U12 kernel (CAPS_AMD_TRSM_COLS:
+// 64, the default, or 128)
+static int trsm_cols = 64;
 
 static void launch_panel(int rows, const BlockDesc* desc, char* arena, const LuSide* side, const BlockOut* out, int nb,
                          int k, int pb, hipStream_t s)
@@ -1267,6 +1259,7 @@ int capsd_lu_init(void)
     if (const char* e = getenv("CAPS_AMD_PANEL_SET")) panel_set = atoi(e);
     if (const char* e = getenv("CAPS_AMD_GEMM_KERNEL")) gemm_kernel = atoi(e);
     if (const char* e = getenv("CAPS_AMD_NARROW_TILES")) narrow_tiles = atoi(e);
+    if (const char* e = getenv("CAPS_AMD_TRSM_COLS")) trsm_cols = atoi(e) == 128 ? 128 : 64;
     char* p;
     size_t bytes;
     return capsd_noalloc_scratch(&p, &bytes);
@@ -1395,9 +1388,12 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         else if (piv)
             hipLaunchKernelGGL(k_lu_swap_trsm<64>, dim3((n + 63) / 64, nblocks), dim3(64), 0, s, desc, arena, sd, out,
                                k, pb, cs_off, cols, swaps);
-        if (dlt)
-            hipLaunchKernelGGL(k_lu_trsm_mfma, dim3((n + 127) / 128, nblocks), dim3(256), 0, s, desc, arena, sd, out,
-                               k, pb, cs_off, cols);
+        if (dlt && trsm_cols == 128)
+            hipLaunchKernelGGL(k_lu_trsm_mfma<128>, dim3((n + 127) / 128, nblocks), dim3(256), 0, s, desc, arena, sd,
+                               out, k, pb, cs_off, cols);
+        else if (dlt)
+            hipLaunchKernelGGL(k_lu_trsm_mfma<64>, dim3((n + 63) / 64, nblocks), dim3(256), 0, s, desc, arena, sd,
+                               out, k, pb, cs_off, cols);
     };
     // pivoted blocks: the one-workgroup (or grid-wide) panel; delta-form
     // blocks: the diagonal block, then L21 = A21 U11^{-1}

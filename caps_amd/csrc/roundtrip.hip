@@ -576,13 +576,18 @@ __device__ __forceinline__ UV uv(int l, int m)
     return {(l + 1.) * (l + m) * r, l * (l - m + 1.) * r};
 }
 
-// flags bit 0: the trace pass: only the diagonal tiles (blockIdx.y unused)
-// and only the diagonal of M (from I pairs with |l1 - l2| <= 2); bit 1:
-// leave out the blocks with out[b].skip set.
+// flags bit 0: the trace pass: only the diagonal tiles and only the
+// diagonal of M (from I pairs with |l1 - l2| <= 2); bit 1: leave out the
+// blocks with out[b].skip set.
+// Full pass: workgroup blockIdx.x takes the tile pair (bx, by), bx >= by
+// (linear index over the lower triangle of 16 x 16 tiles of (l1, l2)): the
+// tile at rows of bx and columns of by and its mirror at rows of by and
+// columns of bx, both from one I window (I is symmetric), so every I value
+// is read about once.  Trace pass: blockIdx.x = bx = by.
 // mode 0 (LU path) writes -M, i.e. 1 - M without its unit diagonal (the LU's
 // delta form; k_dominance puts the unit diagonal back where the LU needs
-// partial pivoting), and for every column of the workgroup's 32 rows the sum
-// of |M_ij| over its rows i != j into off_cpart (row tile blockIdx.x);
+// partial pivoting), and for every column of a tile's 32 rows the sum of
+// |M_ij| over its rows i != j into off_cpart (at the tile's row tile);
 // mode 1 writes M.
 __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                   const double* __restrict__ lfac_tab, int mode,
@@ -591,16 +596,26 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     const BlockDesc& d = desc[blockIdx.z];
     if ((flags & 2) && out[blockIdx.z].skip) return;
     const bool diag_only = flags & 1;
-    const int L1 = d.lmin + blockIdx.x * 16, L2 = d.lmin + (diag_only ? blockIdx.x : blockIdx.y) * 16;
-    if (L1 >= d.lmin + d.ldim || L2 >= d.lmin + d.ldim) return;   // uniform over the workgroup
+    int bx, by;
+    if (diag_only) {
+        bx = by = blockIdx.x;
+    } else {
+        const int64_t t = blockIdx.x;
+        bx = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while ((int64_t)bx * (bx + 1) / 2 > t) bx--;
+        while ((int64_t)(bx + 1) * (bx + 2) / 2 <= t) bx++;
+        by = (int)(t - (int64_t)bx * (bx + 1) / 2);
+    }
+    const int L1 = d.lmin + bx * 16, L2 = d.lmin + by * 16;
+    if (L1 >= d.lmin + d.ldim) return;   // uniform over the workgroup (L2 <= L1)
     const int m = d.m;
-    // the I values of the workgroup's 16 x 16 tiles and their ±1 halo, both
+    // the I values of the workgroup's 16 x 16 tile and its ±1 halo, both
     // polarizations, loaded once into LDS; the window holds I(l1, l2) at
     // [pol][l2 - L2 + 1][l1 - L1 + 1] (rows padded to 19: conflict-free
-    // column writes).  The table keeps I(l1, l2) = I(l2, l1) once, at row
+    // column accesses).  The table keeps I(l1, l2) = I(l2, l1) once, at row
     // max(l1, l2), column min(l1, l2) (row major): below the diagonal
-    // (L1 > L2) consecutive lanes take consecutive l2, above it consecutive
-    // l1, so that the reads are contiguous either way
+    // (L1 > L2) consecutive lanes take consecutive l2, so that the reads are
+    // contiguous
     __shared__ double wm[2][18][19];
     __shared__ int we[2][18][19];
     __shared__ double csum[16][2][16];   // [column pair][parity][row pair]: |M| of the lane's 2 rows
@@ -622,19 +637,28 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     }
     __syncthreads();
     const int c0 = threadIdx.x & 15, r0 = threadIdx.x >> 4;
-    const int i2 = (L1 - d.lmin) + c0, j2 = (L2 - d.lmin) + r0;
     const bool partials = mode == 0 && !diag_only && d.n_rtiles > 0;
-    const bool in = i2 < d.ldim && j2 < d.ldim && (!diag_only || i2 == j2);
-    if (in) {
+    const double* ma = at<double>(arena, d.off_mie) + 2 * d.ldim;   // mantissas of e^{ea}, e^{eb}
+    const double* mb = ma + d.ldim;
+    const int* xa = reinterpret_cast<const int*>(ma + 2 * d.ldim);  // their exponents
+    const int* xb = xa + d.ldim;
+    double* Am = at<double>(arena, d.off_mat);
+    const int64_t lda = d.lda;
+    const double sg = mode == 0 ? -1.0 : 1.0;
+
+    // The 2 x 2 block of matrix rows 2 i2, 2 i2 + 1 and columns 2 j2,
+    // 2 j2 + 1 (l1 = lmin + i2, l2 = lmin + j2), written, and for the
+    // partial column sums its |M| over its two rows per column (i2 == j2:
+    // without the diagonal) into csum[cp][.][rp].  I(l1 + a, l2 + b) is
+    // wm[pol][wr + b][wc + a] (tr = 0) or wm[pol][wr + a][wc + b] (tr = 1,
+    // the mirror tile: the window of (l2, l1)).
+    auto block = [&](int i2, int j2, int wr, int wc, bool tr, int cp, int rp) {
         const int l1 = d.lmin + i2, l2 = d.lmin + j2;
         // I(l1 + a, l2 + b) of polarization pol times the prefactor pm 2^pe
         auto X = [&](int pol, int a, int b, double pm, int pe) {
-            return ldexp(wm[pol][r0 + 1 + b][c0 + 1 + a] * pm, we[pol][r0 + 1 + b][c0 + 1 + a] + pe);
+            const int rr = tr ? wr + a : wr + b, cc = tr ? wc + b : wc + a;
+            return ldexp(wm[pol][rr][cc] * pm, we[pol][rr][cc] + pe);
         };
-        const double* ma = at<double>(arena, d.off_mie) + 2 * d.ldim;   // mantissas of e^{ea}, e^{eb}
-        const double* mb = ma + d.ldim;
-        const int* xa = reinterpret_cast<const int*>(ma + 2 * d.ldim);  // their exponents
-        const int* xb = xa + d.ldim;
         const double ma1 = ma[i2], mb1 = mb[i2], ma2 = ma[j2], mb2 = mb[j2];
         const int xa1 = xa[i2], xb1 = xb[i2], xa2 = xa[j2], xb2 = xb[j2];
         const int TE = 0, TM = 1;
@@ -663,7 +687,7 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
             double* dg = at<double>(arena, d.off_diag);
             dg[2 * i2] = EE;
             dg[2 * i2 + 1] = MM;
-            return;   // no barrier follows in the trace pass
+            return;
         }
         double EM = 0, ME = 0;
         if (m != 0) {
@@ -673,9 +697,7 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
             EM = D(TM, pEM, eEM) - C(TE, pEM, eEM);
             ME = C(TM, pME, eME) - D(TE, pME, eME);
         }
-        double* Am = at<double>(arena, d.off_mat);
-        const int64_t r = 2 * i2, c = 2 * j2, lda = d.lda;
-        const double sg = mode == 0 ? -1.0 : 1.0;
+        const int64_t r = 2 * i2, c = 2 * j2;
         // rows r, r+1 of a column are adjacent: one 16-byte store per column
         // (r and lda even, the matrix 256-byte aligned)
         *reinterpret_cast<double2*>(Am + r + c * lda) = make_double2(sg * EE, sg * ME);
@@ -688,24 +710,57 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
         if (partials) {
             // column c: EE (row r, the diagonal when i2 = j2) and ME (row r+1);
             // column c+1: EM (row r) and MM (row r+1, the diagonal when i2 = j2)
-            csum[r0][0][c0] = (i2 == j2 ? 0.0 : fabs(EE)) + fabs(ME);
-            csum[r0][1][c0] = fabs(EM) + (i2 == j2 ? 0.0 : fabs(MM));
+            csum[cp][0][rp] = (i2 == j2 ? 0.0 : fabs(EE)) + fabs(ME);
+            csum[cp][1][rp] = fabs(EM) + (i2 == j2 ? 0.0 : fabs(MM));
         }
-    } else if (partials) {
-        csum[r0][0][c0] = 0.0;
-        csum[r0][1][c0] = 0.0;
-    }
-    if (!partials) return;
-    __syncthreads();
-    // 32 columns of the workgroup: the sum over its 32 rows in row order
-    if (threadIdx.x < 32) {
-        const int jp = threadIdx.x >> 1, par = threadIdx.x & 1;
-        const int col = 2 * ((L2 - d.lmin) + jp) + par;
-        if (col < d.N) {
-            double s = 0;
+    };
+    // the 32 columns of a tile at column tile ct, row tile rt: the sum over
+    // its 32 rows in row order
+    auto column_sums = [&](int rt, int ct) {
+        if (threadIdx.x < 32) {
+            const int jp = threadIdx.x >> 1, par = threadIdx.x & 1;
+            const int col = 2 * (ct * 16 + jp) + par;
+            if (col < d.N) {
+                double sum = 0;
 #pragma unroll
-            for (int q = 0; q < 16; q++) s += csum[jp][par][q];
-            at<double>(arena, d.off_cpart)[(int64_t)blockIdx.x * d.N + col] = s;
+                for (int q = 0; q < 16; q++) sum += csum[jp][par][q];
+                at<double>(arena, d.off_cpart)[(int64_t)rt * d.N + col] = sum;
+            }
+        }
+    };
+
+    // the tile (rows of bx, columns of by): lane (c0, r0) takes i2 = 16 bx + c0,
+    // j2 = 16 by + r0
+    {
+        const int i2 = bx * 16 + c0, j2 = by * 16 + r0;
+        const bool in = i2 < d.ldim && j2 < d.ldim && (!diag_only || i2 == j2);
+        if (in) block(i2, j2, r0 + 1, c0 + 1, false, r0, c0);
+        else if (partials) {
+            csum[r0][0][c0] = 0.0;
+            csum[r0][1][c0] = 0.0;
+        }
+        if (diag_only) return;   // no barrier follows in the trace pass
+        if (partials) {
+            __syncthreads();
+            column_sums(bx, by);
+        }
+    }
+    if (bx == by) return;   // uniform: a diagonal tile is its own mirror
+    // the mirror (rows of by, columns of bx): lane (c0, r0) takes
+    // i2 = 16 by + c0, j2 = 16 bx + r0, whose I(l1 + a, l2 + b) is the
+    // window's I(l2 + b, l1 + a)
+    {
+        if (partials) __syncthreads();   // csum is reused
+        const int i2 = by * 16 + c0, j2 = bx * 16 + r0;
+        const bool in = i2 < d.ldim && j2 < d.ldim;
+        if (in) block(i2, j2, c0 + 1, r0 + 1, true, r0, c0);
+        else if (partials) {
+            csum[r0][0][c0] = 0.0;
+            csum[r0][1][c0] = 0.0;
+        }
+        if (partials) {
+            __syncthreads();
+            column_sums(by, bx);
         }
     }
 }
@@ -906,7 +961,8 @@ void capsd_launch_I(const BlockDesc* desc, char* arena, const void* groups, cons
 void capsd_launch_assemble(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int max_ldim,
                            int mode, const BlockOut* out, int flags, hipStream_t s)
 {
-    dim3 g((max_ldim + 15) / 16, (flags & 1) ? 1 : (max_ldim + 15) / 16, nblocks);
+    const unsigned T = (unsigned)((max_ldim + 15) / 16);
+    dim3 g((flags & 1) ? T : T * (T + 1) / 2, 1, nblocks);
     hipLaunchKernelGGL(k_assemble, g, dim3(256), 0, s, desc, arena, lfac_tab, mode, out, flags);
 }
 
