@@ -287,7 +287,8 @@ double env_double(const char* name, double dflt)
 // ---- LU schedule configuration ---------------------------------------------
 // levels: 1 = 64-column steps, 2 = 128-column steps of two panels with one
 // K = 128 trailing update, 3 = 256-column steps of four panels with one
-// K = 256 trailing update; streams: LU groups of blocks on separate streams;
+// K = 256 trailing update, 4 = 512-column steps of eight panels with one
+// K = 512 trailing update; streams: LU groups of blocks on separate streams;
 // xcd: trailing-update tile order (1 = all tiles of a block on one XCD);
 // tile: trailing-update workgroup tile, 128 (x 128), 256 (x 128) or 64 (x 64).
 struct LuCfg {
@@ -324,7 +325,7 @@ int load_tuning(const char* path)
         TuneEntry e;
         if (sscanf(line, "%d %d %d %d %d %d", &e.maxN, &e.n, &e.cfg.levels, &e.cfg.streams, &e.cfg.xcd,
                    &e.cfg.tile) == 6 &&
-            e.maxN > 0 && e.n > 0 && e.cfg.levels >= 1 && e.cfg.levels <= 3 && e.cfg.streams >= 1 &&
+            e.maxN > 0 && e.n > 0 && e.cfg.levels >= 1 && e.cfg.levels <= 4 && e.cfg.streams >= 1 &&
             (e.cfg.tile == 64 || e.cfg.tile == 128 || e.cfg.tile == 256))
             t.push_back(e);
     }
@@ -374,7 +375,7 @@ LuCfg lu_config(int maxN, int n)
             c = e.cfg;
         }
     }
-    if (getenv("CAPS_AMD_LU_LEVELS")) c.levels = std::max(1, std::min(3, (int)env_double("CAPS_AMD_LU_LEVELS", 2)));
+    if (getenv("CAPS_AMD_LU_LEVELS")) c.levels = std::max(1, std::min(4, (int)env_double("CAPS_AMD_LU_LEVELS", 2)));
     if (getenv("CAPS_AMD_LU_STREAMS")) c.streams = std::max(1, (int)env_double("CAPS_AMD_LU_STREAMS", 1));
     if (getenv("CAPS_AMD_GEMM_XCD")) c.xcd = env_double("CAPS_AMD_GEMM_XCD", 0) != 0 ? 1 : 0;
     if (getenv("CAPS_AMD_GEMM_TILE")) {
@@ -591,7 +592,7 @@ size_t block_bytes_estimate(caps_t* self, int m)
     size_t lmin, lmax;
     caps_estimate_lminmax(self, m, &lmin, &lmax);
     const size_t nK = 2 * (lmin + ldim) + 4;
-    return (size_t)2 * (ldim + 2) * (ldim + 2) * 12 + lda * N * 8 + lda * 512 * 8 + nK * (24 * 2 + 24 * 64) +
+    return (size_t)2 * (ldim + 2) * (ldim + 2) * 12 + lda * N * 8 + lda * (size_t)(2 * capsd_lu_npanels() * capsd_lu_nb() + 2 * capsd_lu_nb()) * 8 + nK * (24 * 2 + 24 * 64) +
            (size_t)N * 32 + (size_t)((ldim + 15) / 16) * N * 8 + 65536;
 }
 
@@ -2693,7 +2694,7 @@ int capsx_lu_time_form(int dim, int n, int reps, int levels, int streams, int xc
         c->h_out[b].delta = form;
     }
     LuCfg cfg = lu_config(dim, n);
-    if (levels > 0) cfg.levels = std::min(levels, 3);
+    if (levels > 0) cfg.levels = std::min(levels, 4);
     if (streams > 0) cfg.streams = streams;
     if (xcd >= 0) cfg.xcd = xcd ? 1 : 0;
     if (tile > 0) cfg.tile = tile <= 64 ? 64 : tile >= 256 ? 256 : 128;
@@ -2735,7 +2736,7 @@ void capsx_set_graph(caps_t* self, int on)
     }
 }
 
-// Extension: fix the LU schedule of all later batches (levels 1|2|3, streams
+// Extension: fix the LU schedule of all later batches (levels 1|2|3|4, streams
 // >= 1, xcd 0|1, tile 64|128|256); levels = 0 returns to the autotuned table.
 // Returns 0 on success, 1 on invalid values.
 int capsx_lu_config(int levels, int streams, int xcd, int tile)
@@ -2745,7 +2746,7 @@ int capsx_lu_config(int levels, int streams, int xcd, int tile)
         g_have_override = false;
         return 0;
     }
-    if (levels < 1 || levels > 3 || streams < 1 || (xcd != 0 && xcd != 1) ||
+    if (levels < 1 || levels > 4 || streams < 1 || (xcd != 0 && xcd != 1) ||
         (tile != 64 && tile != 128 && tile != 256))
         return 1;
     g_override.levels = levels;

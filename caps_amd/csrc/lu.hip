@@ -62,7 +62,7 @@ struct LuSide {
     int64_t off_inv;     // NPANEL x 2 NB x NB doubles (delta form): per panel
                          // L11^{-1} column-major, then U11^{-1} row-major
 };
-constexpr int NPANEL = 4;    // panels per step of the three-level schedule
+constexpr int NPANEL = 8;    // panels per step of the four-level schedule
 
 __device__ __forceinline__ double* panel_buf(char* arena, const LuSide& sd, int64_t lda, int pb)
 {
@@ -1293,8 +1293,8 @@ int capsd_lu_grid_panel_rows(int rows)
 }
 
 // Upper bound of the number of trailing-update launches of one LU (the
-// three-level schedule has 8 per 256 columns, the others fewer).
-int capsd_lu_max_updates(int maxN) { return 8 * ((maxN + 4 * NB - 1) / (4 * NB)) + 2; }
+// four-level schedule has 20 per 512 columns, the others fewer).
+int capsd_lu_max_updates(int maxN) { return 20 * ((maxN + 8 * NB - 1) / (8 * NB)) + 2; }
 
 // Panel buffers per block (LuSide::off_panel holds this many lda x NB panels).
 int capsd_lu_npanels(void) { return NPANEL; }
@@ -1313,6 +1313,12 @@ int capsd_lu_npanels(void) { return NPANEL; }
 //     panels' swaps and the 256-row TRSM as four 64-row TRSMs with three
 //     strip updates between them; one K = 4NB update of the trailing matrix
 //     (half the read-modify-write of C per flop of levels 2).
+//   levels 4: 8NB-column steps: the levels-3 step on panels 0..3 applied to
+//     the step's 512 columns, the same on panels 4..7 applied to their own,
+//     the pivots of panels 4..7 into the L rows of panels 0..3; for the
+//     trailing columns all eight panels' swaps and the 512-row TRSM (eight
+//     64-row TRSMs with the updates from the panels above between them);
+//     one K = 8NB update of the trailing matrix.
 // tile: 128 (update workgroups of 128 x 128, 4 wavefronts), 256 (256 x 128,
 // 8 wavefronts) or 64 (64 x 64, one wavefront); xcd_order: tile order of the
 // update (k_lu_gemm).
@@ -1429,6 +1435,26 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         trsm(k, h + 1, NB, tail, 0);
         update(k, o + 2 * NB, ALL, o + 2 * NB, tail, o, 2 * NB);
     };
+    // the four-panel step on panels h..h+3 (columns from k0 = k + h NB)
+    // applied to the columns up to k0 + 4NB + tail: panels h, h+1 with the
+    // two-panel step (applied to the step's columns), panels h+2, h+3, their
+    // pivots into the first two panels' L rows; for the columns beyond the
+    // step's own the four panels' swaps and U rows (64-row TRSMs with the
+    // updates from the panels above between them) and one K = 4NB update
+    auto step4 = [&](int k, int h, int tail) {
+        const int o = h * NB;
+        step2(k, h, 2 * NB);
+        step2(k, h + 2, 0);
+        swap_panels(k, h, h + 2, h + 2, h + 3);
+        trsm(k, h, 4 * NB, tail, 15 << h);
+        update(k, o + NB, NB, o + 4 * NB, tail, o, NB);
+        trsm(k, h + 1, 3 * NB, tail, 0);
+        update(k, o + 2 * NB, 2 * NB, o + 4 * NB, tail, o, 2 * NB);
+        trsm(k, h + 2, 2 * NB, tail, 0);
+        update(k, o + 3 * NB, NB, o + 4 * NB, tail, o + 2 * NB, NB);
+        trsm(k, h + 3, NB, tail, 0);
+        update(k, o + 4 * NB, ALL, o + 4 * NB, tail, o, 4 * NB);
+    };
     if (levels <= 1) {
         for (int k = 0; k < maxN; k += NB) {
             panel(k, 0);
@@ -1437,19 +1463,36 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         }
     } else if (levels == 2) {
         for (int k = 0; k < maxN; k += 2 * NB) step2(k, 0, ALL);
+    } else if (levels == 3) {
+        for (int k = 0; k < maxN; k += 4 * NB) step4(k, 0, ALL);
     } else {
-        for (int k = 0; k < maxN; k += 4 * NB) {
-            step2(k, 0, 2 * NB);
-            step2(k, 2, 0);
-            swap_panels(k, 0, 2, 2, 3);
-            trsm(k, 0, 4 * NB, ALL, 15);
-            update(k, NB, NB, 4 * NB, ALL, 0, NB);
-            trsm(k, 1, 3 * NB, ALL, 0);
-            update(k, 2 * NB, 2 * NB, 4 * NB, ALL, 0, 2 * NB);
-            trsm(k, 2, 2 * NB, ALL, 0);
-            update(k, 3 * NB, NB, 4 * NB, ALL, 2 * NB, NB);
-            trsm(k, 3, NB, ALL, 0);
-            update(k, 4 * NB, ALL, 4 * NB, ALL, 0, 4 * NB);
+        // 8NB-column steps: the four-panel step on panels 0..3 applied to
+        // the step's own columns, then on panels 4..7, the later panels'
+        // pivots into the earlier panels' L rows; for the trailing columns
+        // the eight panels' swaps and their 512 U rows (forward
+        // substitution in 64-row TRSMs and updates from the panels above);
+        // one K = 8NB update of the trailing matrix (half the C traffic per
+        // flop of levels 3)
+        for (int k = 0; k < maxN; k += 8 * NB) {
+            step4(k, 0, 4 * NB);
+            step4(k, 4, 0);
+            swap_panels(k, 0, 4, 4, 7);
+            trsm(k, 0, 8 * NB, ALL, 255);
+            update(k, NB, NB, 8 * NB, ALL, 0, NB);
+            trsm(k, 1, 7 * NB, ALL, 0);
+            update(k, 2 * NB, 2 * NB, 8 * NB, ALL, 0, 2 * NB);
+            trsm(k, 2, 6 * NB, ALL, 0);
+            update(k, 3 * NB, NB, 8 * NB, ALL, 2 * NB, NB);
+            trsm(k, 3, 5 * NB, ALL, 0);
+            update(k, 4 * NB, 4 * NB, 8 * NB, ALL, 0, 4 * NB);
+            trsm(k, 4, 4 * NB, ALL, 0);
+            update(k, 5 * NB, NB, 8 * NB, ALL, 4 * NB, NB);
+            trsm(k, 5, 3 * NB, ALL, 0);
+            update(k, 6 * NB, 2 * NB, 8 * NB, ALL, 4 * NB, 2 * NB);
+            trsm(k, 6, 2 * NB, ALL, 0);
+            update(k, 7 * NB, NB, 8 * NB, ALL, 6 * NB, NB);
+            trsm(k, 7, NB, ALL, 0);
+            update(k, 8 * NB, ALL, 8 * NB, ALL, 0, 8 * NB);
         }
     }
     hipLaunchKernelGGL(k_lu_finalize, dim3(nblocks), dim3(64), 0, s, desc, arena, sd, out, nblocks);

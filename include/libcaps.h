@@ -268,7 +268,7 @@ int capsx_lu_time(int dim, int n, int reps, int levels, int streams, int xcd, in
  * dA + b*lda*dim; overwritten), written to the device array d_logdet,
  * enqueued on `stream` (a hipStream_t; NULL: the null stream) without
  * waiting.  It allocates nothing: descriptors and the LU's side buffers
- * (about 8*lda*64 doubles per matrix) live in a fixed 64 MiB device array
+ * (about 16*lda*64 doubles per matrix) live in a fixed 64 MiB device array
  * per GPU; batches needing more are refused (3) before any launch.  Calls
  * that overlap on one GPU share the array and race.  Returns 0; 1 invalid
  * arguments, 2 no scratch address, 3 too large, 4 launch error. */

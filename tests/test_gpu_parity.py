@@ -484,7 +484,7 @@ def test_lu_kernel_variants_bitwise(ca):
         # 2700 and 3100: the 512-lane panels (6 and 12 rows per lane)
         for N, nmat in ((517, 6), (300, 6), (129, 6), (2700, 2), (3100, 2)):
             mats = np.array([np.eye(N) - 0.3 * rng.normal(size=(N, N)) / math.sqrt(N) for _ in range(nmat)])
-            for levels in (1, 2, 3):
+            for levels in (1, 2, 3, 4):
                 at.set_config(levels, 1, 0, 128)
                 L.capsx_lu_kernels(*default, None)
                 ref = ca.lu_logdet_batch(mats)
