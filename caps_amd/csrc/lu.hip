@@ -339,7 +339,7 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
 // For a column diagonally dominant 1 - M (the matrix holds -M, out[b].delta)
 // partial pivoting exchanges no rows, so a panel is the unpivoted block
 // factorization [A11; A21] = [L11; L21] U11: the 64 x 64 diagonal block is
-// factored by one wavefront (k_lu_diag, which also forms L11^{-1} and
+// factored by one workgroup (k_lu_diag, which also forms L11^{-1} and
 // U11^{-1}), then L21 = A21 U11^{-1} (k_lu_l21) and, for the step's U rows,
 // U12 = L11^{-1} A12 (k_lu_trsm_mfma) are products on the FP64 MFMA units,
 // spread over as many workgroups as the panel has row (column) tiles.  The
@@ -348,15 +348,18 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
 // substitutions they replace.
 
 // The 64 x 64 diagonal block of panel pb at step k: unpivoted LU with the
-// pivots u = 1 + d (d the stored diagonal), one lane per row, rows held in
-// registers and the pivot row broadcast through LDS; log|u_ii| = log1p(d);
-// the factored block goes to the panel buffer's top rows; then lane j forms
-// column j of L11^{-1} (forward substitution) and of U11^{-1} (backward), the
-// L11 and U11 entries read as LDS broadcasts.  Rows and columns past the
-// matrix (a last panel narrower than 64) act as identity.
-__global__ void __launch_bounds__(64) k_lu_diag(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                                const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
-                                                int k, int pb)
+// pivots u = 1 + d (d the stored diagonal), one lane of wavefront 0 per
+// row, rows held in registers and the pivot row broadcast through LDS (16-
+// byte accesses); log|u_ii| = log1p(d); the factored block goes to the panel
+// buffer's top rows.  Then the block is stored transposed in LDS and
+// wavefront 0 forms column r of L11^{-1} (forward substitution), wavefront 1
+// column r of U11^{-1} (backward substitution), both column-oriented (each
+// solved x_q is applied to all later entries at once: independent FMAs, the
+// L11 or U11 column read as 16-byte LDS broadcasts).  Rows and columns past
+// the matrix (a last panel narrower than 64) act as identity.
+__global__ void __launch_bounds__(128) k_lu_diag(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+                                                 const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
+                                                 int k, int pb)
 {
     const int b = blockIdx.x;
     const BlockDesc& d = desc[b];
@@ -368,59 +371,99 @@ __global__ void __launch_bounds__(64) k_lu_diag(const BlockDesc* __restrict__ de
     const double* __restrict__ A = at<double>(arena, d.off_mat);
     double* __restrict__ P = panel_buf(arena, side[b], lda, pb);
     double* __restrict__ inv = inv_buf(arena, side[b], pb);
-    __shared__ double S[NB][NB + 1];   // [row][column], odd pitch
-    const int r = threadIdx.x;
+    // LU phase: S[c][j] holds pivot row c; inversion phase: S[j][i] holds
+    // the factored block's entry (i, j) (column j contiguous)
+    constexpr int SP = NB + 2;   // pitch: 16-byte aligned rows
+    __shared__ __align__(16) double S[NB * SP];
+    __shared__ double rinv[NB];   // 1 / (1 + d_i)
+    const int r = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double a[NB];
+    if (wv == 0) {
 #pragma unroll
-    for (int j = 0; j < NB; j++) a[j] = (r < nbk && j < nbk) ? A[(k0 + r) + (int64_t)(k0 + j) * lda] : 0.0;
+        for (int j = 0; j < NB; j++) a[j] = (r < nbk && j < nbk) ? A[(k0 + r) + (int64_t)(k0 + j) * lda] : 0.0;
+    }
 #pragma unroll
     for (int c = 0; c < NB; c++) {
-        if (r == c) {
+        if (wv == 0 && r == c) {
+            // the pivot row from column c & ~1 on, in 16-byte pieces
 #pragma unroll
-            for (int j = c; j < NB; j++) S[c][j] = a[j];
+            for (int j = c & ~1; j < NB; j += 2)
+                *reinterpret_cast<double2*>(&S[c * SP + j]) = make_double2(a[j], a[j + 1]);
         }
         __syncthreads();
-        const double ru = 1.0 / (1.0 + S[c][c]);
-        if (r > c) {
+        if (wv == 0 && r > c) {
+            const double ru = 1.0 / (1.0 + S[c * SP + c]);
             const double l = a[c] * ru;
             a[c] = l;
+            if (c + 1 < NB) {
+                // columns c+1 .. NB-1: the first alone when c+1 is odd
+                int j = c + 1;
+                if (j & 1) {
+                    a[j] -= l * S[c * SP + j];
+                    j++;
+                }
 #pragma unroll
-            for (int j = c + 1; j < NB; j++) a[j] -= l * S[c][j];
+                for (; j < NB; j += 2) {
+                    const double2 u = *reinterpret_cast<const double2*>(&S[c * SP + j]);
+                    a[j] -= l * u.x;
+                    a[j + 1] -= l * u.y;
+                }
+            }
         }
     }
-    __syncthreads();
+    __syncthreads();   // the last pivot row is read; S is reused
+    if (wv == 0) {
+        double dr = 0;   // a[r] (selected: a lane-dependent index would put a[] in scratch)
 #pragma unroll
-    for (int j = 0; j < NB; j++) S[r][j] = a[j];
-    __syncthreads();
-    if (r < nbk) {
-        at<double>(arena, side[b].off_logs)[k0 + r] = log_abs_1p(S[r][r]);
-        moved_buf(arena, side[b], pb)[r] = r;
+        for (int j = 0; j < NB; j++) dr = j == r ? a[j] : dr;
+        if (r < nbk) {
+            at<double>(arena, side[b].off_logs)[k0 + r] = log_abs_1p(dr);
+            moved_buf(arena, side[b], pb)[r] = r;
 #pragma unroll
-        for (int j = 0; j < NB; j++)
-            if (j < nbk) P[r + (int64_t)j * lda] = a[j];
+            for (int j = 0; j < NB; j++)
+                if (j < nbk) P[r + (int64_t)j * lda] = a[j];
+        }
+#pragma unroll
+        for (int j = 0; j < NB; j++) S[j * SP + r] = a[j];   // column j, row r
+        rinv[r] = 1.0 / (1.0 + dr);
     }
-    // column r of L11^{-1} (unit lower): x_i = δ_ir - Σ_{q<i} L_iq x_q
-    {
-        double x[NB];
+    __syncthreads();
+    double x[NB];
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
-            double t = (i == r) ? 1.0 : 0.0;
+    for (int i = 0; i < NB; i++) x[i] = (i == r) ? 1.0 : 0.0;
+    if (wv == 0) {
+        // column r of L11^{-1} (unit lower): x = e_r; for q: x_i -= L_iq x_q, i > q
 #pragma unroll
-            for (int q = 0; q < i; q++) t -= S[i][q] * x[q];
-            x[i] = t;
+        for (int q = 0; q < NB - 1; q++) {
+            const double xq = x[q];
+            int i = q + 1;
+            if (i & 1) {
+                x[i] -= S[q * SP + i] * xq;
+                i++;
+            }
+#pragma unroll
+            for (; i < NB; i += 2) {
+                const double2 l = *reinterpret_cast<const double2*>(&S[q * SP + i]);
+                x[i] -= l.x * xq;
+                x[i + 1] -= l.y * xq;
+            }
         }
 #pragma unroll
         for (int i = 0; i < NB; i++) inv[(int64_t)r * NB + i] = x[i];          // column-major
-    }
-    // column r of U11^{-1}: x_i = (δ_ir - Σ_{q>i} U_iq x_q) / (1 + d_i)
-    {
-        double x[NB];
+    } else {
+        // column r of U11^{-1}: x = e_r; for q from the last: x_q /= u_qq,
+        // x_i -= U_iq x_q, i < q
 #pragma unroll
-        for (int i = NB - 1; i >= 0; i--) {
-            double t = (i == r) ? 1.0 : 0.0;
+        for (int q = NB - 1; q >= 0; q--) {
+            const double xq = x[q] * rinv[q];
+            x[q] = xq;
 #pragma unroll
-            for (int q = i + 1; q < NB; q++) t -= S[i][q] * x[q];
-            x[i] = t / (1.0 + S[i][i]);
+            for (int i = 0; i + 1 < q; i += 2) {
+                const double2 u = *reinterpret_cast<const double2*>(&S[q * SP + i]);
+                x[i] -= u.x * xq;
+                x[i + 1] -= u.y * xq;
+            }
+            if (q & 1) x[q - 1] -= S[q * SP + q - 1] * xq;
         }
 #pragma unroll
         for (int i = 0; i < NB; i++) inv[NB * NB + (int64_t)i * NB + r] = x[i];   // row-major
@@ -1408,7 +1451,7 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         if (k0 >= maxN) return;
         if (piv) launch_panel(maxN - k0, desc, arena, sd, out, nblocks, k0, pb, s);
         if (!dlt) return;
-        hipLaunchKernelGGL(k_lu_diag, dim3(nblocks), dim3(64), 0, s, desc, arena, sd, out, k, pb);
+        hipLaunchKernelGGL(k_lu_diag, dim3(nblocks), dim3(128), 0, s, desc, arena, sd, out, k, pb);
         const int below = maxN - k0 - NB;
         if (below > 0)
             hipLaunchKernelGGL(k_lu_l21, dim3((below + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, sd, out, k,
