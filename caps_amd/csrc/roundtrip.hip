@@ -122,8 +122,10 @@ __global__ void k_bessel_points(int n, const int* __restrict__ order, const doub
 // values P_ν^μ(x) for ν = μ, μ+1, ... follow from the upward recurrence in the
 // degree (DLMF 14.10.3, the recurrence of src/plm.c:72-112 continued over
 // all ν), kept as mantissa × 2^E.  For every ν the wave sums its 64 node
-// terms after aligning them to the wave's largest exponent (ldexp, exact);
-// one KPart per (ν, wave) goes to HBM and k_K_combine adds the waves.
+// terms after aligning them to their largest exponent (ldexp, exact): the
+// terms of 16 ν at a time go through LDS, four lanes per ν add 16 nodes
+// each and combine within their quad (no wave-wide reduction per ν); one
+// KPart per (ν, wave) goes to HBM and k_K_combine adds the waves.
 
 __global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                   const double* __restrict__ lfac_tab)
@@ -161,8 +163,20 @@ __global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ 
     }
 
     KPart* part = at<KPart>(arena, d.off_kpart);
+    // per chunk of 16 ν: each node's term p_ν · 2^E as a mantissa in [1, 2)
+    // and an exponent (rows ν, columns node; odd pitch), then per ν the sum
+    // over the wave's 64 nodes: lane 4j + q adds nodes 16q .. 16q+15 of the
+    // chunk's ν j (exponent-aligned to their largest), the quad combines
+    // its four partial sums
+    constexpr int CH = 16, TP = 65;
     __shared__ double c1s[64], c2s[64];
+    __shared__ double tm_[CH * TP];
+    __shared__ int te_[CH * TP];
+    __shared__ double2 mant_s[64];   // per node: (|r_TE|, |r_TM|) times the weight mantissa
+    mant_s[lane] = make_double2(mant_te, mant_tm);
+    const int rj = lane >> 2, rq = lane & 3;   // reduction role: ν j of the chunk, node quarter q
     double p_prev = 0, p = 1;
+    constexpr int NONE = -1000000;
     for (int base = mu; base <= d.nu_max; base += 64) {
         // recurrence coefficients for the next 64 steps, one division per lane
         {
@@ -173,51 +187,62 @@ __global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ 
         }
         __syncthreads();
         const int nend = min(base + 64, d.nu_max + 1);
-        // ν's terms are p·mant·2^E; the recurrence gives p at ν+1, kept
-        // below 2^600 by moving powers of two into E
-        auto advance = [&](double& pv, double& pp, int& Ev, int j) {
-            const double pn = c1s[j] * x * pv - c2s[j] * pp;
-            pp = pv;
-            pv = pn;
-            if (pv > 0x1p600) {
-                pv *= 0x1p-600;
-                pp *= 0x1p-600;
-                Ev += 600;
+        for (int c0 = base; c0 < nend; c0 += CH) {
+            // this node's terms of ν = c0 .. c0 + CH - 1; the recurrence
+            // gives p at ν+1, kept below 2^600 by moving powers of two into E
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+                const int e = active ? E + ilogb(p) : NONE;
+                tm_[j * TP + lane] = active ? ldexp(p, E - e) : 0.0;
+                te_[j * TP + lane] = e;
+                const int jj = c0 - base + j;
+                if (jj < 64) {
+                    const double pn = c1s[jj] * x * p - c2s[jj] * p_prev;
+                    p_prev = p;
+                    p = pn;
+                    if (p > 0x1p600) {
+                        p *= 0x1p-600;
+                        p_prev *= 0x1p-600;
+                        E += 600;
+                    }
+                }
             }
-        };
-        auto reduce = [&](double pv, int Ev, int nu) {
-            const int e_lane = active ? Ev + ilogb(pv) : -1000000;
-            const int F = wave_max_int(e_lane);
-            const double s_te = wave_sum(active ? ldexp(pv * mant_te, Ev - F) : 0.0);
-            const double s_tm = wave_sum(active ? ldexp(pv * mant_tm, Ev - F) : 0.0);
-            if (lane == 0) {
+            __syncthreads();
+            // partial sums over the lane's 16 nodes
+            const double* rowm = tm_ + rj * TP + 16 * rq;
+            const int* rowe = te_ + rj * TP + 16 * rq;
+            int F = NONE;
+#pragma unroll
+            for (int t = 0; t < 16; t++) F = max(F, rowe[t]);
+            double ste = 0, stm = 0;
+#pragma unroll
+            for (int t = 0; t < 16; t++) {
+                const double2 mt = mant_s[16 * rq + t];
+                const double m = rowm[t];
+                const int sh = rowe[t] - F;
+                ste += ldexp(m * mt.x, sh);
+                stm += ldexp(m * mt.y, sh);
+            }
+            // the quad's four partial sums, aligned to their largest exponent
+            int Fq = max(F, dpp_i<DPP_QUAD_X1>(F));
+            Fq = max(Fq, dpp_i<DPP_QUAD_X2>(Fq));
+            ste = ldexp(ste, F - Fq);
+            stm = ldexp(stm, F - Fq);
+            ste += dpp_d<DPP_QUAD_X1>(ste);
+            stm += dpp_d<DPP_QUAD_X1>(stm);
+            ste += dpp_d<DPP_QUAD_X2>(ste);
+            stm += dpp_d<DPP_QUAD_X2>(stm);
+            const int nu = c0 + rj;
+            if (rq == 0 && nu < nend) {
                 KPart kp;
-                kp.s_te = s_te;
-                kp.s_tm = s_tm;
-                kp.e = F;
+                kp.s_te = ste;
+                kp.s_tm = stm;
+                kp.e = Fq;
                 kp.pad_ = 0;
                 part[(int64_t)(nu - mu) * d.n_waves + wave] = kp;
             }
-        };
-        // two ν per iteration: their reductions are independent, so the
-        // second's cross-lane chains overlap the first's
-        int nu = base;
-        for (; nu + 1 < nend; nu += 2) {
-            double p1 = p, pp1 = p_prev;
-            int E1 = E;
-            advance(p1, pp1, E1, nu - base);   // ν+1
-            reduce(p, E, nu);
-            reduce(p1, E1, nu + 1);
-            p = p1;
-            p_prev = pp1;
-            E = E1;
-            advance(p, p_prev, E, nu + 1 - base);
+            __syncthreads();   // tm_/te_ are rewritten by the next chunk
         }
-        if (nu < nend) {
-            reduce(p, E, nu);
-            advance(p, p_prev, E, nu - base);
-        }
-        __syncthreads();
     }
 }
 
