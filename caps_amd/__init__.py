@@ -86,6 +86,9 @@ def lib():
             "capsx_material_free": (None, [p]),
             "capsx_material_epsilonm1": (d, [d, p]),
             "capsx_material_get_extrapolation": (None, [p, dp, dp, dp, dp]),
+            "capsx_resume_load": (i, [p, ctypes.c_char_p]),
+            "capsx_resume_lookup": (i, [p, d, dp]),
+            "capsx_set_output": (i, [p, ctypes.c_char_p]),
             "capsx_free_energy": (d, [p, d, d, d, i, i, d, i, ctypes.POINTER(ctypes.c_long),
                                       ctypes.POINTER(ctypes.c_long)]),
             "capsx_psd": (i, [i, dp, dp]),
@@ -304,6 +307,26 @@ class Caps:
         F = lib().caps_free_energy(self.ptr, T, cutoff, epsrel, {"plain": 0, "batched": 1, "surrogate": 2}[mode],
                                    1 if drude_xi0 else 0, ctypes.byref(nb), ctypes.byref(nu))
         return F, nb.value, nu.value
+
+    def resume_load(self, path):
+        """Read the (ξ(L+R)/c, F_ξ) lines of an earlier run's output into the
+        resume cache (path None clears it); returns the number read
+        (capsx_resume_load; ValueError for a missing or malformed file)."""
+        n = lib().capsx_resume_load(self.ptr, None if path is None else os.fsencode(path))
+        if n < 0:
+            raise ValueError("%s: %s" % (path, "cannot open" if n == -1 else "wrong format"))
+        return n
+
+    def resume_lookup(self, xi_):
+        """F_ξ from the resume cache, or None."""
+        v = ctypes.c_double()
+        return v.value if lib().capsx_resume_lookup(self.ptr, xi_, ctypes.byref(v)) else None
+
+    def set_output(self, path):
+        """Append a "# xi*(L+R)/c=..., logdetD=..., t=..." line per computed
+        F_ξ to path (None closes it)."""
+        if lib().capsx_set_output(self.ptr, None if path is None else os.fsencode(path)) != 0:
+            raise ValueError("cannot open %s" % path)
 
     def surrogate_report(self, reset=True):
         """Surrogate accuracy on the blocks computed for its plans (mode

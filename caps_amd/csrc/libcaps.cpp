@@ -20,6 +20,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "blocks.h"
@@ -298,6 +299,52 @@ struct TuneEntry {
     int maxN, n;
     LuCfg cfg;
 };
+// ---- resume cache and per-frequency output --------------------------------
+// The reference driver reads the lines "# xi*(L+R)/c=<ξ_>, logdetD=<F_ξ>, ..."
+// of an earlier run's output (src/caps.c:72-101) and returns F_ξ from them
+// for a frequency that matches (ξ_ equal, or |1 - ξ_cached/ξ_| < 1e-11,
+// src/caps.c:316-327); it prints such a line for every F_ξ it computes
+// (src/caps.c:265).  Per caps object, host only.
+struct ResumeState {
+    std::vector<std::pair<double, double>> cache;   // (ξ_, F_ξ)
+    FILE* out = nullptr;
+    double t0 = 0;
+};
+std::mutex g_resume_mu;
+std::unordered_map<const caps_t*, ResumeState> g_resume;
+
+double wall_seconds()
+{
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+bool resume_lookup(const caps_t* self, double xi, double* v)
+{
+    std::lock_guard<std::mutex> lock(g_resume_mu);
+    auto it = g_resume.find(self);
+    if (it == g_resume.end()) return false;
+    for (const auto& e : it->second.cache)
+        if (xi == e.first || fabs(1 - e.first / xi) < 1e-11) {
+            *v = e.second;
+            return true;
+        }
+    return false;
+}
+
+void resume_record(const caps_t* self, double xi, double v)
+{
+    std::lock_guard<std::mutex> lock(g_resume_mu);
+    auto it = g_resume.find(self);
+    if (it == g_resume.end() || !it->second.out) return;
+    if (xi == 0)
+        fprintf(it->second.out, "# xi*(L+R)/c=0, logdetD=%.16g, t=%g\n", v, wall_seconds() - it->second.t0);
+    else
+        fprintf(it->second.out, "# xi*(L+R)/c=%.16g, logdetD=%.16g, t=%g\n", xi, v, wall_seconds() - it->second.t0);
+    fflush(it->second.out);
+}
+
 std::mutex g_cfg_mu;
 LuCfg g_override;                 // used when g_have_override
 bool g_have_override = false;
@@ -926,6 +973,14 @@ caps_t* caps_init(double R, double L)
 void caps_free(caps_t* self)
 {
     if (!self) return;
+    {
+        std::lock_guard<std::mutex> lock(g_resume_mu);
+        auto it = g_resume.find(self);
+        if (it != g_resume.end()) {
+            if (it->second.out) fclose(it->second.out);
+            g_resume.erase(it);
+        }
+    }
     ctx_free((Ctx*)self->priv);
     free(self);
 }
@@ -1263,8 +1318,8 @@ int capsx_bessel_points(int n, const int* order, const double* nu, const double*
 // column diagonally dominant: no pivot search, log1p of u_ii - 1), bit 3
 // set when the HODLR-mode rule |tr M| > |log det| replaced the LU value by
 // -tr M.  Returns 0 on success.
-static int logdetD_multi_ordered(caps_t* const* objs, const double* xi_, const int* m, int n, double* out,
-                                 double* trace, int* flags);
+static int logdetD_multi_ordered(caps_t* host, caps_t* const* objs, const double* xi_, const int* m, int n,
+                                 double* out, double* trace, int* flags);
 
 int capsx_logdetD_multi_info(caps_t* const* objs, const double* xi_, const int* m, int n, double* out, double* trace,
                              int* flags)
@@ -1272,15 +1327,23 @@ int capsx_logdetD_multi_info(caps_t* const* objs, const double* xi_, const int* 
     if (n <= 0) return 0;
     // the blocks of one caps object and m (whose I tables share the a_q,
     // k_I) are run next to each other, so that they fall into one part of
-    // the batch and into full groups
-    std::vector<int> ord(n);
-    for (int i = 0; i < n; i++) ord[i] = i;
+    // the batch and into full groups: objects in the order of their first
+    // block, then m, then the input order.  The batch runs on objs[0]'s
+    // context (device buffers, statistics, graph setting) either way.
+    std::vector<int> ord(n), first(n);
+    {
+        std::unordered_map<caps_t*, int> seen;
+        for (int i = 0; i < n; i++) {
+            first[i] = seen.emplace(objs[i], i).first->second;
+            ord[i] = i;
+        }
+    }
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-        return std::make_tuple((uintptr_t)objs[a], m[a]) < std::make_tuple((uintptr_t)objs[b], m[b]);
+        return std::make_tuple(first[a], m[a]) < std::make_tuple(first[b], m[b]);
     });
     bool same = true;
     for (int i = 0; i < n && same; i++) same = ord[i] == i;
-    if (same) return logdetD_multi_ordered(objs, xi_, m, n, out, trace, flags);
+    if (same) return logdetD_multi_ordered(objs[0], objs, xi_, m, n, out, trace, flags);
     std::vector<caps_t*> po(n);
     std::vector<double> px(n), pout(n), ptr(trace ? n : 0);
     std::vector<int> pm(n), pf(flags ? n : 0);
@@ -1289,8 +1352,8 @@ int capsx_logdetD_multi_info(caps_t* const* objs, const double* xi_, const int* 
         px[i] = xi_[ord[i]];
         pm[i] = m[ord[i]];
     }
-    const int rc = logdetD_multi_ordered(po.data(), px.data(), pm.data(), n, pout.data(), trace ? ptr.data() : nullptr,
-                                         flags ? pf.data() : nullptr);
+    const int rc = logdetD_multi_ordered(objs[0], po.data(), px.data(), pm.data(), n, pout.data(),
+                                         trace ? ptr.data() : nullptr, flags ? pf.data() : nullptr);
     if (rc) return rc;
     for (int i = 0; i < n; i++) {
         out[ord[i]] = pout[i];
@@ -1300,8 +1363,8 @@ int capsx_logdetD_multi_info(caps_t* const* objs, const double* xi_, const int* 
     return 0;
 }
 
-static int logdetD_multi_ordered(caps_t* const* objs, const double* xi_, const int* m, int n, double* out,
-                                 double* trace, int* flags)
+static int logdetD_multi_ordered(caps_t* host, caps_t* const* objs, const double* xi_, const int* m, int n,
+                                 double* out, double* trace, int* flags)
 {
     for (int i = 0; i < n; i++) {
         if (!objs[i] || !(xi_[i] > 0) || m[i] < 0) {
@@ -1315,7 +1378,7 @@ static int logdetD_multi_ordered(caps_t* const* objs, const double* xi_, const i
             return 1;
         }
     }
-    Ctx* c = ctx_of(objs[0]);
+    Ctx* c = ctx_of(host);
     std::lock_guard<std::mutex> lock(c->mu);
     const int lu_mode = dense_mode(objs[0]->detalg);
     double tot[6] = {0, 0, 0, 0, 0, 0};
@@ -1323,7 +1386,7 @@ static int logdetD_multi_ordered(caps_t* const* objs, const double* xi_, const i
         const int k = chunk_blocks_multi(objs + b0, m + b0, n - b0);
         ensure_blocks(c, k);
         BatchPlan p = plan_batch_multi(objs + b0, c, xi_ + b0, m + b0, k, 1);
-        run_batch(objs[0], c, p, 0, lu_mode);
+        run_batch(host, c, p, 0, lu_mode);
         // blocks whose 1 - M is not column dominant were left unfactored by
         // the delta-form LU: run them again with the pivoted LU
         std::vector<int> again;
@@ -1351,7 +1414,7 @@ static int logdetD_multi_ordered(caps_t* const* objs, const double* xi_, const i
             }
             BatchPlan q = plan_batch_multi(ro.data(), c, rx.data(), rm.data(), (int)again.size(), 1);
             q.lu_kinds = 1;
-            run_batch(objs[0], c, q, 0, lu_mode);
+            run_batch(host, c, q, 0, lu_mode);
             for (size_t j = 0; j < again.size(); j++) take(c->h_out[j], again[j]);
             for (int i = 0; i < 6; i++) tot[i] += c->timing[i];
         }
@@ -1934,7 +1997,19 @@ struct Term {
     bool done = false;
     int mstop = -1;
     double value = 0;
+    bool cached = false;     // value taken from the resume cache (no blocks)
 };
+
+// A term whose frequency is in the resume cache: its m-sum is replaced by
+// the cached value (v = {2 F_ξ, 0}: the m-sum of it is F_ξ, m = 0 halved,
+// the next term below any cutoff).
+void term_from_cache(const caps_t* self, Term& t)
+{
+    double val;
+    if (!resume_lookup(self, t.xi, &val)) return;
+    t.v = {2 * val, 0.0};
+    t.cached = true;
+}
 
 // Surrogate predictions of log|logdetD(ξ_, m)| for m = 0..mcap at each of
 // the frequencies xs (row q of P holds frequency q).
@@ -1993,6 +2068,7 @@ void run_requests(caps_t* self, std::vector<Term>& terms, const std::vector<std:
     std::vector<int> ms, owner;
     for (auto& r : req) {
         Term& t = terms[r.first];
+        if (t.cached) continue;
         for (int m = (int)t.v.size(); m < r.second; m++) {
             xs.push_back(t.xi);
             ms.push_back(m);
@@ -2051,6 +2127,12 @@ double free_energy_T0(caps_t* self, double cutoff, double epsrel, int mode, long
             const double xi = c * c / alpha;
             if (xi > xi_cut) {
                 fcache[idx] = 0;
+                mstop_at[idx] = -1;
+                continue;
+            }
+            double cv;
+            if (resume_lookup(self, xi, &cv)) {
+                fcache[idx] = cv;
                 mstop_at[idx] = -1;
                 continue;
             }
@@ -2120,6 +2202,7 @@ double free_energy_T0(caps_t* self, double cutoff, double epsrel, int mode, long
             fcache[slot[q]] = terms[q].value;
             mstop_at[slot[q]] = terms[q].mstop;
             used += terms[q].mstop + 1;
+            resume_record(self, terms[q].xi, terms[q].value);
         }
         double I = 0;
         for (int i = 1; i <= N; i++) {
@@ -2259,16 +2342,24 @@ double free_energy_psd(caps_t* self, double T_scaled, double cutoff, int order, 
     std::vector<double> pole(order), eta(order);
     capsx_psd(order, pole.data(), eta.data());
     std::vector<Term> terms(order);
-    for (int j = 0; j < order; j++) terms[j].xi = pole[j] * T_scaled / (2 * M_PI);
+    for (int j = 0; j < order; j++) {
+        terms[j].xi = pole[j] * T_scaled / (2 * M_PI);
+        term_from_cache(self, terms[j]);
+    }
     long used = 0;
     if (mode == 0) {
-        for (Term& t : terms)
+        for (Term& t : terms) {
+            if (t.cached) {
+                msum(t.v, cutoff, &t.value, &t.mstop);
+                continue;
+            }
             for (int m = 0;; m++) {
                 TERMINATE(m >= 100000, "sum over m did not converge at xi_=%g", t.xi);
                 t.v.push_back(caps_logdetD(self, t.xi, m));
                 computed++;
                 if (msum(t.v, cutoff, &t.value, &t.mstop)) break;
             }
+        }
     } else {
         std::vector<std::pair<int, int>> req;
         const int MCAP = std::min(self->ldim - 1, 4000), per = MCAP + 1;
@@ -2309,13 +2400,15 @@ double free_energy_psd(caps_t* self, double T_scaled, double cutoff, int order, 
                 run_requests(self, terms, {{j, upto}}, &computed);
                 cx->plan_batches++;
             }
-            if (mode == 2) sur_score(cx, &P[(size_t)j * per], MCAP, mpred[j], t);
+            if (mode == 2 && !t.cached) sur_score(cx, &P[(size_t)j * per], MCAP, mpred[j], t);
         }
     }
     std::vector<double> v{v0 / 2};
     for (int j = 0; j < order; j++) {
         v.push_back(eta[j] * terms[j].value);
+        if (terms[j].cached) continue;
         used += terms[j].mstop + 1;
+        resume_record(self, terms[j].xi, terms[j].value);
     }
     if (nblocks) *nblocks = computed;
     if (nused) *nused = used;
@@ -2338,7 +2431,11 @@ double capsx_free_energy(caps_t* self, double T, double cutoff, double epsrel, i
     TERMINATE(!(T >= 0), "the free energy needs T >= 0 (got %g)", T);
     if (T == 0) return free_energy_T0(self, cutoff, epsrel, mode, nblocks, nused);
     long computed = 0;
-    const double v0 = xi0_term(self, xi0, omegap, cutoff, &computed);
+    double v0;
+    if (!resume_lookup(self, 0, &v0)) {
+        v0 = xi0_term(self, xi0, omegap, cutoff, &computed);
+        resume_record(self, 0, v0);
+    }
     const double T_scaled = 2 * M_PI * CAPS_kB * (self->R + self->L) * T / (CAPS_hbar * CAPS_c);
     if (psd_order == 0) return free_energy_msd(self, T_scaled, cutoff, epsrel, mode, v0, nblocks, nused, computed);
     if (psd_order < 0) {
@@ -2346,6 +2443,78 @@ double capsx_free_energy(caps_t* self, double T, double cutoff, double epsrel, i
         psd_order = (int)ceil((1 - 1.5 * log10(epsrel)) / sqrt(Teff));
     }
     return free_energy_psd(self, T_scaled, cutoff, psd_order, mode, v0, nblocks, nused, computed);
+}
+
+// Extension: the resume cache of the free-energy drivers, read from an
+// earlier run's output as the reference driver's --resume does
+// (src/caps.c:72-101): every line holding "# xi*(L+R)/c=" and "logdetD="
+// gives one (ξ_, F_ξ) pair.  The drivers then take F_ξ of a matching
+// frequency from the cache instead of computing its blocks.  Appends to the
+// object's cache; path NULL or "" clears it.  Returns the number of pairs
+// read, -1 if the file cannot be opened, -2 if a line has the wrong format
+// (the reference's two errors, src/caps.c:78, 90, 96).
+int capsx_resume_load(caps_t* self, const char* path)
+{
+    std::vector<std::pair<double, double>> got;
+    if (path && path[0]) {
+        FILE* fh = fopen(path, "r");
+        if (!fh) {
+            fprintf(stderr, "cannot open %s for reading\n", path);
+            return -1;
+        }
+        char line[512];
+        while (fgets(line, sizeof(line), fh)) {
+            char* p1 = strstr(line, "# xi*(L+R)/c=");
+            char* p2 = strstr(line, "logdetD=");
+            if (!p1 || !p2) continue;
+            p1 += 13;
+            char* e1 = strchr(p1, ',');
+            p2 += 8;
+            char* e2 = strchr(p2, ',');
+            if (!e1 || !e2) {
+                fclose(fh);
+                fprintf(stderr, "%s has wrong format\n", path);
+                return -2;
+            }
+            *e1 = '\0';
+            *e2 = '\0';
+            got.emplace_back(atof(p1), atof(p2));
+        }
+        fclose(fh);
+    }
+    std::lock_guard<std::mutex> lock(g_resume_mu);
+    ResumeState& st = g_resume[self];
+    if (!path || !path[0]) st.cache.clear();
+    st.cache.insert(st.cache.end(), got.begin(), got.end());
+    return (int)got.size();
+}
+
+// Extension: F_ξ of frequency xi_ from the resume cache, with the
+// reference's matching rule (src/caps.c:316-327).  Returns 1 and sets
+// *value if found, else 0.
+int capsx_resume_lookup(caps_t* self, double xi_, double* value)
+{
+    double v;
+    if (!resume_lookup(self, xi_, &v)) return 0;
+    if (value) *value = v;
+    return 1;
+}
+
+// Extension: append a line "# xi*(L+R)/c=<ξ_>, logdetD=<F_ξ>, t=<s>" for
+// every F_ξ the free-energy drivers compute (src/caps.c:265) to path (opened
+// for appending; the file capsx_resume_load reads back); NULL or "" closes
+// it.  Returns 0, or -1 if the file cannot be opened.
+int capsx_set_output(caps_t* self, const char* path)
+{
+    std::lock_guard<std::mutex> lock(g_resume_mu);
+    ResumeState& st = g_resume[self];
+    if (st.out) fclose(st.out);
+    st.out = nullptr;
+    if (!path || !path[0]) return 0;
+    st.out = fopen(path, "a");
+    if (!st.out) return -1;
+    st.t0 = wall_seconds();
+    return 0;
 }
 
 double caps_free_energy(caps_t* self, double T, double cutoff, double epsrel, int mode, int drude_xi0,
@@ -2373,6 +2542,7 @@ double free_energy_msd(caps_t* self, double T_scaled, double cutoff, double epsr
         while ((int)terms.size() <= n) {
             Term t;
             t.xi = terms.size() * T_scaled;
+            term_from_cache(self, t);
             terms.push_back(t);
         }
         return terms[n];
@@ -2382,13 +2552,18 @@ double free_energy_msd(caps_t* self, double T_scaled, double cutoff, double epsr
         // plain: one block at a time in F_xi order (src/caps.c:307-396, 836-846)
         for (int n = 1;; n++) {
             Term& t = term(n);
-            for (int m = 0;; m++) {
+            for (int m = 0; !t.cached; m++) {
                 TERMINATE(m >= 100000, "sum over m did not converge at xi_=%g", t.xi);
                 t.v.push_back(caps_logdetD(self, t.xi, m));
                 computed++;
                 if (msum(t.v, cutoff, &t.value, &t.mstop)) break;
             }
-            used += t.mstop + 1;
+            if (t.cached) {
+                msum(t.v, cutoff, &t.value, &t.mstop);
+            } else {
+                used += t.mstop + 1;
+                resume_record(self, t.xi, t.value);
+            }
             v.push_back(t.value);
             if (fabs(v[n] / v[0]) < epsrel) break;
             TERMINATE(n > 100000, "Matsubara sum did not converge");
@@ -2419,7 +2594,10 @@ double free_energy_msd(caps_t* self, double T_scaled, double cutoff, double epsr
         bool stop = false;
         auto accept = [&](int j) {
             Term& t = term(j);
-            used += t.mstop + 1;
+            if (!t.cached) {
+                used += t.mstop + 1;
+                resume_record(self, t.xi, t.value);
+            }
             v.push_back(t.value);
             n = j;
             stop = fabs(v[j] / v[0]) < epsrel;
@@ -2452,7 +2630,7 @@ double free_energy_msd(caps_t* self, double T_scaled, double cutoff, double epsr
                 const int j = req[r].first;
                 Term& t = term(j);
                 if (!msum(t.v, cutoff, &t.value, &t.mstop)) finish_term(j, (int)t.v.size() + 16, 16);
-                sur_score(cx, &P[(size_t)(j - 1) * per], MCAP, mpred[j], t);
+                if (!t.cached) sur_score(cx, &P[(size_t)(j - 1) * per], MCAP, mpred[j], t);
                 accept(j);
             }
         }
@@ -2460,19 +2638,28 @@ double free_energy_msd(caps_t* self, double T_scaled, double cutoff, double epsr
             finish_term(1, 64, 32);
             accept(1);
         }
+        // m-cutoff of term j for planning (a resumed term's is not known)
+        auto mcut = [&](int j) { return term(j).cached ? 64 : term(j).mstop; };
         if (!stop && n < 2) {
-            finish_term(2, term(1).mstop + 12, 16);
+            finish_term(2, mcut(1) + 12, 16);
             accept(2);
         }
         while (!stop) {
+            if (term(n + 1).cached) {
+                // a resumed term: taken in order, no batch planned past it
+                Term& t = term(n + 1);
+                msum(t.v, cutoff, &t.value, &t.mstop);
+                accept(n + 1);
+                continue;
+            }
             // prediction: geometric decay of v_n, m-cutoff growing by at most
             // the last step, 8 blocks of margin, one extra term
             const double r = std::min(0.95, std::max(1e-6, fabs(v[n] / v[n - 1])));
             const double need = epsrel * fabs(v[0]) / fabs(v[n]);
             int more = need < 1 ? (int)ceil(log(need) / log(r)) + 1 : 1;
             more = std::max(1, std::min(more, 64));
-            const int dm = std::max(0, term(n).mstop - term(n - 1).mstop);
-            const int mp = term(n).mstop + dm + 8;
+            const int dm = std::max(0, mcut(n) - mcut(n - 1));
+            const int mp = mcut(n) + dm + 8;
             std::vector<std::pair<int, int>> req;
             for (int j = n + 1; j <= n + more; j++) {
                 term(j);

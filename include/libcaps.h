@@ -314,6 +314,25 @@ double caps_free_energy(caps_t *self, double T, double cutoff, double epsrel, in
  * as for caps_free_energy.  T = 0: as caps_free_energy. */
 double capsx_free_energy(caps_t *self, double T, double cutoff, double epsrel, int mode, int xi0,
                          double omegap, int psd_order, long *nblocks, long *nused);
+/* Extension: the resume cache of the free-energy drivers, read from an
+ * earlier run's output as the reference driver's --resume option does
+ * (src/caps.c:72-101): every line with "# xi*(L+R)/c=" and "logdetD="
+ * gives one (ξ(L+R)/c, F_ξ) pair, appended to this object's cache (path
+ * NULL or "" clears it).  caps_free_energy and capsx_free_energy then take
+ * F_ξ of a matching frequency (equal, or |1 - ξ_cached/ξ| < 1e-11,
+ * src/caps.c:316-327) from the cache and compute no blocks for it.
+ * Returns the number of pairs read; -1 the file cannot be opened, -2 a line
+ * has the wrong format (the reference's two errors, src/caps.c:78, 90, 96). */
+int capsx_resume_load(caps_t *self, const char *path);
+/* Extension: F_ξ of frequency xi_ from the resume cache (the matching rule
+ * above).  Returns 1 and sets *value if found, else 0. */
+int capsx_resume_lookup(caps_t *self, double xi_, double *value);
+/* Extension: the free-energy drivers append a line
+ * "# xi*(L+R)/c=%.16g, logdetD=%.16g, t=%g" for every F_ξ they compute
+ * (the reference's output line, src/caps.c:265, 807, 827; cached ones are
+ * not repeated) to path, opened for appending: the file capsx_resume_load
+ * reads back.  NULL or "" closes it.  Returns 0, -1 if it cannot be opened. */
+int capsx_set_output(caps_t *self, const char *path);
 /* Extension: poles ξ_j (ascending) and residues η_j, j < N, of the Padé
  * spectrum decomposition of the Bose function (Hu, Xu, Yan, J. Chem. Phys.
  * 133, 101106 (2010)): 1/(1 - e^{-x}) ≈ 1/x + 1/2 + Σ_j 2η_j x/(x² + ξ_j²).
