@@ -4,12 +4,14 @@ tolerance each reference test uses, and against closed forms.
 
 Relative error is the reference's criterion: x == y or |1 - x/y| < tol
 (src/tests/unittest.c:64-78)."""
+import json
 import math
+import os
 
 import numpy as np
 import pytest
 
-from conftest import load_golden, relerr
+from conftest import GOLDEN, load_golden, relerr
 import pyoracle as po
 
 
@@ -344,3 +346,38 @@ def test_manual_runs_fixture():
         assert r["xi"][0]["xi_"] == 0
         for k, row in enumerate(r["xi"][1:], 1):
             assert abs(row["xi_"] / (k * Ts) - 1) < 1e-13
+
+
+def _tables():
+    rows = []
+    for name in ("ref_tables.json", "ref_tables_c2.json"):
+        path = os.path.join(GOLDEN, name)
+        if os.path.exists(path):
+            with open(path) as f:
+                rows += json.load(f)["data"]
+    return rows
+
+
+@pytest.mark.parametrize("case", _tables(), ids=lambda c: "%s_ldim%d_xi%g_m%d" % (c["kind"], c["ldim"], c["xi_"], c["m"]))
+def test_oracle_K_I_M_vs_reference_tables(case):
+    """The oracle's K_ν, I_{l1 l2} and matrix elements against the tables the
+    reference compiled from its sources wrote (oracle/build_ref.sh,
+    tests/golden/gen_ref_vectors.py), with the tolerances of the device test:
+    K 1e-9 relative; I 1e-9 times the q-sum's condition Σ_q|a_q K| / |I|
+    (orc_block_I_abs); elements 1e-9 relative plus 1e-14 sqrt|M_ii M_jj|."""
+    import pyoracle as po
+    oc = po.caps(case["R"], case["L"], ldim=case["ldim"], epsrel=1e-12,
+                 drude=None if case["omegap"] is None else (case["omegap"], case["gamma"]))
+    B = po.Block(oc, case["xi_"], case["m"])
+    for nu, p, v, s in case["K"]:
+        g, gs = B.K(nu, p)
+        assert gs == s and abs(math.exp(g - v) - 1) < 1e-9, ("K", nu, p, g, v, gs, s)
+    for l1, l2, p, v, s in case["I"]:
+        g, gs = B.I(l1, l2, p)
+        cond = max(1.0, math.exp(B.I_abs(l1, l2, p) - v))
+        assert gs == s and abs(math.exp(g - v) - 1) < 1e-9 * cond, ("I", l1, l2, p, g, v, cond)
+    assert B.lmin == case["lmin"]
+    for l1, l2, p1, p2, v in case["M"]:
+        g = B.elem(l1, l2, p1, p2)
+        scale = math.sqrt(abs(B.elem(l1, l1, p1, p1) * B.elem(l2, l2, p2, p2)))
+        assert abs(g - v) <= 1e-9 * abs(v) + 1e-14 * scale + 1e-300, ("M", l1, l2, p1, p2, g, v, scale)
