@@ -71,6 +71,9 @@ struct KEnt {
     int e;
     int pad_;
 };
+// zeroed KEnt entries in front of every block's K table (BlockDesc::off_k
+// is at least this far into the arena; 256 bytes are reserved)
+constexpr int K_GUARD = 16;
 
 // Per-block scalar results.
 struct BlockOut {

@@ -571,8 +571,11 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         off = align256(off + (5 * (int64_t)ldim + 3 * (int64_t)(ldim + 2)) * 8);
         d.off_kpart = off;
         off = align256(off + (int64_t)d.n_waves * d.nK * sizeof(KPart));
-        d.off_k = off;
-        off = align256(off + 2 * (int64_t)d.nK * (sizeof(capsd::KEnt) + 8));
+        // K table after a zeroed guard of capsd::K_GUARD entries (k_K_combine):
+        // the I recurrence's chunks read up to 2 (chunk - 1) entries below
+        // index 0 for steps past qmax, whose coefficients are 0
+        d.off_k = off + 256;
+        off = align256(d.off_k + 2 * (int64_t)d.nK * (sizeof(capsd::KEnt) + 8));
         d.off_i = off;
         off = align256(off + 2 * (int64_t)d.nl * d.nl * 12);
         d.off_mat = off;
@@ -639,7 +642,7 @@ size_t block_bytes_estimate(caps_t* self, int m)
     size_t lmin, lmax;
     caps_estimate_lminmax(self, m, &lmin, &lmax);
     const size_t nK = 2 * (lmin + ldim) + 4;
-    return (size_t)2 * (ldim + 2) * (ldim + 2) * 12 + lda * N * 8 + lda * (size_t)(2 * capsd_lu_npanels() * capsd_lu_nb() + 2 * capsd_lu_nb()) * 8 + nK * (24 * 2 + 24 * 64) +
+    return (size_t)2 * (ldim + 2) * (ldim + 2) * 12 + 256 + lda * N * 8 + lda * (size_t)(2 * capsd_lu_npanels() * capsd_lu_nb() + 2 * capsd_lu_nb()) * 8 + nK * (24 * 2 + 24 * 64) +
            (size_t)N * 32 + (size_t)((ldim + 15) / 16) * N * 8 + 65536;
 }
 

@@ -253,6 +253,14 @@ __global__ void __launch_bounds__(256) k_K_combine(const BlockDesc* __restrict__
 {
     const BlockDesc& d = desc[blockIdx.y];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < K_GUARD) {
+        // the zeroed guard in front of the table (read by k_I's last chunk)
+        KEnt z;
+        z.m = 0;
+        z.e = 0;
+        z.pad_ = 0;
+        at<KEnt>(arena, d.off_k)[(int)threadIdx.x - K_GUARD] = z;
+    }
     if (j >= d.nK) return;
     const KPart* part = at<KPart>(arena, d.off_kpart) + (int64_t)j * d.n_waves;
     int F = -2000000000;
@@ -299,6 +307,21 @@ __global__ void __launch_bounds__(256) k_K_combine(const BlockDesc* __restrict__
 // (src/misc.c:95-109).  Symmetric in l1, l2; zero below l = max(m,1);
 // perfect reflectors: I_TE = -|I_TM| (src/integration.c:627-671).
 
+// x / y for the a_q recurrence, whose y are products of integers of
+// moderate size (normal, far from overflow): the hardware reciprocal
+// estimate refined by two Newton-Raphson steps, r <- r (2 - y r), then one
+// residual correction of the quotient, q <- q + r (x - y q) (Markstein's
+// scheme; within an ulp or two of the rounded quotient, 8 instructions
+// against 11 of the IEEE division with its scaling and fix-up steps).
+__device__ __forceinline__ double div_nr(double x, double y)
+{
+    double r = __builtin_amdgcn_rcp(y);
+    r = fma(fma(-y, r, 1.0), r, r);
+    r = fma(fma(-y, r, 1.0), r, r);
+    const double q = x * r;
+    return fma(fma(-y, q, x), r, q);
+}
+
 // α(p, n, ν) = Nα(p)/Dα(p) with Nα = (p² - (n+ν+1)²)(p² - (n-ν)²),
 // Dα = 4p² - 1 (the reference's alpha, src/integration.c:485-488).  The
 // recurrence keeps numerator and denominator apart so that one step of
@@ -327,6 +350,7 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                                            int use_skip)
 {
     static_assert(C >= 3, "the convergence test looks at a chunk's last three terms");
+    static_assert(2 * (C - 1) <= K_GUARD, "a chunk reads at most 2 (C - 1) entries below the K table");
     const IGroup& grp = groups[blockIdx.y];
     const int G = grp.count;
     // the group's blocks that take part (uniform over the workgroup)
@@ -508,9 +532,9 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
                 const double num1 = (Ap * Ap * d23 + k2 * al2.N * al3.D + k3 * al3.N * al2.D) * al4.D;
                 const double num2 = k4 * al4.N * d23;
                 const double den = (pq + 2) * (pq + 3) * (p1 + 1) * (p1 + 2) * al1.N * d23 * al4.D;
-                aq = (num1 * aqm1 - num2 * aqm2) * al1.D / den;
+                aq = div_nr((num1 * aqm1 - num2 * aqm2) * al1.D, den);
             } else {
-                aq = (pq + 1) * (p2 + 2) * al2.N * al1.D * aqm1 / ((pq + 2) * (p1 + 1) * al1.N * al2.D);
+                aq = div_nr((pq + 1) * (p2 + 2) * al2.N * al1.D * aqm1, (pq + 2) * (p1 + 1) * al1.N * al2.D);
             }
             al3 = al1;
             al4 = al2;
@@ -524,10 +548,13 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
             for (int pp = 0; pp < NPOL; pp++) {
                 const unsigned bit = 1u << (g * NPOL + pp);
                 if (!__any(!(done & bit))) continue;   // uniform: converged in every lane
-                const KEnt* kp = kb[g] + (POL0 + pp) * nK;
+                // index jb - 2t; steps past qmax (coefficient 0) may reach
+                // 2 (C - 1) entries below the table: the zeroed guard there
+                // (polarization 0) or the other polarization's last entries
+                const KEnt* kp = kb[g] + (POL0 + pp) * nK + jb;
                 KEnt kv[C];
 #pragma unroll
-                for (int t = 0; t < C; t++) kv[t] = kp[max(jb - 2 * t, 0)];   // steps past qmax: any entry
+                for (int t = 0; t < C; t++) kv[t] = kp[-2 * t];
                 double tm[C];
 #pragma unroll
                 for (int t = 0; t < C; t++) {
