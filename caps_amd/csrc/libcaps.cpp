@@ -602,15 +602,16 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         p.max_nl = std::max(p.max_nl, d.nl);
     }
     p.bytes = (size_t)off;
-    // parts = LU stream groups of the tuned schedule, at most two (one
-    // without an LU, and in profiled runs, which time each update launch
-    // alone): the tuning times the LU alone, where three groups can win,
-    // but in the pipeline every further part adds I work that runs before
-    // its LU can start (C2: three parts 253 ms, two 235 ms per step);
-    // CAPS_AMD_PARTS overrides the cap
+    // parts = LU stream groups of the tuned schedule, at most
+    // CAPS_AMD_PARTS (default 1; one without an LU, and in profiled runs,
+    // which time each update launch alone).  Since the I, K and assembly
+    // kernels and the LU's own kernels got faster, running the I of one
+    // part beside the LU of another no longer pays: C2 step 164.5 ms with
+    // one part, 165.4 ms with two, 165.9 ms with three (the kernels slow
+    // each other down as much as they overlap)
     p.parts = 1;
     if (with_lu && !c->prof) {
-        const int cap = std::max(1, (int)env_double("CAPS_AMD_PARTS", 2));
+        const int cap = std::max(1, (int)env_double("CAPS_AMD_PARTS", 1));
         p.parts = std::max(1, std::min(std::min(lu_config(p.maxN, n).streams, cap), std::min(MAX_PARTS, n / 4)));
     }
     p.groups.clear();
