@@ -393,6 +393,65 @@ long capsx_plan_batches(caps_t *self, int reset);
  * frequencies compared, mean and largest |predicted - actual m-cutoff|}. */
 void capsx_surrogate_report(caps_t *self, double out[6], int reset);
 
+/* ---- further extensions: batched points, diagnostics, test hooks ----------
+ * None has a counterpart in the reference's public API; the point functions
+ * evaluate the reference's per-point routines on many points in one launch. */
+
+/* n blocks as capsx_logdetD_multi, with per block tr M (trace, may be NULL)
+ * and flags (may be NULL): bits 0-1 how the value was obtained (0 LU, 1 trace
+ * shortcut |tr M| < 1e-8, 2 LU-mode Mercator shortcut, 3 non-finite trace;
+ * the rules of src/matrix.c:57-149), bit 2 set when the LU ran in delta form
+ * (1 - M column diagonally dominant: no pivot search), bit 3 set when the
+ * HODLR-mode rule |tr M| > |log det| replaced the LU value by -tr M. */
+int capsx_logdetD_multi_info(caps_t *const *objs, const double *xi_, const int *m, int n, double *out,
+                             double *trace, int *flags);
+/* log|a_l|, log|b_l| of the sphere at n points (xi_[i], l[i]), as caps_mie
+ * (src/libcaps.c:557-602) per point.  Returns 0 (1: n < 0). */
+int capsx_mie_points(caps_t *self, int n, const double *xi_, const int *l, double *lna, double *lnb);
+/* Fresnel r_TE, r_TM of the plate at n points (xi_[i], k_[i]), as
+ * caps_fresnel (src/libcaps.c:616-645) per point.  Returns 0 (1: n < 0). */
+int capsx_fresnel_points(caps_t *self, int n, const double *xi_, const double *k_, double *r_TE, double *r_TM);
+/* log I_{n+1/2}(x), log K_{n+1/2}(x) (order[i] = n >= 0; NaN for order < 0)
+ * and I_nu(x)/I_{nu+1}(x) (nu[i] >= 0; NaN for nu < 0) at n points: the device
+ * functions of the Mie coefficients (src/bessel.c:635-660, 781-833). */
+int capsx_bessel_points(int n, const int *order, const double *nu, const double *x, double *logI, double *logK,
+                        double *ratio);
+/* Number of quadrature nodes of the K integrals of block (xi_, m). */
+int capsx_block_nodes(caps_t *self, double xi_, int m);
+/* Test hook: the K table (log|K_nu^p|, nu = mu..nu_max, 2*nK values) and the I
+ * table (log|I|, sign; 2*nl*nl values, l from lo) of one block as computed
+ * on the device.  Returns 0, 1 if nK > cap_nK or nl > cap_nl. */
+int capsx_block_tables(caps_t *self, double xi_, int m, int *mu, int *nK, double *lnK, int *lo, int *nl,
+                       double *logI, signed char *sgnI, int cap_nK, int cap_nl);
+/* Test hook: copy the N x N matrix M of a caps_M_t (column-major); returns N. */
+int capsx_M_copy(caps_M_t *self, double *out);
+/* Test hook: log|det A| of n column-major dim x dim matrices (lda = dim) on
+ * the batched pivoted LU.  Returns 0 on success. */
+int capsx_lu_logdet_batch(const double *A, int dim, int n, double *out);
+/* Test hook: log|det(1 - M)| of n column-major dim x dim matrices M through
+ * the device's column dominance test (delta[b] = 1: delta form, no pivot
+ * search; 0: pivoted LU).  No trace shortcut.  Returns 0 on success. */
+int capsx_logdet_1mM_batch(const double *M, int dim, int n, double *out, int *delta);
+/* Test hook: 1 = delta form for column-dominant blocks (default), 0 = the
+ * pivoted LU for every block.  Returns the previous setting. */
+int capsx_lu_delta_mode(int mode);
+/* Autotuning: capsx_lu_time with the matrix form: 0 = 1 - 0.3 X/sqrt(dim)
+ * (pivoted LU), 1 = 1 - M, M = 0.3 X/dim (column dominant: delta form). */
+int capsx_lu_time_form(int dim, int n, int reps, int levels, int streams, int xcd, int tile, int form,
+                       double *ms);
+/* Test hook: D = A(16x4) B(4x16) through one FP64 MFMA instruction (the
+ * register layout check).  Returns 0 on success. */
+int capsx_mfma_probe(const double *A, const double *B, double *D);
+/* Diagnostics: HIP events around every trailing-update launch (on != 0). */
+void capsx_profile(caps_t *self, int on);
+/* Accumulated statistics since the last reset: out[0..5] stage milliseconds
+ * (Mie, K, I, assembly, LU, total), out[6] blocks, out[7] blocks factored,
+ * out[8] algorithmic LU flops, out[9] batches, out[10] I integrals; with
+ * capsx_profile on, out[11..14] the trailing updates' milliseconds, flops,
+ * launches and algorithmic bytes. */
+void capsx_stats(caps_t *self, double out[16]);
+void capsx_stats_reset(caps_t *self);
+
 #ifdef __cplusplus
 }
 #endif
