@@ -89,6 +89,7 @@ def lib():
             "capsx_resume_load": (i, [p, ctypes.c_char_p]),
             "capsx_resume_lookup": (i, [p, d, dp]),
             "capsx_set_output": (i, [p, ctypes.c_char_p]),
+            "capsx_set_checkpoint": (i, [p, i, d]),
             "capsx_free_energy": (d, [p, d, d, d, i, i, d, i, ctypes.POINTER(ctypes.c_long),
                                       ctypes.POINTER(ctypes.c_long)]),
             "capsx_psd": (i, [i, dp, dp]),
@@ -327,6 +328,15 @@ class Caps:
         F_ξ to path (None closes it)."""
         if lib().capsx_set_output(self.ptr, None if path is None else os.fsencode(path)) != 0:
             raise ValueError("cannot open %s" % path)
+
+    def set_checkpoint(self, nodes=0, seconds=0.0):
+        """T = 0 free energy in batches of at most `nodes` quadrature nodes
+        (0: a refinement level's new nodes at once), each F_ξ written to the
+        output file as its batch completes; with seconds > 0 the driver stops
+        before its next batch once that time has passed and free_energy
+        returns NaN (continue with resume_load of the output file)."""
+        if lib().capsx_set_checkpoint(self.ptr, int(nodes), float(seconds)) != 0:
+            raise ValueError("nodes and seconds must be >= 0")
 
     def surrogate_report(self, reset=True):
         """Surrogate accuracy on the blocks computed for its plans (mode

@@ -232,14 +232,35 @@ void ensure_blocks(Ctx* c, int n)
     c->cap_blocks = cap;
 }
 
+double env_double(const char* name, double dflt);
+
+// HBM budget of one launch sequence (CAPS_AMD_ARENA_GB, default 160 GB):
+// a batch's blocks go into sequences that fit it.
+size_t arena_budget() { return (size_t)(env_double("CAPS_AMD_ARENA_GB", 160.0) * 1e9); }
+
+// The device arena of a batch, grown to at least `bytes`: by a quarter more
+// than before, so that slowly growing batches do not reallocate each time,
+// but not beyond the budget unless `bytes` itself is more.
 void ensure_arena(Ctx* c, size_t bytes)
 {
     if (bytes <= c->arena_size) return;
     HIPCHECK(hipStreamSynchronize(c->stream));
     if (c->arena) HIPCHECK(hipFree(c->arena));
     c->arena = nullptr;
-    const size_t sz = std::max(bytes, (size_t)(c->arena_size * 1.25));
-    HIPCHECK(hipMalloc(&c->arena, sz));
+    const size_t sz = std::max(bytes, std::min((size_t)(c->arena_size * 1.25), arena_budget()));
+    if (hipMalloc(&c->arena, sz) != hipSuccess) {
+        size_t fr = 0, tot = 0;
+        (void)hipGetLastError();
+        (void)hipMemGetInfo(&fr, &tot);
+        TERMINATE(true, "cannot allocate a batch arena of %.3f GB (previous %.3f GB; device memory free %.3f of %.3f GB)",
+                  sz / 1e9, c->arena_size / 1e9, fr / 1e9, tot / 1e9);
+    }
+    if (env_double("CAPS_AMD_PROGRESS", 0) > 0) {
+        size_t fr = 0, tot = 0;
+        (void)hipMemGetInfo(&fr, &tot);
+        fprintf(stderr, "capsx: arena %.3f GB (needed %.3f GB); device memory free %.3f of %.3f GB\n", sz / 1e9,
+                bytes / 1e9, fr / 1e9, tot / 1e9);
+    }
     c->arena_size = sz;
 }
 
@@ -309,6 +330,11 @@ struct ResumeState {
     std::vector<std::pair<double, double>> cache;   // (ξ_, F_ξ)
     FILE* out = nullptr;
     double t0 = 0;
+    // checkpointing of the T = 0 driver (capsx_set_checkpoint): at most
+    // `nodes` quadrature nodes per batch (0: a level's new nodes at once),
+    // stop before a batch once `seconds` have passed (0: no limit)
+    int nodes = 0;
+    double seconds = 0;
 };
 std::mutex g_resume_mu;
 std::unordered_map<const caps_t*, ResumeState> g_resume;
@@ -331,6 +357,14 @@ bool resume_lookup(const caps_t* self, double xi, double* v)
             return true;
         }
     return false;
+}
+
+void checkpoint_get(const caps_t* self, int* nodes, double* seconds)
+{
+    std::lock_guard<std::mutex> lock(g_resume_mu);
+    auto it = g_resume.find(self);
+    *nodes = it == g_resume.end() ? 0 : it->second.nodes;
+    *seconds = it == g_resume.end() ? 0 : it->second.seconds;
 }
 
 void resume_record(const caps_t* self, double xi, double v)
@@ -934,8 +968,7 @@ bool is_lu_mode(detalg_t a) { return dense_mode(a) != 0; }
 // Largest number of blocks per launch sequence that fits the HBM budget.
 int chunk_blocks_multi(caps_t* const* objs, const int* m, int n)
 {
-    const double gb = env_double("CAPS_AMD_ARENA_GB", 160.0);
-    const size_t budget = (size_t)(gb * 1e9);
+    const size_t budget = arena_budget();
     size_t acc = 0;
     int k = 0;
     for (; k < n; k++) {
@@ -1386,7 +1419,15 @@ static int logdetD_multi_ordered(caps_t* host, caps_t* const* objs, const double
     std::lock_guard<std::mutex> lock(c->mu);
     const int lu_mode = dense_mode(objs[0]->detalg);
     double tot[6] = {0, 0, 0, 0, 0, 0};
+    // a batch of several launch sequences takes the whole budget at once:
+    // HIP kept a freed arena's memory from the next allocation (device memory
+    // free after growing a 92.6 GB arena to 100 GB: 115 of 309 GB), so an
+    // arena grown between sequences held both
+    if (chunk_blocks_multi(objs, m, n) < n) ensure_arena(c, arena_budget());
+    // CAPS_AMD_PROGRESS=1: one line on stderr per launch sequence
+    static const bool progress = env_double("CAPS_AMD_PROGRESS", 0) > 0;
     for (int b0 = 0; b0 < n;) {
+        const double t_seq = progress ? wall_seconds() : 0;
         const int k = chunk_blocks_multi(objs + b0, m + b0, n - b0);
         ensure_blocks(c, k);
         BatchPlan p = plan_batch_multi(objs + b0, c, xi_ + b0, m + b0, k, 1);
@@ -1422,6 +1463,9 @@ static int logdetD_multi_ordered(caps_t* host, caps_t* const* objs, const double
             for (size_t j = 0; j < again.size(); j++) take(c->h_out[j], again[j]);
             for (int i = 0; i < 6; i++) tot[i] += c->timing[i];
         }
+        if (progress)
+            fprintf(stderr, "capsx: blocks %d..%d of %d (dimension <= %d, %zu pivoted) in %.2f s\n", b0, b0 + k - 1,
+                    n, p.maxN, again.size(), wall_seconds() - t_seq);
         b0 += k;
     }
     memcpy(c->timing, tot, sizeof(tot));
@@ -2105,8 +2149,13 @@ double fcqs_weight(double t, int N)
 // R/L > 10 the integrand is taken as 0 above the frequency where the PFA
 // estimate of logdet falls below 1e-100 (src/caps.c:256-263).  The rule's
 // levels M = 5, 10, 20, ... 2560 are refined until two successive levels
-// agree to epsrel; every level's new nodes form one batch (modes 1, 2) or
-// run one block at a time (mode 0).
+// agree to epsrel; every level's new nodes form one batch (modes 1, 2; with
+// capsx_set_checkpoint, batches of at most that many nodes) or run one block
+// at a time (mode 0).  Every F_ξ goes to the output file as soon as its
+// batch is done.  With a time limit set by capsx_set_checkpoint, the driver
+// stops before a batch (mode 0: before a node) once the limit has passed and
+// returns NaN, after at least one batch per call; a later call that loads
+// the output into the resume cache continues from there.
 double free_energy_T0(caps_t* self, double cutoff, double epsrel, int mode, long* nblocks, long* nused)
 {
     const int MMIN = 5, MMAX = 2560;
@@ -2117,6 +2166,70 @@ double free_energy_T0(caps_t* self, double cutoff, double epsrel, int mode, long
     std::vector<double> fcache(MMAX + 1, NAN);
     std::vector<int> mstop_at(MMAX + 1, -1);
     Ctx* cx = ctx_of(self);
+    int ck_nodes;
+    double ck_seconds;
+    checkpoint_get(self, &ck_nodes, &ck_seconds);
+    const double t_start = wall_seconds();
+    int nbatches = 0;   // batches (mode 0: nodes) this call has run: at least one runs
+    auto out_of_time = [&](int M, int done, int todo) {
+        if (!(ck_seconds > 0) || nbatches == 0 || wall_seconds() - t_start < ck_seconds) return false;
+        fprintf(stderr,
+                "caps_free_energy: time limit of %g s reached at level M=%d after %d of its %d new nodes; "
+                "the computed F_xi are in the output file\n",
+                ck_seconds, M, done, todo);
+        if (nblocks) *nblocks = computed;
+        if (nused) *nused = used;
+        return true;
+    };
+    // m-sums of the nodes ts[0..n) in one batch (modes 1, 2): first guess of
+    // each node's m-range: with the surrogate (mode 2) its predicted
+    // m-cutoff plus a margin, else the m-cutoff of the nearest known node at
+    // a lower frequency (m-cutoffs fall with ξ at large ξ) plus a margin,
+    // else 64; then every node is extended until its own cutoff is reached
+    auto batch = [&](std::vector<Term>& ts) {
+        std::vector<std::pair<int, int>> req;
+        const int MCAP = std::min(self->ldim - 1, 4000), per = MCAP + 1;
+        std::vector<float> P;
+        std::vector<int> mpred;
+        if (mode == 2) {
+            TERMINATE(!cx->sur, "caps_free_energy mode 2 needs a surrogate (capsx_set_surrogate)");
+            std::vector<double> xs;
+            for (const Term& t : ts) xs.push_back(t.xi);
+            sur_predict(self, xs, MCAP, P);
+            for (size_t q = 0; q < ts.size(); q++) {
+                const int ms = sur_mcut(&P[q * per], MCAP, cutoff);
+                mpred.push_back(ms);
+                req.push_back({(int)q, std::min(self->ldim * 8, ms + std::max(4, ms / 25) + 1)});
+            }
+        }
+        for (size_t q = 0; q < ts.size() && mode != 2; q++) {
+            int guess = -1;
+            double best = -1;
+            for (int idx = 1; idx <= MMAX; idx++) {
+                if (mstop_at[idx] < 0) continue;
+                const double t = M_PI * idx / MMAX, c = cos(t / 2) / sin(t / 2);
+                const double xk = c * c / alpha;
+                if (xk <= ts[q].xi && xk > best) {
+                    best = xk;
+                    guess = mstop_at[idx];
+                }
+            }
+            req.push_back({(int)q, guess >= 0 ? guess + 8 + guess / 8 : 64});
+        }
+        run_requests(self, ts, req, &computed);
+        cx->plan_batches++;
+        for (size_t q = 0; q < ts.size(); q++) {
+            Term& t = ts[q];
+            int upto = (int)t.v.size();
+            while (!msum(t.v, cutoff, &t.value, &t.mstop)) {
+                TERMINATE(upto >= 100000, "sum over m did not converge at xi_=%g", t.xi);
+                upto += 16;
+                run_requests(self, ts, {{(int)q, upto}}, &computed);
+                cx->plan_batches++;
+            }
+            if (mode == 2) sur_score(cx, &P[q * per], MCAP, mpred[q], t);
+        }
+    };
     double Ilast = NAN;
     for (int M = MMIN; M <= MMAX; M *= 2) {
         const int ratio = MMAX / M, N = M - 1;
@@ -2145,68 +2258,39 @@ double free_energy_T0(caps_t* self, double cutoff, double epsrel, int mode, long
             terms.push_back(tm);
             slot.push_back(idx);
         }
+        auto finish = [&](size_t q) {
+            fcache[slot[q]] = terms[q].value;
+            mstop_at[slot[q]] = terms[q].mstop;
+            used += terms[q].mstop + 1;
+            resume_record(self, terms[q].xi, terms[q].value);
+        };
+        const int todo = (int)terms.size();
         if (mode == 0) {
-            for (Term& t : terms)
+            for (int q = 0; q < todo; q++) {
+                if (out_of_time(M, q, todo)) return NAN;
+                Term& t = terms[q];
                 for (int m = 0;; m++) {
                     TERMINATE(m >= 100000, "sum over m did not converge at xi_=%g", t.xi);
                     t.v.push_back(caps_logdetD(self, t.xi, m));
                     computed++;
                     if (msum(t.v, cutoff, &t.value, &t.mstop)) break;
                 }
-        } else if (!terms.empty()) {
-            // first guess of each node's m-range: with the surrogate (mode
-            // 2) its predicted m-cutoff plus a margin, else the m-cutoff of
-            // the nearest known node at a lower frequency (m-cutoffs fall
-            // with ξ at large ξ) plus a margin, else 64; then every node is
-            // extended until its own cutoff is reached
-            std::vector<std::pair<int, int>> req;
-            const int MCAP = std::min(self->ldim - 1, 4000), per = MCAP + 1;
-            std::vector<float> P;
-            std::vector<int> mpred;
-            if (mode == 2) {
-                TERMINATE(!cx->sur, "caps_free_energy mode 2 needs a surrogate (capsx_set_surrogate)");
-                std::vector<double> xs;
-                for (const Term& t : terms) xs.push_back(t.xi);
-                sur_predict(self, xs, MCAP, P);
-                for (size_t q = 0; q < terms.size(); q++) {
-                    const int ms = sur_mcut(&P[q * per], MCAP, cutoff);
-                    mpred.push_back(ms);
-                    req.push_back({(int)q, std::min(self->ldim * 8, ms + std::max(4, ms / 25) + 1)});
+                finish(q);
+                nbatches++;
+            }
+        } else {
+            const int step = ck_nodes > 0 ? ck_nodes : std::max(todo, 1);
+            for (int q0 = 0; q0 < todo; q0 += step) {
+                if (out_of_time(M, q0, todo)) return NAN;
+                const int q1 = std::min(todo, q0 + step);
+                std::vector<Term> ts(terms.begin() + q0, terms.begin() + q1);
+                batch(ts);
+                nbatches++;
+                for (int q = q0; q < q1; q++) {
+                    terms[q] = std::move(ts[q - q0]);
+                    finish(q);
                 }
             }
-            for (size_t q = 0; q < terms.size() && mode != 2; q++) {
-                int guess = -1;
-                double best = -1;
-                for (int idx = 1; idx <= MMAX; idx++) {
-                    if (mstop_at[idx] < 0) continue;
-                    const double t = M_PI * idx / MMAX, c = cos(t / 2) / sin(t / 2);
-                    const double xk = c * c / alpha;
-                    if (xk <= terms[q].xi && xk > best) {
-                        best = xk;
-                        guess = mstop_at[idx];
-                    }
-                }
-                req.push_back({(int)q, guess >= 0 ? guess + 8 + guess / 8 : 64});
-            }
-            run_requests(self, terms, req, &computed);
-            cx->plan_batches++;
-            for (size_t q = 0; q < terms.size(); q++) {
-                Term& t = terms[q];
-                int upto = (int)t.v.size();
-                while (!msum(t.v, cutoff, &t.value, &t.mstop)) {
-                    TERMINATE(upto >= 100000, "sum over m did not converge at xi_=%g", t.xi);
-                    upto += 16;
-                    run_requests(self, terms, {{(int)q, upto}}, &computed);
-                    cx->plan_batches++;
-                }
-                if (mode == 2) sur_score(cx, &P[q * per], MCAP, mpred[q], t);
-            }
-        }
-        for (size_t q = 0; q < terms.size(); q++) {
-            fcache[slot[q]] = terms[q].value;
-            mstop_at[slot[q]] = terms[q].mstop;
-            used += terms[q].mstop + 1;
-            resume_record(self, terms[q].xi, terms[q].value);
         }
         double I = 0;
         for (int i = 1; i <= N; i++) {
@@ -2518,6 +2602,23 @@ int capsx_set_output(caps_t* self, const char* path)
     st.out = fopen(path, "a");
     if (!st.out) return -1;
     st.t0 = wall_seconds();
+    return 0;
+}
+
+// Extension: checkpointing of the T = 0 free energy, for runs longer than
+// one job: at most `nodes` quadrature nodes per batch (0: every new node of
+// a refinement level in one batch, the default), and a time limit in
+// seconds (0: none) after which caps_free_energy stops before its next batch
+// and returns NaN (every call runs at least one batch).  Each F_ξ is written to the output file (capsx_set_output)
+// when its batch is done; the next job loads that file with
+// capsx_resume_load and continues.  Returns 0, or -1 for negative values.
+int capsx_set_checkpoint(caps_t* self, int nodes, double seconds)
+{
+    if (nodes < 0 || !(seconds >= 0)) return -1;
+    std::lock_guard<std::mutex> lock(g_resume_mu);
+    ResumeState& st = g_resume[self];
+    st.nodes = nodes;
+    st.seconds = seconds;
     return 0;
 }
 

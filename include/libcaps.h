@@ -333,6 +333,17 @@ int capsx_resume_lookup(caps_t *self, double xi_, double *value);
  * not repeated) to path, opened for appending: the file capsx_resume_load
  * reads back.  NULL or "" closes it.  Returns 0, -1 if it cannot be opened. */
 int capsx_set_output(caps_t *self, const char *path);
+/* Extension: checkpointing of the T = 0 free energy for runs longer than one
+ * job (the reference's way is to kill the MPI job and rerun with --resume,
+ * src/caps.c:74-105, losing only the frequencies in flight).  The driver
+ * batches at most `nodes` quadrature nodes at a time (0: all new nodes of a
+ * refinement level, the default), writes each F_ξ to the output file as its
+ * batch completes, and with seconds > 0 stops before its next batch once
+ * that many seconds have passed since the call began, returning NaN (every
+ * call runs at least one batch); a later call after capsx_resume_load of that
+ * file continues.  Returns 0, -1 for a
+ * negative argument. */
+int capsx_set_checkpoint(caps_t *self, int nodes, double seconds);
 /* Extension: poles ξ_j (ascending) and residues η_j, j < N, of the Padé
  * spectrum decomposition of the Bose function (Hu, Xu, Yan, J. Chem. Phys.
  * 133, 101106 (2010)): 1/(1 - e^{-x}) ≈ 1/x + 1/2 + Σ_j 2η_j x/(x² + ξ_j²).

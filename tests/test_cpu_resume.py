@@ -59,3 +59,17 @@ def test_output_line_format(tmp_path):
     assert p.exists()
     with pytest.raises(ValueError):
         c.set_output(str(tmp_path / "no" / "such" / "dir.txt"))
+
+
+def test_checkpoint_arguments():
+    """capsx_set_checkpoint rejects negative node counts and times (host code)."""
+    import caps_amd
+    c = caps_amd.Caps(50e-6, 500e-9, ldim=30)
+    with pytest.raises(ValueError):
+        c.set_checkpoint(nodes=-1)
+    with pytest.raises(ValueError):
+        c.set_checkpoint(seconds=-1.0)
+    with pytest.raises(ValueError):
+        c.set_checkpoint(seconds=float("nan"))
+    c.set_checkpoint(nodes=3, seconds=0.0)
+    c.set_checkpoint()

@@ -53,3 +53,30 @@ def test_resume_plain_mode_matches(ca, tmp_path):
     F1, nb1, _ = _run(ca, 300.0, out=str(full), mode="plain")
     F3, nb3, _ = _run(ca, 300.0, resume=str(full), mode="plain")
     assert nb3 == 0 and abs(F3 - F1) <= 1e-14 * abs(F1)
+
+
+def test_checkpointed_T0_runs_one_node_per_call(ca, tmp_path):
+    """With one quadrature node per batch and a time limit already passed,
+    every call computes one node, writes its line and returns NaN; calls that
+    resume from the growing output file end with the uninterrupted F."""
+    import math
+    F1, nb1, _ = _run(ca, 0.0)
+    out = tmp_path / "ck.out"
+    calls, nb_sum, F = 0, 0, float("nan")
+    while math.isnan(F):
+        calls += 1
+        assert calls <= 200
+        c = ca.Caps(R, L, ldim=LDIM)
+        c.set_output(str(out))
+        if out.exists():
+            c.resume_load(str(out))
+        c.set_checkpoint(nodes=1, seconds=1e-9)
+        F, nb, _ = c.free_energy(0, mode="batched")
+        c.set_output(None)
+        nb_sum += nb
+        lines = [ln for ln in out.read_text().splitlines() if ln.startswith("# xi*(L+R)/c=")]
+        assert len(lines) == calls
+    print("T=0 checkpointed: %d calls, %d blocks (uninterrupted %d), F %.16g vs %.16g" % (calls, nb_sum, nb1, F, F1))
+    assert calls >= 4
+    assert abs(F - F1) <= 1e-14 * abs(F1)
+
