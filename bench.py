@@ -321,23 +321,31 @@ def large_aspect_energy(ca, out, budget):
                 "F_over_PFA_minus_DE": F / F_pfa - (1 + theta * LbyR),
                 "last_two_levels_rel_diff": abs(1 - levels[Ms[-2]] / levels[Ms[-1]]) if len(Ms) > 1 else None,
                 "rule_error_on_pfa_shape": pfa_rule_error(Ms[-1])})
-    # plain-form identity: the largest computed frequency recomputed one block
-    # at a time, every other F_ξ from the file
+    # plain-form identity: the computed frequency nearest ξ(L+R)/c = 1e4
+    # recomputed one block at a time, every other F_ξ from the file
     xs = []
     with open(out) as f:
         for line in f:
             if line.startswith("# xi*(L+R)/c="):
-                xs.append((float(line[13:].split(",")[0]), line))
-    top = max(xs)[0]
-    rest = out + ".rest"
+                a, b = line[13:].split(", logdetD=")
+                xs.append((float(a), float(b.split(",")[0]), line))
+    pick = min(xs, key=lambda r: abs(math.log(r[0] / 1e4)))
+    rest, plain_out = out + ".rest", out + ".plain"
     with open(rest, "w") as f:
-        f.writelines(line for x, line in xs if x != top)
+        f.writelines(r[2] for r in xs if r[0] != pick[0])
     c2 = ca.Caps(LA_R, LA_L, ldim=LA_LDIM)
     c2.resume_load(rest)
+    c2.set_output(plain_out)
     t0 = time.perf_counter()
     Fp, nbp, _ = c2.free_energy(0, cutoff=CUTOFF, epsrel=LA_EPSREL, mode="plain")
-    res["plain_check"] = {"xi_": top, "blocks": nbp, "seconds": time.perf_counter() - t0, "F_plain": Fp,
-                          "rel_diff": abs(Fp - F) / abs(F)}
+    dtp = time.perf_counter() - t0
+    c2.set_output(None)
+    with open(plain_out) as f:
+        fxi = [float(line.split(", logdetD=")[1].split(",")[0]) for line in f if line.startswith("# xi*(L+R)/c=")]
+    res["plain_check"] = {"xi_": pick[0], "blocks": nbp, "seconds": dtp, "F_xi_batched": pick[1],
+                          "F_xi_plain": fxi[-1] if fxi else None,
+                          "F_xi_rel_diff": abs(fxi[-1] - pick[1]) / abs(pick[1]) if fxi else None,
+                          "F_plain": Fp, "F_rel_diff": abs(Fp - F) / abs(F)}
     return res
 
 
