@@ -119,6 +119,7 @@ struct Ctx {
     hipStream_t lu_stream[MAX_LU_STREAMS];
     hipEvent_t lu_done[MAX_LU_STREAMS];
     hipEvent_t part_i_done[MAX_PARTS];   // part g's I finished
+    hipEvent_t lu_ready = nullptr;        // a one-part batch's matrices assembled
     char* arena = nullptr;
     size_t arena_size = 0;
     BlockDesc* d_desc = nullptr;
@@ -171,6 +172,7 @@ Ctx* ctx_of(caps_t* self)
             HIPCHECK(hipEventCreateWithFlags(&c->lu_done[g], hipEventDisableTiming));
         }
         for (auto& e : c->part_i_done) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&c->lu_ready, hipEventDisableTiming));
         HIPCHECK(hipMalloc(&c->d_status, sizeof(int)));
         HIPCHECK(hipHostMalloc((void**)&c->h_status, sizeof(int), hipHostMallocDefault));
         self->priv = c;
@@ -202,6 +204,7 @@ void ctx_free(Ctx* c)
         hipEventDestroy(c->lu_done[g]);
     }
     for (auto& e : c->part_i_done) hipEventDestroy(e);
+    hipEventDestroy(c->lu_ready);
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -501,7 +504,7 @@ void grid_for(BlockDesc& d)
 // while a second factored the parts in turn was slower, 0.31-0.44 s for
 // 2-8 parts).
 struct BatchPlan {
-    int n = 0, maxN = 0, max_ldim = 0, max_waves = 0, max_nK = 0, max_nl = 0;
+    int n = 0, maxN = 0, minN = 0, max_ldim = 0, max_waves = 0, max_nK = 0, max_nl = 0;
     int64_t max_ntri = 0;
     size_t bytes = 0;
     std::vector<IGroup> groups;   // per part, ordered by class (see build_igroups)
@@ -629,6 +632,7 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         } else
             s.off_panel = s.off_logs = s.off_moved = s.off_ut = s.off_inv = 0;
         p.maxN = std::max(p.maxN, N);
+        p.minN = b == 0 ? N : std::min(p.minN, N);
         p.max_ldim = std::max(p.max_ldim, ldim);
         p.max_waves = std::max(p.max_waves, d.n_waves);
         p.max_nK = std::max(p.max_nK, d.nK);
@@ -687,9 +691,34 @@ size_t block_bytes_estimate(caps_t* self, int m)
 // events around every trailing-update launch (one group).  kinds: the
 // determinant forms the blocks may take (capsd_launch_lu).  Returns the
 // number of update launches of the first group.
+// The LU's stream groups of blocks [0, n) (descriptors in c->h_desc):
+// contiguous runs of about equal LU cost (N^3; larger blocks come first in a
+// batch, so a dominant block may run alone), bnd[g] .. bnd[g+1], each up to
+// its own largest N, gN[g] (at most maxN).  Returns the number of groups.
+int lu_groups(const Ctx* c, int n, int maxN, int streams, int bnd[MAX_LU_STREAMS + 1], int gN[MAX_LU_STREAMS])
+{
+    const int groups = std::max(1, std::min(streams, std::min(MAX_LU_STREAMS, n / 4)));
+    std::vector<double> cum(n + 1, 0.0);
+    for (int b = 0; b < n; b++) cum[b + 1] = cum[b] + std::pow((double)c->h_desc[b].N, 3);
+    bnd[0] = 0;
+    for (int g = 1; g < groups; g++) {
+        int b = bnd[g - 1] + 1;
+        while (b < n - (groups - g) && cum[b] < cum[n] * g / groups) b++;
+        bnd[g] = b;
+    }
+    bnd[groups] = n;
+    for (int g = 0; g < groups; g++) {
+        gN[g] = 0;
+        for (int b = bnd[g]; b < bnd[g + 1]; b++) gN[g] = std::max(gN[g], c->h_desc[b].N);
+        gN[g] = std::min(gN[g], maxN);
+    }
+    return groups;
+}
+
 int launch_lu_groups(Ctx* c, int n, int maxN, const LuCfg& cfg, hipEvent_t ready, bool prof, int kinds)
 {
-    const int groups = std::max(1, std::min(cfg.streams, std::min(MAX_LU_STREAMS, n / 4)));
+    int bnd[MAX_LU_STREAMS + 1], gN[MAX_LU_STREAMS];
+    const int groups = lu_groups(c, n, maxN, cfg.streams, bnd, gN);
     if (groups == 1)
         return capsd_launch_lu(c->d_desc, c->arena, c->d_side, c->d_out, n, maxN, c->stream, cfg.levels, cfg.xcd,
                                cfg.tile, prof ? c->gemm_ev.data() : nullptr, prof ? c->gemm_info.data() : nullptr,
@@ -700,10 +729,10 @@ int launch_lu_groups(Ctx* c, int n, int maxN, const LuCfg& cfg, hipEvent_t ready
     // the library's stream holds a queue of its own
     int launched = 0;
     for (int g = groups - 1; g >= 0; g--) {
-        const int b0 = (int)((int64_t)n * g / groups), b1 = (int)((int64_t)n * (g + 1) / groups);
+        const int b0 = bnd[g], b1 = bnd[g + 1];
         hipStream_t sg = g == 0 ? c->stream : c->lu_stream[g - 1];
         if (g > 0) HIPCHECK(hipStreamWaitEvent(sg, ready, 0));
-        const int k = capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, b1 - b0, maxN, sg,
+        const int k = capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, b1 - b0, gN[g], sg,
                                       cfg.levels, cfg.xcd, cfg.tile, nullptr, nullptr, kinds);
         if (g == 0) launched = k;
     }
@@ -712,6 +741,31 @@ int launch_lu_groups(Ctx* c, int n, int maxN, const LuCfg& cfg, hipEvent_t ready
         HIPCHECK(hipStreamWaitEvent(c->stream, c->lu_done[g], 0));
     }
     return launched;
+}
+
+// LU schedule of a planned one-part batch.  Blocks of unequal N run in
+// MAX_LU_STREAMS groups of about equal cost, whose chains of dependent panel
+// steps then overlap, each with the schedule of its share of n_eff =
+// sum N^3 / maxN^3 blocks of the largest N (mixed-aspect batch of 64 blocks,
+// N 614-5462: LU 15.9 ms in one group with the table's schedule for 64
+// blocks, 12.4 ms this way).  One group when profiled: HIP events time the
+// launches of one group.
+LuCfg batch_lu_config(const Ctx* c, const BatchPlan& p, bool prof)
+{
+    LuCfg cfg = lu_config(p.maxN, p.n);
+    bool fixed;
+    {
+        std::lock_guard<std::mutex> lock(g_cfg_mu);
+        fixed = g_have_override;   // a schedule fixed by capsx_lu_config applies as it is
+    }
+    if (p.parts == 1 && p.minN < p.maxN && !fixed) {
+        double w = 0;
+        for (int b = 0; b < p.n; b++) w += std::pow((double)c->h_desc[b].N / p.maxN, 3);
+        cfg = lu_config(p.maxN, std::max(1, (int)lround(w / MAX_LU_STREAMS)));
+        cfg.streams = MAX_LU_STREAMS;
+    }
+    if (prof) cfg.streams = 1;
+    return cfg;
 }
 
 // Run the pipeline for n blocks already planned; mode 0: LU path, 1: M only.
@@ -758,7 +812,13 @@ int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, b
         if (events && last) HIPCHECK(hipEventRecord(c->ev[4], sg));
         if (mode != 0) continue;
         const bool prof = c->prof && p.parts == 1;
-        const LuCfg cfg = lu_config(p.maxN, n);
+        const LuCfg cfg = batch_lu_config(c, p, prof);
+        if (p.parts == 1) {
+            // one part: the LU in the schedule's stream groups
+            HIPCHECK(hipEventRecord(c->lu_ready, s));
+            nlaunched = launch_lu_groups(c, n, p.maxN, cfg, c->lu_ready, prof, p.lu_kinds);
+            continue;
+        }
         const int k = capsd_launch_lu(c->d_desc + b0, c->arena, c->d_side + b0, c->d_out + b0, nb, p.maxN, sg,
                                       cfg.levels, cfg.xcd, cfg.tile, prof ? c->gemm_ev.data() : nullptr,
                                       prof ? c->gemm_info.data() : nullptr, p.lu_kinds);
@@ -776,12 +836,13 @@ int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, b
 // are read by the kernels at run time and are not part of it.
 std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts)
 {
-    const LuCfg cfg = lu_config(p.maxN, p.n);
+    const LuCfg cfg = batch_lu_config(c, p, c->prof && p.parts == 1);
     struct {
         int n, maxN, max_ldim, max_waves, max_nK, mode, shortcuts, lu_kinds, delta_mode;
         int64_t max_ntri;
         int parts, part_b[MAX_PARTS + 1], part_goff[MAX_PARTS], cls[MAX_PARTS][8];
         int levels, streams, xcd, tile;
+        int lu_bnd[MAX_LU_STREAMS + 1], lu_gN[MAX_LU_STREAMS];   // the LU's stream groups (lu_groups)
         int lu_kernels[3], grid_panel_rows;   // LU kernel variants (capsx_lu_kernels, capsx_lu_grid_panel_rows)
         const void *arena, *desc, *side, *out, *groups;
     } k;
@@ -804,6 +865,7 @@ std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_sh
     k.streams = cfg.streams;
     k.xcd = cfg.xcd;
     k.tile = cfg.tile;
+    if (p.parts == 1 && mode == 0) lu_groups(c, p.n, p.maxN, cfg.streams, k.lu_bnd, k.lu_gN);
     capsd_lu_kernels(-1, -1, -1, k.lu_kernels);
     k.grid_panel_rows = capsd_lu_grid_panel_rows(0);
     k.arena = c->arena;
@@ -1364,9 +1426,11 @@ int capsx_logdetD_multi_info(caps_t* const* objs, const double* xi_, const int* 
     if (n <= 0) return 0;
     // the blocks of one caps object and m (whose I tables share the a_q,
     // k_I) are run next to each other, so that they fall into one part of
-    // the batch and into full groups: objects in the order of their first
-    // block, then m, then the input order.  The batch runs on objs[0]'s
-    // context (device buffers, statistics, graph setting) either way.
+    // the batch and into full groups; larger blocks first (the LU's stream
+    // groups are contiguous runs of blocks, balanced by cost): ldim
+    // descending, then objects in the order of their first block, then m,
+    // then the input order.  The batch runs on objs[0]'s context (device
+    // buffers, statistics, graph setting) either way.
     std::vector<int> ord(n), first(n);
     {
         std::unordered_map<caps_t*, int> seen;
@@ -1375,8 +1439,9 @@ int capsx_logdetD_multi_info(caps_t* const* objs, const double* xi_, const int* 
             ord[i] = i;
         }
     }
+    auto ldim_of = [&](int i) { return objs[i] ? objs[i]->ldim : 0; };
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
-        return std::make_tuple(first[a], m[a]) < std::make_tuple(first[b], m[b]);
+        return std::make_tuple(-ldim_of(a), first[a], m[a]) < std::make_tuple(-ldim_of(b), first[b], m[b]);
     });
     bool same = true;
     for (int i = 0; i < n && same; i++) same = ord[i] == i;

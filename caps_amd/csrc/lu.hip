@@ -23,7 +23,7 @@
 //                   buffer), double-buffered;
 //   k_lu_gemm       the same product with register-staged operands, for
 //                   128 x 64, 64 x 128, 64 x 64 and 256 x 128 tiles.
-// k_lu_finalize sums log|u_ii| with Kahan compensation.
+// k_lu_finalize sums log|u_ii| in double-double arithmetic.
 // Matrices are column-major with leading dimension lda.
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -347,20 +347,33 @@ __device__ __forceinline__ void lu_panel_body(const BlockDesc* __restrict__ desc
 // 1), so products with the triangular inverses are as accurate as the
 // substitutions they replace.
 
-// The 64 x 64 diagonal block of panel pb at step k: unpivoted LU with the
-// pivots u = 1 + d (d the stored diagonal), one lane of wavefront 0 per
-// row, rows held in registers and the pivot row broadcast through LDS (16-
-// byte accesses); log|u_ii| = log1p(d); the factored block goes to the panel
-// buffer's top rows.  Then the block is stored transposed in LDS and
-// wavefront 0 forms column r of L11^{-1} (forward substitution), wavefront 1
-// column r of U11^{-1} (backward substitution), both column-oriented (each
-// solved x_q is applied to all later entries at once: independent FMAs, the
-// L11 or U11 column read as 16-byte LDS broadcasts).  Rows and columns past
-// the matrix (a last panel narrower than 64) act as identity.
-__global__ void __launch_bounds__(128) k_lu_diag(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+// The 64 x 64 diagonal block of panel pb at step k, by one workgroup of four
+// wavefronts with the block in LDS, in compact loops (a fully unrolled
+// 64-step form is larger than the instruction cache, and a lone block's LU
+// then waits on instruction fetch).  Unpivoted blocked LU over four 16-column
+// panels, pivots u = 1 + d (d the stored diagonal), log|u_ii| = log1p(d):
+// (a) wavefront 0 factors the panel's rows c0..63 (one lane per row, the 16
+//     columns in registers, pivot rows broadcast by readlane);
+// (b) U12 of the panel's 16 rows: forward substitution with the panel's unit
+//     lower block, one lane per column;
+// (c) the trailing block minus L21 U12 on the FP64 MFMA units (16 x 16
+//     tiles, K = 16, spread over the wavefronts).
+// The factored block goes to the panel buffer's top rows.  Then the
+// triangular inverses, blocked by 16: the diagonal blocks of L11^{-1} and
+// U11^{-1} by substitution (lane = column of one of the four blocks), the
+// blocks at distance 1, 2, 3 from the diagonal as MFMA products,
+// X_ij = -X_ii sum_{k=j}^{i-1} L_ik X_kj and Y_ij = -Y_ii sum_{k=i+1}^{j}
+// U_ik Y_kj; L11^{-1} is stored column-major, U11^{-1} row-major.  Rows and
+// columns past the matrix (a last panel narrower than 64) act as identity.
+__global__ void __launch_bounds__(256, 2) k_lu_diag(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                  const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
                                                  int k, int pb)
 {
+    constexpr int WP = NB + 1;   // LDS pitch (doubles)
+    // W: the block, after the LU L (strict lower) and U - I (upper);
+    // Z: L11^{-1} below the diagonal, U11^{-1} on and above it
+    __shared__ double W[NB * WP];
+    __shared__ double Z[NB * WP];
     const int b = blockIdx.x;
     const BlockDesc& d = desc[b];
     if (out[b].skip || !out[b].delta) return;
@@ -371,157 +384,252 @@ __global__ void __launch_bounds__(128) k_lu_diag(const BlockDesc* __restrict__ d
     const double* __restrict__ A = at<double>(arena, d.off_mat);
     double* __restrict__ P = panel_buf(arena, side[b], lda, pb);
     double* __restrict__ inv = inv_buf(arena, side[b], pb);
-    // LU phase: S[c][j] holds pivot row c; inversion phase: S[j][i] holds
-    // the factored block's entry (i, j) (column j contiguous)
-    constexpr int SP = NB + 2;   // pitch: 16-byte aligned rows
-    __shared__ __align__(16) double S[NB * SP];
-    __shared__ double rinv[NB];   // 1 / (1 + d_i)
-    const int r = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    double a[NB];
-    if (wv == 0) {
-#pragma unroll
-        for (int j = 0; j < NB; j++) a[j] = (r < nbk && j < nbk) ? A[(k0 + r) + (int64_t)(k0 + j) * lda] : 0.0;
-    }
-#pragma unroll
-    for (int c = 0; c < NB; c++) {
-        if (wv == 0 && r == c) {
-            // the pivot row from column c & ~1 on, in 16-byte pieces
-#pragma unroll
-            for (int j = c & ~1; j < NB; j += 2)
-                *reinterpret_cast<double2*>(&S[c * SP + j]) = make_double2(a[j], a[j + 1]);
-        }
-        __syncthreads();
-        if (wv == 0 && r > c) {
-            const double ru = 1.0 / (1.0 + S[c * SP + c]);
-            const double l = a[c] * ru;
-            a[c] = l;
-            if (c + 1 < NB) {
-                // columns c+1 .. NB-1: the first alone when c+1 is odd
-                int j = c + 1;
-                if (j & 1) {
-                    a[j] -= l * S[c * SP + j];
-                    j++;
-                }
-#pragma unroll
-                for (; j < NB; j += 2) {
-                    const double2 u = *reinterpret_cast<const double2*>(&S[c * SP + j]);
-                    a[j] -= l * u.x;
-                    a[j + 1] -= l * u.y;
-                }
-            }
-        }
-    }
-    __syncthreads();   // the last pivot row is read; S is reused
-    if (wv == 0) {
-        double dr = 0;   // a[r] (selected: a lane-dependent index would put a[] in scratch)
-#pragma unroll
-        for (int j = 0; j < NB; j++) dr = j == r ? a[j] : dr;
-        if (r < nbk) {
-            at<double>(arena, side[b].off_logs)[k0 + r] = log_abs_1p(dr);
-            moved_buf(arena, side[b], pb)[r] = r;
-#pragma unroll
-            for (int j = 0; j < NB; j++)
-                if (j < nbk) P[r + (int64_t)j * lda] = a[j];
-        }
-#pragma unroll
-        for (int j = 0; j < NB; j++) S[j * SP + r] = a[j];   // column j, row r
-        rinv[r] = 1.0 / (1.0 + dr);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;   // MFMA operand row / k index
+#pragma unroll 4
+    for (int s = 0; s < 16; s++) {
+        const int j = wv + 4 * s;
+        W[lane * WP + j] = (lane < nbk && j < nbk) ? A[(k0 + lane) + (int64_t)(k0 + j) * lda] : 0.0;
     }
     __syncthreads();
-    double x[NB];
+#pragma unroll 1
+    for (int p = 0; p < 4; p++) {
+        const int c0 = 16 * p;
+        // (a) panel columns c0 .. c0+15, rows c0 .. 63
+        if (wv == 0) {
+            double a[16];
 #pragma unroll
-    for (int i = 0; i < NB; i++) x[i] = (i == r) ? 1.0 : 0.0;
-    if (wv == 0) {
-        // column r of L11^{-1} (unit lower): x = e_r; for q: x_i -= L_iq x_q, i > q
+            for (int t = 0; t < 16; t++) a[t] = W[lane * WP + c0 + t];
 #pragma unroll
-        for (int q = 0; q < NB - 1; q++) {
-            const double xq = x[q];
-            int i = q + 1;
-            if (i & 1) {
-                x[i] -= S[q * SP + i] * xq;
-                i++;
+            for (int t = 0; t < 16; t++) {
+                const int pr = c0 + t;
+                double u[16];
+#pragma unroll
+                for (int j = t; j < 16; j++) u[j] = lane_d(a[j], pr);
+                const double ru = 1.0 / (1.0 + u[t]);
+                if (lane > pr) {
+                    const double l = a[t] * ru;
+                    a[t] = l;
+#pragma unroll
+                    for (int j = t + 1; j < 16; j++) a[j] -= l * u[j];
+                }
             }
+            if (lane >= c0) {
 #pragma unroll
-            for (; i < NB; i += 2) {
-                const double2 l = *reinterpret_cast<const double2*>(&S[q * SP + i]);
-                x[i] -= l.x * xq;
-                x[i + 1] -= l.y * xq;
+                for (int t = 0; t < 16; t++) W[lane * WP + c0 + t] = a[t];
             }
         }
+        __syncthreads();
+        if (p == 3) break;
+        // (b) rows c0 .. c0+15, columns c0+16 .. 63: y_t -= L_ts y_s, s < t
+        if (wv == 0 && lane < 48 - c0) {
+            const int j = c0 + 16 + lane;
+            double y[16];
 #pragma unroll
-        for (int i = 0; i < NB; i++) inv[(int64_t)r * NB + i] = x[i];          // column-major
-    } else {
-        // column r of U11^{-1}: x = e_r; for q from the last: x_q /= u_qq,
-        // x_i -= U_iq x_q, i < q
+            for (int t = 0; t < 16; t++) y[t] = W[(c0 + t) * WP + j];
 #pragma unroll
-        for (int q = NB - 1; q >= 0; q--) {
-            const double xq = x[q] * rinv[q];
-            x[q] = xq;
+            for (int t = 1; t < 16; t++)
 #pragma unroll
-            for (int i = 0; i + 1 < q; i += 2) {
-                const double2 u = *reinterpret_cast<const double2*>(&S[q * SP + i]);
-                x[i] -= u.x * xq;
-                x[i + 1] -= u.y * xq;
-            }
-            if (q & 1) x[q - 1] -= S[q * SP + q - 1] * xq;
+                for (int s2 = 0; s2 < t; s2++) y[t] -= W[(c0 + t) * WP + c0 + s2] * y[s2];
+#pragma unroll
+            for (int t = 1; t < 16; t++) W[(c0 + t) * WP + j] = y[t];
         }
+        __syncthreads();
+        // (c) trailing tiles: W -= L21 U12 (K = 16)
+        const int nt = 3 - p;
+#pragma unroll 1
+        for (int tile = wv; tile < nt * nt; tile += 4) {
+            const int i0 = c0 + 16 + 16 * (tile / nt), j0 = c0 + 16 + 16 * (tile % nt);
+            double4_t acc;
 #pragma unroll
-        for (int i = 0; i < NB; i++) inv[NB * NB + (int64_t)i * NB + r] = x[i];   // row-major
+            for (int q = 0; q < 4; q++) acc[q] = W[(i0 + lk + 4 * q) * WP + j0 + li];
+#pragma unroll
+            for (int kk = 0; kk < 16; kk += 4) {
+                const double av = -W[(i0 + li) * WP + c0 + kk + lk];
+                const double bv = W[(c0 + kk + lk) * WP + j0 + li];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) W[(i0 + lk + 4 * q) * WP + j0 + li] = acc[q];
+        }
+        __syncthreads();
+    }
+    // the factored block, log|u_ii|, the (identity) row order
+#pragma unroll 4
+    for (int s = 0; s < 16; s++) {
+        const int j = wv + 4 * s;
+        if (lane < nbk && j < nbk) P[lane + (int64_t)j * lda] = W[lane * WP + j];
+    }
+    if (wv == 0 && lane < nbk) {
+        at<double>(arena, side[b].off_logs)[k0 + lane] = log_abs_1p(W[lane * WP + lane]);
+        moved_buf(arena, side[b], pb)[lane] = lane;
+    }
+    // diagonal blocks of the inverses: wavefront 0 L^{-1}, wavefront 1 U^{-1};
+    // lane = 16 * block + column
+    {
+        const int bb = 16 * lk, c = li;
+        double x[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) x[i] = (i == c) ? 1.0 : 0.0;
+        if (wv == 0) {
+            // x = e_c; for q: x_i -= L_iq x_q, i > q
+#pragma unroll
+            for (int q = 0; q < 15; q++)
+#pragma unroll
+                for (int i = q + 1; i < 16; i++) x[i] -= W[(bb + i) * WP + bb + q] * x[q];
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if (i > c) Z[(bb + i) * WP + bb + c] = x[i];
+        } else if (wv == 1) {
+            // x = e_c; for q from the last: x_q /= u_qq, x_i -= U_iq x_q, i < q
+            double ru[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) ru[q] = 1.0 / (1.0 + W[(bb + q) * WP + bb + q]);
+#pragma unroll
+            for (int q = 15; q >= 0; q--) {
+                const double xq = x[q] * ru[q];
+                x[q] = xq;
+#pragma unroll
+                for (int i = 0; i < q; i++) x[i] -= W[(bb + i) * WP + bb + q] * xq;
+            }
+#pragma unroll
+            for (int i = 0; i < 16; i++)
+                if (i <= c) Z[(bb + i) * WP + bb + c] = x[i];
+        }
+    }
+    __syncthreads();
+    // blocks at distance dd from the diagonal: tasks 0 .. 3-dd of L^{-1}
+    // (block (dd + t, t)), then those of U^{-1} (block (t, dd + t))
+#pragma unroll 1
+    for (int dd = 1; dd < 4; dd++) {
+        const int ntask = 4 - dd;
+#pragma unroll 1
+        for (int task = wv; task < 2 * ntask; task += 4) {
+            const bool lower = task < ntask;
+            const int t = lower ? task : task - ntask;
+            const int bi = lower ? dd + t : t, bj = lower ? t : dd + t;   // block row, column
+            const int kb0 = lower ? bj : bi + 1, kb1 = lower ? bi - 1 : bj;
+            double4_t acc = (double4_t){0, 0, 0, 0};
+#pragma unroll 1
+            for (int kb = kb0; kb <= kb1; kb++) {
+#pragma unroll
+                for (int kk = 0; kk < 16; kk += 4) {
+                    // W block (bi, kb) times Z block (kb, bj); a diagonal
+                    // Z block is unit lower (L^{-1}) or upper (U^{-1})
+                    const double av = W[(16 * bi + li) * WP + 16 * kb + kk + lk];
+                    const int zr = kk + lk;
+                    double bv = Z[(16 * kb + zr) * WP + 16 * bj + li];
+                    if (kb == bj) bv = lower ? (zr > li ? bv : (zr == li ? 1.0 : 0.0)) : (zr <= li ? bv : 0.0);
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                }
+            }
+            // times -(diagonal block bi of the inverse); the sum's register q
+            // holds rows 4q .. 4q+3 of K: the B operand of step q
+            double4_t res = (double4_t){0, 0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int zc = 4 * q + lk;
+                const double z = Z[(16 * bi + li) * WP + 16 * bi + zc];
+                const double av = lower ? (li > zc ? -z : (li == zc ? -1.0 : 0.0)) : (li <= zc ? -z : 0.0);
+                res = __builtin_amdgcn_mfma_f64_16x16x4f64(av, acc[q], res, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) Z[(16 * bi + lk + 4 * q) * WP + 16 * bj + li] = res[q];
+        }
+        __syncthreads();
+    }
+    // L11^{-1} column-major (unit diagonal, zero above), U11^{-1} row-major
+    // (zero below)
+#pragma unroll 4
+    for (int s = 0; s < 16; s++) {
+        const int c = wv + 4 * s;
+        const int r = lane;
+        inv[(int64_t)c * NB + r] = r > c ? Z[r * WP + c] : (r == c ? 1.0 : 0.0);
+        inv[NB * NB + (int64_t)c * NB + r] = c <= r ? Z[c * WP + r] : 0.0;
     }
 }
 
-// L21 = A21 U11^{-1} for the rows below the diagonal block of panel pb:
-// 4 wavefronts of 64 rows per workgroup, each 4 x 4 MFMA tiles of 16 x 16
-// over K = 64; A21 read from the matrix (16-row column segments), U11^{-1}
-// (row-major, L2-resident) as the B operand; the product, with swapped
-// operands so each register holds whole 16-row column segments, goes to the
-// panel buffer (rows relative to the panel's first row), the trailing
-// updates' L operand.
+// L21 = A21 U11^{-1} for the rows below the diagonal block of panel pb, in
+// MFMA tiles of 16 x 16 over K = 64.  A wavefront takes 64 rows and CT of
+// the four 16-column tiles of the result: CT = 4 (workgroups of 256 rows,
+// 256 MFMAs per wavefront) when the batch has rows enough to fill the GPU,
+// CT = 1 (workgroups of 64 rows, the four wavefronts one column tile each,
+// 64 MFMAs per wavefront, all operands loaded before the first product) for
+// a short grid, whose time is the latency of one workgroup.  A21 read from
+// the matrix (16-row column segments), U11^{-1} (row-major, L2-resident) as
+// the B operand; the product, with swapped operands so each register holds
+// whole 16-row column segments, goes to the panel buffer (rows relative to
+// the panel's first row), the trailing updates' L operand.
+template <int CT>
 __global__ void __launch_bounds__(256) k_lu_l21(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
                                                int k, int pb)
 {
+    static_assert(CT == 1 || CT == 4, "column tiles per wavefront");
+    constexpr int WPR = 4 / CT;   // wavefronts per 64-row group
     const int b = blockIdx.y;
     const BlockDesc& d = desc[b];
     if (out[b].skip || !out[b].delta) return;
     const int N = d.N, k0 = k + pb * NB;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int m0 = k0 + NB + blockIdx.x * 256 + wv * 64;   // this wave's first row
+    const int m0 = k0 + NB + blockIdx.x * (64 * CT) + (wv / WPR) * 64;   // this wave's first row
     if (m0 >= N) return;
+    const int j0 = (wv % WPR) * CT;   // first column tile
     const int64_t lda = d.lda;
     const double* __restrict__ A = at<double>(arena, d.off_mat);
     double* __restrict__ P = panel_buf(arena, side[b], lda, pb);
     const double* __restrict__ Ui = inv_buf(arena, side[b], pb) + NB * NB;
     const int li = lane & 15, lk = lane >> 4;
-    double4_t acc[4][4];
+    double4_t acc[4][CT];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = (double4_t){0, 0, 0, 0};
+        for (int j = 0; j < CT; j++) acc[i][j] = (double4_t){0, 0, 0, 0};
+    if constexpr (CT == 4) {
 #pragma unroll
-    for (int kk = 0; kk < NB; kk += 4) {
-        double av[4], bv[4];
+        for (int kk = 0; kk < NB; kk += 4) {
+            double av[4], bv[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = m0 + 16 * i + li;
-            av[i] = row < N ? A[row + (int64_t)(k0 + kk + lk) * lda] : 0.0;
+            for (int i = 0; i < 4; i++) {
+                const int row = m0 + 16 * i + li;
+                av[i] = row < N ? A[row + (int64_t)(k0 + kk + lk) * lda] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) bv[j] = Ui[(kk + lk) * NB + 16 * j + li];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+        }
+    } else {
+        double av[NB / 4][4], bv[NB / 4];
+#pragma unroll
+        for (int s = 0; s < NB / 4; s++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = m0 + 16 * i + li;
+                av[s][i] = row < N ? A[row + (int64_t)(k0 + 4 * s + lk) * lda] : 0.0;
+            }
+            bv[s] = Ui[(4 * s + lk) * NB + 16 * j0 + li];
         }
 #pragma unroll
-        for (int j = 0; j < 4; j++) bv[j] = Ui[(kk + lk) * NB + 16 * j + li];
+        for (int s = 0; s < NB / 4; s++)
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < 4; i++)
+                acc[i][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[s], av[s][i], acc[i][0], 0, 0, 0);
     }
-    // lane l, register q of acc[i][j]: row m0 + 16i + l%16, column 16j + l/16 + 4q
+    // lane l, register q of acc[i][j]: row m0 + 16i + l%16, column
+    // 16 (j0 + j) + l/16 + 4q
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int row = m0 + 16 * i + li;
         if (row < N) {
 #pragma unroll
-            for (int j = 0; j < 4; j++)
+            for (int j = 0; j < CT; j++)
 #pragma unroll
-                for (int q = 0; q < 4; q++) P[(row - k0) + (int64_t)(16 * j + lk + 4 * q) * lda] = acc[i][j][q];
+                for (int q = 0; q < 4; q++)
+                    P[(row - k0) + (int64_t)(16 * (j0 + j) + lk + 4 * q) * lda] = acc[i][j][q];
         }
     }
 }
@@ -1130,14 +1238,15 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
     gemm_store_tile(A, lda, acc, m0 + wr, n0 + wc, rr, cq, rend, cend, pre);
 }
 
-// Kahan sum of log|u_ii| (src/matrix.c:462-471).
+// Sum of log|u_ii| (the reference sums them with Kahan compensation,
+// src/matrix.c:462-471), in double-double arithmetic over the wavefront.
 __global__ void __launch_bounds__(64) k_lu_finalize(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                    const LuSide* __restrict__ side, BlockOut* __restrict__ out,
                                                    int nblocks)
 {
     const int b = blockIdx.x;   // one wavefront per block
     if (b >= nblocks || out[b].skip) return;
-    const double s = wave_kahan_sum(at<double>(arena, side[b].off_logs), desc[b].N);
+    const double s = wave_sum_dd(at<double>(arena, side[b].off_logs), desc[b].N);
     if (threadIdx.x == 0) out[b].logdet = s;
 }
 
@@ -1451,11 +1560,18 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         if (k0 >= maxN) return;
         if (piv) launch_panel(maxN - k0, desc, arena, sd, out, nblocks, k0, pb, s);
         if (!dlt) return;
-        hipLaunchKernelGGL(k_lu_diag, dim3(nblocks), dim3(128), 0, s, desc, arena, sd, out, k, pb);
+        hipLaunchKernelGGL(k_lu_diag, dim3(nblocks), dim3(256), 0, s, desc, arena, sd, out, k, pb);
         const int below = maxN - k0 - NB;
-        if (below > 0)
-            hipLaunchKernelGGL(k_lu_l21, dim3((below + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, sd, out, k,
-                               pb);
+        // whole 64 x 64 tiles per wavefront while they give two workgroups
+        // per CU or more, else one column tile per wavefront
+        if (below > 0) {
+            if ((int64_t)((below + 255) / 256) * nblocks >= 512)
+                hipLaunchKernelGGL(k_lu_l21<4>, dim3((below + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, sd,
+                                   out, k, pb);
+            else
+                hipLaunchKernelGGL(k_lu_l21<1>, dim3((below + 63) / 64, nblocks), dim3(256), 0, s, desc, arena, sd,
+                                   out, k, pb);
+        }
     };
     // buffers [qbegin, qend) take the pivots of the later panels pfirst..plast
     auto swap_panels = [&](int k, int qbegin, int qend, int pfirst, int plast) {

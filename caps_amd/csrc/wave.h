@@ -78,4 +78,38 @@ __device__ __forceinline__ double wave_kahan_sum(const double* __restrict__ x, i
     return s;
 }
 
+// Sum of x[0..n) by a whole wavefront in double-double arithmetic: lane l
+// adds x[l], x[l+64], ... with error-free transformations (TwoSum) into a
+// head and a tail, then the 64 head/tail pairs are combined in a butterfly;
+// the rounded head + tail is returned in every lane.  As accurate as a
+// compensated (Kahan) sum in index order, with a dependent chain of n/64
+// steps instead of n.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ double wave_sum_dd(const double* __restrict__ x, int n)
+{
+    const int lane = threadIdx.x & 63;
+    double s = 0, c = 0;
+    for (int i = lane; i < n; i += 64) {
+        double t, e;
+        two_sum(s, x[i], t, e);
+        s = t;
+        c += e;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double so = __shfl_xor(s, off), co = __shfl_xor(c, off);
+        double t, e;
+        two_sum(s, so, t, e);
+        s = t;
+        c = (c + co) + e;
+    }
+    return s + c;
+}
+
 }  // namespace capsd

@@ -541,6 +541,34 @@ def test_graph_replay_matches_eager(ca, streams):
         at.set_config(0)
 
 
+def test_graph_replay_uneven_batches(ca):
+    """Batches of blocks of unequal dimension (two geometries) run their LU in
+    cost-balanced stream groups; captured and replayed they give the eager
+    values bit for bit, also for a second batch of the same size and largest
+    dimension whose groups split differently, and with replays interleaved."""
+    def objs():
+        return make(ca, 20e-6, 1e-6, 120), make(ca, 10e-6, 1e-6, 60)
+    big, small = objs()
+    gbig, gsmall = objs()
+    b1 = [big] * 8 + [small] * 12
+    b2 = [big] * 12 + [small] * 8
+    xs = np.linspace(0.5, 3.0, 20)
+    ms = np.arange(20) % 7
+    ref1 = ca.logdetD_multi(b1, xs, ms)
+    ref2 = ca.logdetD_multi(b2, xs, ms)
+    g1 = [gbig if o is big else gsmall for o in b1]
+    g2 = [gbig if o is big else gsmall for o in b2]
+    gbig.set_graph(True)
+    try:
+        for rep in range(3):
+            v1 = ca.logdetD_multi(g1, xs, ms)
+            assert np.array_equal(v1, ref1), (rep, np.abs(v1 - ref1).max())
+            v2 = ca.logdetD_multi(g2, xs, ms)
+            assert np.array_equal(v2, ref2), (rep, np.abs(v2 - ref2).max())
+    finally:
+        gbig.set_graph(False)
+
+
 def test_lu_grid_panel_small_threshold(ca):
     """The grid-wide column panel (used for tall panels) on small matrices,
     with and without row interchanges, both LU schedules."""
