@@ -910,7 +910,9 @@ __global__ void __launch_bounds__(64) k_trace(const BlockDesc* __restrict__ desc
 // the two-term series.  lower != 0 (CHOLESKY): both sums over the matrix the
 // reference builds for that mode, M with its upper triangle left at zero
 // (src/matrix.c:93-102), i.e. ||M||_F² over i >= j and tr M² over the
-// diagonal.  One workgroup per block.
+// diagonal.  One workgroup per block, over pairs of 64 x 64 tiles (I, J),
+// I <= J: both tiles read coalesced (one column segment per load) into LDS,
+// where the transposed products m_ij m_ji are formed.
 __global__ void __launch_bounds__(256) k_frobenius(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                    BlockOut* __restrict__ out, int lower)
 {
@@ -918,19 +920,45 @@ __global__ void __launch_bounds__(256) k_frobenius(const BlockDesc* __restrict__
     if (out[blockIdx.x].skip == 1) return;   // trace shortcut already taken
     const double* A = at<double>(arena, d.off_mat);
     const double* dg = at<double>(arena, d.off_diag);
+    const int N = d.N, nt = (N + 63) / 64;
+    const int64_t lda = d.lda;
+    __shared__ double T1[64 * 65], T2[64 * 65];   // tiles (I, J) and (J, I), [row][column]
+    const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+    auto m = [&](int i, int j) { return i >= N || j >= N ? 0.0 : i == j ? dg[i] : -A[i + (int64_t)j * lda]; };
     double f = 0, t2 = 0;
-    for (int64_t idx = threadIdx.x; idx < (int64_t)d.N * d.N; idx += blockDim.x) {
-        const int i = (int)(idx % d.N), j = (int)(idx / d.N);
-        if (lower && i < j) continue;
-        const double mij = i == j ? dg[i] : -A[i + (int64_t)j * d.lda];
-        f += mij * mij;
-        if (!lower) {
-            const double mji = i == j ? dg[i] : -A[j + (int64_t)i * d.lda];
-            t2 += mij * mji;
-        } else if (i == j) {
-            t2 += mij * mij;
+    for (int I = 0; I < nt; I++)
+        for (int J = I; J < nt; J++) {
+#pragma unroll 4
+            for (int s2 = 0; s2 < 16; s2++) {
+                const int c = w + 4 * s2;
+                T1[r * 65 + c] = m(64 * I + r, 64 * J + c);
+                if (J != I) T2[r * 65 + c] = m(64 * J + r, 64 * I + c);
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int s2 = 0; s2 < 16; s2++) {
+                const int c = w + 4 * s2;   // element (64I + r, 64J + c) and its mirror
+                const double a = T1[r * 65 + c];
+                if (J == I) {
+                    if (!lower) {
+                        f += a * a;
+                        t2 += a * T1[c * 65 + r];
+                    } else if (r >= c) {
+                        f += a * a;
+                        if (r == c) t2 += a * a;
+                    }
+                } else {
+                    const double b = T2[c * 65 + r];   // element (64J + c, 64I + r)
+                    if (!lower) {
+                        f += a * a + b * b;
+                        t2 += 2 * (a * b);
+                    } else {
+                        f += b * b;   // below the diagonal: rows of tile J > columns of tile I
+                    }
+                }
+            }
+            __syncthreads();
         }
-    }
     __shared__ double sf[256], st[256];
     sf[threadIdx.x] = f;
     st[threadIdx.x] = t2;
