@@ -438,16 +438,20 @@ def main():
                  "speedup_vs_plain": (nb / (total / args.steps)) / (nbp / dtp)}
         print("bench: plain step %.3f s, %d blocks" % (dtp, nbp), file=sys.stderr, flush=True)
 
-    # dominant kernel: the LU's trailing update (k_lu_gemm_dma), FP64 MFMA
+    # dominant kernel: the LU's trailing update (k_lu_gemm_dma), FP64 MFMA;
+    # its launches only (the trailing updates of regions one panel wide or
+    # high run on k_lu_gemm tiles and are reported beside it)
     gemm_s = prof["gemm_ms"] / 1e3
     achieved = prof["gemm_flops"] / gemm_s / 1e12 if gemm_s > 0 else None
     roof = {"kernel": "k_lu_gemm_dma", "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS if achieved else None, "traffic": gemm_traffic(),
+            "traffic_source": "stored: profiles/k_lu_gemm_dma_c2_traffic.txt (FETCH_SIZE + WRITE_SIZE runs)",
             "launches": prof["gemm_launches"],
             "avg_launch_us": 1e3 * prof["gemm_ms"] / max(1, prof["gemm_launches"]),
             "flops_per_launch": prof["gemm_flops"] / max(1, prof["gemm_launches"]),
             "algorithmic_bytes_per_launch": prof["gemm_bytes"] / max(1, prof["gemm_launches"]),
-            "arithmetic_intensity_flop_per_byte": prof["gemm_flops"] / max(1.0, prof["gemm_bytes"])}
+            "arithmetic_intensity_flop_per_byte": prof["gemm_flops"] / max(1.0, prof["gemm_bytes"]),
+            "other_update_launches_ms": prof["other_update_ms"]}
     lu_s = st["stage_ms"]["lu"] / 1e3
     mixed = None if args.no_mixed else mixed_aspect(caps_amd, args.steps)
     large = None if args.no_large else large_aspect(caps_amd)

@@ -208,7 +208,8 @@ class Caps:
         names = ["mie", "K", "I", "assemble", "lu", "total"]
         return {"stage_ms": dict(zip(names, a[:6])), "blocks": int(a[6]), "blocks_factored": int(a[7]),
                 "lu_flops": a[8], "batches": int(a[9]), "I_integrals": a[10],
-                "gemm_ms": a[11], "gemm_flops": a[12], "gemm_launches": int(a[13]), "gemm_bytes": a[14]}
+                "gemm_ms": a[11], "gemm_flops": a[12], "gemm_launches": int(a[13]), "gemm_bytes": a[14],
+                "other_update_ms": a[15]}
 
     def profile(self, on=True):
         """Time every launch of the LU's trailing-update kernel with HIP events."""

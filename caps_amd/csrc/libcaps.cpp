@@ -895,7 +895,7 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
     if (c->prof && (int)c->gemm_ev.size() < 2 * nupd) {
         const size_t old = c->gemm_ev.size();
         c->gemm_ev.resize(2 * nupd);
-        c->gemm_info.resize(6 * nupd);
+        c->gemm_info.resize(7 * nupd);
         for (size_t i = old; i < c->gemm_ev.size(); i++) HIPCHECK(hipEventCreate(&c->gemm_ev[i]));
     }
     int nlaunched = 0;
@@ -944,10 +944,15 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
               *c->h_status, n, c->h_desc[0].xi_, c->h_desc[0].m);
     for (int i = 0; i < 6; i++) c->acc[i] += c->timing[i];
     if (c->prof && mode == 0) {
-        // algorithmic flops of each update launch: 2 x rows x columns x depth
+        // algorithmic flops of each k_lu_gemm_dma launch: 2 x rows x columns
+        // x depth; the time of the other update launches (k_lu_gemm tiles)
         for (int pi = 0; pi < nlaunched; pi++) {
-            const int* g = &c->gemm_info[6 * pi];   // k, r_off, r_lim, c_off, c_lim, KK
+            const int* g = &c->gemm_info[7 * pi];   // k, r_off, r_lim, c_off, c_lim, KK, dma
             HIPCHECK(hipEventElapsedTime(&ms, c->gemm_ev[2 * pi], c->gemm_ev[2 * pi + 1]));
+            if (!g[6]) {
+                c->acc[15] += ms;
+                continue;
+            }
             c->acc[11] += ms;
             c->acc[13] += 1;
             for (int b = 0; b < n; b++) {

@@ -1475,8 +1475,8 @@ int capsd_lu_npanels(void) { return NPANEL; }
 // 8 wavefronts) or 64 (64 x 64, one wavefront); xcd_order: tile order of the
 // update (k_lu_gemm).
 // gemm_ev: NULL, or 2 events per update launch recorded around it;
-// gemm_info receives (k, r_off, r_lim, c_off, c_lim, KK) of each update
-// launch.  kinds: bit 0 set if the batch may hold blocks for the pivoted LU
+// gemm_info receives (k, r_off, r_lim, c_off, c_lim, KK, dma) of each update
+// launch (dma = 1 for k_lu_gemm_dma, 0 for the k_lu_gemm tiles).  kinds: bit 0 set if the batch may hold blocks for the pivoted LU
 // (out[b].delta = 0), bit 1 if it may hold delta-form blocks; kernels for a
 // kind the caller rules out are not launched.  Returns the number of update
 // launches.
@@ -1494,6 +1494,7 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         const int cols = min(maxN, k + c_off + min(c_lim, maxN)) - (k + c_off);
         if (rows <= 0 || cols <= 0) return;
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p], s);
+        int dma = 0;   // 1: this launch is k_lu_gemm_dma
         if (tile == 64) {
             const int gx = (rows + 63) / 64, gy = (cols + 63) / 64;
             hipLaunchKernelGGL((k_lu_gemm<1, 1>), dim3(gx * gy * nb8), dim3(64), 0, s, desc, arena, sd, out, k, r_off,
@@ -1518,6 +1519,7 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
                                r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
         } else {
             const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
+            dma = 1;
             if (gemm_kernel == 2)
                 hipLaunchKernelGGL((k_lu_gemm_dma<8, 3>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out,
                                    k, r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
@@ -1527,8 +1529,8 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         }
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
         if (gemm_info) {
-            int* g = gemm_info + 6 * p;
-            g[0] = k, g[1] = r_off, g[2] = r_lim, g[3] = c_off, g[4] = c_lim, g[5] = KK;
+            int* g = gemm_info + 7 * p;
+            g[0] = k, g[1] = r_off, g[2] = r_lim, g[3] = c_off, g[4] = c_lim, g[5] = KK, g[6] = dma;
         }
         p++;
     };

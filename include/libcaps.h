@@ -458,8 +458,10 @@ void capsx_profile(caps_t *self, int on);
 /* Accumulated statistics since the last reset: out[0..5] stage milliseconds
  * (Mie, K, I, assembly, LU, total), out[6] blocks, out[7] blocks factored,
  * out[8] algorithmic LU flops, out[9] batches, out[10] I integrals; with
- * capsx_profile on, out[11..14] the trailing updates' milliseconds, flops,
- * launches and algorithmic bytes. */
+ * capsx_profile on, out[11..14] the milliseconds, algorithmic flops,
+ * launches and algorithmic bytes of the trailing updates run by
+ * k_lu_gemm_dma, out[15] the milliseconds of the other trailing-update
+ * launches (k_lu_gemm tiles for regions one panel wide or high). */
 void capsx_stats(caps_t *self, double out[16]);
 void capsx_stats_reset(caps_t *self);
 
