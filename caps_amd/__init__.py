@@ -62,6 +62,13 @@ def lib():
             "caps_last_timing": (None, [p, dp]),
             "caps_M_init": (p, [p, i, d]), "caps_M_elem": (d, [p, i, i, ctypes.c_char, ctypes.c_char]),
             "caps_M_free": (None, [p]), "caps_kernel_M": (d, [i, i, p]),
+            "caps_integrate_init": (p, [p, d, i, d]), "caps_integrate_free": (None, [p]),
+            "caps_integrate_K": (d, [p, i, i, ctypes.POINTER(ctypes.c_byte)]),
+            "caps_integrate_I": (d, [p, i, i, i, ctypes.POINTER(ctypes.c_byte)]),
+            "caps_integrate_A": (d, [p, i, i, i, ctypes.POINTER(ctypes.c_byte)]),
+            "caps_integrate_B": (d, [p, i, i, i, ctypes.POINTER(ctypes.c_byte)]),
+            "caps_integrate_C": (d, [p, i, i, i, ctypes.POINTER(ctypes.c_byte)]),
+            "caps_integrate_D": (d, [p, i, i, i, ctypes.POINTER(ctypes.c_byte)]),
             "caps_ht_drude": (d, [p]),
             "caps_free_energy": (d, [p, d, d, d, i, i, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
             "caps_logdetD0": (None, [p, i, d, dp, dp, dp]),

@@ -1555,9 +1555,21 @@ int caps_logdetD_batch(caps_t* self, const double* xi_, const int* m, int n, dou
 
 // Host copy of a block's matrix (the reference's integration_t slot of
 // caps_M_t).
+// The K and I tables of one block (xi_, m) copied from the device, behind
+// the integration object of caps_integrate_init.
+struct IntegTables {
+    int m = 0;
+    bool is_pr = false;
+    int mu = 0, nK = 0, lo = 0, nl = 0;
+    std::vector<double> lnK;            // [2][nK]: log|K_nu^p|, nu = mu ..
+    std::vector<double> logI;           // [2][nl][nl]: log|I_{l1 l2}^p|, l from lo
+    std::vector<signed char> sgnI;      // [2][nl][nl]
+};
+
 struct caps_block_state {
-    double* M;   // N x N, column-major, lda = N
+    double* M;   // N x N, column-major, lda = N (caps_M_init)
     int N;
+    IntegTables* tab;   // caps_integrate_init; NULL for a caps_M_t's state
 };
 
 double caps_logdetD(caps_t* self, double xi_, int m)
@@ -3187,8 +3199,10 @@ int capsx_mfma_probe(const double* A, const double* B, double* D)
 
 // Extension (test hook): the K table (log|K_ν^p|, ν = mu..nu_max) and the I
 // table (log|I|, sign) of one block, from the values the kernels compute.
-int capsx_block_tables(caps_t* self, double xi_, int m, int* mu, int* nK, double* lnK /* 2*nK */, int* lo,
-                       int* nl, double* logI /* 2*nl*nl */, signed char* sgnI /* 2*nl*nl */, int cap_nK, int cap_nl)
+// The K table (log|K|) and the full I table (log|I|, sign; the device keeps
+// the lower triangle, the upper one is its mirror image) of block (xi_, m)
+// as computed on the device.
+static void block_tables_host(caps_t* self, double xi_, int m, IntegTables& t)
 {
     Ctx* c = ctx_of(self);
     std::lock_guard<std::mutex> lock(c->mu);
@@ -3196,39 +3210,196 @@ int capsx_block_tables(caps_t* self, double xi_, int m, int* mu, int* nK, double
     BatchPlan p = plan_batch(self, c, &xi_, &m, 1, 0);
     run_batch(self, c, p, 1, 0);
     const BlockDesc& d = c->h_desc[0];
-    *mu = d.mu;
-    *nK = d.nK;
-    *lo = d.lo;
-    *nl = d.nl;
-    if (d.nK > cap_nK || d.nl > cap_nl) return 1;
-    if (lnK) {
-        HIPCHECK(hipMemcpy(lnK, c->arena + d.off_k + 2 * sizeof(capsd::KEnt) * (int64_t)d.nK, 2 * d.nK * sizeof(double),
-                           hipMemcpyDeviceToHost));
+    t.m = m;
+    t.is_pr = d.is_pr != 0;
+    t.mu = d.mu;
+    t.nK = d.nK;
+    t.lo = d.lo;
+    t.nl = d.nl;
+    t.lnK.resize(2 * (size_t)d.nK);
+    HIPCHECK(hipMemcpy(t.lnK.data(), c->arena + d.off_k + 2 * sizeof(capsd::KEnt) * (int64_t)d.nK,
+                       2 * d.nK * sizeof(double), hipMemcpyDeviceToHost));
+    // the table holds I = mant 2^exp; kept as log|I| and sign
+    const size_t nl_ = (size_t)d.nl, nn = 2 * nl_ * nl_;
+    std::vector<int> ex(nn);
+    t.logI.resize(nn);
+    t.sgnI.resize(nn);
+    HIPCHECK(hipMemcpy(t.logI.data(), c->arena + d.off_i, nn * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(ex.data(), c->arena + d.off_i + 16 * (int64_t)d.nl * d.nl, nn * sizeof(int),
+                       hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nn; i++) {
+        const double mt = t.logI[i];
+        t.sgnI[i] = (signed char)((mt > 0) - (mt < 0));
+        t.logI[i] = mt == 0 ? -INFINITY : log(fabs(mt)) + ex[i] * M_LN2;
     }
+    for (size_t pol = 0; pol < 2; pol++)
+        for (size_t r = 0; r < nl_; r++)
+            for (size_t col = r + 1; col < nl_; col++) {
+                const size_t i = pol * nl_ * nl_ + r * nl_ + col, src = pol * nl_ * nl_ + col * nl_ + r;
+                t.logI[i] = t.logI[src];
+                t.sgnI[i] = t.sgnI[src];
+            }
+}
+
+int capsx_block_tables(caps_t* self, double xi_, int m, int* mu, int* nK, double* lnK /* 2*nK */, int* lo,
+                       int* nl, double* logI /* 2*nl*nl */, signed char* sgnI /* 2*nl*nl */, int cap_nK, int cap_nl)
+{
+    IntegTables t;
+    block_tables_host(self, xi_, m, t);
+    *mu = t.mu;
+    *nK = t.nK;
+    *lo = t.lo;
+    *nl = t.nl;
+    if (t.nK > cap_nK || t.nl > cap_nl) return 1;
+    if (lnK) memcpy(lnK, t.lnK.data(), t.lnK.size() * sizeof(double));
     if (logI) {
-        // the table holds I = mant 2^exp; returned as log|I| and sign
-        const size_t nn = 2 * (size_t)d.nl * d.nl;
-        std::vector<int> ex(nn);
-        HIPCHECK(hipMemcpy(logI, c->arena + d.off_i, nn * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(ex.data(), c->arena + d.off_i + 16 * (int64_t)d.nl * d.nl, nn * sizeof(int),
-                           hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < nn; i++) {
-            const double mt = logI[i];
-            sgnI[i] = (signed char)((mt > 0) - (mt < 0));
-            logI[i] = mt == 0 ? -INFINITY : log(fabs(mt)) + ex[i] * M_LN2;
-        }
-        // the device keeps the lower triangle (row >= column); the upper one
-        // is its mirror image
-        const size_t nl_ = (size_t)d.nl;
-        for (size_t pol = 0; pol < 2; pol++)
-            for (size_t r = 0; r < nl_; r++)
-                for (size_t col = r + 1; col < nl_; col++) {
-                    const size_t i = pol * nl_ * nl_ + r * nl_ + col, src = pol * nl_ * nl_ + col * nl_ + r;
-                    logI[i] = logI[src];
-                    sgnI[i] = sgnI[src];
-                }
+        memcpy(logI, t.logI.data(), t.logI.size() * sizeof(double));
+        memcpy(sgnI, t.sgnI.data(), t.sgnI.size());
     }
     return 0;
+}
+
+// ---- the k-integrals of one block: src/include/integration.h:13-23 --------
+// The reference evaluates K_ν and I_{l1 l2} on demand by quadrature and caches
+// them; here caps_integrate_init computes the block's whole K and I tables on
+// the device (the tables caps_logdetD builds: ν from 2m, or 2 for m = 0, up
+// to 2(lmin + ldim) + 1, and l1, l2 over the block's ℓ-window with one more
+// on either side), and the functions read them.  epsrel is accepted and
+// unused (the device quadrature has a fixed accuracy, 1e-11 relative).
+// Values outside the tables abort with a message.
+
+// Reference: src/integration.c:695-731 (NULL for xi_ < 0, m < 0 or
+// epsrel <= 0; here also for xi_ = 0, which has no device tables).
+integration_t* caps_integrate_init(caps_t* caps, double xi_, int m, double epsrel)
+{
+    if (!caps || !(xi_ > 0) || m < 0 || !(epsrel > 0)) return NULL;
+    TERMINATE(2 * caps->ldim > capsd_lu_max_dim(), "ldim=%d exceeds the maximum %d", caps->ldim,
+              capsd_lu_max_dim() / 2);
+    caps_block_state* st = (caps_block_state*)calloc(1, sizeof(caps_block_state));
+    TERMINATE(!st, "out of memory");
+    st->tab = new IntegTables();
+    block_tables_host(caps, xi_, m, *st->tab);
+    return st;
+}
+
+// Reference: src/integration.c:737-746.
+void caps_integrate_free(integration_t* integration)
+{
+    if (!integration) return;
+    delete integration->tab;
+    free(integration->M);
+    free(integration);
+}
+
+static const IntegTables& integ_tables(const integration_t* self)
+{
+    TERMINATE(!self || !self->tab, "not an object of caps_integrate_init");
+    return *self->tab;
+}
+
+// log|K_nu^p| and its sign (TE negative, TM positive: the sign of r_p).
+// Reference: src/integration.c:452-490.
+double caps_integrate_K(integration_t* self, int nu, polarization_t p, sign_t* sign)
+{
+    const IntegTables& t = integ_tables(self);
+    TERMINATE(nu < t.mu || nu >= t.mu + t.nK, "K: nu=%d outside the table [%d, %d]", nu, t.mu, t.mu + t.nK - 1);
+    // perfect reflectors: r_TE = -r_TM, K_TE = -K_TM (the device computes TM)
+    const int pol = (p == TE && !t.is_pr) ? 0 : 1;
+    const double v = t.lnK[(size_t)pol * t.nK + (nu - t.mu)];
+    *sign = std::isinf(v) && v < 0 ? 0 : (p == TE ? -1 : 1);
+    return v;
+}
+
+// log|I_{l1 l2}^p| and its sign; 0 (log -inf) when l1 < m or l2 < m.
+// Reference: src/integration.c:627-693.
+double caps_integrate_I(integration_t* self, int l1, int l2, polarization_t p, sign_t* sign)
+{
+    const IntegTables& t = integ_tables(self);
+    if (l1 < t.m || l2 < t.m) {
+        *sign = 0;
+        return -INFINITY;
+    }
+    TERMINATE(l1 < t.lo || l2 < t.lo || l1 >= t.lo + t.nl || l2 >= t.lo + t.nl,
+              "I: (l1, l2) = (%d, %d) outside the table [%d, %d]", l1, l2, t.lo, t.lo + t.nl - 1);
+    const size_t i = ((size_t)(p == TE ? 0 : 1) * t.nl + (l1 - t.lo)) * t.nl + (l2 - t.lo);
+    *sign = t.sgnI[i];
+    return t.logI[i];
+}
+
+namespace {
+// log|Σ_i c_i s_i e^{x_i}| and its sign, for the A..D combinations of I
+double log_combination(int n, const double* c, const double* x, const sign_t* s, sign_t* sign)
+{
+    double mx = -INFINITY;
+    for (int i = 0; i < n; i++)
+        if (s[i] != 0 && c[i] != 0) mx = std::max(mx, x[i]);
+    if (std::isinf(mx)) {
+        *sign = 0;
+        return -INFINITY;
+    }
+    double sum = 0;
+    for (int i = 0; i < n; i++)
+        if (s[i] != 0 && c[i] != 0) sum += c[i] * s[i] * exp(x[i] - mx);
+    *sign = (sign_t)((sum > 0) - (sum < 0));
+    return sum == 0 ? -INFINITY : mx + log(fabs(sum));
+}
+}  // namespace
+
+// A = m² I_{l1 l2} (0 for m = 0).  Reference: src/integration.c:762-776.
+double caps_integrate_A(integration_t* self, int l1, int l2, polarization_t p, sign_t* sign)
+{
+    const int m = integ_tables(self).m;
+    if (m == 0) {
+        *sign = 0;
+        return -INFINITY;
+    }
+    const double v = caps_integrate_I(self, l1, l2, p, sign);
+    return *sign == 0 ? -INFINITY : 2 * log((double)m) + v;
+}
+
+// B = Σ over l1' = l1 ∓ 1, l2' = l2 ∓ 1 of the recurrence coefficients of
+// P_l^m' times I_{l1' l2'} (m = 0: B = I_{l1 l2}).  Reference:
+// src/integration.c:794-831.
+double caps_integrate_B(integration_t* self, int l1, int l2, polarization_t p, sign_t* sign)
+{
+    const int m = integ_tables(self).m;
+    if (m == 0) return caps_integrate_I(self, l1, l2, p, sign);
+    // P_l^m'-recurrence weights: u_l = (l+1)(l+m)/(2l+1) on l-1, v_l = l(l-m+1)/(2l+1) on l+1
+    const double u1 = (l1 + 1.0) * (l1 + m) / (2 * l1 + 1.0), v1 = l1 * (l1 - m + 1.0) / (2 * l1 + 1.0);
+    const double u2 = (l2 + 1.0) * (l2 + m) / (2 * l2 + 1.0), v2 = l2 * (l2 - m + 1.0) / (2 * l2 + 1.0);
+    double x[4];
+    sign_t s[4];
+    x[0] = caps_integrate_I(self, l1 - 1, l2 - 1, p, &s[0]);
+    x[1] = caps_integrate_I(self, l1 + 1, l2 - 1, p, &s[1]);
+    x[2] = caps_integrate_I(self, l1 - 1, l2 + 1, p, &s[2]);
+    x[3] = caps_integrate_I(self, l1 + 1, l2 + 1, p, &s[3]);
+    const double c[4] = {u1 * u2, -v1 * u2, -u1 * v2, v1 * v2};
+    return log_combination(4, c, x, s, sign);
+}
+
+// C = m [-u_{l2} I_{l1, l2-1} + v_{l2} I_{l1, l2+1}] (0 for m = 0).
+// Reference: src/integration.c:842-874.
+double caps_integrate_C(integration_t* self, int l1, int l2, polarization_t p, sign_t* sign)
+{
+    const int m = integ_tables(self).m;
+    if (m == 0) {
+        *sign = 0;
+        return -INFINITY;
+    }
+    const double u2 = (l2 + 1.0) * (l2 + m) / (2 * l2 + 1.0), v2 = l2 * (l2 - m + 1.0) / (2 * l2 + 1.0);
+    double x[2];
+    sign_t s[2];
+    x[0] = caps_integrate_I(self, l1, l2 - 1, p, &s[0]);
+    x[1] = caps_integrate_I(self, l1, l2 + 1, p, &s[1]);
+    const double c[2] = {-u2, v2};
+    const double v = log_combination(2, c, x, s, sign);
+    return *sign == 0 ? -INFINITY : log((double)m) + v;
+}
+
+// D_{l1 l2} = C_{l2 l1}.  Reference: src/integration.c:885-888.
+double caps_integrate_D(integration_t* self, int l1, int l2, polarization_t p, sign_t* sign)
+{
+    return caps_integrate_C(self, l2, l1, p, sign);
 }
 
 // Extension (test hook): copy the N x N matrix M of a caps_M_t (column-major).

@@ -187,6 +187,23 @@ double caps_kernel_M0_MM_plasma(int i, int j, void *args);
 integration_plasma_t *caps_integrate_plasma_init(caps_t *caps, double omegap, double epsrel);
 double caps_integrate_plasma(integration_plasma_t *self, int l1, int l2, int m, double *ratio1, double *ratio2);
 void caps_integrate_plasma_free(integration_plasma_t *self);
+/* The k-integrals of one block (ξ, m): caps_integrate_init computes the
+ * block's whole K_ν and I_{l1 l2} tables on the GPU (ν from 2m, or 2 for
+ * m = 0, to 2(lmin + ldim) + 1; l1, l2 over the block's ℓ-window and one
+ * more on either side); K, I, A, B, C, D return log|value| and set *sign.
+ * I is 0 (sign 0) for l1 < m or l2 < m; arguments outside the tables abort
+ * with a message.  epsrel is accepted for compatibility (the device
+ * quadrature has a fixed accuracy); NULL for xi_ <= 0, m < 0 or
+ * epsrel <= 0.  Reference: src/include/integration.h:13-22,
+ * src/integration.c:452-493, 627-888. */
+integration_t *caps_integrate_init(caps_t *self, double xi_, int m, double epsrel);
+void caps_integrate_free(integration_t *integration);
+double caps_integrate_I(integration_t *self, int l1, int l2, polarization_t p, sign_t *sign);
+double caps_integrate_K(integration_t *self, int nu, polarization_t p, sign_t *sign);
+double caps_integrate_A(integration_t *self, int l1, int l2, polarization_t p, sign_t *sign);
+double caps_integrate_B(integration_t *self, int l1, int l2, polarization_t p, sign_t *sign);
+double caps_integrate_C(integration_t *self, int l1, int l2, polarization_t p, sign_t *sign);
+double caps_integrate_D(integration_t *self, int l1, int l2, polarization_t p, sign_t *sign);
 /* ξ = 0 blocks EE, MM and the plasma-model MM (omegap in rad/s), each
  * ldim x ldim, filled and factored on the GPU; NULL outputs are skipped.
  * Reference: src/libcaps.c:984-1017. */

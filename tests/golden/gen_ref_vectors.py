@@ -52,6 +52,10 @@ def lib():
         L.caps_integrate_K.argtypes = [p, i, i, ctypes.POINTER(ctypes.c_byte)]
         L.caps_integrate_I.restype = d
         L.caps_integrate_I.argtypes = [p, i, i, i, ctypes.POINTER(ctypes.c_byte)]
+        for f in "ABCD":
+            fn = getattr(L, "caps_integrate_" + f)
+            fn.restype = d
+            fn.argtypes = [p, i, i, i, ctypes.POINTER(ctypes.c_byte)]
         L.caps_M_init.restype = p
         L.caps_M_init.argtypes = [p, i, d]
         L.caps_M_elem.restype = d
@@ -163,6 +167,15 @@ def gen_tables(cases=TABLE_CASES):
                 for p in (0, 1):
                     v = L_.caps_integrate_I(integ, l1, l2, p, ctypes.byref(s))
                     I.append([l1, l2, p, v, s.value])
+        # the combinations A, B, C, D of src/integration.c:762-888 (their
+        # I arguments l ± 1 stay inside the block's window and one more)
+        ABCD = []
+        for l1 in ls:
+            for l2 in ls:
+                for p in (0, 1):
+                    for f in "ABCD":
+                        v = getattr(L_, "caps_integrate_" + f)(integ, l1, l2, p, ctypes.byref(s))
+                        ABCD.append([l1, l2, p, f, v, s.value])
         Mobj = L_.caps_M_init(c, m, xi)
         E = []
         for l1 in ls:
@@ -173,7 +186,7 @@ def gen_tables(cases=TABLE_CASES):
         L_.caps_M_free(Mobj)
         out.append({"kind": tag, "R": R, "L": L, "ldim": ldim, "omegap": None if drude is None else drude[0],
                     "gamma": None if drude is None else drude[1], "xi_": xi, "m": m, "lmin": lmin,
-                    "K": K, "I": I, "M": E})
+                    "K": K, "I": I, "ABCD": ABCD, "M": E})
     return out
 
 

@@ -203,6 +203,79 @@ def test_K_I_M_vs_reference(ca, case):
         len(case["M"])))
 
 
+@pytest.mark.parametrize("case", _ref("ref_tables.json"),
+                         ids=lambda c: "%s_ldim%d_xi%g_m%d" % (c["kind"], c["ldim"], c["xi_"], c["m"]))
+def test_integrate_exports_vs_reference(ca, case):
+    """caps_integrate_init / K / I / A / B / C / D (src/include/integration.h)
+    read from the device tables, against the reference's own functions.  K
+    and I as in test_K_I_M_vs_reference; A..D are sums c_i I_i of I values
+    at l ± 1: each I carries up to 1e-9 of its q-sum Σ_q |a_q K| (the
+    oracle's I_abs), so |ΔV| <= 1e-9 Σ_i |c_i| I_abs_i."""
+    import ctypes
+    import pyoracle as po
+    L = ca.lib()
+    c = make(ca, case["R"], case["L"], case["ldim"], case["omegap"], case["gamma"])
+    t = c.tables(case["xi_"], case["m"])
+    lo, nl, mu, nK = t["lo"], t["logI"].shape[1], t["mu"], t["lnK"].shape[1]
+    m = case["m"]
+    integ = L.caps_integrate_init(c.ptr, case["xi_"], m, 1e-8)
+    assert integ
+    s = ctypes.c_byte()
+    oc = po.caps(case["R"], case["L"], ldim=case["ldim"], epsrel=1e-12,
+                 drude=None if case["omegap"] is None else (case["omegap"], case["gamma"]))
+    B = po.Block(oc, case["xi_"], m)
+    try:
+        nk = 0
+        for nu, p, v, sg in case["K"]:
+            if not mu <= nu < mu + nK:
+                continue
+            g = L.caps_integrate_K(integ, nu, p, ctypes.byref(s))
+            assert s.value == sg and abs(math.exp(g - v) - 1) < 1e-9, ("K", nu, p, g, v, s.value, sg)
+            nk += 1
+        for l1, l2, p, v, sg in case["I"]:
+            g = L.caps_integrate_I(integ, l1, l2, p, ctypes.byref(s))
+            cond = max(1.0, math.exp(B.I_abs(l1, l2, p) - v))
+            assert s.value == sg and abs(math.exp(g - v) - 1) < 1e-9 * cond, ("I", l1, l2, p, g, v, cond)
+
+        def iabs(l1, l2, p):
+            # I_abs / |V| (logs: the values span far beyond double range)
+            return 0.0 if l1 < m or l2 < m else math.exp(B.I_abs(l1, l2, p) - v)
+
+        def uv(l):
+            return (l + 1.0) * (l + m) / (2 * l + 1.0), l * (l - m + 1.0) / (2 * l + 1.0)
+        worst, nabcd = 0.0, 0
+        for l1, l2, p, f, v, sg in case["ABCD"]:
+            if min(l1, l2) - 1 < lo or max(l1, l2) + 1 >= lo + nl:
+                continue
+            nabcd += 1
+            g = getattr(L, "caps_integrate_" + f)(integ, l1, l2, p, ctypes.byref(s))
+            assert s.value == sg, (f, l1, l2, p, s.value, sg)
+            if sg == 0:
+                assert g == v == -math.inf
+                continue
+            (u1, v1), (u2, v2) = uv(l1), uv(l2)
+            # bound: sum of |c_i| I_abs_i relative to |V|
+            if f == "A":
+                bound = m * m * iabs(l1, l2, p)
+            elif f == "B":
+                bound = iabs(l1, l2, p) if m == 0 else (
+                    u1 * u2 * iabs(l1 - 1, l2 - 1, p) + v1 * u2 * iabs(l1 + 1, l2 - 1, p)
+                    + u1 * v2 * iabs(l1 - 1, l2 + 1, p) + v1 * v2 * iabs(l1 + 1, l2 + 1, p))
+            elif f == "C":
+                bound = m * (u2 * iabs(l1, l2 - 1, p) + v2 * iabs(l1, l2 + 1, p))
+            else:
+                bound = m * (u1 * iabs(l1 - 1, l2, p) + v1 * iabs(l1 + 1, l2, p))
+            err = abs(math.exp(g - v) - 1)
+            tol = 1e-9 * max(1.0, bound)
+            worst = max(worst, err / tol)
+            assert err < tol, (f, l1, l2, p, g, v, tol)
+    finally:
+        L.caps_integrate_free(integ)
+    print("%s m=%d: %d K, %d I, %d A..D against the reference; largest A..D error / tolerance %.3f"
+          % (case["kind"], m, nk, len(case["I"]), nabcd, worst))
+    assert nabcd >= 100
+
+
 BLOCKS = _ref("ref_blocks.json")
 
 
