@@ -3415,10 +3415,11 @@ int capsx_M_copy(caps_M_t* self, double* out)
 // blocks factored, out[8] algorithmic LU flops Σ(2/3 N³ - 1/2 N² + 5/6 N),
 // out[9] batches, out[10] (l1 >= l2, polarization) I integrals.
 // With capsx_profile on: out[11] milliseconds in the LU's trailing-update
-// kernel (k_lu_gemm, HIP events around each launch), out[12] its algorithmic
-// flops (2 rows cols depth per updated region, over factored blocks),
-// out[13] its launches, out[14] its algorithmic HBM bytes (C read and
-// written once, both operands read once).
+// kernel k_lu_gemm_dma (HIP events around each launch), out[12] its
+// algorithmic flops (2 rows cols depth per updated region, over factored
+// blocks), out[13] its launches, out[14] its algorithmic HBM bytes (C read
+// and written once, both operands read once); out[15] milliseconds of the
+// other trailing-update launches (k_lu_gemm tiles).
 void capsx_stats(caps_t* self, double out[16])
 {
     Ctx* c = ctx_of(self);
