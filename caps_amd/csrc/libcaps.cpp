@@ -36,7 +36,7 @@ struct capsx_mlp;
 extern "C" {
 void capsx_mlp_predict(capsx_mlp* h, const float* X, int n, float* Y);
 void capsd_launch_mie(const BlockDesc*, char*, const double*, int, int, hipStream_t);
-void capsd_launch_K(const BlockDesc*, char*, const double*, int, int, int, int*, hipStream_t);
+void capsd_launch_K(const BlockDesc*, char*, const double*, int, int, int, int, int*, hipStream_t);
 void capsd_launch_I(const BlockDesc*, char*, const void*, const int*, const double*, int64_t, int*, const BlockOut*,
                     int, int, hipStream_t);
 void capsd_launch_assemble(const BlockDesc*, char*, const double*, int, int, int, const BlockOut*, int, hipStream_t);
@@ -514,6 +514,7 @@ void grid_for(BlockDesc& d)
 // 2-8 parts).
 struct BatchPlan {
     int n = 0, maxN = 0, minN = 0, max_ldim = 0, max_waves = 0, max_nK = 0, max_nl = 0;
+    int sum_waves = 0;   // K quadrature waves over the batch's blocks
     int64_t max_ntri = 0;
     size_t bytes = 0;
     std::vector<IGroup> groups;   // per part, ordered by class (see build_igroups)
@@ -644,6 +645,7 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
         p.minN = b == 0 ? N : std::min(p.minN, N);
         p.max_ldim = std::max(p.max_ldim, ldim);
         p.max_waves = std::max(p.max_waves, d.n_waves);
+        p.sum_waves += d.n_waves;
         p.max_nK = std::max(p.max_nK, d.nK);
         p.max_ntri = std::max(p.max_ntri, (int64_t)d.nl * (d.nl + 1) / 2);
         p.max_nl = std::max(p.max_nl, d.nl);
@@ -777,6 +779,18 @@ LuCfg batch_lu_config(const Ctx* c, const BatchPlan& p, bool prof)
     return cfg;
 }
 
+// Segments of the K quadrature's ν loop per wave: a batch of few waves
+// (fewer than two per SIMD: one block, or a few large ones) splits each
+// wave's ν range over up to 8 workgroups, whose steps are otherwise one
+// latency-bound chain per wave (CAPS_AMD_K_SEGMENTS fixes the number).
+int k_segments(const BatchPlan& p)
+{
+    if (getenv("CAPS_AMD_K_SEGMENTS")) return std::max(1, std::min(8, (int)env_double("CAPS_AMD_K_SEGMENTS", 1)));
+    const int target = 2048;
+    if (p.sum_waves >= target || p.sum_waves <= 0) return 1;
+    return std::min(8, (target + p.sum_waves - 1) / p.sum_waves);
+}
+
 // Run the pipeline for n blocks already planned; mode 0: LU path, 1: M only.
 // The kernels of one planned batch on the library's stream (and the LU
 // streams); stage events between the stages when `events`.  Returns the
@@ -788,7 +802,7 @@ int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, b
     if (events) HIPCHECK(hipEventRecord(c->ev[0], s));
     capsd_launch_mie(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, s);
     if (events) HIPCHECK(hipEventRecord(c->ev[1], s));
-    capsd_launch_K(c->d_desc, c->arena, g_lfac_dev, n, p.max_waves, p.max_nK, c->d_status, s);
+    capsd_launch_K(c->d_desc, c->arena, g_lfac_dev, n, p.max_waves, p.max_nK, k_segments(p), c->d_status, s);
     if (events) HIPCHECK(hipEventRecord(c->ev[2], s));
     int nlaunched = 0;
     // part g on stream g (part 0 on the library's stream); its I waits for
@@ -847,7 +861,7 @@ std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_sh
 {
     const LuCfg cfg = batch_lu_config(c, p, c->prof && p.parts == 1);
     struct {
-        int n, maxN, max_ldim, max_waves, max_nK, mode, shortcuts, lu_kinds, delta_mode;
+        int n, maxN, max_ldim, max_waves, max_nK, kseg, mode, shortcuts, lu_kinds, delta_mode;
         int64_t max_ntri;
         int parts, part_b[MAX_PARTS + 1], part_goff[MAX_PARTS], cls[MAX_PARTS][8];
         int levels, streams, xcd, tile;
@@ -860,6 +874,7 @@ std::vector<char> graph_key(Ctx* c, const BatchPlan& p, int mode, int lu_mode_sh
     k.maxN = p.maxN;
     k.max_ldim = p.max_ldim;
     k.max_waves = p.max_waves;
+    k.kseg = k_segments(p);
     k.max_nK = p.max_nK;
     k.mode = mode;
     k.shortcuts = lu_mode_shortcuts;

@@ -127,11 +127,16 @@ __global__ void k_bessel_points(int n, const int* __restrict__ order, const doub
 // each and combine within their quad (no wave-wide reduction per ν); one
 // KPart per (ν, wave) goes to HBM and k_K_combine adds the waves.
 
+__device__ __forceinline__ double plm_step(double c1, double c2, double x, double p, double p_prev)
+{
+    return c1 * x * p - c2 * p_prev;
+}
+
 __global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
-                                                  const double* __restrict__ lfac_tab)
+                                                  const double* __restrict__ lfac_tab, int nseg)
 {
     const BlockDesc& d = desc[blockIdx.y];
-    const int wave = blockIdx.x;
+    const int wave = blockIdx.x / nseg, seg = blockIdx.x % nseg;
     if (wave >= d.n_waves) return;  // uniform over the workgroup
     const int lane = threadIdx.x;
     const int k = wave * 64 + lane;
@@ -177,7 +182,34 @@ __global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ 
     const int rj = lane >> 2, rq = lane & 3;   // reduction role: ν j of the chunk, node quarter q
     double p_prev = 0, p = 1;
     constexpr int NONE = -1000000;
-    for (int base = mu; base <= d.nu_max; base += 64) {
+    // this workgroup's ν range: segment seg of nseg of the 64-ν chunks (a
+    // short grid splits the ν loop, whose steps are latency-bound, over
+    // more waves); the recurrence is run alone up to the segment's start,
+    // the same steps as the full loop, so the terms are the same bits
+    const int nch = (d.nu_max - mu + 64) / 64;
+    const int nu_beg = mu + 64 * (int)((int64_t)nch * seg / nseg);
+    const int nu_end = min(d.nu_max, mu + 64 * (int)((int64_t)nch * (seg + 1) / nseg) - 1);
+    for (int base = mu; base < nu_beg; base += 64) {
+        {
+            const int nu = base + lane;
+            const double ll = (double)(nu + 1 - mu);
+            c1s[lane] = (2.0 * nu + 1) / ll;
+            c2s[lane] = (double)(nu + mu) / ll;
+        }
+        __syncthreads();
+        for (int jj = 0; jj < 64; jj++) {
+            const double pn = plm_step(c1s[jj], c2s[jj], x, p, p_prev);
+            p_prev = p;
+            p = pn;
+            if (p > 0x1p600) {
+                p *= 0x1p-600;
+                p_prev *= 0x1p-600;
+                E += 600;
+            }
+        }
+        __syncthreads();
+    }
+    for (int base = nu_beg; base <= nu_end; base += 64) {
         // recurrence coefficients for the next 64 steps, one division per lane
         {
             const int nu = base + lane;
@@ -186,7 +218,7 @@ __global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ 
             c2s[lane] = (double)(nu + mu) / ll;
         }
         __syncthreads();
-        const int nend = min(base + 64, d.nu_max + 1);
+        const int nend = min(base + 64, nu_end + 1);
         for (int c0 = base; c0 < nend; c0 += CH) {
             // this node's terms of ν = c0 .. c0 + CH - 1; the recurrence
             // gives p at ν+1, kept below 2^600 by moving powers of two into E
@@ -197,7 +229,7 @@ __global__ void __launch_bounds__(64) k_K_partial(const BlockDesc* __restrict__ 
                 te_[j * TP + lane] = e;
                 const int jj = c0 - base + j;
                 if (jj < 64) {
-                    const double pn = c1s[jj] * x * p - c2s[jj] * p_prev;
+                    const double pn = plm_step(c1s[jj], c2s[jj], x, p, p_prev);
                     p_prev = p;
                     p = pn;
                     if (p > 0x1p600) {
@@ -999,9 +1031,9 @@ void capsd_launch_mie(const BlockDesc* desc, char* arena, const double* lfac_tab
 }
 
 void capsd_launch_K(const BlockDesc* desc, char* arena, const double* lfac_tab, int nblocks, int max_waves,
-                    int max_nK, int* status, hipStream_t s)
+                    int max_nK, int nseg, int* status, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_K_partial, dim3(max_waves, nblocks), dim3(64), 0, s, desc, arena, lfac_tab);
+    hipLaunchKernelGGL(k_K_partial, dim3(max_waves * nseg, nblocks), dim3(64), 0, s, desc, arena, lfac_tab, nseg);
     hipLaunchKernelGGL(k_K_combine, dim3((max_nK + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, status);
 }
 

@@ -862,3 +862,23 @@ def test_profiled_update_statistics(ca):
     print("profiled updates: %d k_lu_gemm_dma launches, %.2f TF/s, other updates %.3f ms"
           % (st["gemm_launches"], tflops, st["other_update_ms"]))
     assert 0 < tflops < 78.6
+
+
+def test_K_segments_bitwise(ca):
+    """The K quadrature's ν loop split over several workgroups per wave (short
+    grids) runs the same recurrence steps: the K and I tables are the same
+    bits for 1, 3 and the automatic number of segments."""
+    c = make(ca, 200e-6, 1e-6, 1200, 9.0 / HBAR_EV, 0.035 / HBAR_EV)
+    out = []
+    try:
+        for seg in ("1", "3", None):
+            if seg is None:
+                os.environ.pop("CAPS_AMD_K_SEGMENTS", None)
+            else:
+                os.environ["CAPS_AMD_K_SEGMENTS"] = seg
+            out.append(c.tables(165.4563560875888, 125))
+    finally:
+        os.environ.pop("CAPS_AMD_K_SEGMENTS", None)
+    for t in out[1:]:
+        assert np.array_equal(t["lnK"], out[0]["lnK"])
+        assert np.array_equal(t["logI"], out[0]["logI"]) and np.array_equal(t["sgnI"], out[0]["sgnI"])
