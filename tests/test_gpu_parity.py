@@ -566,6 +566,19 @@ def test_lu_kernel_variants_bitwise(ca):
                     assert L.capsx_lu_kernels(gk, ps, nt, None) == 0
                     out = ca.lu_logdet_batch(mats)
                     assert np.array_equal(out, ref), (N, levels, gk, ps, nt, np.abs(out - ref).max())
+        # the delta form (column dominant 1 - M: C tiles preloaded into the
+        # accumulators), short grids: split slices, narrow and wide tiles
+        for N, nmat in ((517, 3), (2700, 2)):
+            Ms = np.array([0.3 * rng.normal(size=(N, N)) / N for _ in range(nmat)])
+            for levels in (1, 2, 3, 4):
+                at.set_config(levels, 1, 0, 128)
+                L.capsx_lu_kernels(*default, None)
+                ref, fl = ca.logdet_1mM_batch(Ms)
+                assert fl.all()
+                for gk, nt in itertools.product((0, 1, 2), (0, 1)):
+                    assert L.capsx_lu_kernels(gk, -1, nt, None) == 0
+                    out, _ = ca.logdet_1mM_batch(Ms)
+                    assert np.array_equal(out, ref), ("delta", N, levels, gk, nt, np.abs(out - ref).max())
     finally:
         L.capsx_lu_kernels(*default, None)
         at.set_config(0)
