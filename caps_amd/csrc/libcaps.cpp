@@ -524,7 +524,9 @@ struct BatchPlan {
     int part_cls[MAX_PARTS][8] = {};    // I groups per class of each part
     // determinant forms the LU launches kernels for (capsd_launch_lu kinds):
     // 2 = delta form only (round-trip blocks; one that turns out not column
-    // dominant is left unfactored and run again with 1 = pivoted)
+    // dominant is left unfactored and run again with 1 = pivoted), 6 = the
+    // same in the symmetric form (1 - M of a round trip is symmetric: the
+    // reference factors it as such, sym_spd in src/libcaps.c:954)
     int lu_kinds = 2;
 };
 
@@ -578,6 +580,8 @@ BatchPlan plan_batch_multi(caps_t* const* objs, Ctx* c, const double* xi_, const
 {
     BatchPlan p;
     p.n = n;
+    // CAPS_AMD_LU_SYM=0: the general (non-symmetric) delta form
+    p.lu_kinds = env_double("CAPS_AMD_LU_SYM", 1) != 0 ? 6 : 2;
     const int nb = capsd_lu_nb();
     int64_t off = 0;
     double last_xi = NAN, ep = 0, es = 0;
@@ -979,17 +983,29 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
             }
             c->acc[11] += ms;
             c->acc[13] += 1;
+            const bool sym = p.lu_kinds & 4;
             for (int b = 0; b < n; b++) {
                 if (c->h_out[b].skip) continue;
                 const double N = c->h_desc[b].N;
                 const double r0 = g[0] + g[1], c0 = g[0] + g[3];
-                // region rows x cols, depth kk: flops 2 rows cols kk; HBM
-                // bytes: C read and written once, L and U read once
+                // region rows x cols, depth kk: flops 2 x elements x kk; HBM
+                // bytes: C read and written once, L and U read once.  The
+                // symmetric form needs the elements on and below the
+                // diagonal only (row >= column)
                 const double rows = std::max(0.0, std::min(N, r0 + g[2]) - r0);
                 const double cols = std::max(0.0, std::min(N, c0 + g[4]) - c0);
                 const double kk = rows > 0 && cols > 0 ? g[5] : 0;
-                c->acc[12] += 2.0 * rows * cols * kk;
-                c->acc[14] += 16.0 * rows * cols + 8.0 * (rows + cols) * kk;
+                double elems = rows * cols;
+                if (sym && rows > 0 && cols > 0) {
+                    // columns j in [c0, c0 + cols): rows max(r0, j) .. e - 1
+                    const double e = r0 + rows, c1 = c0 + cols - 1;
+                    const double ja = std::min(c1, r0);   // j <= r0: all rows
+                    elems = ja >= c0 ? (ja - c0 + 1) * rows : 0;
+                    const double jb0 = std::max(c0, r0 + 1), jb1 = std::min(c1, e - 1);
+                    if (jb1 >= jb0) elems += (jb1 - jb0 + 1) * (e - (jb0 + jb1) / 2);
+                }
+                c->acc[12] += 2.0 * elems * kk;
+                c->acc[14] += 16.0 * elems + 8.0 * (rows + cols) * kk;
             }
         }
     }
@@ -1001,7 +1017,12 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
         if (mode == 0 && c->h_out[b].skip == 0) {
             const double N = d.N;
             c->acc[7] += 1;
-            c->acc[8] += 2.0 / 3 * N * N * N - 0.5 * N * N + 5.0 / 6 * N;
+            // LU: 2/3 N^3 - 1/2 N^2 + 5/6 N flops; the symmetric (LDL^T)
+            // form of a dominant block: 1/3 N^3 + 1/2 N^2 - 5/6 N
+            if ((p.lu_kinds & 4) && c->h_out[b].delta)
+                c->acc[8] += 1.0 / 3 * N * N * N + 0.5 * N * N - 5.0 / 6 * N;
+            else
+                c->acc[8] += 2.0 / 3 * N * N * N - 0.5 * N * N + 5.0 / 6 * N;
         }
     }
 }

@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(256, 2) k_lu_diag(const BlockDesc* __restrict_
 template <int CT>
 __global__ void __launch_bounds__(256) k_lu_l21(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
-                                               int k, int pb)
+                                               int k, int pb, int sym)
 {
     static_assert(CT == 1 || CT == 4, "column tiles per wavefront");
     constexpr int WPR = 4 / CT;   // wavefronts per 64-row group
@@ -631,6 +631,24 @@ __global__ void __launch_bounds__(256) k_lu_l21(const BlockDesc* __restrict__ de
                 for (int q = 0; q < 4; q++)
                     P[(row - k0) + (int64_t)(16 * (j0 + j) + lk + 4 * q) * lda] = acc[i][j][q];
         }
+    }
+    if (sym) {
+        // symmetric form: U12 = D11 L21^T, u_cc = 1 + the factored block's
+        // stored diagonal; U row k0 + c goes to the U buffer at
+        // (pb NB + c) lda, its column `row` (16 consecutive rows per store)
+        double* Ut = at<double>(arena, side[b].off_ut) + (int64_t)pb * NB * lda;
+#pragma unroll
+        for (int j = 0; j < CT; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int c = 16 * (j0 + j) + lk + 4 * q;
+                const double u = 1.0 + P[c + (int64_t)c * lda];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int row = m0 + 16 * i + li;
+                    if (row < N) Ut[(int64_t)c * lda + row] = u * acc[i][j][q];
+                }
+            }
     }
 }
 
@@ -992,7 +1010,7 @@ template <int WM, int WN>
 __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
           const BlockOut* __restrict__ out, int k, int r_off, int r_lim, int c_off, int c_lim, int kk0, int KK,
-          int gx, int gy, int nblocks, int xcd_order)
+          int gx, int gy, int nblocks, int xcd_order, int lower)
 {
     constexpr int NT = 64 * WM * WN, GTM = 64 * WM, GTN = 64 * WN;
     constexpr int KS = NT >= 256 ? LU_KS : 8;   // K per LDS stage (prefetch registers)
@@ -1026,6 +1044,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     const int m0 = r0 + (tile % gx) * GTM;   // rows of the update
     const int n0 = c0 + (tile / gx) * GTN;   // columns of the update
     if (m0 >= rend || n0 >= cend) return;
+    if (lower && m0 + GTM <= n0) return;   // symmetric form: a tile wholly above the diagonal
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
     const double* Pb = panel_buf(arena, side[b], lda, 0);
@@ -1141,7 +1160,7 @@ template <int KS, int OCC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
               const BlockOut* __restrict__ out, int k, int r_off, int r_lim, int c_off, int c_lim, int kk0, int KK,
-              int gx, int gy, int nblocks, int xcd_order)
+              int gx, int gy, int nblocks, int xcd_order, int lower)
 {
     constexpr int GT = 128, PITCH = GT + 16;
     constexpr int KW = KS / 4;   // kk rows of a stage moved by each wavefront
@@ -1164,6 +1183,7 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
     const int m0 = r0 + (tile % gx) * GT;
     const int n0 = c0 + (tile / gx) * GT;
     if (m0 >= rend || n0 >= cend) return;
+    if (lower && m0 + GT <= n0) return;   // symmetric form: a tile wholly above the diagonal
     const int64_t lda = d.lda;
     double* A = at<double>(arena, d.off_mat);
     const double* Pb = panel_buf(arena, side[b], lda, 0);
@@ -1478,12 +1498,19 @@ int capsd_lu_npanels(void) { return NPANEL; }
 // gemm_info receives (k, r_off, r_lim, c_off, c_lim, KK, dma) of each update
 // launch (dma = 1 for k_lu_gemm_dma, 0 for the k_lu_gemm tiles).  kinds: bit 0 set if the batch may hold blocks for the pivoted LU
 // (out[b].delta = 0), bit 1 if it may hold delta-form blocks; kernels for a
-// kind the caller rules out are not launched.  Returns the number of update
-// launches.
+// kind the caller rules out are not launched; bit 2 (with bit 1 alone): the
+// delta-form blocks are symmetric (1 - M of a round trip): U12 = D11 L21^T
+// comes from k_lu_l21, no TRSM or update of U rows runs, and the trailing
+// updates skip the tiles wholly above the diagonal (the diagonal blocks are
+// kept whole for k_lu_diag).  Returns the number of update launches.
 int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockOut* out, int nblocks, int maxN,
                     hipStream_t s, int levels, int xcd_order, int tile, hipEvent_t* gemm_ev, int* gemm_info, int kinds)
 {
     const bool piv = kinds & 1, dlt = kinds & 2;
+    // symmetric delta form (bit 2, delta-form blocks only): U12 = D11 L21^T
+    // is written by k_lu_l21, so no TRSM and no update of U rows runs, and
+    // the trailing updates keep the lower triangle
+    const int sym = (kinds & 4) && dlt && !piv ? 1 : 0;
     const LuSide* sd = (const LuSide*)side;
     const int nb8 = xcd_order ? (nblocks + 7) & ~7 : nblocks;
     constexpr int ALL = 1 << 30;   // region limit: up to N
@@ -1493,39 +1520,40 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         const int rows = min(maxN, k + r_off + min(r_lim, maxN)) - (k + r_off);
         const int cols = min(maxN, k + c_off + min(c_lim, maxN)) - (k + c_off);
         if (rows <= 0 || cols <= 0) return;
+        if (sym && r_off + rows <= c_off) return;   // wholly above the diagonal: U rows
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p], s);
         int dma = 0;   // 1: this launch is k_lu_gemm_dma
         if (tile == 64) {
             const int gx = (rows + 63) / 64, gy = (cols + 63) / 64;
             hipLaunchKernelGGL((k_lu_gemm<1, 1>), dim3(gx * gy * nb8), dim3(64), 0, s, desc, arena, sd, out, k, r_off,
-                               r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+                               r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym);
         } else if (tile == 256) {
             const int gx = (rows + 255) / 256, gy = (cols + 127) / 128;
             hipLaunchKernelGGL((k_lu_gemm<4, 2>), dim3(gx * gy * nb8), dim3(512), 0, s, desc, arena, sd, out, k,
-                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym);
         } else if (narrow_tiles && cols <= 64 && rows > 64) {
             // regions one panel wide: 128 x 64 tiles (a 128 x 128 tile would
             // compute half of its product for nothing)
             const int gx = (rows + 127) / 128, gy = (cols + 63) / 64;
             hipLaunchKernelGGL((k_lu_gemm<2, 1>), dim3(gx * gy * nb8), dim3(128), 0, s, desc, arena, sd, out, k,
-                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym);
         } else if (narrow_tiles && rows <= 64 && cols > 64) {
             const int gx = (rows + 63) / 64, gy = (cols + 127) / 128;
             hipLaunchKernelGGL((k_lu_gemm<1, 2>), dim3(gx * gy * nb8), dim3(128), 0, s, desc, arena, sd, out, k,
-                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym);
         } else if (gemm_kernel == 0) {
             const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
             hipLaunchKernelGGL((k_lu_gemm<2, 2>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k,
-                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+                               r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym);
         } else {
             const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
             dma = 1;
             if (gemm_kernel == 2)
                 hipLaunchKernelGGL((k_lu_gemm_dma<8, 3>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out,
-                                   k, r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+                                   k, r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym);
             else
                 hipLaunchKernelGGL((k_lu_gemm_dma<16, 2>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out,
-                                   k, r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order);
+                                   k, r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym);
         }
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
         if (gemm_info) {
@@ -1548,6 +1576,7 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         else if (piv)
             hipLaunchKernelGGL(k_lu_swap_trsm<64>, dim3((n + 63) / 64, nblocks), dim3(64), 0, s, desc, arena, sd, out,
                                k, pb, cs_off, cols, swaps);
+        if (sym) return;   // the U rows come from k_lu_l21
         if (dlt && trsm_cols == 128)
             hipLaunchKernelGGL(k_lu_trsm_mfma<128>, dim3((n + 127) / 128, nblocks), dim3(256), 0, s, desc, arena, sd,
                                out, k, pb, cs_off, cols);
@@ -1569,10 +1598,10 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         if (below > 0) {
             if ((int64_t)((below + 255) / 256) * nblocks >= 512)
                 hipLaunchKernelGGL(k_lu_l21<4>, dim3((below + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, sd,
-                                   out, k, pb);
+                                   out, k, pb, sym);
             else
                 hipLaunchKernelGGL(k_lu_l21<1>, dim3((below + 63) / 64, nblocks), dim3(256), 0, s, desc, arena, sd,
-                                   out, k, pb);
+                                   out, k, pb, sym);
         }
     };
     // buffers [qbegin, qend) take the pivots of the later panels pfirst..plast

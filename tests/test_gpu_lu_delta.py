@@ -152,3 +152,32 @@ def test_pivoted_fallback_on_round_trip_blocks(ca):
                 assert abs(a - b) <= 1e-12 * abs(a) + n * 2.2e-16, (a, b, n)
             else:
                 assert a == b
+
+
+def test_symmetric_form_matches_general(ca):
+    """Round-trip blocks (1 - M symmetric) factored in the symmetric delta
+    form (U12 = D11 L21^T, lower-triangle updates; the default) and in the
+    general delta form (CAPS_AMD_LU_SYM=0) agree to the rounding of the
+    factorization (N eps relative; the two forms round the symmetric pairs
+    differently), for the R/L = 20 blocks and for a batch of blocks of
+    unequal dimension."""
+    import os
+    from caps_amd.workloads import mixed_aspect_blocks, make_objects
+    c = ca.Caps(20e-6, 1e-6, ldim=120)
+    ms = np.arange(121)
+    blocks = mixed_aspect_blocks()[:24]
+    objs = make_objects(blocks)
+    xs, mm = [b["xi_"] for b in blocks], [b["m"] for b in blocks]
+    res = {}
+    try:
+        for sym in ("1", "0"):
+            os.environ["CAPS_AMD_LU_SYM"] = sym
+            res[sym] = (c.logdetD_batch(np.ones(121), ms), ca.logdetD_multi(objs, xs, mm))
+    finally:
+        os.environ.pop("CAPS_AMD_LU_SYM", None)
+    worst = 0.0
+    for a, b, N in ((res["1"][0], res["0"][0], 240), (res["1"][1], res["0"][1], 2 * max(b_["ldim"] for b_ in blocks))):
+        rel = np.abs(a - b) / np.abs(b)
+        worst = max(worst, rel.max())
+        assert rel.max() < N * 2.3e-16 * 8, (rel.max(), N)
+    print("symmetric vs general delta form: largest relative difference %.2e" % worst)
