@@ -561,7 +561,7 @@ __global__ void __launch_bounds__(256, 2) k_lu_diag(const BlockDesc* __restrict_
 // whole 16-row column segments, goes to the panel buffer (rows relative to
 // the panel's first row), the trailing updates' L operand.
 template <int CT>
-__global__ void __launch_bounds__(256) k_lu_l21(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
+__global__ void __launch_bounds__(256, 2) k_lu_l21(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
                                                int k, int pb, int sym)
 {
@@ -580,6 +580,19 @@ __global__ void __launch_bounds__(256) k_lu_l21(const BlockDesc* __restrict__ de
     double* __restrict__ P = panel_buf(arena, side[b], lda, pb);
     const double* __restrict__ Ui = inv_buf(arena, side[b], pb) + NB * NB;
     const int li = lane & 15, lk = lane >> 4;
+    // symmetric form: the pivots u_cc = 1 + the factored block's stored
+    // diagonal of this lane's U12 rows, loaded before any store (a load
+    // after a store to the arena would wait for it)
+    double u[CT][4];
+    if (sym) {
+#pragma unroll
+        for (int j = 0; j < CT; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int c = 16 * (j0 + j) + lk + 4 * q;
+                u[j][q] = 1.0 + P[c + (int64_t)c * lda];
+            }
+    }
     double4_t acc[4][CT];
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -642,11 +655,10 @@ __global__ void __launch_bounds__(256) k_lu_l21(const BlockDesc* __restrict__ de
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int c = 16 * (j0 + j) + lk + 4 * q;
-                const double u = 1.0 + P[c + (int64_t)c * lda];
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const int row = m0 + 16 * i + li;
-                    if (row < N) Ut[(int64_t)c * lda + row] = u * acc[i][j][q];
+                    if (row < N) Ut[(int64_t)c * lda + row] = u[j][q] * acc[i][j][q];
                 }
             }
     }
