@@ -831,8 +831,11 @@ int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, b
                        0, mode == 0, sg);
         if (!last) HIPCHECK(hipEventRecord(c->part_i_done[g], sg));
         if (events && last) HIPCHECK(hipEventRecord(c->ev[3], sg));
+        // the symmetric delta-form LU reads nothing above the diagonal outside
+        // its 64 x 64 diagonal blocks: those mirror tiles are not assembled
+        const bool sym_only = mode == 0 && !lu_mode_shortcuts && (p.lu_kinds & 4) && !(p.lu_kinds & 1);
         capsd_launch_assemble(c->d_desc + b0, c->arena, g_lfac_dev, nb, p.max_ldim, mode, c->d_out + b0,
-                              mode == 0 ? 2 : 0, sg);
+                              (mode == 0 ? 2 : 0) | (sym_only ? 4 : 0), sg);
         if (mode == 0) capsd_launch_dominance(c->d_desc + b0, c->arena, c->d_out + b0, nb, p.maxN, g_delta_mode, sg);
         if (mode == 0 && lu_mode_shortcuts)
             capsd_launch_frobenius(c->d_desc + b0, c->arena, c->d_out + b0, nb, lu_mode_shortcuts == 2, sg);

@@ -662,7 +662,9 @@ __device__ __forceinline__ UV uv(int l, int m)
 
 // flags bit 0: the trace pass: only the diagonal tiles and only the
 // diagonal of M (from I pairs with |l1 - l2| <= 2); bit 1: leave out the
-// blocks with out[b].skip set.
+// blocks with out[b].skip set; bit 2: the symmetric delta-form LU follows,
+// so a mirror tile outside the 64 x 64 diagonal blocks (tile pairs 2k, 2k+1)
+// is not written, only its column partial sums (from the lower tile's rows).
 // Full pass: workgroup blockIdx.x takes the tile pair (bx, by), bx >= by
 // (linear index over the lower triangle of 16 x 16 tiles of (l1, l2)): the
 // tile at rows of bx and columns of by and its mirror at rows of by and
@@ -703,6 +705,7 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     __shared__ double wm[2][18][19];
     __shared__ int we[2][18][19];
     __shared__ double csum[16][2][16];   // [column pair][parity][row pair]: |M| of the lane's 2 rows
+    __shared__ double rsum[16][2][16];   // [row pair][parity][column pair]: |M| of the lane's 2 columns
     {
         const double* Im = at<double>(arena, d.off_i);
         const int* Ie = at<int>(arena, d.off_i + 16 * (int64_t)d.nl * d.nl);
@@ -722,6 +725,10 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
     __syncthreads();
     const int c0 = threadIdx.x & 15, r0 = threadIdx.x >> 4;
     const bool partials = mode == 0 && !diag_only && d.n_rtiles > 0;
+    // flags bit 2: its columns' partial sums are the lower tile's row sums
+    // (|M_ij| = |M_ji|)
+    const bool skip_mirror = (flags & 4) && !diag_only && bx != by && (bx >> 1) != (by >> 1);
+    const bool rows_too = partials && skip_mirror;
     const double* ma = at<double>(arena, d.off_mie) + 2 * d.ldim;   // mantissas of e^{ea}, e^{eb}
     const double* mb = ma + d.ldim;
     const int* xa = reinterpret_cast<const int*>(ma + 2 * d.ldim);  // their exponents
@@ -796,6 +803,11 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
             // column c+1: EM (row r) and MM (row r+1, the diagonal when i2 = j2)
             csum[cp][0][rp] = (i2 == j2 ? 0.0 : fabs(EE)) + fabs(ME);
             csum[cp][1][rp] = fabs(EM) + (i2 == j2 ? 0.0 : fabs(MM));
+            if (rows_too) {
+                // row r: EE, EM; row r+1: ME, MM (no diagonal in such a tile)
+                rsum[rp][0][cp] = fabs(EE) + fabs(EM);
+                rsum[rp][1][cp] = fabs(ME) + fabs(MM);
+            }
         }
     };
     // the 32 columns of a tile at column tile ct, row tile rt: the sum over
@@ -822,6 +834,10 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
         else if (partials) {
             csum[r0][0][c0] = 0.0;
             csum[r0][1][c0] = 0.0;
+            if (rows_too) {
+                rsum[c0][0][r0] = 0.0;
+                rsum[c0][1][r0] = 0.0;
+            }
         }
         if (diag_only) return;   // no barrier follows in the trace pass
         if (partials) {
@@ -830,6 +846,21 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
         }
     }
     if (bx == by) return;   // uniform: a diagonal tile is its own mirror
+    if (skip_mirror) {
+        // the mirror's 32 columns (rows of bx) over the 32 rows of by: the
+        // lower tile's row sums in column order
+        if (rows_too && threadIdx.x >= 32 && threadIdx.x < 64) {
+            const int ip = (threadIdx.x - 32) >> 1, par = threadIdx.x & 1;
+            const int col = 2 * (bx * 16 + ip) + par;
+            if (col < d.N) {
+                double sum = 0;
+#pragma unroll
+                for (int q = 0; q < 16; q++) sum += rsum[ip][par][q];
+                at<double>(arena, d.off_cpart)[(int64_t)by * d.N + col] = sum;
+            }
+        }
+        return;
+    }
     // the mirror (rows of by, columns of bx): lane (c0, r0) takes
     // i2 = 16 by + c0, j2 = 16 bx + r0, whose I(l1 + a, l2 + b) is the
     // window's I(l2 + b, l1 + a)

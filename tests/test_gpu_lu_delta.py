@@ -160,7 +160,10 @@ def test_symmetric_form_matches_general(ca):
     general delta form (CAPS_AMD_LU_SYM=0) agree to the rounding of the
     factorization (N eps relative; the two forms round the symmetric pairs
     differently), for the R/L = 20 blocks and for a batch of blocks of
-    unequal dimension."""
+    unequal dimension.  The symmetric form leaves the mirror tiles outside
+    the 64 x 64 diagonal blocks unassembled and takes their column sums for
+    the dominance test from the lower tiles' row sums: the blocks' flags
+    (which blocks took the delta form) agree too."""
     import os
     from caps_amd.workloads import mixed_aspect_blocks, make_objects
     c = ca.Caps(20e-6, 1e-6, ldim=120)
@@ -172,9 +175,11 @@ def test_symmetric_form_matches_general(ca):
     try:
         for sym in ("1", "0"):
             os.environ["CAPS_AMD_LU_SYM"] = sym
-            res[sym] = (c.logdetD_batch(np.ones(121), ms), ca.logdetD_multi(objs, xs, mm))
+            v, _, flags = ca.logdetD_multi(objs, xs, mm, info=True)
+            res[sym] = (c.logdetD_batch(np.ones(121), ms), v, flags)
     finally:
         os.environ.pop("CAPS_AMD_LU_SYM", None)
+    assert np.array_equal(res["1"][2], res["0"][2]), (res["1"][2], res["0"][2])
     worst = 0.0
     for a, b, N in ((res["1"][0], res["0"][0], 240), (res["1"][1], res["0"][1], 2 * max(b_["ldim"] for b_ in blocks))):
         rel = np.abs(a - b) / np.abs(b)
