@@ -463,6 +463,10 @@ __global__ void __launch_bounds__(256, 2) k_lu_diag(const BlockDesc* __restrict_
     if (wv == 0 && lane < nbk) {
         at<double>(arena, side[b].off_logs)[k0 + lane] = log_abs_1p(W[lane * WP + lane]);
         moved_buf(arena, side[b], pb)[lane] = lane;
+        // the pivots u = 1 + d for the symmetric form's updates, contiguous at
+        // columns 0 .. 63 of the panel's first U-buffer row (never a U12
+        // entry: those lie at columns k0 + NB and beyond)
+        at<double>(arena, side[b].off_ut)[(int64_t)pb * NB * lda + lane] = 1.0 + W[lane * WP + lane];
     }
     // diagonal blocks of the inverses: wavefront 0 L^{-1}, wavefront 1 U^{-1};
     // lane = 16 * block + column
@@ -559,11 +563,12 @@ __global__ void __launch_bounds__(256, 2) k_lu_diag(const BlockDesc* __restrict_
 // the matrix (16-row column segments), U11^{-1} (row-major, L2-resident) as
 // the B operand; the product, with swapped operands so each register holds
 // whole 16-row column segments, goes to the panel buffer (rows relative to
-// the panel's first row), the trailing updates' L operand.
+// the panel's first row), the trailing updates' L operand (and, in the
+// symmetric form, their U operand: U12 = D11 L21^T).
 template <int CT>
 __global__ void __launch_bounds__(256, 2) k_lu_l21(const BlockDesc* __restrict__ desc, char* __restrict__ arena,
                                                const LuSide* __restrict__ side, const BlockOut* __restrict__ out,
-                                               int k, int pb, int sym)
+                                               int k, int pb)
 {
     static_assert(CT == 1 || CT == 4, "column tiles per wavefront");
     constexpr int WPR = 4 / CT;   // wavefronts per 64-row group
@@ -580,19 +585,6 @@ __global__ void __launch_bounds__(256, 2) k_lu_l21(const BlockDesc* __restrict__
     double* __restrict__ P = panel_buf(arena, side[b], lda, pb);
     const double* __restrict__ Ui = inv_buf(arena, side[b], pb) + NB * NB;
     const int li = lane & 15, lk = lane >> 4;
-    // symmetric form: the pivots u_cc = 1 + the factored block's stored
-    // diagonal of this lane's U12 rows, loaded before any store (a load
-    // after a store to the arena would wait for it)
-    double u[CT][4];
-    if (sym) {
-#pragma unroll
-        for (int j = 0; j < CT; j++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int c = 16 * (j0 + j) + lk + 4 * q;
-                u[j][q] = 1.0 + P[c + (int64_t)c * lda];
-            }
-    }
     double4_t acc[4][CT];
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -644,23 +636,6 @@ __global__ void __launch_bounds__(256, 2) k_lu_l21(const BlockDesc* __restrict__
                 for (int q = 0; q < 4; q++)
                     P[(row - k0) + (int64_t)(16 * (j0 + j) + lk + 4 * q) * lda] = acc[i][j][q];
         }
-    }
-    if (sym) {
-        // symmetric form: U12 = D11 L21^T, u_cc = 1 + the factored block's
-        // stored diagonal; U row k0 + c goes to the U buffer at
-        // (pb NB + c) lda, its column `row` (16 consecutive rows per store)
-        double* Ut = at<double>(arena, side[b].off_ut) + (int64_t)pb * NB * lda;
-#pragma unroll
-        for (int j = 0; j < CT; j++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int c = 16 * (j0 + j) + lk + 4 * q;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int row = m0 + 16 * i + li;
-                    if (row < N) Ut[(int64_t)c * lda + row] = u[j][q] * acc[i][j][q];
-                }
-            }
     }
 }
 
@@ -1018,6 +993,18 @@ __device__ __forceinline__ void gemm_store_tile(double* __restrict__ A, int64_t 
     }
 }
 
+// The pivots u_j = 1 + d_jj of the step's columns k + kk0 .. k + kk0 + KK
+// into LDS (k_lu_diag keeps panel q's 64 at the start of U-buffer row
+// q NB); the first barrier after it publishes them.
+__device__ __forceinline__ void load_pivots(double* su, const double* Ut, int64_t lda, int kk0, int KK, int tid,
+                                            int nthreads)
+{
+    for (int t = tid; t < KK; t += nthreads) {
+        const int j = kk0 + t;
+        su[t] = Ut[(int64_t)(j / NB) * NB * lda + j % NB];
+    }
+}
+
 template <int WM, int WN>
 __global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
@@ -1064,6 +1051,10 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
     // one LDS buffer: the A and B stages
     constexpr int SMEM = KS * (GTPM + GTPN);
     __shared__ double smem[SMEM];
+    // symmetric form: U row j = u_j L column j (U12 = D11 L21^T), with the
+    // pivots u_j = 1 + the factored diagonal blocks' stored diagonals
+    __shared__ double su[NPANEL * NB];
+    if (lower) load_pivots(su, Ut, lda, kk0, KK, threadIdx.x, NT);
     double(*As)[GTPM] = reinterpret_cast<double(*)[GTPM]>(smem);             // [kk][row]
     double(*Bs)[GTPN] = reinterpret_cast<double(*)[GTPN]>(smem + KS * GTPM);  // [kk][col]
     const int tid = threadIdx.x;
@@ -1108,14 +1099,17 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
             const int row = m0 + r;
             pa[t] = row < rend ? Lp[row + (int64_t)kk * lda] : 0.0;
         }
-        // U rows from the step's row-major U buffer (consecutive lanes read
+        // U rows from the step's row-major U buffer, or (symmetric form) L
+        // columns at rows n0 .. to be scaled by u_j (consecutive lanes read
         // consecutive columns)
 #pragma unroll
         for (int t = 0; t < PERB; t++) {
             const int idx = tid + NT * t;
             const int kk2 = idx / GTN, c = idx % GTN;
             const int colg = n0 + c;
-            pb[t] = colg < cend ? Ut[(int64_t)(kk0 + ks + kk2) * lda + colg] : 0.0;
+            pb[t] = colg >= cend ? 0.0
+                  : lower        ? Lp[colg + (int64_t)kk2 * lda]
+                                 : Ut[(int64_t)(kk0 + ks + kk2) * lda + colg];
         }
     };
     fetch(0);
@@ -1129,7 +1123,7 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
 #pragma unroll
         for (int t = 0; t < PERB; t++) {
             const int idx = tid + NT * t;
-            Bs[idx / GTN][idx % GTN] = pb[t];
+            Bs[idx / GTN][idx % GTN] = lower ? pb[t] * su[ks + idx / GTN] : pb[t];
         }
         __syncthreads();
         if (ks + KS < KK) fetch(ks + KS);
@@ -1202,6 +1196,10 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
     const double* Ut = at<double>(arena, side[b].off_ut);
     __shared__ double sA[2][KS][PITCH];   // [stage][kk][row]
     __shared__ double sB[2][KS][PITCH];   // [stage][kk][col]
+    // symmetric form: U row j = u_j L column j (U12 = D11 L21^T); the
+    // stage holds the L column at rows n0 .., scaled when read
+    __shared__ double su[NPANEL * NB];
+    if (lower) load_pivots(su, Ut, lda, kk0, KK, threadIdx.x, 256);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wr = (wv & 1) * 64, wc = (wv >> 1) * 64;
     const int li = lane & 15, lk = lane >> 4;
@@ -1210,8 +1208,9 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
     // rows KW wv .. KW wv + KW - 1 of both operands
     auto issue = [&](int ks, int buf) {
         const int j = kk0 + ks, q = j / NB;
-        const double* Lp = Pb + (int64_t)q * lda * NB + (int64_t)(j % NB) * lda - (k + q * NB) + m0 + 2 * lane;
-        const double* Up = Ut + (int64_t)j * lda + n0 + 2 * lane;
+        const double* Lq = Pb + (int64_t)q * lda * NB + (int64_t)(j % NB) * lda - (k + q * NB);
+        const double* Lp = Lq + m0 + 2 * lane;
+        const double* Up = lower ? Lq + n0 + 2 * lane : Ut + (int64_t)j * lda + n0 + 2 * lane;
 #pragma unroll
         for (int t = 0; t < KW; t++) {
             const int kk = KW * wv + t;
@@ -1260,6 +1259,11 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
             for (int i = 0; i < 4; i++) av[i] = -sA[buf][kk + lk][wr + 16 * i + li];
 #pragma unroll
             for (int j = 0; j < 4; j++) bv[j] = sB[buf][kk + lk][wc + 16 * j + li];
+            if (lower) {
+                const double uj = su[s * KS + kk + lk];
+#pragma unroll
+                for (int j = 0; j < 4; j++) bv[j] *= uj;
+            }
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -1511,17 +1515,18 @@ int capsd_lu_npanels(void) { return NPANEL; }
 // launch (dma = 1 for k_lu_gemm_dma, 0 for the k_lu_gemm tiles).  kinds: bit 0 set if the batch may hold blocks for the pivoted LU
 // (out[b].delta = 0), bit 1 if it may hold delta-form blocks; kernels for a
 // kind the caller rules out are not launched; bit 2 (with bit 1 alone): the
-// delta-form blocks are symmetric (1 - M of a round trip): U12 = D11 L21^T
-// comes from k_lu_l21, no TRSM or update of U rows runs, and the trailing
-// updates skip the tiles wholly above the diagonal (the diagonal blocks are
-// kept whole for k_lu_diag).  Returns the number of update launches.
+// delta-form blocks are symmetric (1 - M of a round trip): the updates take
+// U12 = D11 L21^T from the L21 panel buffers and the pivots (no U buffer),
+// no TRSM or update of U rows runs, and the trailing updates skip the tiles
+// wholly above the diagonal (the diagonal blocks are kept whole for
+// k_lu_diag).  Returns the number of update launches.
 int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockOut* out, int nblocks, int maxN,
                     hipStream_t s, int levels, int xcd_order, int tile, hipEvent_t* gemm_ev, int* gemm_info, int kinds)
 {
     const bool piv = kinds & 1, dlt = kinds & 2;
     // symmetric delta form (bit 2, delta-form blocks only): U12 = D11 L21^T
-    // is written by k_lu_l21, so no TRSM and no update of U rows runs, and
-    // the trailing updates keep the lower triangle
+    // is formed by the updates from k_lu_l21's L21, so no TRSM and no update
+    // of U rows runs, and the trailing updates keep the lower triangle
     const int sym = (kinds & 4) && dlt && !piv ? 1 : 0;
     const LuSide* sd = (const LuSide*)side;
     const int nb8 = xcd_order ? (nblocks + 7) & ~7 : nblocks;
@@ -1610,10 +1615,10 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         if (below > 0) {
             if ((int64_t)((below + 255) / 256) * nblocks >= 512)
                 hipLaunchKernelGGL(k_lu_l21<4>, dim3((below + 255) / 256, nblocks), dim3(256), 0, s, desc, arena, sd,
-                                   out, k, pb, sym);
+                                   out, k, pb);
             else
                 hipLaunchKernelGGL(k_lu_l21<1>, dim3((below + 63) / 64, nblocks), dim3(256), 0, s, desc, arena, sd,
-                                   out, k, pb, sym);
+                                   out, k, pb);
         }
     };
     // buffers [qbegin, qend) take the pivots of the later panels pfirst..plast
