@@ -1252,17 +1252,22 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wavefront's stage-s loads are in LDS
         __syncthreads();                       // everyone's are, and stage s-1's buffer is free
         if (s + 1 < S) issue((s + 1) * KS, buf ^ 1);
+        // symmetric form: this lane's pivots of the stage (rows kk + lk)
+        double us[KS / 4];
+        if (lower) {
+#pragma unroll
+            for (int t = 0; t < KS / 4; t++) us[t] = su[s * KS + 4 * t + lk];
+        }
 #pragma unroll
         for (int kk = 0; kk < KS; kk += 4) {
             double av[4], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) av[i] = -sA[buf][kk + lk][wr + 16 * i + li];
-#pragma unroll
             for (int j = 0; j < 4; j++) bv[j] = sB[buf][kk + lk][wc + 16 * j + li];
-            if (lower) {
-                const double uj = su[s * KS + kk + lk];
 #pragma unroll
-                for (int j = 0; j < 4; j++) bv[j] *= uj;
+            for (int i = 0; i < 4; i++) av[i] = -sA[buf][kk + lk][wr + 16 * i + li];
+            if (lower) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) bv[j] *= us[kk / 4];
             }
 #pragma unroll
             for (int i = 0; i < 4; i++)
