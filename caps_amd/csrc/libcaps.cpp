@@ -994,7 +994,9 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
                 // region rows x cols, depth kk: flops 2 x elements x kk; HBM
                 // bytes: C read and written once, L and U read once.  The
                 // symmetric form needs the elements on and below the
-                // diagonal only (row >= column)
+                // diagonal only (row >= column), and its U rows are L rows
+                // (U12 = D11 L21^T): the operands are the L rows of the union
+                // of the region's row and column ranges, read once
                 const double rows = std::max(0.0, std::min(N, r0 + g[2]) - r0);
                 const double cols = std::max(0.0, std::min(N, c0 + g[4]) - c0);
                 const double kk = rows > 0 && cols > 0 ? g[5] : 0;
@@ -1007,8 +1009,13 @@ void run_batch(caps_t* self, Ctx* c, const BatchPlan& p, int mode, int lu_mode_s
                     const double jb0 = std::max(c0, r0 + 1), jb1 = std::min(c1, e - 1);
                     if (jb1 >= jb0) elems += (jb1 - jb0 + 1) * (e - (jb0 + jb1) / 2);
                 }
+                double operand_rows = rows + cols;
+                if (sym && rows > 0 && cols > 0) {
+                    const double lo = std::max(r0, c0), hi = std::min(r0 + rows, c0 + cols);
+                    if (hi > lo) operand_rows -= hi - lo;   // rows in both ranges are read once
+                }
                 c->acc[12] += 2.0 * elems * kk;
-                c->acc[14] += 16.0 * elems + 8.0 * (rows + cols) * kk;
+                c->acc[14] += 16.0 * elems + 8.0 * operand_rows * kk;
             }
         }
     }

@@ -1233,6 +1233,11 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
     // scale of C per update instead of one per MFMA.
     const int rr = lane & 15, cq = lane >> 4;
     const bool pre = out[b].delta;
+    // symmetric form: a wavefront whose 64 x 64 quadrant lies above the
+    // diagonal and outside the 64 x 64 diagonal blocks (the upper right one
+    // of a diagonal tile; nothing reads those entries) computes and stores
+    // nothing; it still moves its share of every stage
+    const bool idle = lower && (m0 + wr + 63) / NB < (n0 + wc) / NB;
     double4_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -1242,7 +1247,7 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int col = n0 + wc + 16 * j + cq + 4 * q;
-                acc[i][j][q] = (pre && row < rend && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
+                acc[i][j][q] = (pre && !idle && row < rend && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
             }
     }
     issue(0, 0);
@@ -1252,6 +1257,7 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): this wavefront's stage-s loads are in LDS
         __syncthreads();                       // everyone's are, and stage s-1's buffer is free
         if (s + 1 < S) issue((s + 1) * KS, buf ^ 1);
+        if (idle) continue;   // uniform over the wavefront
         // symmetric form: this lane's pivots of the stage (rows kk + lk)
         double us[KS / 4];
         if (lower) {
@@ -1276,7 +1282,7 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
         }
     }
-    gemm_store_tile(A, lda, acc, m0 + wr, n0 + wc, rr, cq, rend, cend, pre);
+    if (!idle) gemm_store_tile(A, lda, acc, m0 + wr, n0 + wc, rr, cq, rend, cend, pre);
 }
 
 // Sum of log|u_ii| (the reference sums them with Kahan compensation,
