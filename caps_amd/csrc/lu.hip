@@ -952,7 +952,8 @@ __global__ void __launch_bounds__(64 * (NPANEL - 1))
 // MFMA layout: lane l, register q of acc[i][j] holds row r0 + 16i + rr,
 // column c0 + 16j + cq + 4q, rr = l%16, cq = l/16): acc holds C - L U
 // (pre) or -L U, to be added to C (loads of half a 16-row band issued
-// before its stores).
+// before its stores).  NT: the pre tile's stores are non-temporal.
+template <bool NT = false>
 __device__ __forceinline__ void gemm_store_tile(double* __restrict__ A, int64_t lda, const double4_t (&acc)[4][4],
                                                 int r0, int c0, int rr, int cq, int rend, int cend, bool pre)
 {
@@ -966,7 +967,12 @@ __device__ __forceinline__ void gemm_store_tile(double* __restrict__ A, int64_t 
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const int col = c0 + 16 * j + cq + 4 * q;
-                    if (col < cend) A[row + (int64_t)col * lda] = acc[i][j][q];
+                    if (col < cend) {
+                        if constexpr (NT)
+                            __builtin_nontemporal_store(acc[i][j][q], &A[row + (int64_t)col * lda]);
+                        else
+                            A[row + (int64_t)col * lda] = acc[i][j][q];
+                    }
                 }
         } else {
             // half a band at a time (8 loads in flight: no spills beside
@@ -1230,7 +1236,9 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
     // tile (loaded beside the first stage, one latency for both) and the
     // tile ends with stores only.  Pivoted blocks (unit diagonal): they
     // start at zero and C + acc is formed at the end, one rounding at the
-    // scale of C per update instead of one per MFMA.
+    // scale of C per update instead of one per MFMA.  The C tile is read
+    // and written once per launch, non-temporally: it does not evict the L
+    // and U rows that the block's other tiles read again from L2.
     const int rr = lane & 15, cq = lane >> 4;
     const bool pre = out[b].delta;
     // symmetric form: a wavefront whose 64 x 64 quadrant lies above the
@@ -1247,7 +1255,8 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int col = n0 + wc + 16 * j + cq + 4 * q;
-                acc[i][j][q] = (pre && !idle && row < rend && col < cend) ? A[row + (int64_t)col * lda] : 0.0;
+                acc[i][j][q] = (pre && !idle && row < rend && col < cend)
+                                   ? __builtin_nontemporal_load(&A[row + (int64_t)col * lda]) : 0.0;
             }
     }
     issue(0, 0);
@@ -1282,7 +1291,7 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
         }
     }
-    if (!idle) gemm_store_tile(A, lda, acc, m0 + wr, n0 + wc, rr, cq, rend, cend, pre);
+    if (!idle) gemm_store_tile<true>(A, lda, acc, m0 + wr, n0 + wc, rr, cq, rend, cend, pre);
 }
 
 // Sum of log|u_ii| (the reference sums them with Kahan compensation,
