@@ -619,11 +619,12 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
             int e2;
             const double mt = frexp(sv * a0m, &e2);
             const int ex = e2 + a0e + S - sh[g][pp];
-            Im[pol * nl2 + a] = mt;
-            Ie[pol * nl2 + a] = ex;
+            // non-temporal: a batch's I tables are far larger than the caches
+            __builtin_nontemporal_store(mt, &Im[pol * nl2 + a]);
+            __builtin_nontemporal_store(ex, &Ie[pol * nl2 + a]);
             if (NPOL == 1) {
-                Im[a] = -fabs(mt);
-                Ie[a] = ex;
+                __builtin_nontemporal_store(-fabs(mt), &Im[a]);
+                __builtin_nontemporal_store(ex, &Ie[a]);
             }
         }
     }
@@ -653,6 +654,8 @@ __global__ void __launch_bounds__(256) k_I(const BlockDesc* __restrict__ desc, c
 struct UV {
     double u, v;
 };
+
+typedef double nt_double2 __attribute__((ext_vector_type(2)));   // 16-byte non-temporal stores
 
 __device__ __forceinline__ UV uv(int l, int m)
 {
@@ -791,8 +794,11 @@ __global__ void __launch_bounds__(256) k_assemble(const BlockDesc* __restrict__ 
         const int64_t r = 2 * i2, c = 2 * j2;
         // rows r, r+1 of a column are adjacent: one 16-byte store per column
         // (r and lda even, the matrix 256-byte aligned)
-        *reinterpret_cast<double2*>(Am + r + c * lda) = make_double2(sg * EE, sg * ME);
-        *reinterpret_cast<double2*>(Am + r + (c + 1) * lda) = make_double2(sg * EM, sg * MM);
+        // (non-temporal: the matrices of a batch are far larger than the
+        // caches, and these lines would only evict the I windows)
+        __builtin_nontemporal_store((nt_double2){sg * EE, sg * ME}, reinterpret_cast<nt_double2*>(Am + r + c * lda));
+        __builtin_nontemporal_store((nt_double2){sg * EM, sg * MM},
+                                    reinterpret_cast<nt_double2*>(Am + r + (c + 1) * lda));
         if (i2 == j2) {
             double* dg = at<double>(arena, d.off_diag);
             dg[r] = EE;
