@@ -207,7 +207,9 @@ class Caps:
         return dict(zip(["mie", "K", "I", "assemble", "lu", "total"], t))
 
     def stats(self, reset=False):
-        """Accumulated stage times and work counters of this object's batches."""
+        """Accumulated stage times and work counters of this object's batches
+        (the Mie kernel runs beside the K quadrature: "K" is the time the K
+        stage adds after the Mie kernel ends)."""
         a = np.zeros(16)
         lib().capsx_stats(self.ptr, _dptr(a))
         if reset:

@@ -120,6 +120,8 @@ struct Ctx {
     hipEvent_t lu_done[MAX_LU_STREAMS];
     hipEvent_t part_i_done[MAX_PARTS];   // part g's I finished
     hipEvent_t lu_ready = nullptr;        // a one-part batch's matrices assembled
+    hipEvent_t head_fork = nullptr;       // a batch's start, for the Mie kernel's stream
+    hipEvent_t mie_done = nullptr;        // the batch's Mie coefficients written
     char* arena = nullptr;
     size_t arena_size = 0;
     BlockDesc* d_desc = nullptr;
@@ -182,6 +184,8 @@ Ctx* ctx_of(caps_t* self)
         }
         for (auto& e : c->part_i_done) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&c->lu_ready, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&c->head_fork, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&c->mie_done, hipEventDisableTiming));
         HIPCHECK(hipMalloc(&c->d_status, sizeof(int)));
         HIPCHECK(hipHostMalloc((void**)&c->h_status, sizeof(int), hipHostMallocDefault));
         self->priv = c;
@@ -214,6 +218,8 @@ void ctx_free(Ctx* c)
     }
     for (auto& e : c->part_i_done) hipEventDestroy(e);
     hipEventDestroy(c->lu_ready);
+    hipEventDestroy(c->head_fork);
+    hipEventDestroy(c->mie_done);
     hipStreamDestroy(c->stream);
     delete c;
 }
@@ -804,9 +810,17 @@ int enqueue_batch(Ctx* c, const BatchPlan& p, int mode, int lu_mode_shortcuts, b
     const int n = p.n;
     hipStream_t s = c->stream;
     if (events) HIPCHECK(hipEventRecord(c->ev[0], s));
-    capsd_launch_mie(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, s);
-    if (events) HIPCHECK(hipEventRecord(c->ev[1], s));
+    // the Mie coefficients beside the K quadrature (independent; the I
+    // tables need both), on the LU stream that no LU group or part uses:
+    // the K stage's time is then what it adds after the Mie kernel
+    hipStream_t sm = c->lu_stream[MAX_LU_STREAMS - 1];
+    HIPCHECK(hipEventRecord(c->head_fork, s));
+    HIPCHECK(hipStreamWaitEvent(sm, c->head_fork, 0));
+    capsd_launch_mie(c->d_desc, c->arena, g_lfac_dev, n, p.max_ldim, sm);
+    if (events) HIPCHECK(hipEventRecord(c->ev[1], sm));
+    HIPCHECK(hipEventRecord(c->mie_done, sm));
     capsd_launch_K(c->d_desc, c->arena, g_lfac_dev, n, p.max_waves, p.max_nK, k_segments(p), c->d_status, s);
+    HIPCHECK(hipStreamWaitEvent(s, c->mie_done, 0));
     if (events) HIPCHECK(hipEventRecord(c->ev[2], s));
     int nlaunched = 0;
     // part g on stream g (part 0 on the library's stream); its I waits for

@@ -473,7 +473,9 @@ int capsx_mfma_probe(const double *A, const double *B, double *D);
 /* Diagnostics: HIP events around every trailing-update launch (on != 0). */
 void capsx_profile(caps_t *self, int on);
 /* Accumulated statistics since the last reset: out[0..5] stage milliseconds
- * (Mie, K, I, assembly, LU, total), out[6] blocks, out[7] blocks factored,
+ * (Mie, K, I, assembly, LU, total; the Mie kernel runs beside the K
+ * quadrature, whose time is what it adds after the Mie kernel ends),
+ * out[6] blocks, out[7] blocks factored,
  * out[8] algorithmic LU flops, out[9] batches, out[10] I integrals; with
  * capsx_profile on, out[11..14] the milliseconds, algorithmic flops,
  * launches and algorithmic bytes of the trailing updates run by
