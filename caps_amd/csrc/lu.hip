@@ -1167,8 +1167,9 @@ k_lu_gemm(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const Lu
 // (columns) of the product that are not stored.
 typedef __attribute__((address_space(3))) void lds_void;
 // KS = 16: 73.7 KB of LDS, two workgroups per CU; KS = 8: 36.9 KB, OCC
-// workgroups per CU as the registers allow
-template <int KS, int OCC>
+// workgroups per CU as the registers allow.  NT: the C tile is loaded and
+// stored non-temporally (launches whose C region is larger than the caches)
+template <int KS, int OCC, bool NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
               const BlockOut* __restrict__ out, int k, int r_off, int r_lim, int c_off, int c_lim, int kk0, int KK,
@@ -1237,8 +1238,8 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
     // tile ends with stores only.  Pivoted blocks (unit diagonal): they
     // start at zero and C + acc is formed at the end, one rounding at the
     // scale of C per update instead of one per MFMA.  The C tile is read
-    // and written once per launch, non-temporally: it does not evict the L
-    // and U rows that the block's other tiles read again from L2.
+    // and written once per launch; non-temporally (NT) it does not evict the
+    // L and U rows that the block's other tiles read again from L2.
     const int rr = lane & 15, cq = lane >> 4;
     const bool pre = out[b].delta;
     // symmetric form: a wavefront whose 64 x 64 quadrant lies above the
@@ -1255,8 +1256,9 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int col = n0 + wc + 16 * j + cq + 4 * q;
+                const double* cp = &A[row + (int64_t)col * lda];
                 acc[i][j][q] = (pre && !idle && row < rend && col < cend)
-                                   ? __builtin_nontemporal_load(&A[row + (int64_t)col * lda]) : 0.0;
+                                   ? (NT ? __builtin_nontemporal_load(cp) : *cp) : 0.0;
             }
     }
     issue(0, 0);
@@ -1291,7 +1293,7 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
         }
     }
-    if (!idle) gemm_store_tile<true>(A, lda, acc, m0 + wr, n0 + wc, rr, cq, rend, cend, pre);
+    if (!idle) gemm_store_tile<NT>(A, lda, acc, m0 + wr, n0 + wc, rr, cq, rend, cend, pre);
 }
 
 // Sum of log|u_ii| (the reference sums them with Kahan compensation,
@@ -1585,12 +1587,24 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
         } else {
             const int gx = (rows + 127) / 128, gy = (cols + 127) / 128;
             dma = 1;
-            if (gemm_kernel == 2)
-                hipLaunchKernelGGL((k_lu_gemm_dma<8, 3>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out,
-                                   k, r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym);
+            // non-temporal C tiles when the launch's C region (the blocks'
+            // trailing matrices, the lower half in the symmetric form) is
+            // larger than the 256 MB memory-side cache: then nothing keeps
+            // it for the next panel's kernels anyway, and cached it would
+            // evict the L rows the tiles share
+            const bool nt = (double)rows * cols * 8.0 * nblocks * (sym ? 0.5 : 1.0) > 256e6;
+#define GEMM_DMA(KSV, OCCV, NTV)                                                                                      \
+    hipLaunchKernelGGL((k_lu_gemm_dma<KSV, OCCV, NTV>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k, \
+                       r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym)
+            if (gemm_kernel == 2 && nt)
+                GEMM_DMA(8, 3, true);
+            else if (gemm_kernel == 2)
+                GEMM_DMA(8, 3, false);
+            else if (nt)
+                GEMM_DMA(16, 2, true);
             else
-                hipLaunchKernelGGL((k_lu_gemm_dma<16, 2>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out,
-                                   k, r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym);
+                GEMM_DMA(16, 2, false);
+#undef GEMM_DMA
         }
         if (gemm_ev) (void)hipEventRecord(gemm_ev[2 * p + 1], s);
         if (gemm_info) {
