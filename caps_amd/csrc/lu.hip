@@ -1256,9 +1256,11 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int col = n0 + wc + 16 * j + cq + 4 * q;
-                const double* cp = &A[row + (int64_t)col * lda];
-                acc[i][j][q] = (pre && !idle && row < rend && col < cend)
-                                   ? (NT ? __builtin_nontemporal_load(cp) : *cp) : 0.0;
+                acc[i][j][q] = 0.0;
+                if (pre && !idle && row < rend && col < cend) {
+                    const double* cp = A + row + (int64_t)col * lda;
+                    acc[i][j][q] = NT ? __builtin_nontemporal_load(cp) : *cp;
+                }
             }
     }
     issue(0, 0);
