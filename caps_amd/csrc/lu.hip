@@ -1173,11 +1173,13 @@ template <int KS, int OCC, bool NT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, OCC)))
 k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, const LuSide* __restrict__ side,
               const BlockOut* __restrict__ out, int k, int r_off, int r_lim, int c_off, int c_lim, int kk0, int KK,
-              int gx, int gy, int nblocks, int xcd_order, int lower)
+              int gx, int gy, int nblocks, int xcd_order, int lower, int tri)
 {
     constexpr int GT = 128, PITCH = GT + 16;
     constexpr int KW = KS / 4;   // kk rows of a stage moved by each wavefront
-    const int wg = blockIdx.x, tiles = gx * gy;
+    // tri: a square region of the symmetric form (gx = gy), launched over
+    // its lower triangle of tiles only (row tile >= column tile)
+    const int wg = blockIdx.x, tiles = tri ? gx * (gx + 1) / 2 : gx * gy;
     int b, tile;
     if (xcd_order) {
         const int xcd = wg & 7, local = wg >> 3;
@@ -1193,8 +1195,19 @@ k_lu_gemm_dma(const BlockDesc* __restrict__ desc, char* __restrict__ arena, cons
     const int N = d.N;
     const int r0 = k + r_off, rend = min(N, r0 + r_lim);
     const int c0 = k + c_off, cend = min(N, c0 + c_lim);
-    const int m0 = r0 + (tile % gx) * GT;
-    const int n0 = c0 + (tile / gx) * GT;
+    int ti = tile % gx, tj = tile / gx;
+    if (tri) {
+        // column-major over the triangle: column tile j holds row tiles j .. gx - 1
+        tj = 0;
+        ti = tile;
+        while (ti >= gx - tj) {
+            ti -= gx - tj;
+            tj++;
+        }
+        ti += tj;
+    }
+    const int m0 = r0 + ti * GT;
+    const int n0 = c0 + tj * GT;
     if (m0 >= rend || n0 >= cend) return;
     if (lower && m0 + GT <= n0) return;   // symmetric form: a tile wholly above the diagonal
     const int64_t lda = d.lda;
@@ -1595,9 +1608,13 @@ int capsd_launch_lu(const BlockDesc* desc, char* arena, const void* side, BlockO
             // it for the next panel's kernels anyway, and cached it would
             // evict the L rows the tiles share
             const bool nt = (double)rows * cols * 8.0 * nblocks * (sym ? 0.5 : 1.0) > 256e6;
-#define GEMM_DMA(KSV, OCCV, NTV)                                                                                      \
-    hipLaunchKernelGGL((k_lu_gemm_dma<KSV, OCCV, NTV>), dim3(gx * gy * nb8), dim3(256), 0, s, desc, arena, sd, out, k, \
-                       r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym)
+            // a square region of the symmetric form: only its lower
+            // triangle of tiles is launched (the others would exit at once)
+            const int tri = sym && r_off == c_off && rows == cols ? 1 : 0;
+            const unsigned ntile = tri ? (unsigned)(gx * (gx + 1) / 2) : (unsigned)(gx * gy);
+#define GEMM_DMA(KSV, OCCV, NTV)                                                                                    \
+    hipLaunchKernelGGL((k_lu_gemm_dma<KSV, OCCV, NTV>), dim3(ntile * nb8), dim3(256), 0, s, desc, arena, sd, out, k, \
+                       r_off, r_lim, c_off, c_lim, kk0, KK, gx, gy, nblocks, xcd_order, sym, tri)
             if (gemm_kernel == 2 && nt)
                 GEMM_DMA(8, 3, true);
             else if (gemm_kernel == 2)
