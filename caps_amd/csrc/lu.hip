@@ -23,6 +23,13 @@
 //                   buffer), double-buffered;
 //   k_lu_gemm       the same product with register-staged operands, for
 //                   128 x 64, 64 x 128, 64 x 64 and 256 x 128 tiles.
+// Column-dominant blocks take the delta form instead (no pivot search, the
+// matrix holding -M with u = 1 + d): k_lu_diag factors the 64 x 64 diagonal
+// block and inverts its triangles, k_lu_l21 forms L21 = A21 U11^{-1} and
+// k_lu_trsm_mfma U12 = L11^{-1} A12.  Round-trip blocks are symmetric: there
+// the updates form U12 = D11 L21^T from the L21 panel buffers and the
+// pivots (no TRSM, no U buffer), keep the lower triangle, and a square
+// region launches only its lower triangle of tiles.
 // k_lu_finalize sums log|u_ii| in double-double arithmetic.
 // Matrices are column-major with leading dimension lda.
 #include <hip/hip_runtime.h>
@@ -58,7 +65,9 @@ struct LuSide {
                          // panel's first row (at column c rows c and ipiv[c]
                          // were exchanged)
     int64_t off_ut;      // NPANEL NB x lda doubles: the step's U rows, row
-                         // k + i at i lda (columns at their matrix index)
+                         // k + i at i lda (columns at their matrix index);
+                         // symmetric form: only panel q's pivots, at row q NB,
+                         // columns 0 .. 63 (never U12 entries)
     int64_t off_inv;     // NPANEL x 2 NB x NB doubles (delta form): per panel
                          // L11^{-1} column-major, then U11^{-1} row-major
 };
