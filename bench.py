@@ -462,7 +462,7 @@ def main():
     errors["F_rel_vs_reference"] = abs(F / ref_F - 1)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
-        "metric": "logdetD blocks/s @R/L=200 FP64 (batched+surrogate form; plain form and LU FP64 TF/s alongside)",
+        "metric": "logdetD blocks/s @R/L=200 FP64, batched+surrogate vs plain; FP64 MFMA TF/s",
         "value": value, "unit": "blocks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1000 * total / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (analytic Drude material, no dataset)",
